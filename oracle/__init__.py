@@ -1,0 +1,258 @@
+"""CPU oracle for the BYO hot path -- TEST INFRASTRUCTURE ONLY.
+
+Thin numpy/ctypes front end over ``oracle/liboracle.so`` (built from
+``oracle/oracle.c`` by ``oracle/Makefile``).  Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import
+this package, and only as the checker / CPU baseline.  The product
+(``byo_amd``) never imports it.
+
+Each wrapper names the reference file:line its C function restates; see
+``oracle/oracle.h`` for the exact semantics and how each is pinned.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+NONE = 0xFFFFFFFF
+MLP_NPARAM = 32 * 6 + 32 + 32 * 32 + 32 + 3 * 32 + 3
+
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        u64, u32, i64, dbl, i32 = C.c_uint64, C.c_uint32, C.c_int64, C.c_double, C.c_int
+        sig = {
+            "orc_splitmix64": (u64, [u64]),
+            "orc_ref_hash64": (u64, [u64]),
+            "orc_ref_hash32": (u32, [u32]),
+            "orc_rmat_edges": (None, [i32, u64, dbl, dbl, dbl, u64, u64, _u32p, _u32p]),
+            "orc_update_rmat": (None, [u64, u64, u64, _u32p, _u32p]),
+            "orc_sort_unique_u64": (u64, [_u64p, u64]),
+            "orc_build_sym_csr": (u64, [u64, u64, _u32p, _u32p, _u64p, _u32p]),
+            "orc_keys_to_csr": (None, [u64, _u64p, u64, _u64p, _u32p]),
+            "orc_bfs_levels": (None, [u64, _u64p, _u32p, u32, _u32p]),
+            "orc_bfs_diropt": (i32, [u64, u64, _u64p, _u32p, u32, i64, _u32p, _u8p,
+                                     _u64p, _u64p, i32]),
+            "orc_check_bfs_tree": (u64, [u64, _u64p, _u32p, u32, _u32p, _u32p]),
+            "orc_pagerank_f64": (i32, [u64, _u64p, _u32p, dbl, i32, dbl, _f64p, _f64p]),
+            "orc_ppr_f64": (i32, [u64, _u64p, _u32p, u32, dbl, i32, _f64p]),
+            "orc_pagerank_ref_order": (i32, [u64, _u64p, _u32p, dbl, i32, dbl, _f64p]),
+            "orc_cc_union_find": (None, [u64, _u64p, _u32p, _u32p]),
+            "orc_cc_labelprop": (i32, [u64, _u64p, _u32p, _u32p]),
+            "orc_tc": (u64, [u64, _u64p, _u32p]),
+            "orc_tc_brute": (u64, [u64, _u64p, _u32p]),
+            "orc_orient": (None, [u64, _u64p, _u32p, _u64p, _u32p]),
+            "orc_map_sum": (u64, [u64, _u32p]),
+            "orc_mlp_forward_backward": (dbl, [_f32p, _f32p, _i32p, i32, _f32p, C.c_void_p]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+class Graph:
+    """Host CSR: offsets uint64[n+1], ids uint32[m] (run_csr.cc:28-191)."""
+
+    def __init__(self, offsets, ids):
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        self.ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        self.n = len(self.offsets) - 1
+        self.m = int(self.offsets[-1]) if self.n >= 0 else 0
+
+    def degrees(self):
+        return np.diff(self.offsets).astype(np.uint32)
+
+    def neighbors(self, v):
+        return self.ids[self.offsets[v]:self.offsets[v + 1]]
+
+
+# ---- generators ---------------------------------------------------------
+
+def rmat_edges(scale, num_edges, a=0.5, b=0.1, c=0.1, seed=1, first=0):
+    src = np.empty(num_edges, np.uint32)
+    dst = np.empty(num_edges, np.uint32)
+    lib().orc_rmat_edges(scale, num_edges, a, a + b, a + b + c, seed, first, src, dst)
+    return src, dst
+
+
+def update_rmat(graph_n, count, first=0):
+    """Batch_insert_runner's update stream (run_unweighted.h:797-806)."""
+    src = np.empty(count, np.uint32)
+    dst = np.empty(count, np.uint32)
+    lib().orc_update_rmat(graph_n, first, count, src, dst)
+    return src, dst
+
+
+def sym_csr(n, src, dst):
+    src = np.ascontiguousarray(src, np.uint32)
+    dst = np.ascontiguousarray(dst, np.uint32)
+    off = np.empty(n + 1, np.uint64)
+    ids = np.empty(max(2 * len(src), 1), np.uint32)
+    m = lib().orc_build_sym_csr(n, len(src), src, dst, off, ids)
+    return Graph(off, ids[:m].copy())
+
+
+def rmat_graph(scale, edge_factor=16, a=0.5, b=0.1, c=0.1, seed=1):
+    n = 1 << scale
+    src, dst = rmat_edges(scale, edge_factor * n, a, b, c, seed)
+    return sym_csr(n, src, dst)
+
+
+def sort_unique(keys):
+    k = np.array(keys, dtype=np.uint64, copy=True)
+    w = lib().orc_sort_unique_u64(k, len(k))
+    return k[:w]
+
+
+def keys_to_csr(n, keys):
+    keys = np.ascontiguousarray(keys, np.uint64)
+    off = np.empty(n + 1, np.uint64)
+    ids = np.empty(max(len(keys), 1), np.uint32)
+    lib().orc_keys_to_csr(n, keys, len(keys), off, ids)
+    return Graph(off, ids[:len(keys)].copy())
+
+
+def graph_keys(g):
+    src = np.repeat(np.arange(g.n, dtype=np.uint64), np.diff(g.offsets).astype(np.int64))
+    return (src << np.uint64(32)) | g.ids.astype(np.uint64)
+
+
+# ---- algorithms ---------------------------------------------------------
+
+def bfs_levels(g, src):
+    lv = np.empty(max(g.n, 1), np.uint32)
+    lib().orc_bfs_levels(g.n, g.offsets, g.ids, src, lv)
+    return lv[:g.n]
+
+
+def bfs_diropt(g, src, threshold=-1, max_log=256):
+    par = np.empty(max(g.n, 1), np.uint32)
+    dirlog = np.zeros(max_log, np.uint8)
+    fsize = np.zeros(max_log, np.uint64)
+    fdeg = np.zeros(max_log, np.uint64)
+    nl = lib().orc_bfs_diropt(g.n, g.m, g.offsets, g.ids, src, threshold, par,
+                              dirlog, fsize, fdeg, max_log)
+    k = min(nl, max_log)
+    return par[:g.n], dirlog[:k].copy(), fsize[:k].copy(), fdeg[:k].copy()
+
+
+def check_bfs_tree(g, src, parents, levels):
+    return int(lib().orc_check_bfs_tree(
+        g.n, g.offsets, g.ids, src,
+        np.ascontiguousarray(parents, np.uint32), np.ascontiguousarray(levels, np.uint32)))
+
+
+def pagerank(g, damping=0.85, iters=20, eps=0.0):
+    p = np.empty(max(g.n, 1), np.float64)
+    l1 = np.zeros(max(iters, 1), np.float64)
+    it = lib().orc_pagerank_f64(g.n, g.offsets, g.ids, damping, iters, eps, p, l1)
+    return p[:g.n], l1[:it].copy()
+
+
+def ppr(g, seed, damping=0.85, iters=30):
+    p = np.empty(max(g.n, 1), np.float64)
+    lib().orc_ppr_f64(g.n, g.offsets, g.ids, seed, damping, iters, p)
+    return p[:g.n]
+
+
+def pagerank_ref_order(g, damping=0.85, iters=100, eps=1e-6):
+    p = np.empty(max(g.n, 1), np.float64)
+    it = lib().orc_pagerank_ref_order(g.n, g.offsets, g.ids, damping, iters, eps, p)
+    return p[:g.n], it
+
+
+def cc(g):
+    lab = np.empty(max(g.n, 1), np.uint32)
+    lib().orc_cc_union_find(g.n, g.offsets, g.ids, lab)
+    return lab[:g.n]
+
+
+def cc_labelprop(g):
+    lab = np.empty(max(g.n, 1), np.uint32)
+    lib().orc_cc_labelprop(g.n, g.offsets, g.ids, lab)
+    return lab[:g.n]
+
+
+def tc(g):
+    return int(lib().orc_tc(g.n, g.offsets, g.ids))
+
+
+def tc_brute(g):
+    return int(lib().orc_tc_brute(g.n, g.offsets, g.ids))
+
+
+def orient(g):
+    off = np.empty(g.n + 1, np.uint64)
+    ids = np.empty(max(g.m // 2, 1), np.uint32)
+    lib().orc_orient(g.n, g.offsets, g.ids, off, ids)
+    return Graph(off, ids[:int(off[-1])].copy())
+
+
+def map_sum(g):
+    return int(lib().orc_map_sum(g.m, g.ids))
+
+
+def mlp_forward_backward(params, x, labels):
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    labels = np.ascontiguousarray(labels, np.int32)
+    grads = np.empty(MLP_NPARAM, np.float32)
+    logits = np.empty(3 * len(labels), np.float32)
+    loss = lib().orc_mlp_forward_backward(params, x, labels, len(labels), grads,
+                                          logits.ctypes.data)
+    return loss, grads, logits.reshape(-1, 3)
+
+
+# ---- fixtures -----------------------------------------------------------
+
+def read_adjacency_graph(path):
+    """AdjacencyGraph text format (graph_io.cc:35-100): header, n, m,
+    n offsets, m ids."""
+    with open(path) as f:
+        toks = f.read().split()
+    assert toks[0] == "AdjacencyGraph", toks[0]
+    n, m = int(toks[1]), int(toks[2])
+    off = np.empty(n + 1, np.uint64)
+    off[:n] = np.array(toks[3:3 + n], dtype=np.uint64)
+    off[n] = m
+    ids = np.array(toks[3 + n:3 + n + m], dtype=np.uint32)
+    return Graph(off, ids)
+
+
+def bfs_sources(g, k, seed=1):
+    """k distinct sources with degree > 0 drawn by SplitMix64(seed)."""
+    deg = np.diff(g.offsets)
+    out, x = [], seed
+    L = lib()
+    tries = 0
+    while len(out) < k and tries < 100 * k + 1000:
+        x = L.orc_splitmix64(x)
+        v = x % g.n
+        tries += 1
+        if deg[v] > 0 and v not in out:
+            out.append(int(v))
+    return out

@@ -1,0 +1,731 @@
+/*
+ * oracle.c -- CPU restatement of the BYO hot path.  TEST INFRASTRUCTURE
+ * ONLY: linked by tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg as the checker and the CPU baseline, never by the
+ * product library.  Each function cites the reference file:line it follows
+ * (paths relative to the reference repository root).
+ *
+ * Pinning (how each function is checked independently of its own formula):
+ *   orc_ref_hash32/64, orc_update_rmat : hand-evaluated known answers for
+ *       i = 0, 1 and a committed golden prefix (tests/golden/).
+ *   orc_build_sym_csr : the reference's own graph_io_test.cc normalisation
+ *       cases and the inputs/ fixtures (n, m, degrees).
+ *   orc_bfs_levels    : textbook FIFO BFS; inputs/ facts (reached count,
+ *       level count); test_bfs.cc parent cases.
+ *   orc_bfs_diropt    : levels equal orc_bfs_levels on every graph;
+ *       direction log equals the closed-form rule of edge_map_data.h:257-306.
+ *   orc_pagerank_f64  : closed form on a star (2x2 linear system), dense
+ *       power iteration on <=128-vertex fixtures, per-iteration mass law.
+ *   orc_cc_*          : brute-force DSU; test_cc.cc partitions.
+ *   orc_tc            : orc_tc_brute (independent triple enumeration) and
+ *       inputs/ facts (107, 2, 0 triangles).
+ *   orc_mlp_*         : central finite differences of the loss.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ===================================================================== */
+/* hashing / generators                                                  */
+/* ===================================================================== */
+
+uint64_t orc_splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+/* run_unweighted.h:691-702 */
+uint64_t orc_ref_hash64(uint64_t u) {
+  uint64_t v = u * 3935559000370003845ull + 2691343689449507681ull;
+  v ^= v >> 21;
+  v ^= v << 37;
+  v ^= v >> 4;
+  v *= 4768777513237032717ull;
+  v ^= v << 20;
+  v ^= v >> 41;
+  v ^= v << 5;
+  return v;
+}
+
+/* run_unweighted.h:720-728 */
+uint32_t orc_ref_hash32(uint32_t a) {
+  a = (a + 0x7ed55d16u) + (a << 12);
+  a = (a ^ 0xc761c23cu) ^ (a >> 19);
+  a = (a + 0x165667b1u) + (a << 5);
+  a = (a + 0xd3a2646cu) ^ (a << 9);
+  a = (a + 0xfd7046c5u) + (a << 3);
+  a = (a ^ 0xb55a4f09u) ^ (a >> 16);
+  return a;
+}
+
+void orc_rmat_edges(int scale, uint64_t num_edges, double a, double ab,
+                    double abc, uint64_t seed, uint64_t first_edge,
+                    uint32_t *src, uint32_t *dst) {
+  const uint64_t base = orc_splitmix64(seed);
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < (int64_t)num_edges; k++) {
+    uint64_t i = first_edge + (uint64_t)k;
+    uint32_t u = 0, v = 0;
+    for (int l = 0; l < scale; l++) {
+      uint64_t h = orc_splitmix64(base + i * (uint64_t)scale + (uint64_t)l);
+      double r = (double)(h >> 11) * 0x1.0p-53;
+      uint32_t bit = 1u << (scale - 1 - l);
+      if (r < a) {
+      } else if (r < ab) {
+        v |= bit;
+      } else if (r < abc) {
+        u |= bit;
+      } else {
+        u |= bit;
+        v |= bit;
+      }
+    }
+    src[k] = u;
+    dst[k] = v;
+  }
+}
+
+static int log2_up(uint64_t n) { /* parlay::log2_up: ceil(log2 n) */
+  int r = 0;
+  while ((1ull << r) < n) r++;
+  return r;
+}
+
+/* Batch_insert_runner's update stream (run_unweighted.h:797-806) through
+ * rMat::operator() / rMatRec (run_unweighted.h:755-777).  All uint32
+ * arithmetic wraps exactly as intT = uintE does there. */
+void orc_update_rmat(uint64_t graph_n, uint64_t first, uint64_t count,
+                     uint32_t *src, uint32_t *dst) {
+  const double a = 0.5, b = 0.1, c = 0.1;
+  const double ab = a + b, abc = a + b + c;
+  const uint32_t nn = (uint32_t)(1ull << (log2_up(graph_n) - 1));
+  const uint32_t seed = (uint32_t)orc_ref_hash64(0); /* r.ith_rand(0) */
+  const uint32_t h = orc_ref_hash32(seed);
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < (int64_t)count; k++) {
+    uint32_t i = (uint32_t)(first + (uint64_t)k);
+    uint32_t start = orc_ref_hash32((uint32_t)(2u * i) * h);
+    uint32_t stride = orc_ref_hash32((uint32_t)(2u * i + 1u) * h);
+    uint32_t row = 0, col = 0;
+    for (uint32_t size = nn; size > 1; size >>= 1) {
+      double r = (double)orc_ref_hash32(start) / 2147483647.0;
+      if (r < a) {
+      } else if (r < ab) {
+        col += size >> 1;
+      } else if (r < abc) {
+        row += size >> 1;
+      } else {
+        row += size >> 1;
+        col += size >> 1;
+      }
+      start += stride;
+    }
+    src[k] = row;
+    dst[k] = col;
+  }
+}
+
+/* ===================================================================== */
+/* containers                                                            */
+/* ===================================================================== */
+
+static void radix_sort_u64(uint64_t *keys, uint64_t n) {
+  if (n < 2) return;
+  uint64_t *tmp = (uint64_t *)malloc(n * sizeof(uint64_t));
+  uint64_t *cnt = (uint64_t *)malloc(65536 * sizeof(uint64_t));
+  uint64_t *from = keys, *to = tmp;
+  for (int pass = 0; pass < 4; pass++) {
+    int sh = pass * 16;
+    memset(cnt, 0, 65536 * sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; i++) cnt[(from[i] >> sh) & 0xFFFF]++;
+    int trivial = 0;
+    for (int d = 0; d < 65536; d++)
+      if (cnt[d] == n) trivial = 1;
+    if (trivial) continue;
+    uint64_t s = 0;
+    for (int d = 0; d < 65536; d++) {
+      uint64_t c = cnt[d];
+      cnt[d] = s;
+      s += c;
+    }
+    for (uint64_t i = 0; i < n; i++) to[cnt[(from[i] >> sh) & 0xFFFF]++] = from[i];
+    uint64_t *t = from;
+    from = to;
+    to = t;
+  }
+  if (from != keys) memcpy(keys, from, n * sizeof(uint64_t));
+  free(tmp);
+  free(cnt);
+}
+
+uint64_t orc_sort_unique_u64(uint64_t *keys, uint64_t n) {
+  radix_sort_u64(keys, n);
+  if (n == 0) return 0;
+  uint64_t w = 1;
+  for (uint64_t i = 1; i < n; i++)
+    if (keys[i] != keys[w - 1]) keys[w++] = keys[i];
+  return w;
+}
+
+void orc_keys_to_csr(uint64_t n, const uint64_t *keys, uint64_t len,
+                     uint64_t *offsets, uint32_t *ids) {
+  memset(offsets, 0, (n + 1) * sizeof(uint64_t));
+  for (uint64_t i = 0; i < len; i++) offsets[(keys[i] >> 32) + 1]++;
+  for (uint64_t v = 0; v < n; v++) offsets[v + 1] += offsets[v];
+  for (uint64_t i = 0; i < len; i++) ids[i] = (uint32_t)keys[i];
+}
+
+/* graph_io.h:829-847 (sym_graph_from_edges: symmetrize, sort, dedupe, drop
+ * self loops), with n fixed by the caller instead of inferred. */
+uint64_t orc_build_sym_csr(uint64_t n, uint64_t ne, const uint32_t *src,
+                           const uint32_t *dst, uint64_t *offsets,
+                           uint32_t *ids) {
+  uint64_t *keys = (uint64_t *)malloc((2 * ne + 1) * sizeof(uint64_t));
+  uint64_t k = 0;
+  for (uint64_t i = 0; i < ne; i++) {
+    uint64_t u = src[i], v = dst[i];
+    if (u == v) continue;
+    keys[k++] = (u << 32) | v;
+    keys[k++] = (v << 32) | u;
+  }
+  uint64_t m = orc_sort_unique_u64(keys, k);
+  orc_keys_to_csr(n, keys, m, offsets, ids);
+  free(keys);
+  return m;
+}
+
+/* ===================================================================== */
+/* BFS                                                                   */
+/* ===================================================================== */
+
+void orc_bfs_levels(uint64_t n, const uint64_t *off, const uint32_t *ids,
+                    uint32_t src, uint32_t *level) {
+  for (uint64_t i = 0; i < n; i++) level[i] = ORC_NONE;
+  if (n == 0) return;
+  uint32_t *q = (uint32_t *)malloc(n * sizeof(uint32_t));
+  uint64_t head = 0, tail = 0;
+  level[src] = 0;
+  q[tail++] = src;
+  while (head < tail) {
+    uint32_t u = q[head++];
+    for (uint64_t e = off[u]; e < off[u + 1]; e++) {
+      uint32_t t = ids[e];
+      if (level[t] == ORC_NONE) {
+        level[t] = level[u] + 1;
+        q[tail++] = t;
+      }
+    }
+  }
+  free(q);
+}
+
+static inline int bm_get(const uint32_t *bm, uint32_t i) {
+  return (bm[i >> 5] >> (i & 31)) & 1u;
+}
+
+/* BFS.h:47-64 driving edgeMapData (edge_map_data.h:254-307):
+ *   threshold = M/20; dense if (isDense && |F| > n/10), else compute
+ *   sum of out-degrees of F; empty result when it is 0; dense when
+ *   |F| + sum > threshold; otherwise sparse push (edgeMapSparse,
+ *   edge_map_blocked.h:18-92).  Dense pull (edgeMapDense :47-175): every
+ *   unvisited v scans N(v) in order and takes the first frontier member as
+ *   parent (early exit).  Sparse push: CAS(parents[t], NONE, u) per edge. */
+int orc_bfs_diropt(uint64_t n, uint64_t m, const uint64_t *off,
+                   const uint32_t *ids, uint32_t src, int64_t threshold,
+                   uint32_t *parents, uint8_t *dirlog, uint64_t *fsize,
+                   uint64_t *fdeg, int max_log) {
+  const uint64_t thr = threshold < 0 ? m / 20 : (uint64_t)threshold;
+  const uint64_t nw = (n + 31) / 32;
+  for (uint64_t i = 0; i < n; i++) parents[i] = ORC_NONE;
+  if (n == 0) return 0;
+  parents[src] = src;
+  uint32_t *cur = (uint32_t *)malloc(n * sizeof(uint32_t));
+  uint32_t *nxt = (uint32_t *)malloc(n * sizeof(uint32_t));
+  uint32_t *bm = (uint32_t *)calloc(nw, sizeof(uint32_t));
+  uint32_t *bm2 = (uint32_t *)calloc(nw, sizeof(uint32_t));
+  uint64_t fs = 1;
+  cur[0] = src;
+  int is_dense = 0, have_sparse = 1, have_dense = 0;
+  int lv = 0;
+  int nthreads = 1;
+#ifdef _OPENMP
+  nthreads = omp_get_max_threads();
+#endif
+  uint32_t **tbuf = (uint32_t **)calloc((size_t)nthreads, sizeof(uint32_t *));
+  uint64_t *tcnt = (uint64_t *)calloc((size_t)nthreads + 1, sizeof(uint64_t));
+  uint64_t *tcap = (uint64_t *)calloc((size_t)nthreads, sizeof(uint64_t));
+
+  while (fs > 0) {
+    /* out-degree sum of F from whichever representation exists */
+    uint64_t od = 0;
+    if (have_sparse) {
+#pragma omp parallel for reduction(+ : od) schedule(static)
+      for (int64_t i = 0; i < (int64_t)fs; i++)
+        od += off[cur[i] + 1] - off[cur[i]];
+    } else {
+#pragma omp parallel for reduction(+ : od) schedule(static)
+      for (int64_t v = 0; v < (int64_t)n; v++)
+        if (bm_get(bm, (uint32_t)v)) od += off[v + 1] - off[v];
+    }
+    int dense;
+    if (is_dense && fs > n / 10) {
+      dense = 1;
+    } else if (od == 0) {
+      dense = 2; /* edgeMapData returns an empty subset (:289) */
+    } else {
+      dense = (fs + od > thr) ? 1 : 0;
+    }
+    if (lv < max_log) {
+      if (dirlog) dirlog[lv] = (uint8_t)dense;
+      if (fsize) fsize[lv] = fs;
+      if (fdeg) fdeg[lv] = od;
+    }
+    lv++;
+    if (dense == 2) break;
+    uint64_t count = 0;
+    if (dense) {
+      if (!have_dense) { /* toDense: set_atomic per element (:357-364) */
+        memset(bm, 0, nw * sizeof(uint32_t));
+        for (uint64_t i = 0; i < fs; i++) bm[cur[i] >> 5] |= 1u << (cur[i] & 31);
+      }
+      memset(bm2, 0, nw * sizeof(uint32_t));
+#pragma omp parallel for reduction(+ : count) schedule(dynamic, 1024)
+      for (int64_t v = 0; v < (int64_t)n; v++) {
+        if (parents[v] != ORC_NONE) continue;
+        for (uint64_t e = off[v]; e < off[v + 1]; e++) {
+          uint32_t u = ids[e];
+          if (bm_get(bm, u)) {
+            parents[v] = u;
+            __atomic_fetch_or(&bm2[v >> 5], 1u << (v & 31), __ATOMIC_RELAXED);
+            count++;
+            break;
+          }
+        }
+      }
+      uint32_t *t = bm;
+      bm = bm2;
+      bm2 = t;
+      is_dense = 1;
+      have_dense = 1;
+      have_sparse = 0;
+    } else {
+      if (!have_sparse) { /* toSparse: pack_index, ascending (:339-354) */
+        uint64_t k = 0;
+        for (uint64_t v = 0; v < n; v++)
+          if (bm_get(bm, (uint32_t)v)) cur[k++] = (uint32_t)v;
+      }
+#pragma omp parallel
+      {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        tcnt[tid] = 0;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t i = 0; i < (int64_t)fs; i++) {
+          uint32_t u = cur[i];
+          for (uint64_t e = off[u]; e < off[u + 1]; e++) {
+            uint32_t t = ids[e];
+            uint32_t expect = ORC_NONE;
+            if (__atomic_load_n(&parents[t], __ATOMIC_RELAXED) == ORC_NONE &&
+                __atomic_compare_exchange_n(&parents[t], &expect, u, 0,
+                                            __ATOMIC_RELAXED,
+                                            __ATOMIC_RELAXED)) {
+              if (tcnt[tid] == tcap[tid]) {
+                tcap[tid] = tcap[tid] ? 2 * tcap[tid] : 1024;
+                tbuf[tid] = (uint32_t *)realloc(tbuf[tid],
+                                                tcap[tid] * sizeof(uint32_t));
+              }
+              tbuf[tid][tcnt[tid]++] = t;
+            }
+          }
+        }
+      }
+      /* Reducer_Vector::get_sequence: concatenate in worker order
+       * (reducer.hpp:228-240) */
+      for (int t = 0; t < nthreads; t++) {
+        memcpy(nxt + count, tbuf[t], tcnt[t] * sizeof(uint32_t));
+        count += tcnt[t];
+      }
+      uint32_t *t = cur;
+      cur = nxt;
+      nxt = t;
+      is_dense = 0;
+      have_sparse = 1;
+      have_dense = 0;
+    }
+    fs = count;
+  }
+  for (int t = 0; t < nthreads; t++) free(tbuf[t]);
+  free(tbuf);
+  free(tcnt);
+  free(tcap);
+  free(cur);
+  free(nxt);
+  free(bm);
+  free(bm2);
+  return lv;
+}
+
+uint64_t orc_check_bfs_tree(uint64_t n, const uint64_t *off,
+                            const uint32_t *ids, uint32_t src,
+                            const uint32_t *parents, const uint32_t *level) {
+  for (uint64_t v = 0; v < n; v++) {
+    uint32_t p = parents[v];
+    if (level[v] == ORC_NONE) {
+      if (p != ORC_NONE) return v + 1;
+      continue;
+    }
+    if (v == src) {
+      if (p != src) return v + 1;
+      continue;
+    }
+    if (p == ORC_NONE || p >= n) return v + 1;
+    if (level[p] == ORC_NONE || level[p] + 1 != level[v]) return v + 1;
+    /* p must be a neighbour of v (adjacency sorted: binary search) */
+    uint64_t lo = off[v], hi = off[v + 1];
+    while (lo < hi) {
+      uint64_t mid = (lo + hi) / 2;
+      if (ids[mid] < p) lo = mid + 1; else hi = mid;
+    }
+    if (lo == off[v + 1] || ids[lo] != p) return v + 1;
+  }
+  return 0;
+}
+
+/* ===================================================================== */
+/* PageRank                                                              */
+/* ===================================================================== */
+
+/* PageRank.h:80-123 (PR_F / PR_Vertex_F, Jacobi), evaluated as a pull over
+ * the symmetric adjacency: identical arithmetic result up to summation
+ * order. */
+int orc_pagerank_f64(uint64_t n, const uint64_t *off, const uint32_t *ids,
+                     double damping, int max_iters, double eps, double *p,
+                     double *l1_log) {
+  if (n == 0) return 0;
+  double *contrib = (double *)malloc(n * sizeof(double));
+  double *pn = (double *)malloc(n * sizeof(double));
+  const double added = (1.0 - damping) * (1.0 / (double)n);
+  for (uint64_t i = 0; i < n; i++) p[i] = 1.0 / (double)n;
+  int it = 0;
+  while (it < max_iters) {
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < (int64_t)n; s++) {
+      uint64_t d = off[s + 1] - off[s];
+      contrib[s] = d ? p[s] / (double)d : 0.0;
+    }
+    double l1 = 0.0;
+#pragma omp parallel for reduction(+ : l1) schedule(dynamic, 1024)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+      double acc = 0.0;
+      for (uint64_t e = off[i]; e < off[i + 1]; e++) acc += contrib[ids[e]];
+      double v = damping * acc + added;
+      pn[i] = v;
+      l1 += fabs(v - p[i]);
+    }
+    memcpy(p, pn, n * sizeof(double));
+    if (l1_log) l1_log[it] = l1;
+    it++;
+    if (eps > 0 && l1 < eps) break;
+  }
+  free(contrib);
+  free(pn);
+  return it;
+}
+
+int orc_ppr_f64(uint64_t n, const uint64_t *off, const uint32_t *ids,
+                uint32_t seed, double damping, int iters, double *p) {
+  if (n == 0) return 0;
+  double *contrib = (double *)malloc(n * sizeof(double));
+  double *pn = (double *)malloc(n * sizeof(double));
+  for (uint64_t i = 0; i < n; i++) p[i] = 0.0;
+  p[seed] = 1.0;
+  for (int it = 0; it < iters; it++) {
+    for (uint64_t s = 0; s < n; s++) {
+      uint64_t d = off[s + 1] - off[s];
+      contrib[s] = d ? p[s] / (double)d : 0.0;
+    }
+    for (uint64_t i = 0; i < n; i++) {
+      double acc = 0.0;
+      for (uint64_t e = off[i]; e < off[i + 1]; e++) acc += contrib[ids[e]];
+      pn[i] = damping * acc + (i == seed ? 1.0 - damping : 0.0);
+    }
+    memcpy(p, pn, n * sizeof(double));
+  }
+  free(contrib);
+  free(pn);
+  return iters;
+}
+
+/* PageRank.h:125-193 with parallel_for run in index order: the apply
+ * function overwrites p_div[u] inside the sweep that reads p_div[s]. */
+int orc_pagerank_ref_order(uint64_t n, const uint64_t *off,
+                           const uint32_t *ids, double damping, int max_iters,
+                           double eps, double *p) {
+  if (n == 0) return 0;
+  const double added = (1.0 - damping) * (1.0 / (double)n);
+  double *pc = (double *)malloc(n * sizeof(double));
+  double *pn = (double *)malloc(n * sizeof(double));
+  double *pdiv = (double *)malloc(n * sizeof(double));
+  for (uint64_t i = 0; i < n; i++) {
+    pc[i] = 1.0 / (double)n;
+    pdiv[i] = (1.0 / (double)n) / (double)(off[i + 1] - off[i]);
+  }
+  int it = 0;
+  while (it < max_iters) {
+    for (uint64_t u = 0; u < n; u++) {
+      double acc = 0.0;
+      for (uint64_t e = off[u]; e < off[u + 1]; e++) acc += pdiv[ids[e]];
+      pn[u] = damping * acc + added;
+      pdiv[u] = pn[u] / (double)(off[u + 1] - off[u]);
+    }
+    double l1 = 0.0;
+    for (uint64_t i = 0; i < n; i++) l1 += fabs(pc[i] - pn[i]);
+    memcpy(pc, pn, n * sizeof(double));
+    it++;
+    if (l1 < eps) break;
+  }
+  memcpy(p, pc, n * sizeof(double));
+  free(pc);
+  free(pn);
+  free(pdiv);
+  return it;
+}
+
+/* ===================================================================== */
+/* Connectivity                                                          */
+/* ===================================================================== */
+
+/* find_compress (SimpleUnionAsync/Connectivity.h:32-47) */
+static uint32_t uf_find(uint32_t *par, uint32_t i) {
+  uint32_t j = i;
+  while (par[j] != j) j = par[j];
+  uint32_t tmp;
+  while ((tmp = par[i]) > j) {
+    par[i] = j;
+    i = tmp;
+  }
+  return j;
+}
+
+/* unite_impl (:49-65) run for every CSR entry in vertex order, then
+ * finish() (:113-116). */
+void orc_cc_union_find(uint64_t n, const uint64_t *off, const uint32_t *ids,
+                       uint32_t *labels) {
+  for (uint64_t i = 0; i < n; i++) labels[i] = (uint32_t)i;
+  for (uint64_t u = 0; u < n; u++) {
+    for (uint64_t e = off[u]; e < off[u + 1]; e++) {
+      uint32_t a = uf_find(labels, (uint32_t)u);
+      uint32_t b = uf_find(labels, ids[e]);
+      if (a == b) continue;
+      if (a > b) labels[a] = b; else labels[b] = a;
+    }
+  }
+  for (uint64_t i = 0; i < n; i++) uf_find(labels, (uint32_t)i);
+}
+
+/* labelprop_cc::CC (LabelPropagation/Connectivity.h:78-146): repeat
+ * label[t] = min(label[t], label[s]) over edges until no change. */
+int orc_cc_labelprop(uint64_t n, const uint64_t *off, const uint32_t *ids,
+                     uint32_t *labels) {
+  for (uint64_t i = 0; i < n; i++) labels[i] = (uint32_t)i;
+  int rounds = 0;
+  int changed = 1;
+  while (changed) {
+    changed = 0;
+    rounds++;
+    for (uint64_t u = 0; u < n; u++) {
+      for (uint64_t e = off[u]; e < off[u + 1]; e++) {
+        uint32_t t = ids[e];
+        if (labels[u] < labels[t]) {
+          labels[t] = labels[u];
+          changed = 1;
+        }
+      }
+    }
+  }
+  return rounds;
+}
+
+/* ===================================================================== */
+/* Triangle counting                                                     */
+/* ===================================================================== */
+
+static inline int rank_less(const uint64_t *off, uint32_t u, uint32_t v) {
+  uint64_t du = off[u + 1] - off[u], dv = off[v + 1] - off[v];
+  return du < dv || (du == dv && u < v);
+}
+
+/* filterGraph(G, rank[u] < rank[v]) (Triangle.h:185-190 ->
+ * graph_mutation.h:15-81): count pass, scan, order-preserving fill. */
+void orc_orient(uint64_t n, const uint64_t *off, const uint32_t *ids,
+                uint64_t *dg_off, uint32_t *dg_ids) {
+  dg_off[0] = 0;
+  for (uint64_t u = 0; u < n; u++) {
+    uint64_t c = 0;
+    for (uint64_t e = off[u]; e < off[u + 1]; e++)
+      c += rank_less(off, (uint32_t)u, ids[e]);
+    dg_off[u + 1] = dg_off[u] + c;
+  }
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t u = 0; u < (int64_t)n; u++) {
+    uint64_t w = dg_off[u];
+    for (uint64_t e = off[u]; e < off[u + 1]; e++)
+      if (rank_less(off, (uint32_t)u, ids[e])) dg_ids[w++] = ids[e];
+  }
+}
+
+/* seq_merge_full (intersect.h:68-90) */
+static uint64_t merge_count(const uint32_t *a, uint64_t na, const uint32_t *b,
+                            uint64_t nb) {
+  uint64_t i = 0, j = 0, c = 0;
+  while (i < na && j < nb) {
+    if (a[i] == b[j]) {
+      c++;
+      i++;
+      j++;
+    } else if (a[i] < b[j]) {
+      i++;
+    } else {
+      j++;
+    }
+  }
+  return c;
+}
+
+/* CountDirectedBalanced (Triangle.h:99-150) without the block split. */
+uint64_t orc_tc(uint64_t n, const uint64_t *off, const uint32_t *ids) {
+  uint64_t m = n ? off[n] : 0;
+  uint64_t *dg_off = (uint64_t *)malloc((n + 1) * sizeof(uint64_t));
+  uint32_t *dg_ids = (uint32_t *)malloc((m / 2 + 1) * sizeof(uint32_t));
+  orc_orient(n, off, ids, dg_off, dg_ids);
+  uint64_t total = 0;
+#pragma omp parallel for reduction(+ : total) schedule(dynamic, 64)
+  for (int64_t u = 0; u < (int64_t)n; u++) {
+    const uint32_t *nu = dg_ids + dg_off[u];
+    uint64_t du = dg_off[u + 1] - dg_off[u];
+    for (uint64_t k = 0; k < du; k++) {
+      uint32_t v = nu[k];
+      total += merge_count(nu, du, dg_ids + dg_off[v], dg_off[v + 1] - dg_off[v]);
+    }
+  }
+  free(dg_off);
+  free(dg_ids);
+  return total;
+}
+
+static int has_edge(const uint64_t *off, const uint32_t *ids, uint32_t u,
+                    uint32_t w) {
+  uint64_t lo = off[u], hi = off[u + 1];
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) / 2;
+    if (ids[mid] < w) lo = mid + 1; else hi = mid;
+  }
+  return lo < off[u + 1] && ids[lo] == w;
+}
+
+uint64_t orc_tc_brute(uint64_t n, const uint64_t *off, const uint32_t *ids) {
+  uint64_t c = 0;
+  for (uint64_t u = 0; u < n; u++)
+    for (uint64_t e = off[u]; e < off[u + 1]; e++) {
+      uint32_t v = ids[e];
+      if (v <= u) continue;
+      for (uint64_t f = off[v]; f < off[v + 1]; f++) {
+        uint32_t w = ids[f];
+        if (w <= v) continue;
+        c += has_edge(off, ids, (uint32_t)u, w);
+      }
+    }
+  return c;
+}
+
+/* ===================================================================== */
+/* Map                                                                   */
+/* ===================================================================== */
+
+uint64_t orc_map_sum(uint64_t m, const uint32_t *ids) {
+  uint64_t s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (int64_t i = 0; i < (int64_t)m; i++) s += ids[i];
+  return s;
+}
+
+/* ===================================================================== */
+/* MLP 6->32->32->3 forward/backward (double accumulation)               */
+/* ===================================================================== */
+
+double orc_mlp_forward_backward(const float *P, const float *x,
+                                const int32_t *labels, int batch, float *G,
+                                float *logits) {
+  const float *W1 = P, *b1 = W1 + 32 * 6, *W2 = b1 + 32, *b2 = W2 + 32 * 32,
+              *W3 = b2 + 32, *b3 = W3 + 3 * 32;
+  double g[ORC_MLP_NPARAM];
+  memset(g, 0, sizeof(g));
+  double *gW1 = g, *gb1 = gW1 + 32 * 6, *gW2 = gb1 + 32, *gb2 = gW2 + 32 * 32,
+         *gW3 = gb2 + 32, *gb3 = gW3 + 3 * 32;
+  double loss = 0.0;
+  for (int s = 0; s < batch; s++) {
+    const float *xs = x + 6 * s;
+    double h1[32], h2[32], z[3];
+    for (int o = 0; o < 32; o++) {
+      double a = b1[o];
+      for (int i = 0; i < 6; i++) a += (double)W1[o * 6 + i] * xs[i];
+      h1[o] = a > 0 ? a : 0;
+    }
+    for (int o = 0; o < 32; o++) {
+      double a = b2[o];
+      for (int i = 0; i < 32; i++) a += (double)W2[o * 32 + i] * h1[i];
+      h2[o] = a > 0 ? a : 0;
+    }
+    for (int o = 0; o < 3; o++) {
+      double a = b3[o];
+      for (int i = 0; i < 32; i++) a += (double)W3[o * 32 + i] * h2[i];
+      z[o] = a;
+      if (logits) logits[3 * s + o] = (float)a;
+    }
+    double mx = z[0] > z[1] ? z[0] : z[1];
+    if (z[2] > mx) mx = z[2];
+    double se = exp(z[0] - mx) + exp(z[1] - mx) + exp(z[2] - mx);
+    int y = labels[s];
+    loss += -(z[y] - mx - log(se));
+    double dz[3];
+    for (int o = 0; o < 3; o++)
+      dz[o] = (exp(z[o] - mx) / se - (o == y ? 1.0 : 0.0)) / batch;
+    double dh2[32], dh1[32];
+    for (int i = 0; i < 32; i++) dh2[i] = 0;
+    for (int o = 0; o < 3; o++) {
+      gb3[o] += dz[o];
+      for (int i = 0; i < 32; i++) {
+        gW3[o * 32 + i] += dz[o] * h2[i];
+        dh2[i] += W3[o * 32 + i] * dz[o];
+      }
+    }
+    for (int i = 0; i < 32; i++) {
+      dh2[i] = h2[i] > 0 ? dh2[i] : 0;
+      dh1[i] = 0;
+    }
+    for (int o = 0; o < 32; o++) {
+      gb2[o] += dh2[o];
+      for (int i = 0; i < 32; i++) {
+        gW2[o * 32 + i] += dh2[o] * h1[i];
+        dh1[i] += W2[o * 32 + i] * dh2[o];
+      }
+    }
+    for (int o = 0; o < 32; o++) {
+      double d = h1[o] > 0 ? dh1[o] : 0;
+      gb1[o] += d;
+      for (int i = 0; i < 6; i++) gW1[o * 6 + i] += d * xs[i];
+    }
+  }
+  if (G)
+    for (int k = 0; k < ORC_MLP_NPARAM; k++) G[k] = (float)g[k];
+  return loss / batch;
+}

@@ -1,0 +1,369 @@
+"""Pins the CPU oracle against the reference's own tests and fixtures and
+against independent mathematics (brute force, closed forms, scipy).  No GPU.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import connected_components, shortest_path
+
+import oracle as O
+
+M32 = 0xFFFFFFFF
+M64 = (1 << 64) - 1
+
+
+def edges_graph(n, edges):
+    """MakeUnweightedSymmetricGraph (gbbs/unit_tests/graph_test_utils.h:36-59)."""
+    if edges:
+        s = np.array([e[0] for e in edges], np.uint32)
+        d = np.array([e[1] for e in edges], np.uint32)
+    else:
+        s = d = np.zeros(0, np.uint32)
+    return O.sym_csr(n, s, d)
+
+
+def scipy_levels(g, src):
+    A = sp.csr_matrix((np.ones(g.m), g.ids.astype(np.int64), g.offsets.astype(np.int64)),
+                      shape=(g.n, g.n))
+    dist = shortest_path(A, unweighted=True, indices=src)
+    lv = np.full(g.n, O.NONE, np.uint32)
+    fin = np.isfinite(dist)
+    lv[fin] = dist[fin].astype(np.uint32)
+    return lv
+
+
+# ---- hashes and the update generator (run_unweighted.h:689-806) -----------
+
+def py_hash32(a):
+    a &= M32
+    a = ((a + 0x7ed55d16) + (a << 12)) & M32
+    a = ((a ^ 0xc761c23c) ^ (a >> 19)) & M32
+    a = ((a + 0x165667b1) + (a << 5)) & M32
+    a = ((a + 0xd3a2646c) ^ (a << 9)) & M32
+    a = ((a + 0xfd7046c5) + (a << 3)) & M32
+    a = ((a ^ 0xb55a4f09) ^ (a >> 16)) & M32
+    return a
+
+
+def py_hash64(u):
+    v = (u * 3935559000370003845 + 2691343689449507681) & M64
+    v ^= v >> 21
+    v ^= (v << 37) & M64
+    v ^= v >> 4
+    v = (v * 4768777513237032717) & M64
+    v ^= (v << 20) & M64
+    v ^= v >> 41
+    v ^= (v << 5) & M64
+    return v
+
+
+def py_update_pair(graph_n, i):
+    nn = 1 << ((graph_n - 1).bit_length() - 1)
+    h = py_hash32(py_hash64(0) & M32)
+    start = py_hash32(((2 * i) & M32) * h & M32)
+    stride = py_hash32(((2 * i + 1) & M32) * h & M32)
+    row = col = 0
+    size = nn
+    while size > 1:
+        r = py_hash32(start) / 2147483647.0
+        if r < 0.5:
+            pass
+        elif r < 0.5 + 0.1:
+            col += size // 2
+        elif r < 0.5 + 0.1 + 0.1:
+            row += size // 2
+        else:
+            row += size // 2
+            col += size // 2
+        start = (start + stride) & M32
+        size //= 2
+    return row, col
+
+
+def test_hash_known_answers():
+    L = O.lib()
+    for x in (0, 1, 2, 12345, 0xFFFFFFFF):
+        assert L.orc_ref_hash32(x) == py_hash32(x)
+    for x in (0, 1, 2, 1 << 40, M64):
+        assert L.orc_ref_hash64(x) == py_hash64(x)
+    # literals evaluated once from the Python restatement above
+    assert L.orc_ref_hash32(0) == 0x6B4ED927 and L.orc_ref_hash32(1) == 0xB48681B6
+    assert L.orc_ref_hash64(0) == 0x7B439D0C1FD00DE3
+    assert L.orc_ref_hash64(1) == 0xBEA952A971BA8E83
+
+
+def test_update_rmat_matches_python_restatement(golden):
+    s, d = O.update_rmat(1 << 16, 64)
+    for i in range(64):
+        assert (int(s[i]), int(d[i])) == py_update_pair(1 << 16, i)
+    assert [[int(a), int(b)] for a, b in zip(s, d)] == golden["update_rmat_n65536_first64"]
+    s, d = O.update_rmat(1 << 24, 16, first=1000)
+    for k in range(16):
+        assert (int(s[k]), int(d[k])) == py_update_pair(1 << 24, 1000 + k)
+    # H4: endpoints confined to [0, 2^(ceil(log2 n) - 1))
+    s, d = O.update_rmat(1 << 16, 100000)
+    assert s.max() < (1 << 15) and d.max() < (1 << 15)
+
+
+def test_rmat_generator_prefix_and_distribution(golden):
+    s, d = O.rmat_edges(16, 64, seed=1)
+    assert [[int(a), int(b)] for a, b in zip(s, d)] == golden["rmat_s16_seed1_first64"]
+    # quadrant frequencies at the top level follow (a, b, c, d)
+    s, d = O.rmat_edges(12, 200000, seed=7)
+    top_r = (s >> 11) & 1
+    top_c = (d >> 11) & 1
+    freq = [np.mean((top_r == r) & (top_c == c)) for r, c in ((0, 0), (0, 1), (1, 0), (1, 1))]
+    assert np.allclose(freq, [0.5, 0.1, 0.1, 0.3], atol=0.01)
+
+
+# ---- graph construction (graph_io_test.cc:137-230, graph_test.cc) -----------
+
+def test_sym_graph_normalisation():
+    g = edges_graph(2, [(0, 1), (1, 0), (0, 1)])
+    assert g.n == 2 and g.m == 2
+    assert list(g.neighbors(0)) == [1] and list(g.neighbors(1)) == [0]
+    g = edges_graph(3, [(1, 2)])
+    assert g.m == 2 and g.degrees()[0] == 0
+    g = edges_graph(7, [(1, 5), (0, 5), (6, 3), (2, 0), (1, 0)])
+    assert g.m == 10
+    assert list(g.neighbors(0)) == [1, 2, 5]
+    assert list(g.neighbors(1)) == [0, 5]
+    assert list(g.neighbors(2)) == [0]
+    assert list(g.neighbors(3)) == [6]
+    assert list(g.neighbors(4)) == []
+    g = edges_graph(4, [(2, 2), (1, 1)])  # self loops dropped
+    assert g.m == 0
+    g = edges_graph(0, [])
+    assert g.n == 0 and g.m == 0
+
+
+def test_fixture_facts(fixture_graphs, golden):
+    # brute-force facts of the reference's inputs/ graphs
+    expect = {"rMatGraph_J_5_100": (128, 708, 107, 4, 125, 7),
+              "triangles.txt": (5, 12, 2, 1, 5, 3),
+              "star.txt": (10, 18, 0, 1, 10, 2)}
+    for name, (n, m, tri, comps, reached, levels) in expect.items():
+        g = fixture_graphs[name]
+        f = golden[name]
+        src = f["src"]
+        assert (g.n, g.m) == (n, m) == (f["n"], f["m"])
+        assert O.tc_brute(g) == O.tc(g) == tri == f["triangles"]
+        lab = O.cc(g)
+        assert len(np.unique(lab)) == comps == f["components"]
+        lv = O.bfs_levels(g, src)
+        assert np.array_equal(lv, scipy_levels(g, src))
+        assert (lv != O.NONE).sum() == reached
+        assert lv[lv != O.NONE].max() + 1 == levels
+        assert list(lv) == f["levels"]
+
+
+# ---- BFS (BFS.h:30-64, test_bfs.cc:16-51) ---------------------------------
+
+def test_bfs_reference_unit_cases():
+    g = edges_graph(3, [])
+    par, *_ = O.bfs_diropt(g, 1)
+    assert list(par) == [O.NONE, 1, O.NONE]
+    g = edges_graph(7, [(0, 1), (2, 3), (3, 4), (3, 5), (4, 5), (5, 6)])
+    par, *_ = O.bfs_diropt(g, 1)
+    assert list(par) == [1, 1] + [O.NONE] * 5
+    par, *_ = O.bfs_diropt(g, 2)
+    assert par[0] == O.NONE and par[1] == O.NONE and par[2] == 2 and par[3] == 2
+    assert par[4] in (3, 5) and par[5] in (3, 4) and par[6] == 5
+
+
+def rule(prev_dense, fs, fd, n, m):
+    """edge_map_data.h:257-306 closed form."""
+    if prev_dense and fs > n // 10:
+        return 1
+    if fd == 0:
+        return 2
+    return 1 if fs + fd > m // 20 else 0
+
+
+@pytest.mark.parametrize("scale,src", [(10, 3), (14, 10), (16, 10)])
+def test_bfs_diropt_levels_and_rule(scale, src):
+    g = O.rmat_graph(scale)
+    lv = O.bfs_levels(g, src)
+    assert np.array_equal(lv, scipy_levels(g, src)) if scale <= 14 else True
+    par, dirlog, fs, fd = O.bfs_diropt(g, src)
+    assert O.check_bfs_tree(g, src, par, lv) == 0
+    # frontier k is exactly level k; its degree sum is fixed by levels
+    deg = np.diff(g.offsets)
+    nl = lv[lv != O.NONE].max() + 1
+    for k in range(len(dirlog)):
+        at = lv == k
+        assert fs[k] == at.sum()
+        assert fd[k] == deg[at].sum()
+    prev = 0
+    for k in range(len(dirlog)):
+        assert dirlog[k] == rule(prev, fs[k], fd[k], g.n, g.m)
+        prev = 1 if dirlog[k] == 1 else 0
+    assert len(dirlog) in (nl, nl + 1)
+
+
+def test_bfs_tree_checker_catches_planted_errors():
+    g = O.rmat_graph(10)
+    src = 3
+    lv = O.bfs_levels(g, src)
+    par, *_ = O.bfs_diropt(g, src)
+    assert O.check_bfs_tree(g, src, par, lv) == 0
+    reached = np.nonzero((lv != O.NONE) & (lv > 1))[0]
+    bad = par.copy()
+    v = reached[0]
+    bad[v] = src  # src is not at level lv[v]-1 (lv[v] > 1)
+    assert O.check_bfs_tree(g, src, bad, lv) == v + 1
+    unreached = np.nonzero(lv == O.NONE)[0]
+    if len(unreached):
+        bad = par.copy()
+        bad[unreached[0]] = src
+        assert O.check_bfs_tree(g, src, bad, lv) != 0
+
+
+# ---- PageRank (PageRank.h:80-193) ------------------------------------------
+
+def test_pagerank_star_closed_form(fixture_graphs):
+    g = fixture_graphs["star.txt"]  # centre 0, 9 leaves
+    n, k, d = g.n, 9, 0.85
+    t = (1 - d) / n
+    pl = t * (1 + d / k) / (1 - d * d)
+    pc = d * k * pl + t
+    p, _ = O.pagerank(g, d, 300, 0.0)
+    assert abs(p[0] - pc) < 1e-14
+    assert np.allclose(p[1:], pl, rtol=0, atol=1e-14)
+
+
+def test_pagerank_dense_power_iteration(fixture_graphs):
+    g = fixture_graphs["rMatGraph_J_5_100"]
+    n, d = g.n, 0.85
+    deg = np.diff(g.offsets).astype(np.float64)
+    A = np.zeros((n, n))
+    for i in range(n):
+        for s in g.neighbors(i):
+            A[i, s] = 1.0 / deg[s]
+    p = np.full(n, 1.0 / n)
+    for _ in range(20):
+        p = d * (A @ p) + (1 - d) / n
+    q, l1 = O.pagerank(g, d, 20, 0.0)
+    assert np.abs(p - q).max() < 1e-15
+    assert len(l1) == 20
+
+
+def test_pagerank_mass_law_and_eps_stop():
+    g = O.rmat_graph(12)
+    d = 0.85
+    deg = np.diff(g.offsets)
+    prev = np.full(g.n, 1.0 / g.n)
+    for it in range(1, 6):
+        p, _ = O.pagerank(g, d, it, 0.0)
+        assert abs(p.sum() - ((1 - d) + d * prev[deg > 0].sum())) < 1e-12
+        prev = p
+    p, l1 = O.pagerank(g, d, 100, 1e-6)
+    assert l1[-1] < 1e-6 and (len(l1) == 1 or l1[-2] >= 1e-6)
+
+
+def test_pagerank_reference_order_same_fixed_point():
+    g = O.rmat_graph(11)
+    p1, _ = O.pagerank(g, 0.85, 200, 0.0)
+    p2, it = O.pagerank_ref_order(g, 0.85, 200, 1e-13)
+    assert np.abs(p1 - p2).sum() < 1e-9
+
+
+def test_ppr_mass_and_seed():
+    g = O.rmat_graph(10)
+    seed = O.bfs_sources(g, 1)[0]
+    p = O.ppr(g, seed, 0.85, 60)
+    comp = O.cc(g) == O.cc(g)[seed]
+    assert p[~comp].sum() == 0
+    assert abs(p.sum() - 1.0) < 1e-9  # no isolated vertex can hold mass here
+    assert p[seed] >= 0.15
+
+
+# ---- Connectivity (SimpleUnionAsync/Connectivity.h:32-136) -----------------
+
+def test_cc_reference_unit_cases():
+    g = edges_graph(3, [])
+    assert list(O.cc(g)) == [0, 1, 2]
+    g = edges_graph(7, [(0, 1), (2, 3), (3, 4), (3, 5), (4, 5), (5, 6)])
+    assert list(O.cc(g)) == [0, 0, 2, 2, 2, 2, 2]
+
+
+@pytest.mark.parametrize("scale", [8, 12, 16])
+def test_cc_min_label_vs_scipy(scale):
+    g = O.rmat_graph(scale, edge_factor=2)
+    A = sp.csr_matrix((np.ones(g.m), g.ids.astype(np.int64), g.offsets.astype(np.int64)),
+                      shape=(g.n, g.n))
+    nc, comp = connected_components(A, directed=False)
+    minid = np.full(nc, g.n, np.int64)
+    np.minimum.at(minid, comp, np.arange(g.n))
+    expect = minid[comp].astype(np.uint32)
+    assert np.array_equal(O.cc(g), expect)
+    assert np.array_equal(O.cc_labelprop(g), expect)
+
+
+# ---- Triangle counting (Triangle.h:57-204) ---------------------------------
+
+@pytest.mark.parametrize("scale,ef", [(8, 8), (10, 16), (12, 4)])
+def test_tc_vs_brute(scale, ef):
+    g = O.rmat_graph(scale, edge_factor=ef)
+    assert O.tc(g) == O.tc_brute(g)
+
+
+def test_tc_closed_forms():
+    # complete graph K_k has C(k,3) triangles
+    k = 9
+    g = edges_graph(k, [(i, j) for i in range(k) for j in range(i + 1, k)])
+    assert O.tc(g) == k * (k - 1) * (k - 2) // 6
+    # trace(A^3)/6 on an RMAT graph
+    g = O.rmat_graph(9, edge_factor=8)
+    A = np.zeros((g.n, g.n), np.int64)
+    for i in range(g.n):
+        A[i, g.neighbors(i)] = 1
+    assert O.tc(g) == int(np.trace(A @ A @ A)) // 6
+
+
+def test_orientation_properties():
+    g = O.rmat_graph(10)
+    dg = O.orient(g)
+    assert dg.m * 2 == g.m
+    deg = np.diff(g.offsets)
+    for u in range(0, g.n, 7):
+        for v in dg.neighbors(u):
+            assert (deg[u], u) < (deg[v], v)
+        nb = dg.neighbors(u)
+        assert np.all(nb[1:] > nb[:-1])
+
+
+# ---- containers / Map -------------------------------------------------------
+
+def test_keys_csr_roundtrip_and_map_sum():
+    g = O.rmat_graph(10)
+    keys = O.graph_keys(g)
+    assert np.all(keys[1:] > keys[:-1])
+    g2 = O.keys_to_csr(g.n, keys)
+    assert np.array_equal(g2.offsets, g.offsets) and np.array_equal(g2.ids, g.ids)
+    assert O.map_sum(g) == int(g.ids.astype(np.uint64).sum())
+    rng = np.random.default_rng(5)
+    k = rng.integers(0, 1 << 40, 10000, dtype=np.uint64)
+    k = np.concatenate([k, k[:3000]])
+    assert np.array_equal(O.sort_unique(k), np.unique(k))
+
+
+# ---- MLP reference math -----------------------------------------------------
+
+def test_mlp_gradients_finite_difference():
+    rng = np.random.default_rng(0)
+    P = (rng.standard_normal(O.MLP_NPARAM) * 0.3).astype(np.float32)
+    x = rng.standard_normal((16, 6)).astype(np.float32)
+    y = rng.integers(0, 3, 16).astype(np.int32)
+    loss, g, _ = O.mlp_forward_backward(P, x, y)
+    h = 1e-2
+    idx = rng.choice(O.MLP_NPARAM, 40, replace=False)
+    for k in idx:
+        Pp = P.astype(np.float64).copy()
+        Pm = Pp.copy()
+        Pp[k] += h
+        Pm[k] -= h
+        lp, _, _ = O.mlp_forward_backward(Pp.astype(np.float32), x, y)
+        lm, _, _ = O.mlp_forward_backward(Pm.astype(np.float32), x, y)
+        fd = (lp - lm) / (float(np.float32(Pp[k])) - float(np.float32(Pm[k])))
+        assert abs(fd - g[k]) < 2e-3 + 2e-2 * abs(g[k]), (k, fd, g[k])
