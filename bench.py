@@ -1,0 +1,220 @@
+"""Benchmark of the BYO hot path on one MI355X.
+
+Workload (one "step"): direction-optimizing BFS from 8 seeded sources over a
+device-resident RMAT scale-22 static CSR (a=.5 b=.1 c=.1, edge factor 16,
+symmetrised, deduplicated, uint32 ids).  `value` is BFS GTEPS over the timed
+steps (Graph500 convention: undirected edges reached / time).  The same JSON
+line carries FP32-Kahan and FP64 PageRank it/s (20 iterations) with their L1
+difference, CC and TC times, CPMA insert/remove edges/s at batch 1e7, the
+roofline of the BFS traversal, and the CPU baseline (the oracle's OpenMP
+direction-optimizing BFS on the host, on a bounded sample of sources).
+
+    python bench.py [--steps K] [--warmup W] [--scale 22] [--cpma-scale 22]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def splitmix64(x):
+    M = (1 << 64) - 1
+    z = (x + 0x9E3779B97F4A7C15) & M
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+    return z ^ (z >> 31)
+
+
+def pick_sources(deg, k, seed=1):
+    """k distinct sources with degree > 0 from the SplitMix64(seed) stream."""
+    out, x, n = [], seed, len(deg)
+    while len(out) < k:
+        x = splitmix64(x)
+        v = x % n
+        if deg[v] > 0 and v not in out:
+            out.append(int(v))
+    return out
+
+
+def bfs_bytes(n, reached, edges_reached):
+    """Algorithmic bytes of one top-down-equivalent BFS: offsets of reached
+    vertices + their adjacency once + parent init + parent writes."""
+    return 8 * reached + 4 * edges_reached + 4 * n + 4 * reached
+
+
+def sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def cpu_baseline(G, sources, budget_s):
+    """The oracle's OpenMP direction-optimizing BFS (restating BFS.h over
+    edgeMapData) on this host's cores, on as many of the sources as fit in
+    budget_s; returns GTEPS over that sample."""
+    import oracle as O
+    off, ids = G.export_csr()
+    g = O.Graph(off, ids)
+    cores = os.cpu_count()
+    done, t_total, edges = 0, 0.0, 0
+    O.bfs_diropt(g, sources[0])  # warm (page in)
+    for s in sources:
+        t0 = time.perf_counter()
+        par, _, _, fd = O.bfs_diropt(g, s)
+        dt = time.perf_counter() - t0
+        t_total += dt
+        edges += int(fd.sum()) // 2
+        done += 1
+        if t_total > budget_s:
+            break
+    return {"value": edges / t_total / 1e9, "unit": "GTEPS", "cores": cores, "kind": "port",
+            "sample": "oracle OpenMP diropt BFS (M/20 rule) on the same RMAT-%d graph, "
+                      "%d of %d sources, %.2f s" % (int(np.log2(g.n)), done, len(sources), t_total)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--ef", type=int, default=16)
+    ap.add_argument("--sources", type=int, default=8)
+    ap.add_argument("--cpma-scale", type=int, default=22)
+    ap.add_argument("--cpma-batch", type=int, default=10_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import byo_amd as B
+    if B.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    t = time.time()
+    G = B.CsrGraph.rmat(args.scale, args.ef)
+    n, m = G.N(), G.M()
+    deg = G.degrees()
+    srcs = pick_sources(deg, args.sources)
+    print("# graph RMAT-%d ef%d: n=%d m=%d (built on device in %.1fs)" % (args.scale, args.ef, n, m,
+                                                                          time.time() - t), file=sys.stderr)
+
+    for _ in range(args.warmup):
+        G.bfs_multi(srcs, want_parents=False)
+    G.profile(True)
+    G.profile_reset()
+    sync()
+    t0 = time.perf_counter()
+    edges = 0
+    bytes_total = 0
+    for _ in range(args.steps):
+        _, stats = G.bfs_multi(srcs, want_parents=False)
+        for st in stats:
+            edges += st["edges_reached"] // 2
+            bytes_total += bfs_bytes(n, st["reached"], st["edges_reached"])
+    sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    nl, bfs_ms = G.profile_get("bfs")
+    avg_ms = bfs_ms / max(nl, 1)
+    gteps = edges / wall / 1e9
+    avg_bytes = bytes_total / max(nl, 1)
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    dirs = stats[0]["dir"]
+
+    out = {
+        "metric": "BFS GTEPS learned vs threshold; FP32 PR it/s @1e-6 L1; insert edges/s",
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic RMAT generated on the device (seed 1)",
+        "config": {"workload": "BFS x%d sources, RMAT scale %d ef %d (a=.5,b=.1,c=.1), symmetric CSR, "
+                               "direction rule M/20" % (args.sources, args.scale, args.ef),
+                   "n": n, "m": m, "sources": srcs, "levels_dirs_src0": dirs},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "one BFS traversal (all its launches)", "avg_ms": round(avg_ms, 4),
+                     "algorithmic_bytes": int(avg_bytes)},
+    }
+
+    if not args.no_extras:
+        ex = {}
+        p32, l1_32, ms32 = G.pagerank(0.85, 20, 0.0, B.PR_FP32)
+        p32, l1_32, ms32 = G.pagerank(0.85, 20, 0.0, B.PR_FP32)
+        p64, l1_64, ms64 = G.pagerank(0.85, 20, 0.0, B.PR_FP64)
+        p64, l1_64, ms64 = G.pagerank(0.85, 20, 0.0, B.PR_FP64)
+        ex["pr_fp32_it_s"] = round(20 / (ms32 * 1e-3), 1)
+        ex["pr_fp64_it_s"] = round(20 / (ms64 * 1e-3), 1)
+        ex["pr_fp32_vs_fp64_speedup"] = round(ms64 / ms32, 3)
+        ex["pr_l1_fp32_vs_fp64"] = float(np.abs(p32 - p64).sum())
+        w = 4
+        pr_bytes = 8 * (n + 1) + (4 + w) * m + 3 * w * n
+        ex["pr_fp32_GBs"] = round(pr_bytes / (ms32 / 20 * 1e-3) / 1e9, 1)
+        G.cc()
+        lab, cc_ms = G.cc()
+        ex["cc_ms"] = round(cc_ms, 3)
+        ex["cc_components"] = int(len(np.unique(lab)))
+        tri, tc_ms = G.tc()
+        ex["tc_ms"] = round(tc_ms, 3)
+        ex["triangles"] = int(tri)
+        s, map_ms = G.map_sum()
+        s, map_ms = G.map_sum()
+        ex["map_GBs"] = round((8 * (n + 1) + 4 * m) / (map_ms * 1e-3) / 1e9, 1)
+        # CPMA batch updates at batch size cpma_batch on RMAT-cpma_scale
+        if args.cpma_scale == args.scale:
+            H = G
+        else:
+            H = B.CsrGraph.rmat(args.cpma_scale, args.ef)
+        C = B.CpmaGraph.from_graph(H)
+        keys = B.update_rmat_keys(C.N(), args.cpma_batch)
+        C.profile(True)
+        C.insert_batch(keys)  # round 0 (untimed, as Batch_insert_runner)
+        C.remove_batch(keys)
+        C.profile_reset()
+        ins_t, rem_t = [], []
+        for r in range(2):
+            sync()
+            a = time.perf_counter()
+            C.insert_batch(keys)
+            b = time.perf_counter()
+            C.remove_batch(keys)
+            c = time.perf_counter()
+            ins_t.append(b - a)
+            rem_t.append(c - b)
+        ex["cpma_batch"] = args.cpma_batch
+        ex["cpma_insert_edges_s"] = round(args.cpma_batch / np.mean(ins_t), 1)
+        ex["cpma_remove_edges_s"] = round(args.cpma_batch / np.mean(rem_t), 1)
+        ex["cpma_scale"] = args.cpma_scale
+        out["extras"] = ex
+        C.close()
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(G, srcs, args.cpu_budget)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

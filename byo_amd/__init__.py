@@ -1,0 +1,413 @@
+"""byo_amd -- MI355X-native BYO hot path (GBBS edgeMap + BFS / PageRank /
+Connectivity / TriangleCounting over a device CSR and a device CPMA).
+
+Python mirror of the reference's container API (``Graph<Representation>``
+in gbbs/graph.h:2154-3150 over ``CSR`` / ``PMA_graph``,
+benchmarks/run_structures/run_csr.cc:28-191 and run_single_pma.cc:29-140)
+over the C ABI of ``libbyo.so`` (declared in include/byo.h).  Every
+computation runs in the gfx950 HIP kernels of the library; importing this
+package fails loudly when the library has not been built, and every call
+raises ``ByoError`` with the library's message when the library reports an
+error (no device included).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbyo.so")
+
+OK, E_INVAL, E_OOM, E_HIP, E_STATE, E_NODEV = 0, -1, -2, -3, -4, -5
+NONE = 0xFFFFFFFF
+DIR_THRESHOLD, DIR_MLP, DIR_SPARSE, DIR_DENSE = 0, 1, 2, 3
+PR_FP64, PR_FP32 = 0, 1
+MLP_NPARAM = 1379
+
+
+class ByoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("byo error %d: %s" % (code, msg))
+        self.code = code
+
+
+class BfsStats(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("reached", C.c_uint32),
+                ("edges_reached", C.c_uint64), ("device_ms", C.c_double),
+                ("fsize", C.c_uint64 * 64), ("fdeg", C.c_uint64 * 64),
+                ("dir", C.c_uint8 * 64), ("level_ms", C.c_float * 64)]
+
+    def as_dict(self):
+        k = min(self.levels, 64)
+        return {"levels": self.levels, "reached": self.reached,
+                "edges_reached": self.edges_reached, "device_ms": self.device_ms,
+                "fsize": list(self.fsize[:k]), "fdeg": list(self.fdeg[:k]),
+                "dir": list(self.dir[:k]), "level_ms": list(self.level_ms[:k])}
+
+
+_vp = C.c_void_p
+_u64 = C.c_uint64
+_u32 = C.c_uint32
+_i32 = C.c_int
+_dbl = C.c_double
+_P = C.POINTER
+
+# name -> (restype, argtypes); every symbol include/byo.h declares
+SIGNATURES = {
+    "byo_last_error": (C.c_char_p, []),
+    "byo_version": (C.c_char_p, []),
+    "byo_device_count": (_i32, []),
+    "byo_csr_create": (_i32, [_vp, _vp, _u64, _u64, _P(_vp)]),
+    "byo_csr_create_rmat": (_i32, [_i32, _u64, _dbl, _dbl, _dbl, _u64, _P(_vp)]),
+    "byo_cpma_create": (_i32, [_vp, _vp, _u64, _u64, _P(_vp)]),
+    "byo_cpma_create_from": (_i32, [_vp, _P(_vp)]),
+    "byo_graph_destroy": (_i32, [_vp]),
+    "byo_graph_kind": (_i32, [_vp, _P(_i32)]),
+    "byo_num_vertices": (_i32, [_vp, _P(_u64)]),
+    "byo_num_edges": (_i32, [_vp, _P(_u64)]),
+    "byo_degrees": (_i32, [_vp, _vp]),
+    "byo_degree": (_i32, [_vp, _u32, _P(_u32)]),
+    "byo_neighbors": (_i32, [_vp, _u32, _vp, _u64, _P(_u64)]),
+    "byo_export_csr": (_i32, [_vp, _vp, _vp]),
+    "byo_memory_size": (_i32, [_vp, _P(_u64)]),
+    "byo_insert_batch": (_i32, [_vp, _vp, _u64]),
+    "byo_remove_batch": (_i32, [_vp, _vp, _u64]),
+    "byo_insert_batch_dev": (_i32, [_vp, _vp, _u64]),
+    "byo_remove_batch_dev": (_i32, [_vp, _vp, _u64]),
+    "byo_update_rmat_keys": (_i32, [_u64, _u64, _u64, _vp]),
+    "byo_set_direction_policy": (_i32, [_vp, _i32, C.c_int64]),
+    "byo_bfs": (_i32, [_vp, _u32, _vp, _P(BfsStats)]),
+    "byo_bfs_multi": (_i32, [_vp, _vp, _u32, _vp, _vp]),
+    "byo_bfs_levels": (_i32, [_vp, _vp]),
+    "byo_pagerank": (_i32, [_vp, _dbl, _i32, _dbl, _i32, _vp, _vp, _P(_i32), _P(_dbl)]),
+    "byo_ppr_multi": (_i32, [_vp, _vp, _u32, _dbl, _i32, _vp, _P(_dbl)]),
+    "byo_cc": (_i32, [_vp, _vp, _P(_dbl)]),
+    "byo_cc_labelprop": (_i32, [_vp, _vp, _P(_u32), _P(_dbl)]),
+    "byo_tc": (_i32, [_vp, _P(_u64), _P(_dbl)]),
+    "byo_map_sum": (_i32, [_vp, _P(_u64), _P(_dbl)]),
+    "byo_profile_enable": (_i32, [_vp, _i32]),
+    "byo_profile_get": (_i32, [_vp, C.c_char_p, _P(_u64), _P(_dbl)]),
+    "byo_profile_reset": (_i32, [_vp]),
+    "byo_selector_load": (_i32, [_vp, _vp, _vp, _vp]),
+    "byo_trainer_create": (_i32, [_vp, C.c_float, _P(_vp)]),
+    "byo_trainer_destroy": (_i32, [_vp]),
+    "byo_trainer_step": (_i32, [_vp, _vp, _vp, _i32, _P(C.c_float)]),
+    "byo_trainer_grad": (_i32, [_vp, _vp, _vp, _i32, _P(C.c_float), _vp, _vp]),
+    "byo_trainer_params": (_i32, [_vp, _vp]),
+    "byo_trainer_predict": (_i32, [_vp, _vp, _i32, _vp]),
+}
+
+_LIB = None
+
+
+def lib():
+    """The loaded libbyo.so (raises ImportError when it was never built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libbyo.so not built: run `python -m byo_amd.build` "
+                              "(or __graft_entry__.build()) first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != OK:
+        raise ByoError(rc, lib().byo_last_error().decode())
+    return rc
+
+
+def device_count():
+    return lib().byo_device_count()
+
+
+def update_rmat_keys(graph_n, count, first=0):
+    """Batch_insert_runner's update stream (run_unweighted.h:797-806) as
+    uint64 keys (src<<32)|dst, generated on the device."""
+    k = np.empty(max(count, 1), np.uint64)
+    check(lib().byo_update_rmat_keys(graph_n, first, count, k.ctypes.data))
+    return k[:count]
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+class Graph:
+    """A device-resident container with the reference's Graph<> API:
+    N(), M(), degree(i), map_neighbors(i, f), insert_batch / remove_batch,
+    plus the whole-algorithm entry points."""
+
+    KIND = None
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    # -- lifetime ------------------------------------------------------
+    def close(self):
+        if self._h:
+            check(lib().byo_graph_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if self._h and _LIB is not None:
+                _LIB.byo_graph_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- container API (gbbs/graph.h:2179-2308) --------------------------
+    def N(self):
+        n = _u64()
+        check(lib().byo_num_vertices(self._h, C.byref(n)))
+        return n.value
+
+    def M(self):
+        m = _u64()
+        check(lib().byo_num_edges(self._h, C.byref(m)))
+        return m.value
+
+    def degree(self, i):
+        d = _u32()
+        check(lib().byo_degree(self._h, i, C.byref(d)))
+        return d.value
+
+    def degrees(self):
+        out = np.empty(max(self.N(), 1), np.uint32)
+        check(lib().byo_degrees(self._h, _ptr(out)))
+        return out[:self.N()]
+
+    def neighbors(self, i):
+        d = self.degree(i)
+        out = np.empty(max(d, 1), np.uint32)
+        cnt = _u64()
+        check(lib().byo_neighbors(self._h, i, _ptr(out), d, C.byref(cnt)))
+        return out[:cnt.value]
+
+    def map_neighbors(self, i, f):
+        """Calls f(i, dst, None) for every neighbour in ascending order
+        (run_csr.cc:38-43)."""
+        for d in self.neighbors(i):
+            f(i, int(d), None)
+
+    def map_neighbors_early_exit(self, i, f):
+        for d in self.neighbors(i):
+            if f(i, int(d), None):
+                break
+
+    def export_csr(self):
+        n, m = self.N(), self.M()
+        off = np.empty(n + 1, np.uint64)
+        ids = np.empty(max(m, 1), np.uint32)
+        check(lib().byo_export_csr(self._h, _ptr(off), _ptr(ids)))
+        return off, ids[:m]
+
+    def get_memory_size(self):
+        b = _u64()
+        check(lib().byo_memory_size(self._h, C.byref(b)))
+        return b.value
+
+    def insert_batch(self, keys):
+        """Graph::insert_batch (gbbs/graph.h:3071-3097); keys are uint64
+        (src<<32)|dst, or an (k, 2) array of (src, dst)."""
+        k = as_keys(keys)
+        check(lib().byo_insert_batch(self._h, _ptr(k), len(k)))
+
+    def remove_batch(self, keys):
+        k = as_keys(keys)
+        check(lib().byo_remove_batch(self._h, _ptr(k), len(k)))
+
+    # -- algorithms -------------------------------------------------------
+    def set_direction_policy(self, policy, threshold=-1):
+        check(lib().byo_set_direction_policy(self._h, policy, threshold))
+
+    def bfs(self, src, want_parents=True):
+        n = self.N()
+        par = np.empty(max(n, 1), np.uint32) if want_parents else None
+        st = BfsStats()
+        check(lib().byo_bfs(self._h, src, _ptr(par), C.byref(st)))
+        return (par[:n] if want_parents else None), st.as_dict()
+
+    def bfs_multi(self, srcs, want_parents=True):
+        srcs = np.ascontiguousarray(srcs, np.uint32)
+        n, k = self.N(), len(srcs)
+        par = np.empty(max(n * k, 1), np.uint32) if want_parents else None
+        st = (BfsStats * max(k, 1))()
+        check(lib().byo_bfs_multi(self._h, _ptr(srcs), k, _ptr(par), C.cast(st, C.c_void_p)))
+        return (par[:n * k].reshape(k, n) if want_parents else None), [st[i].as_dict() for i in range(k)]
+
+    def bfs_levels(self):
+        n = self.N()
+        lv = np.empty(max(n, 1), np.uint32)
+        check(lib().byo_bfs_levels(self._h, _ptr(lv)))
+        return lv[:n]
+
+    def pagerank(self, damping=0.85, iters=20, eps=0.0, precision=PR_FP32, want_p=True):
+        n = self.N()
+        p = np.empty(max(n, 1), np.float64) if want_p else None
+        l1 = np.empty(max(iters, 1), np.float64)
+        it = _i32()
+        ms = _dbl()
+        check(lib().byo_pagerank(self._h, damping, iters, eps, precision, _ptr(p), _ptr(l1),
+                                 C.byref(it), C.byref(ms)))
+        return (p[:n] if want_p else None), l1[:it.value].copy(), ms.value
+
+    def ppr(self, seeds, damping=0.85, iters=30, want_p=True):
+        seeds = np.ascontiguousarray(seeds, np.uint32)
+        n, k = self.N(), len(seeds)
+        p = np.empty(max(n * k, 1), np.float32) if want_p else None
+        ms = _dbl()
+        check(lib().byo_ppr_multi(self._h, _ptr(seeds), k, damping, iters, _ptr(p), C.byref(ms)))
+        return (p[:n * k].reshape(k, n) if want_p else None), ms.value
+
+    def cc(self):
+        n = self.N()
+        lab = np.empty(max(n, 1), np.uint32)
+        ms = _dbl()
+        check(lib().byo_cc(self._h, _ptr(lab), C.byref(ms)))
+        return lab[:n], ms.value
+
+    def cc_labelprop(self):
+        n = self.N()
+        lab = np.empty(max(n, 1), np.uint32)
+        r = _u32()
+        ms = _dbl()
+        check(lib().byo_cc_labelprop(self._h, _ptr(lab), C.byref(r), C.byref(ms)))
+        return lab[:n], r.value, ms.value
+
+    def tc(self):
+        c = _u64()
+        ms = _dbl()
+        check(lib().byo_tc(self._h, C.byref(c), C.byref(ms)))
+        return c.value, ms.value
+
+    def map_sum(self):
+        s = _u64()
+        ms = _dbl()
+        check(lib().byo_map_sum(self._h, C.byref(s), C.byref(ms)))
+        return s.value, ms.value
+
+    # -- profiling --------------------------------------------------------
+    def profile(self, on=True):
+        check(lib().byo_profile_enable(self._h, 1 if on else 0))
+
+    def profile_get(self, phase):
+        n = _u64()
+        ms = _dbl()
+        check(lib().byo_profile_get(self._h, phase.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def profile_reset(self):
+        check(lib().byo_profile_reset(self._h))
+
+    def load_selector(self, params, mean, inv_std):
+        params = np.ascontiguousarray(params, np.float32)
+        mean = np.ascontiguousarray(mean, np.float32)
+        inv_std = np.ascontiguousarray(inv_std, np.float32)
+        assert params.size == MLP_NPARAM and mean.size == 6 and inv_std.size == 6
+        check(lib().byo_selector_load(self._h, _ptr(params), _ptr(mean), _ptr(inv_std)))
+
+
+class CsrGraph(Graph):
+    """Static CSR (run_csr.cc:28-191)."""
+
+    @classmethod
+    def from_csr(cls, offsets, ids):
+        off = np.ascontiguousarray(offsets, np.uint64)
+        ids = np.ascontiguousarray(ids, np.uint32)
+        h = C.c_void_p()
+        check(lib().byo_csr_create(_ptr(off), _ptr(ids), len(off) - 1, len(ids), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor=16, a=0.5, b=0.1, c=0.1, seed=1):
+        h = C.c_void_p()
+        check(lib().byo_csr_create_rmat(scale, edge_factor, a, b, c, seed, C.byref(h)))
+        return cls(h.value)
+
+
+class CpmaGraph(Graph):
+    """CPMA-style packed-memory-array graph (run_single_pma.cc:29-140)."""
+
+    @classmethod
+    def from_csr(cls, offsets, ids):
+        off = np.ascontiguousarray(offsets, np.uint64)
+        ids = np.ascontiguousarray(ids, np.uint32)
+        h = C.c_void_p()
+        check(lib().byo_cpma_create(_ptr(off), _ptr(ids), len(off) - 1, len(ids), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_graph(cls, g):
+        h = C.c_void_p()
+        check(lib().byo_cpma_create_from(g._h, C.byref(h)))
+        return cls(h.value)
+
+
+def as_keys(keys):
+    a = np.asarray(keys)
+    if a.ndim == 2:
+        a = (a[:, 0].astype(np.uint64) << np.uint64(32)) | a[:, 1].astype(np.uint64)
+    return np.ascontiguousarray(a, np.uint64)
+
+
+class Trainer:
+    """Device trainer of the direction-selector MLP (own fwd/bwd/Adam kernels)."""
+
+    def __init__(self, params, lr=1e-3):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == MLP_NPARAM
+        h = C.c_void_p()
+        check(lib().byo_trainer_create(_ptr(p), lr, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            check(lib().byo_trainer_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if self._h and _LIB is not None:
+                _LIB.byo_trainer_destroy(self._h)
+        except Exception:
+            pass
+
+    def step(self, x, y):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.int32)
+        loss = C.c_float()
+        check(lib().byo_trainer_step(self._h, _ptr(x), _ptr(y), len(y), C.byref(loss)))
+        return loss.value
+
+    def grad(self, x, y):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.int32)
+        loss = C.c_float()
+        g = np.empty(MLP_NPARAM, np.float32)
+        lg = np.empty(3 * len(y), np.float32)
+        check(lib().byo_trainer_grad(self._h, _ptr(x), _ptr(y), len(y), C.byref(loss), _ptr(g), _ptr(lg)))
+        return loss.value, g, lg.reshape(-1, 3)
+
+    def params(self):
+        p = np.empty(MLP_NPARAM, np.float32)
+        check(lib().byo_trainer_params(self._h, _ptr(p)))
+        return p
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.empty(len(x), np.int32)
+        check(lib().byo_trainer_predict(self._h, _ptr(x), len(x), _ptr(out)))
+        return out

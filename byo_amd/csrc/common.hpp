@@ -1,0 +1,239 @@
+// common.hpp -- shared internals of libbyo: error plumbing, device buffers,
+// the graph handle, and wave64 helpers for gfx950.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/byo.h"
+
+namespace byo {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &m);
+
+#define BYO_HIP(expr)                                                        \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if (_e != hipSuccess)                                                    \
+      throw ::byo::Error(_e == hipErrorOutOfMemory ? BYO_E_OOM : BYO_E_HIP,  \
+                         std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+#define BYO_REQUIRE(cond, code, msg)                    \
+  do {                                                  \
+    if (!(cond)) throw ::byo::Error((code), (msg));     \
+  } while (0)
+
+#define BYO_LAUNCH_CHECK() BYO_HIP(hipGetLastError())
+
+// Runs f, mapping exceptions to a status code for the C ABI.
+template <class F>
+int guard(F &&f) {
+  try {
+    f();
+    set_last_error("");
+    return BYO_OK;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return BYO_E_OOM;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return BYO_E_HIP;
+  }
+}
+
+// ---------------------------------------------------------- device memory
+// Owning device allocation.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t b) { alloc(b); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t b) {
+    release();
+    if (b == 0) b = 256;
+    BYO_HIP(hipMalloc(&p, b));
+    bytes = b;
+  }
+  // Grows (discarding contents) when smaller than b.
+  void ensure(size_t b) {
+    if (bytes < b) alloc(b + b / 8);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Event-timed phases (device time on the handle's stream).
+struct Profiler {
+  bool on = false;
+  struct Acc { uint64_t launches = 0; double ms = 0; };
+  std::map<std::string, Acc> acc;
+};
+
+// RAII timing of one phase with HIP events on stream s.
+struct PhaseTimer {
+  hipEvent_t a{}, b{};
+  hipStream_t s;
+  bool active;
+  PhaseTimer(hipStream_t s_, bool active_) : s(s_), active(active_) {
+    if (active) {
+      BYO_HIP(hipEventCreate(&a));
+      BYO_HIP(hipEventCreate(&b));
+      BYO_HIP(hipEventRecord(a, s));
+    }
+  }
+  // Records the end event and returns elapsed ms (synchronises on it).
+  double stop() {
+    if (!active) return 0.0;
+    BYO_HIP(hipEventRecord(b, s));
+    BYO_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    BYO_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms;
+  }
+  ~PhaseTimer() {
+    if (active) { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+  }
+};
+
+}  // namespace byo
+
+// ------------------------------------------------------------ the handle
+struct byo_graph {
+  int kind = 0;  // 0 CSR, 1 CPMA
+  int device = 0;
+  uint64_t n = 0, m = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+
+  // CSR arrays (kind 0), or the CSR view exported from the CPMA (kind 1).
+  byo::DevBuf off;  // uint64[n+1]
+  byo::DevBuf ids;  // uint32[m]
+  bool csr_valid = false;
+
+  // CPMA state (kind 1); owned by cpma.hip.
+  void *cpma = nullptr;
+
+  // Direction policy.
+  int dir_policy = BYO_DIR_THRESHOLD;
+  int64_t dir_threshold = -1;
+  // Learned selector weights (device, 1379 floats + 12 normalisation).
+  byo::DevBuf sel_params;
+  std::vector<float> sel_host;  // host copy for the host-side decision
+  bool sel_loaded = false;
+
+  // Scratch owned by algorithms (grown on demand, kept across calls).
+  std::map<std::string, byo::DevBuf> scratch;
+  byo::DevBuf &buf(const std::string &name, size_t bytes) {
+    auto &b = scratch[name];
+    b.ensure(bytes);
+    return b;
+  }
+  // Pinned host staging for small readbacks.
+  void *pinned = nullptr;
+  size_t pinned_bytes = 0;
+
+  byo::Profiler prof;
+  void account(const char *phase, double ms) {
+    auto &a = prof.acc[phase];
+    a.launches++;
+    a.ms += ms;
+  }
+};
+
+namespace byo {
+// Ensures a device exists and selects the handle's device on this thread.
+void require_device();
+void bind(byo_graph *g);
+byo_graph *new_handle(int kind);
+void *pinned(byo_graph *g, size_t bytes);
+// CSR construction (graph.hip).
+void csr_from_keys(byo_graph *g, uint64_t *keys, uint64_t *alt, uint64_t nk, uint64_t n);
+void csr_upload(byo_graph *g, const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m);
+void check_csr_input(const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m);
+// Current CSR view of any container (the CPMA materialises one lazily).
+void ensure_csr(byo_graph *g);
+// Internal hooks implemented in other translation units.
+void cpma_destroy(byo_graph *g);
+void cpma_ensure_csr(byo_graph *g);  // refresh g->off/g->ids from the PMA
+uint64_t cpma_bytes(byo_graph *g);
+}  // namespace byo
+
+// ------------------------------------------------------ device helpers
+namespace byo {
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint32_t mbcnt(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    T w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+// Inclusive prefix sum across the wave.
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const int l = __lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    T w = __shfl_up(v, o, 64);
+    if (l >= o) v += w;
+  }
+  return v;
+}
+__device__ __forceinline__ uint64_t grid_threads() {
+  return static_cast<uint64_t>(gridDim.x) * blockDim.x;
+}
+__device__ __forceinline__ uint64_t global_tid() {
+  return static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+}
+}  // namespace byo
+
+// Launch helper: grid size capped for grid-stride kernels.
+inline unsigned byo_grid(uint64_t work, unsigned block, unsigned cap = 8192) {
+  uint64_t g = (work + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g);
+}

@@ -1,0 +1,583 @@
+// cpma.hip -- CPMA-style packed-memory-array graph container on the device.
+//
+// Replaces PMA_graph over CPMA<PMA_SETTINGS<uint64_t>> (benchmarks/
+// run_structures/run_single_pma.cc:29-140): the edge set is the set of
+// uint64 keys (src<<32)|dst (run_single_pma.cc:83), kept sorted in fixed
+// size leaves of kLeaf slots.  Leaf l holds cnt[l] keys packed at its start
+// (the rest are the empty key ~0), all in [sep[l], sep[l+1]).
+//
+// insert_batch / remove_batch (Graph::insert_batch, gbbs/graph.h:3071-3124:
+// sort + unique, then insert_sorted_batch):
+//   1. the batch is radix sorted and deduplicated on the device;
+//   2. every key finds its leaf by binary search over sep[];
+//   3. the sorted batch splits into one contiguous segment per touched leaf;
+//   4. insert: a leaf whose keys plus new keys fit is merged in place by one
+//      workgroup in LDS (only touched leaves move); if any leaf would
+//      overflow, the whole array is rebalanced: leaves are compacted, merged
+//      with the batch by merge path, deduplicated and spread evenly over a
+//      (possibly grown) leaf array at the target density;
+//      remove: each touched leaf drops the keys found in its segment (binary
+//      search) and compacts in place; the array shrinks and rebalances when
+//      its density falls below kMinDensity.
+// Degrees are maintained per update (the stale-degree problem of
+// Graph::finilize caching, gbbs/graph.h:2559-2598, does not arise), and a
+// CSR view (offsets + ids) for the traversal kernels is materialised from
+// the leaves when an algorithm asks for it after an update.
+#include <algorithm>
+
+#include "primitives.hpp"
+
+namespace byo {
+
+constexpr uint32_t kLeaf = 512;
+constexpr double kTargetDensity = 0.6;
+constexpr double kMinDensity = 0.2;
+constexpr uint64_t kEmpty = ~0ull;
+
+struct Cpma {
+  uint64_t nleaves = 0;
+  uint64_t total = 0;
+  DevBuf slots;  // uint64[nleaves * kLeaf]
+  DevBuf cnt;    // uint32[nleaves]
+  DevBuf sep;    // uint64[nleaves + 1]
+  DevBuf deg;    // uint32[n]
+  uint64_t rebalances = 0;
+};
+
+static Cpma *state(byo_graph *g) { return static_cast<Cpma *>(g->cpma); }
+
+namespace {
+
+struct UpdCtl {
+  unsigned long long changed;   // keys inserted / removed
+  unsigned long long overflow;  // a leaf would overflow
+  unsigned long long bad;       // an id out of range
+  unsigned long long pad[5];
+};
+
+__global__ void k_check_ids(const uint64_t *keys, uint64_t b, uint64_t n, UpdCtl *ctl) {
+  for (uint64_t i = global_tid(); i < b; i += grid_threads()) {
+    const uint64_t k = keys[i];
+    if ((k >> 32) >= n || (k & 0xFFFFFFFFull) >= n) atomicOr(&ctl->bad, 1ull);
+  }
+}
+
+// Leaf l receives keys [l*m/L, (l+1)*m/L) of the sorted unique array.
+__global__ void k_spread(const uint64_t *keys, uint64_t m, uint64_t nleaves, uint64_t *slots, uint32_t *cnt,
+                         uint64_t *sep) {
+  const int lane = lane_id();
+  const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
+  for (uint64_t l = wave; l < nleaves; l += nwaves) {
+    const uint64_t s = (uint64_t)(((unsigned __int128)m * l) / nleaves);
+    const uint64_t e = (uint64_t)(((unsigned __int128)m * (l + 1)) / nleaves);
+    uint64_t *leaf = slots + l * kLeaf;
+    for (uint32_t i = lane; i < kLeaf; i += 64) leaf[i] = (s + i < e) ? keys[s + i] : kEmpty;
+    if (lane == 0) {
+      cnt[l] = (uint32_t)(e - s);
+      sep[l] = (l == 0) ? 0ull : (s < m ? keys[s] : kEmpty);
+      if (l == nleaves - 1) sep[nleaves] = kEmpty;
+    }
+  }
+}
+
+__global__ void k_degrees_from_off(const uint64_t *off, uint64_t n, uint32_t *deg) {
+  for (uint64_t v = global_tid(); v < n; v += grid_threads()) deg[v] = (uint32_t)(off[v + 1] - off[v]);
+}
+
+// leaf index of each batch key: last l with sep[l] <= key
+__global__ void k_leaf_of(const uint64_t *keys, uint64_t b, const uint64_t *sep, uint64_t nleaves, uint32_t *leaf) {
+  for (uint64_t i = global_tid(); i < b; i += grid_threads()) {
+    const uint64_t k = keys[i];
+    uint64_t lo = 0, hi = nleaves;  // sep[lo] <= k < sep[hi] (sep[nleaves] = ~0 > any key)
+    while (hi - lo > 1) {
+      uint64_t mid = (lo + hi) >> 1;
+      if (sep[mid] <= k) lo = mid; else hi = mid;
+    }
+    leaf[i] = (uint32_t)lo;
+  }
+}
+
+__global__ void k_seg_flags(const uint32_t *leaf, uint64_t b, uint32_t *flag) {
+  for (uint64_t i = global_tid(); i < b; i += grid_threads()) flag[i] = (i == 0 || leaf[i] != leaf[i - 1]) ? 1u : 0u;
+}
+
+__global__ void k_seg_starts(const uint32_t *flag, const uint32_t *pos, uint64_t b, uint32_t *starts) {
+  for (uint64_t i = global_tid(); i < b; i += grid_threads())
+    if (flag[i]) starts[pos[i]] = (uint32_t)i;
+}
+
+__device__ __forceinline__ bool in_sorted(const uint64_t *a, uint64_t len, uint64_t x) {
+  uint64_t lo = 0, hi = len;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo < len && a[lo] == x;
+}
+
+__device__ __forceinline__ uint32_t lower_rank(const uint64_t *a, uint32_t len, uint64_t x) {
+  uint32_t lo = 0, hi = len;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Pass 1 of an insert: count the segment keys absent from their leaf.
+__global__ void __launch_bounds__(256) k_ins_count(const uint64_t *keys, uint64_t b, const uint32_t *starts,
+                                                   uint64_t nseg, const uint32_t *leaf_of, const uint64_t *slots,
+                                                   const uint32_t *cnt, uint32_t *newc, UpdCtl *ctl) {
+  __shared__ uint32_t acc;
+  for (uint64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
+    const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
+    const uint32_t l = leaf_of[s];
+    const uint64_t *leaf = slots + (uint64_t)l * kLeaf;
+    const uint32_t c = cnt[l];
+    if (threadIdx.x == 0) acc = 0;
+    __syncthreads();
+    uint32_t mine = 0;
+    for (uint64_t i = s + threadIdx.x; i < e; i += blockDim.x) mine += !in_sorted(leaf, c, keys[i]);
+    mine = wave_sum(mine);
+    if ((threadIdx.x & 63) == 0 && mine) atomicAdd(&acc, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      newc[j] = acc;
+      if ((uint64_t)c + acc > kLeaf) atomicOr(&ctl->overflow, 1ull);
+    }
+    __syncthreads();
+  }
+}
+
+// Pass 2 of an insert (no leaf overflows): merge in LDS and write back.
+__global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, uint64_t b, const uint32_t *starts,
+                                                   uint64_t nseg, const uint32_t *leaf_of, uint64_t *slots,
+                                                   uint32_t *cnt, uint32_t *deg, UpdCtl *ctl) {
+  __shared__ uint64_t old_keys[kLeaf];
+  __shared__ uint64_t add[kLeaf];
+  __shared__ uint32_t wcnt[4];
+  __shared__ uint32_t nadd;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (uint64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
+    const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
+    const uint32_t l = leaf_of[s];
+    uint64_t *leaf = slots + (uint64_t)l * kLeaf;
+    const uint32_t c = cnt[l];
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) old_keys[i] = leaf[i];
+    if (threadIdx.x == 0) nadd = 0;
+    __syncthreads();
+    // stable compaction of the absent keys, 256 at a time
+    for (uint64_t base = s; base < e; base += blockDim.x) {
+      const uint64_t i = base + threadIdx.x;
+      uint64_t k = 0;
+      bool keep = false;
+      if (i < e) {
+        k = keys[i];
+        keep = !in_sorted(old_keys, c, k);
+      }
+      const uint64_t mask = __ballot(keep);
+      if (lane == 0) wcnt[wv] = __popcll(mask);
+      __syncthreads();
+      uint32_t before = nadd;
+      for (int w = 0; w < wv; w++) before += wcnt[w];
+      if (keep) add[before + mbcnt(mask)] = k;
+      __syncthreads();
+      if (threadIdx.x == 0) nadd += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+      __syncthreads();
+    }
+    const uint32_t na = nadd;
+    // merged position of each key = own index + rank in the other list
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) {
+      const uint64_t k = old_keys[i];
+      leaf[i + lower_rank(add, na, k)] = k;
+    }
+    for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) {
+      const uint64_t k = add[i];
+      leaf[i + lower_rank(old_keys, c, k)] = k;
+      atomicAdd(&deg[k >> 32], 1u);
+    }
+    if (threadIdx.x == 0) {
+      cnt[l] = c + na;
+      if (na) atomicAdd(&ctl->changed, (unsigned long long)na);
+    }
+    __syncthreads();
+  }
+}
+
+// Removal: each touched leaf keeps the keys absent from its segment.
+__global__ void __launch_bounds__(256) k_remove(const uint64_t *keys, uint64_t b, const uint32_t *starts,
+                                                uint64_t nseg, const uint32_t *leaf_of, uint64_t *slots,
+                                                uint32_t *cnt, uint32_t *deg, UpdCtl *ctl) {
+  __shared__ uint64_t kept[kLeaf];
+  __shared__ uint32_t wcnt[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (uint64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
+    const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
+    const uint32_t l = leaf_of[s];
+    uint64_t *leaf = slots + (uint64_t)l * kLeaf;
+    const uint32_t c = cnt[l];
+    uint32_t run = 0;
+    for (uint32_t base = 0; base < c; base += blockDim.x) {
+      const uint32_t i = base + threadIdx.x;
+      uint64_t k = 0;
+      bool keep = false, gone = false;
+      if (i < c) {
+        k = leaf[i];
+        gone = in_sorted(keys + s, e - s, k);
+        keep = !gone;
+      }
+      if (gone) atomicSub(&deg[k >> 32], 1u);
+      const uint64_t mask = __ballot(keep);
+      if (lane == 0) wcnt[wv] = __popcll(mask);
+      __syncthreads();
+      uint32_t before = run;
+      for (int w = 0; w < wv; w++) before += wcnt[w];
+      if (keep) kept[before + mbcnt(mask)] = k;
+      run += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+      __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) leaf[i] = i < run ? kept[i] : kEmpty;
+    if (threadIdx.x == 0) {
+      cnt[l] = run;
+      if (c > run) atomicAdd(&ctl->changed, (unsigned long long)(c - run));
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_widen(const uint32_t *a, uint64_t *b, uint64_t n) {
+  for (uint64_t i = global_tid(); i < n; i += grid_threads()) b[i] = a[i];
+}
+
+// Compaction of all leaves into a flat sorted array (pos = prefix of cnt).
+__global__ void k_compact(const uint64_t *slots, const uint32_t *cnt, const uint64_t *pos, uint64_t nleaves,
+                          uint64_t *flat) {
+  const int lane = lane_id();
+  const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
+  for (uint64_t l = wave; l < nleaves; l += nwaves) {
+    const uint32_t c = cnt[l];
+    const uint64_t p = pos[l];
+    const uint64_t *leaf = slots + l * kLeaf;
+    for (uint32_t i = lane; i < c; i += 64) flat[p + i] = leaf[i];
+  }
+}
+
+// Merge path of sorted A[na] and B[nb] into out[na+nb] (A first on ties).
+__global__ void k_merge(const uint64_t *A, uint64_t na, const uint64_t *B, uint64_t nb, uint64_t *out) {
+  constexpr int K = 8;
+  const uint64_t total = na + nb;
+  for (uint64_t d = global_tid() * K; d < total; d += grid_threads() * K) {
+    uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
+    while (lo < hi) {
+      uint64_t mid = (lo + hi) >> 1;
+      if (A[mid] <= B[d - mid - 1]) lo = mid + 1; else hi = mid;
+    }
+    uint64_t i = lo, j = d - lo;
+    for (int k = 0; k < K && d + k < total; k++) {
+      if (j >= nb || (i < na && A[i] <= B[j])) out[d + k] = A[i++];
+      else out[d + k] = B[j++];
+    }
+  }
+}
+
+__global__ void k_slot_keys_to_csr_off(const uint64_t *flat, uint64_t m, uint64_t n, uint64_t *off, uint32_t *ids) {
+  for (uint64_t i = global_tid(); i < m; i += grid_threads()) {
+    const uint64_t key = flat[i];
+    const uint64_t s = key >> 32;
+    ids[i] = (uint32_t)key;
+    const uint64_t lo = (i == 0) ? 0 : (flat[i - 1] >> 32) + 1;
+    for (uint64_t v = lo; v <= s; v++) off[v] = i;
+    if (i == m - 1)
+      for (uint64_t v = s + 1; v <= n; v++) off[v] = m;
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ host side
+
+static uint64_t leaves_for(uint64_t m) {
+  uint64_t nl = (uint64_t)((double)m / (kLeaf * kTargetDensity)) + 1;
+  return nl;
+}
+
+// (Re)builds the leaves from a sorted unique flat key array; also refreshes
+// the CSR view and the degrees.  flat must hold m keys.
+static void cpma_build(byo_graph *g, const uint64_t *flat, uint64_t m) {
+  Cpma *c = state(g);
+  hipStream_t s = g->stream;
+  const uint64_t nl = leaves_for(m);
+  if (c->nleaves != nl || !c->slots.p) {
+    c->slots.alloc(nl * kLeaf * sizeof(uint64_t));
+    c->cnt.alloc(nl * sizeof(uint32_t));
+    c->sep.alloc((nl + 1) * sizeof(uint64_t));
+    c->nleaves = nl;
+  }
+  k_spread<<<byo_grid(nl * 64, 256, 16384), 256, 0, s>>>(flat, m, nl, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+                                                         c->sep.as<uint64_t>());
+  BYO_LAUNCH_CHECK();
+  c->total = m;
+  g->m = m;
+  // CSR view and degrees straight from the flat array
+  g->off.ensure((g->n + 1) * sizeof(uint64_t));
+  g->ids.ensure(m * sizeof(uint32_t) + 16);
+  prim::keys_to_csr(flat, m, g->n, g->off.as<uint64_t>(), g->ids.as<uint32_t>(), s);
+  c->deg.ensure(g->n * sizeof(uint32_t) + 64);
+  if (g->n)
+    k_degrees_from_off<<<byo_grid(g->n, 256, 16384), 256, 0, s>>>(g->off.as<uint64_t>(), g->n, c->deg.as<uint32_t>());
+  BYO_LAUNCH_CHECK();
+  g->csr_valid = true;
+}
+
+// Compacts the leaves into flat[total] (sorted, unique).
+static void cpma_flatten(byo_graph *g, uint64_t *flat) {
+  Cpma *c = state(g);
+  hipStream_t s = g->stream;
+  uint64_t *cnt64 = g->buf("cpma_cnt64", (c->nleaves + 1) * 8).as<uint64_t>();
+  uint64_t *pos = g->buf("cpma_pos", (c->nleaves + 2) * 8).as<uint64_t>();
+  void *tmp = g->buf("cpma_scan_tmp", prim::scan_tmp_bytes(c->nleaves + 1)).p;
+  // widen counts to uint64 for the scan
+  k_widen<<<byo_grid(c->nleaves, 256, 16384), 256, 0, s>>>(c->cnt.as<uint32_t>(), cnt64, c->nleaves);
+  prim::exclusive_scan_u64(cnt64, pos, c->nleaves, true, tmp, s);
+  k_compact<<<byo_grid(c->nleaves * 64, 256, 16384), 256, 0, s>>>(c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(), pos,
+                                                                  c->nleaves, flat);
+  BYO_LAUNCH_CHECK();
+}
+
+void cpma_ensure_csr(byo_graph *g) {
+  Cpma *c = state(g);
+  hipStream_t s = g->stream;
+  uint64_t *flat = g->buf("cpma_flat", c->total * 8 + 64).as<uint64_t>();
+  cpma_flatten(g, flat);
+  g->off.ensure((g->n + 1) * sizeof(uint64_t));
+  g->ids.ensure(c->total * sizeof(uint32_t) + 16);
+  prim::keys_to_csr(flat, c->total, g->n, g->off.as<uint64_t>(), g->ids.as<uint32_t>(), s);
+  BYO_HIP(hipStreamSynchronize(s));
+  g->csr_valid = true;
+}
+
+void cpma_destroy(byo_graph *g) {
+  delete state(g);
+  g->cpma = nullptr;
+}
+
+uint64_t cpma_bytes(byo_graph *g) {
+  Cpma *c = state(g);
+  // leaves + counts + separators + degrees (get_memory_size counts the PMA
+  // and two per-vertex arrays, run_single_pma.cc:126-131)
+  return c->nleaves * kLeaf * 8 + c->nleaves * 4 + (c->nleaves + 1) * 8 + g->n * 4;
+}
+
+// Sorts + dedupes keys_in[b] (device) into a batch buffer; returns it and
+// the unique count.
+static const uint64_t *prepare_batch(byo_graph *g, const uint64_t *keys_in, uint64_t b, uint64_t *bu, UpdCtl *ctl,
+                                     UpdCtl *hctl) {
+  hipStream_t s = g->stream;
+  uint64_t *k0 = g->buf("upd_k0", b * 8 + 64).as<uint64_t>();
+  uint64_t *k1 = g->buf("upd_k1", b * 8 + 64).as<uint64_t>();
+  BYO_HIP(hipMemsetAsync(ctl, 0, sizeof(UpdCtl), s));
+  k_check_ids<<<byo_grid(b, 256, 8192), 256, 0, s>>>(keys_in, b, g->n, ctl);
+  BYO_HIP(hipMemcpyAsync(k0, keys_in, b * 8, hipMemcpyDeviceToDevice, s));
+  int idbits = 1;
+  while ((1ull << idbits) < g->n) idbits++;
+  int shifts[16];
+  const int np = prim::edge_key_shifts(idbits, shifts);
+  void *rtmp = g->buf("upd_sort_tmp", prim::radix_tmp_bytes(b)).p;
+  uint64_t *sorted = prim::radix_sort_u64(k0, k1, b, shifts, np, rtmp, s);
+  uint64_t *uniq = (sorted == k0) ? k1 : k0;
+  void *stmp = g->buf("upd_sel_tmp", prim::select_tmp_bytes(b) + 256).p;
+  prim::unique_u64(sorted, uniq, b, kEmpty, reinterpret_cast<uint64_t *>(&ctl->pad[0]), stmp, s);
+  BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
+  BYO_HIP(hipStreamSynchronize(s));
+  BYO_REQUIRE(hctl->bad == 0, BYO_E_INVAL, "batch holds a vertex id >= n");
+  *bu = hctl->pad[0];
+  return uniq;
+}
+
+// Segments of the sorted batch per leaf; returns the segment count.
+static uint64_t segment_batch(byo_graph *g, const uint64_t *keys, uint64_t b, uint32_t **leaf_of, uint32_t **starts) {
+  Cpma *c = state(g);
+  hipStream_t s = g->stream;
+  uint32_t *lf = g->buf("upd_leaf", b * 4 + 64).as<uint32_t>();
+  uint32_t *flag = g->buf("upd_flag", b * 4 + 64).as<uint32_t>();
+  uint32_t *pos = g->buf("upd_pos", (b + 1) * 4 + 64).as<uint32_t>();
+  uint32_t *st = g->buf("upd_starts", b * 4 + 64).as<uint32_t>();
+  void *tmp = g->buf("upd_scan_tmp", prim::scan_tmp_bytes(b + 1)).p;
+  const unsigned grid = byo_grid(b, 256, 8192);
+  k_leaf_of<<<grid, 256, 0, s>>>(keys, b, c->sep.as<uint64_t>(), c->nleaves, lf);
+  k_seg_flags<<<grid, 256, 0, s>>>(lf, b, flag);
+  prim::exclusive_scan_u32(flag, pos, b, true, tmp, s);
+  k_seg_starts<<<grid, 256, 0, s>>>(flag, pos, b, st);
+  BYO_LAUNCH_CHECK();
+  uint32_t *h = static_cast<uint32_t *>(pinned(g, 64));
+  BYO_HIP(hipMemcpyAsync(h, pos + b, 4, hipMemcpyDeviceToHost, s));
+  BYO_HIP(hipStreamSynchronize(s));
+  *leaf_of = lf;
+  *starts = st;
+  return h[0];
+}
+
+static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool insert) {
+  Cpma *c = state(g);
+  hipStream_t s = g->stream;
+  if (b == 0) return;
+  UpdCtl *ctl = g->buf("upd_ctl", sizeof(UpdCtl)).as<UpdCtl>();
+  UpdCtl *hctl = static_cast<UpdCtl *>(pinned(g, sizeof(UpdCtl)));
+  PhaseTimer t(s, g->prof.on);
+  uint64_t bu = 0;
+  const uint64_t *batch = prepare_batch(g, keys_dev, b, &bu, ctl, hctl);
+  if (bu == 0) return;
+  uint32_t *leaf_of = nullptr, *starts = nullptr;
+  const uint64_t nseg = segment_batch(g, batch, bu, &leaf_of, &starts);
+  BYO_HIP(hipMemsetAsync(ctl, 0, sizeof(UpdCtl), s));
+  const unsigned grid = (unsigned)std::min<uint64_t>(nseg, 16384);
+  if (insert) {
+    uint32_t *newc = g->buf("upd_newc", nseg * 4 + 64).as<uint32_t>();
+    k_ins_count<<<grid, 256, 0, s>>>(batch, bu, starts, nseg, leaf_of, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+                                     newc, ctl);
+    BYO_LAUNCH_CHECK();
+    BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
+    BYO_HIP(hipStreamSynchronize(s));
+    if (!hctl->overflow) {
+      k_ins_merge<<<grid, 256, 0, s>>>(batch, bu, starts, nseg, leaf_of, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+                                       c->deg.as<uint32_t>(), ctl);
+      BYO_LAUNCH_CHECK();
+      BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
+      BYO_HIP(hipStreamSynchronize(s));
+      c->total += hctl->changed;
+      g->m = c->total;
+      if (hctl->changed) g->csr_valid = false;
+    } else {
+      // global rebalance: flatten, merge with the batch, dedupe, respread
+      c->rebalances++;
+      const uint64_t tot = c->total + bu;
+      uint64_t *flat = g->buf("cpma_flat", c->total * 8 + 64).as<uint64_t>();
+      uint64_t *merged = g->buf("cpma_merged", tot * 8 + 64).as<uint64_t>();
+      uint64_t *uniq = g->buf("cpma_uniq", tot * 8 + 64).as<uint64_t>();
+      cpma_flatten(g, flat);
+      k_merge<<<byo_grid((tot + 7) / 8, 256, 16384), 256, 0, s>>>(flat, c->total, batch, bu, merged);
+      BYO_LAUNCH_CHECK();
+      void *stmp = g->buf("cpma_sel_tmp", prim::select_tmp_bytes(tot) + 256).p;
+      prim::unique_u64(merged, uniq, tot, kEmpty, reinterpret_cast<uint64_t *>(&ctl->pad[1]), stmp, s);
+      BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
+      BYO_HIP(hipStreamSynchronize(s));
+      const uint64_t m2 = hctl->pad[1];
+      cpma_build(g, uniq, m2);
+      BYO_HIP(hipStreamSynchronize(s));
+    }
+  } else {
+    k_remove<<<grid, 256, 0, s>>>(batch, bu, starts, nseg, leaf_of, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+                                  c->deg.as<uint32_t>(), ctl);
+    BYO_LAUNCH_CHECK();
+    BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
+    BYO_HIP(hipStreamSynchronize(s));
+    c->total -= hctl->changed;
+    g->m = c->total;
+    if (hctl->changed) g->csr_valid = false;
+    if (c->nleaves > 1 && (double)c->total < kMinDensity * (double)(c->nleaves * kLeaf)) {
+      c->rebalances++;
+      uint64_t *flat = g->buf("cpma_flat", c->total * 8 + 64).as<uint64_t>();
+      cpma_flatten(g, flat);
+      cpma_build(g, flat, c->total);
+      BYO_HIP(hipStreamSynchronize(s));
+    }
+  }
+  if (g->prof.on) g->account(insert ? "cpma_insert" : "cpma_remove", t.stop());
+}
+
+}  // namespace byo
+
+using namespace byo;
+
+namespace {
+byo_graph *new_cpma_handle(uint64_t n) {
+  byo_graph *g = new_handle(1);
+  g->n = n;
+  g->cpma = new Cpma();
+  return g;
+}
+}  // namespace
+
+extern "C" {
+
+int byo_cpma_create(const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m, byo_graph_t *out) {
+  return guard([&] {
+    BYO_REQUIRE(out != nullptr, BYO_E_INVAL, "out is null");
+    check_csr_input(offsets, ids, n, m);
+    for (uint64_t v = 0; v < n; v++)
+      for (uint64_t e = offsets[v] + 1; e < offsets[v + 1]; e++)
+        BYO_REQUIRE(ids[e - 1] < ids[e], BYO_E_INVAL, "adjacency lists must be sorted without duplicates");
+    byo_graph *g = new_cpma_handle(n);
+    try {
+      csr_upload(g, offsets, ids, n, m);
+      uint64_t *flat = g->buf("cpma_flat", m * 8 + 64).as<uint64_t>();
+      prim::csr_to_keys(g->off.as<uint64_t>(), g->ids.as<uint32_t>(), n, m, flat, g->stream);
+      cpma_build(g, flat, m);
+      BYO_HIP(hipStreamSynchronize(g->stream));
+    } catch (...) {
+      byo_graph_destroy(g);
+      throw;
+    }
+    *out = g;
+  });
+}
+
+int byo_cpma_create_from(byo_graph_t src, byo_graph_t *out) {
+  return guard([&] {
+    BYO_REQUIRE(src && out, BYO_E_INVAL, "null argument");
+    std::lock_guard<std::mutex> lk(src->mu);
+    bind(src);
+    ensure_csr(src);
+    byo_graph *g = new_cpma_handle(src->n);
+    try {
+      const uint64_t m = src->m;
+      uint64_t *flat = g->buf("cpma_flat", m * 8 + 64).as<uint64_t>();
+      BYO_HIP(hipStreamSynchronize(src->stream));
+      prim::csr_to_keys(src->off.as<uint64_t>(), src->ids.as<uint32_t>(), src->n, m, flat, g->stream);
+      cpma_build(g, flat, m);
+      BYO_HIP(hipStreamSynchronize(g->stream));
+    } catch (...) {
+      byo_graph_destroy(g);
+      throw;
+    }
+    *out = g;
+  });
+}
+
+int byo_insert_batch_dev(byo_graph_t g, const uint64_t *keys_dev, uint64_t b) {
+  return guard([&] {
+    BYO_REQUIRE(g && (keys_dev || b == 0), BYO_E_INVAL, "null argument");
+    BYO_REQUIRE(g->kind == 1, BYO_E_STATE, "insert_batch needs a CPMA container");
+    std::lock_guard<std::mutex> lk(g->mu);
+    bind(g);
+    cpma_update(g, keys_dev, b, true);
+  });
+}
+
+int byo_remove_batch_dev(byo_graph_t g, const uint64_t *keys_dev, uint64_t b) {
+  return guard([&] {
+    BYO_REQUIRE(g && (keys_dev || b == 0), BYO_E_INVAL, "null argument");
+    BYO_REQUIRE(g->kind == 1, BYO_E_STATE, "remove_batch needs a CPMA container");
+    std::lock_guard<std::mutex> lk(g->mu);
+    bind(g);
+    cpma_update(g, keys_dev, b, false);
+  });
+}
+
+static int host_batch(byo_graph_t g, const uint64_t *keys, uint64_t b, bool insert) {
+  return guard([&] {
+    BYO_REQUIRE(g && (keys || b == 0), BYO_E_INVAL, "null argument");
+    BYO_REQUIRE(g->kind == 1, BYO_E_STATE, "batch updates need a CPMA container");
+    std::lock_guard<std::mutex> lk(g->mu);
+    bind(g);
+    if (b == 0) return;
+    uint64_t *d = g->buf("upd_host_in", b * 8 + 64).as<uint64_t>();
+    BYO_HIP(hipMemcpyAsync(d, keys, b * 8, hipMemcpyHostToDevice, g->stream));
+    cpma_update(g, d, b, insert);
+  });
+}
+
+int byo_insert_batch(byo_graph_t g, const uint64_t *keys, uint64_t b) { return host_batch(g, keys, b, true); }
+int byo_remove_batch(byo_graph_t g, const uint64_t *keys, uint64_t b) { return host_batch(g, keys, b, false); }
+
+}  // extern "C"

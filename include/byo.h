@@ -1,0 +1,237 @@
+/*
+ * byo.h -- C ABI of the MI355X-native BYO hot path (libbyo.so).
+ *
+ * One handle (byo_graph_t) owns one device-resident graph container -- a
+ * static CSR or a CPMA-style packed-memory-array -- plus the HIP stream and
+ * scratch memory its algorithms use.  Every entry point returns an int
+ * status (BYO_OK = 0, negative on error) and never throws; the message of
+ * the last error on the calling thread is returned by byo_last_error().
+ * Pointers are plain host pointers unless the name says "dev".  Calls on
+ * one handle are serialised by the library; different handles may be used
+ * from different threads.
+ *
+ * Each declaration names the reference interface it replaces (paths are
+ * relative to the reference repository root).  Algorithms run device-side
+ * through whole-algorithm entry points because the reference's functor
+ * arguments (F in map_neighbors(i, F)) cannot cross a C ABI; the host C++
+ * classes in byo_graph.hpp put the functor API back on top.
+ */
+#ifndef BYO_H
+#define BYO_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BYO_OK 0
+#define BYO_E_INVAL -1 /* bad argument (null pointer, id out of range, ...) */
+#define BYO_E_OOM -2   /* device allocation failed                           */
+#define BYO_E_HIP -3   /* HIP runtime error                                  */
+#define BYO_E_STATE -4 /* operation not valid for this container             */
+#define BYO_E_NODEV -5 /* no usable gfx950 device                            */
+
+#define BYO_NONE 0xFFFFFFFFu /* UINT_E_MAX (gbbs/macros.h): unreached / none */
+
+typedef struct byo_graph *byo_graph_t;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *byo_last_error(void);
+/* Library version string. */
+const char *byo_version(void);
+/* Number of visible devices (0 when none); never fails. */
+int byo_device_count(void);
+
+/* ---------------------------------------------------------------------- */
+/* Containers                                                              */
+/* ---------------------------------------------------------------------- */
+
+/* Static CSR from host arrays (copied to HBM).  Replaces
+ * CSR::CSR(v_data, n, m, deletion_fn, e0) (benchmarks/run_structures/
+ * run_csr.cc:87-106): offsets[n+1] (uint64, -DLONG) and ids[m] (uint32),
+ * each adjacency list sorted ascending without duplicates. */
+int byo_csr_create(const uint64_t *offsets, const uint32_t *ids, uint64_t n,
+                   uint64_t m, byo_graph_t *out);
+
+/* Static CSR generated on the device: 2^scale vertices, edge_factor*2^scale
+ * RMAT edges (a, b, c; d = 1-a-b-c) from this library's counter-based
+ * generator with the given seed, symmetrised, deduplicated, self loops
+ * dropped.  The graph a reference run would read from an AdjacencyGraph file
+ * produced by gbbs_io (gbbs/graph_io.h:829-847 normalisation). */
+int byo_csr_create_rmat(int scale, uint64_t edge_factor, double a, double b,
+                        double c, uint64_t seed, byo_graph_t *out);
+
+/* CPMA-style packed-memory-array graph from host CSR arrays.  Replaces
+ * PMA_graph::PMA_graph(v_data, n, m, ...) (benchmarks/run_structures/
+ * run_single_pma.cc:75-89): keys (src<<32)|dst. */
+int byo_cpma_create(const uint64_t *offsets, const uint32_t *ids, uint64_t n,
+                    uint64_t m, byo_graph_t *out);
+/* CPMA built from an existing device container's edge set (device to device). */
+int byo_cpma_create_from(byo_graph_t src, byo_graph_t *out);
+
+/* Frees the container and everything it owns (the container destructor,
+ * run_csr.cc:159). */
+int byo_graph_destroy(byo_graph_t g);
+
+/* 0 = CSR, 1 = CPMA. */
+int byo_graph_kind(byo_graph_t g, int *kind);
+/* Graph::N() / num_vertices() (gbbs/graph.h:2179-2183, run_csr.cc:31). */
+int byo_num_vertices(byo_graph_t g, uint64_t *n);
+/* Graph::M() / num_edges() (gbbs/graph.h:2266-2271, run_csr.cc:32). */
+int byo_num_edges(byo_graph_t g, uint64_t *m);
+/* degree(i) for all vertices into deg[n] (run_csr.cc:34-36; the stored
+ * degrees of Graph::finilize, gbbs/graph.h:2559-2588, always current). */
+int byo_degrees(byo_graph_t g, uint32_t *deg);
+/* degree(i) of one vertex. */
+int byo_degree(byo_graph_t g, uint32_t v, uint32_t *deg);
+/* Neighbours of v in ascending order: the sequence map_neighbors(v, F)
+ * visits (run_csr.cc:38-43, run_single_pma.cc:46-61).  Writes at most cap
+ * ids and the full degree to *count. */
+int byo_neighbors(byo_graph_t g, uint32_t v, uint32_t *out, uint64_t cap,
+                  uint64_t *count);
+/* Whole edge set as host CSR: offsets[n+1], ids[m]. */
+int byo_export_csr(byo_graph_t g, uint64_t *offsets, uint32_t *ids);
+/* Device bytes held by the container (get_memory_size, run_csr.cc:156-158,
+ * run_single_pma.cc:126-131). */
+int byo_memory_size(byo_graph_t g, uint64_t *bytes);
+
+/* Batch updates.  Replace Graph::insert_batch / remove_batch
+ * (gbbs/graph.h:3071-3124) over PMA_graph::insert_sorted_batch /
+ * remove_sorted_batch (run_single_pma.cc:63-70).  keys[b] use the
+ * std::tuple<uint32,uint32> byte layout of the reference, i.e. the uint64
+ * (src<<32)|dst, so a reference harness can pass its tuple array directly.
+ * The batch may be unsorted with duplicates (the library sorts and
+ * dedupes, as graph.h:3090-3092 does).  Edges are directed, as in the
+ * reference.  Every vertex id must be < n (BYO_E_INVAL otherwise, container
+ * unchanged).  CPMA only (BYO_E_STATE for CSR). */
+int byo_insert_batch(byo_graph_t g, const uint64_t *keys, uint64_t b);
+int byo_remove_batch(byo_graph_t g, const uint64_t *keys, uint64_t b);
+/* The same with the batch already in device memory (keys_dev, b entries). */
+int byo_insert_batch_dev(byo_graph_t g, const uint64_t *keys_dev, uint64_t b);
+int byo_remove_batch_dev(byo_graph_t g, const uint64_t *keys_dev, uint64_t b);
+
+/* Update stream of Batch_insert_runner (benchmarks/run_unweighted.h:781-806,
+ * rMat at :730-778), generated on the device: keys[k] = (src<<32)|dst of
+ * updates first .. first+count-1 for a graph of graph_n vertices.  The
+ * reference's arithmetic is kept exactly (uint32 hash32 / INT32_MAX, so the
+ * effective quadrant weights are skewed and endpoints stay below
+ * 2^(ceil(log2 graph_n)-1)). */
+int byo_update_rmat_keys(uint64_t graph_n, uint64_t first, uint64_t count,
+                         uint64_t *keys);
+
+/* ---------------------------------------------------------------------- */
+/* Algorithms                                                              */
+/* ---------------------------------------------------------------------- */
+
+/* Per-level record of one BFS: frontier size, sum of frontier out-degrees
+ * and the direction taken (0 sparse push, 1 dense pull, 2 empty because
+ * the frontier has no out-edges -- edge_map_data.h:289). */
+typedef struct {
+  uint32_t levels;      /* edgeMap calls made                               */
+  uint32_t reached;     /* vertices with a parent                           */
+  uint64_t edges_reached; /* sum of degrees of reached vertices (m_r)       */
+  double device_ms;     /* device time of the traversal (HIP events)        */
+  uint64_t fsize[64];
+  uint64_t fdeg[64];
+  uint8_t dir[64];
+  float level_ms[64];   /* per-level device time when profiling is on      */
+} byo_bfs_stats;
+
+/* Direction policies for edgeMap (edge_map_data.h:254-307). */
+#define BYO_DIR_THRESHOLD 0 /* GBBS rule: dense iff |F|+deg(F) > M/20 ...    */
+#define BYO_DIR_MLP 1       /* learned selector, rule as fallback            */
+#define BYO_DIR_SPARSE 2    /* always sparse push (edgeMap no_dense)         */
+#define BYO_DIR_DENSE 3     /* always dense pull (edgeMap dense_only)        */
+/* threshold < 0 means M/20 (the -1 default of edgeMap). */
+int byo_set_direction_policy(byo_graph_t g, int policy, int64_t threshold);
+
+/* BFS from src.  Replaces BFS(G, src) (benchmarks/BFS/NonDeterministicBFS/
+ * BFS.h:47-64): parents[n] with parents[src] = src and BYO_NONE for
+ * unreached vertices (parents chosen by push rounds are any valid BFS
+ * parent, as CAS order makes them in the reference).  parents and stats
+ * may be NULL (result stays on the device). */
+int byo_bfs(byo_graph_t g, uint32_t src, uint32_t *parents,
+            byo_bfs_stats *stats);
+/* k independent BFS runs in one call (one device launch sequence);
+ * parents[k*n] (may be NULL), stats[k] (may be NULL). */
+int byo_bfs_multi(byo_graph_t g, const uint32_t *srcs, uint32_t k,
+                  uint32_t *parents, byo_bfs_stats *stats);
+/* BFS levels (hop distance, BYO_NONE unreached) of the last byo_bfs call,
+ * derived on the device from its parent tree (run_unweighted.h:142-163). */
+int byo_bfs_levels(byo_graph_t g, uint32_t *levels);
+
+#define BYO_PR_FP64 0 /* FP64 Jacobi                                        */
+#define BYO_PR_FP32 1 /* FP32 values, Kahan per-vertex sums, FP64 residual  */
+/* PageRank.  Replaces PageRank(G, eps, max_iters) (benchmarks/PageRank/
+ * PageRank.h:125-193) with the Jacobi semantics of PageRank_edgeMap
+ * (:80-123): p0 = 1/n, p' = damping * sum_{s in N(i)} p[s]/deg(s) +
+ * (1-damping)/n, stop after max_iters or when the L1 change < eps
+ * (eps <= 0: never early).  p[n] (FP64 on output for both precisions, may
+ * be NULL), l1_log[max_iters] (may be NULL), *iters_run and *device_ms
+ * (may be NULL). */
+int byo_pagerank(byo_graph_t g, double damping, int max_iters, double eps,
+                 int precision, double *p, double *l1_log, int *iters_run,
+                 double *device_ms);
+/* Personalized PageRank for k seeds in one launch sequence (FP32 values,
+ * Kahan sums): p0 = e_seed, p' = damping * sum p[s]/deg(s) +
+ * (1-damping)[i == seed].  p[k*n] as FP32 (may be NULL). */
+int byo_ppr_multi(byo_graph_t g, const uint32_t *seeds, uint32_t k,
+                  double damping, int iters, float *p, double *device_ms);
+
+/* Connected components.  Replaces SimpleUnionAsync(G) (benchmarks/
+ * Connectivity/SimpleUnionAsync/Connectivity.h:122-136): labels[n] with
+ * label = smallest vertex id of the component. */
+int byo_cc(byo_graph_t g, uint32_t *labels, double *device_ms);
+/* Min-label propagation (LabelPropagation/Connectivity.h:132-146) over
+ * edgeMap dense-forward rounds; same labels; *rounds may be NULL. */
+int byo_cc_labelprop(byo_graph_t g, uint32_t *labels, uint32_t *rounds,
+                     double *device_ms);
+
+/* Triangle count.  Replaces Triangle_degree_ordering(G, f)
+ * (benchmarks/TriangleCounting/ShunTangwongsan15/Triangle.h:167-204):
+ * rank by (degree, id), orient low to high rank, count sorted-list
+ * intersections.  CSR only (the reference never runs it on the CPMA). */
+int byo_tc(byo_graph_t g, uint64_t *count, double *device_ms);
+
+/* Test_Map (benchmarks/Map/Base/Map.h:8-18): sum of dst over all edges. */
+int byo_map_sum(byo_graph_t g, uint64_t *sum, double *device_ms);
+
+/* ---------------------------------------------------------------------- */
+/* Profiling: HIP events around each launch of the named device phases.    */
+/* ---------------------------------------------------------------------- */
+int byo_profile_enable(byo_graph_t g, int on);
+/* Accumulated launches and device milliseconds of one phase ("bfs",
+ * "pr_fp32", "pr_fp64", "cc", "tc_count", "map", "cpma_insert", ...). */
+int byo_profile_get(byo_graph_t g, const char *phase, uint64_t *launches,
+                    double *total_ms);
+int byo_profile_reset(byo_graph_t g);
+
+/* ---------------------------------------------------------------------- */
+/* Learned direction selector (MLP 6->32->32->3, FP32)                     */
+/* ---------------------------------------------------------------------- */
+#define BYO_MLP_NPARAM 1379
+/* Loads selector weights (layout W1[32x6] b1 W2[32x32] b2 W3[3x32] b3,
+ * row-major) and the 6-feature normalisation (mean[6], inv_std[6]). */
+int byo_selector_load(byo_graph_t g, const float *params, const float *mean,
+                      const float *inv_std);
+/* One training step on the device: forward, softmax cross-entropy,
+ * backward and an Adam update over a minibatch x[B*6], labels[B].  The
+ * parameter and moment state live in a device-side trainer object. */
+typedef struct byo_trainer *byo_trainer_t;
+int byo_trainer_create(const float *init_params, float lr, byo_trainer_t *out);
+int byo_trainer_destroy(byo_trainer_t t);
+int byo_trainer_step(byo_trainer_t t, const float *x, const int32_t *labels,
+                     int batch, float *loss);
+/* Forward + backward only (no update): grads[NPARAM], logits[B*3]. */
+int byo_trainer_grad(byo_trainer_t t, const float *x, const int32_t *labels,
+                     int batch, float *loss, float *grads, float *logits);
+int byo_trainer_params(byo_trainer_t t, float *params);
+/* Predicted class per row of x[B*6] (argmax of logits). */
+int byo_trainer_predict(byo_trainer_t t, const float *x, int batch,
+                        int32_t *cls);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

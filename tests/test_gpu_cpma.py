@@ -1,0 +1,134 @@
+"""GPU parity of the CPMA container: after every insert_batch / remove_batch
+the edge set equals the closed form E u unique(B) / E minus unique(B) as
+sorted uint64 keys (bit-exact), degrees and M follow, and BFS / PageRank /
+CC rerun on the updated graph agree with the oracle on the same edge set."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def B():
+    import byo_amd
+    assert byo_amd.device_count() > 0, "no GPU visible"
+    return byo_amd
+
+
+def keys_of(G):
+    off, ids = G.export_csr()
+    g = O.Graph(off, ids)
+    return O.graph_keys(g), g
+
+
+def upd_keys(n, count, first=0):
+    s, d = O.update_rmat(n, count, first)
+    return (s.astype(np.uint64) << np.uint64(32)) | d.astype(np.uint64)
+
+
+def test_build_roundtrip(B):
+    g = O.rmat_graph(14)
+    G = B.CpmaGraph.from_csr(g.offsets, g.ids)
+    k, h = keys_of(G)
+    assert np.array_equal(k, O.graph_keys(g))
+    assert G.M() == g.m and G.N() == g.n
+    assert np.array_equal(G.degrees(), g.degrees())
+    C = B.CsrGraph.from_csr(g.offsets, g.ids)
+    G2 = B.CpmaGraph.from_graph(C)
+    assert np.array_equal(keys_of(G2)[0], O.graph_keys(g))
+    assert G.get_memory_size() > 8 * g.m
+
+
+def test_batches_follow_set_semantics(B):
+    """Batch_insert_runner (run_unweighted.h:781-893): sizes 1..1e5 in
+    powers of 10, insert then remove the same (shuffled) batch."""
+    g = O.rmat_graph(14)
+    G = B.CpmaGraph.from_csr(g.offsets, g.ids)
+    E = O.graph_keys(g)
+    rng = np.random.default_rng(0)
+    b = 1
+    while b <= 100000:
+        batch = upd_keys(g.n, b)
+        G.insert_batch(batch)
+        E = np.union1d(E, batch)
+        k, _ = keys_of(G)
+        assert np.array_equal(k, E), ("insert", b)
+        assert G.M() == len(E)
+        shuffled = batch.copy()
+        rng.shuffle(shuffled)
+        G.remove_batch(shuffled)
+        E = np.setdiff1d(E, batch)
+        k, h = keys_of(G)
+        assert np.array_equal(k, E), ("remove", b)
+        assert np.array_equal(G.degrees(), h.degrees())
+        b *= 10
+
+
+def test_overflow_rebalance_and_growth(B):
+    """A batch concentrated on few vertices overflows leaves and forces the
+    global rebalance; a large random batch grows the leaf array."""
+    g = O.rmat_graph(12)
+    G = B.CpmaGraph.from_csr(g.offsets, g.ids)
+    E = O.graph_keys(g)
+    hub = np.arange(0, g.n, dtype=np.uint64)
+    batch = (np.uint64(7) << np.uint64(32)) | hub  # vertex 7 gains n edges
+    G.insert_batch(batch)
+    E = np.union1d(E, batch)
+    assert np.array_equal(keys_of(G)[0], E)
+    rng = np.random.default_rng(1)
+    big = rng.integers(0, g.n, (400000, 2), dtype=np.uint64)
+    bk = (big[:, 0] << np.uint64(32)) | big[:, 1]
+    G.insert_batch(bk)
+    E = np.union1d(E, bk)
+    assert np.array_equal(keys_of(G)[0], E)
+    # remove almost everything: shrink path
+    G.remove_batch(E[: len(E) - 1000])
+    E = E[len(E) - 1000:]
+    assert np.array_equal(keys_of(G)[0], E)
+    G.remove_batch(E)
+    assert G.M() == 0
+    assert len(keys_of(G)[0]) == 0
+    G.insert_batch(bk[:100])
+    assert np.array_equal(keys_of(G)[0], np.unique(bk[:100]))
+
+
+def test_duplicates_self_loops_and_bad_ids(B):
+    g = O.rmat_graph(10)
+    G = B.CpmaGraph.from_csr(g.offsets, g.ids)
+    E = O.graph_keys(g)
+    batch = np.array([(3 << 32) | 3, (3 << 32) | 3, (5 << 32) | 9, (5 << 32) | 9], np.uint64)
+    G.insert_batch(batch)
+    E = np.union1d(E, batch)
+    assert np.array_equal(keys_of(G)[0], E)
+    with pytest.raises(B.ByoError) as e:
+        G.insert_batch(np.array([(np.uint64(g.n) << np.uint64(32)) | np.uint64(1)], np.uint64))
+    assert e.value.code == B.E_INVAL
+    assert np.array_equal(keys_of(G)[0], E)  # unchanged after the rejected batch
+    G.insert_batch(np.zeros(0, np.uint64))
+    G.remove_batch(np.zeros(0, np.uint64))
+    assert np.array_equal(keys_of(G)[0], E)
+
+
+def test_algorithms_after_updates(B):
+    g = O.rmat_graph(14)
+    G = B.CpmaGraph.from_csr(g.offsets, g.ids)
+    # symmetric batches keep the graph undirected, so BFS pull (which reads
+    # out-neighbours as in-neighbours on a symmetric Graph<>) stays defined
+    one = upd_keys(g.n, 25000)
+    rev = ((one & np.uint64(0xFFFFFFFF)) << np.uint64(32)) | (one >> np.uint64(32))
+    batch = np.concatenate([one, rev])
+    G.insert_batch(batch)
+    G.remove_batch(np.concatenate([one[:10000], rev[:10000]]))
+    _, h = keys_of(G)
+    assert np.array_equal(G.degrees(), h.degrees())
+    src = 10
+    lv = O.bfs_levels(h, src)
+    par, st = G.bfs(src)
+    assert O.check_bfs_tree(h, src, par, lv) == 0
+    p_ref, _ = O.pagerank(h, 0.85, 20, 0.0)
+    p, _, _ = G.pagerank(0.85, 20, 0.0, B.PR_FP64)
+    assert np.abs(p - p_ref).sum() < 1e-12
+    s, _ = G.map_sum()
+    assert s == O.map_sum(h)
