@@ -20,12 +20,15 @@
 // hits a 0.5 MiB (scale 22) array that stays in L2.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "primitives.hpp"
 
 namespace byo {
 
 void mlp_eval_host(const std::vector<float> &p, const float *x, float *logits);  // selector.hip
+void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all, byo_bfs_stats *stats,
+                  uint64_t maxdeg, double *ms_out);  // bfs_coop.hip
 
 namespace {
 
@@ -452,8 +455,36 @@ int byo_bfs_multi(byo_graph_t g, const uint32_t *srcs, uint32_t k, uint32_t *par
     ensure_csr(g);
     for (uint32_t i = 0; i < k; i++) BYO_REQUIRE(srcs[i] < g->n, BYO_E_INVAL, "source out of range");
     if (k == 0) return;
-    BfsRun r = bfs_setup(g);
     const uint64_t mx = (g->dir_policy == BYO_DIR_MLP) ? max_degree(g) : 0;
+    const char *mode = std::getenv("BYO_BFS_MODE");
+    if (!(mode && std::strcmp(mode, "levels") == 0)) {
+      // default: every source in one persistent cooperative launch
+      const bool keep_all = parents != nullptr && k > 1;
+      std::vector<byo_bfs_stats> tmp(stats ? 0 : k);
+      byo_bfs_stats *st = stats ? stats : tmp.data();
+      double ms = 0;
+      bfs_coop_run(g, srcs, k, keep_all, st, mx, &ms);
+      if (g->prof.on) {
+        auto &acc = g->prof.acc["bfs"];
+        acc.launches += k;
+        acc.ms += ms;
+        g->account("bfs_launch", ms);
+      }
+      const uint64_t n = g->n;
+      if (keep_all) {
+        uint32_t *all = g->scratch["bfs_parents_all"].as<uint32_t>();
+        BYO_HIP(hipMemcpyAsync(parents, all, (uint64_t)k * n * 4, hipMemcpyDeviceToHost, g->stream));
+        uint32_t *last = g->buf("bfs_parents", n * 4 + 64).as<uint32_t>();
+        BYO_HIP(hipMemcpyAsync(last, all + (uint64_t)(k - 1) * n, n * 4, hipMemcpyDeviceToDevice, g->stream));
+      } else if (parents) {
+        BYO_HIP(hipMemcpyAsync(parents, g->scratch["bfs_parents"].p, n * 4, hipMemcpyDeviceToHost, g->stream));
+      }
+      g->scratch["bfs_last_src"].ensure(8);
+      BYO_HIP(hipMemcpyAsync(g->scratch["bfs_last_src"].p, &srcs[k - 1], 4, hipMemcpyHostToDevice, g->stream));
+      BYO_HIP(hipStreamSynchronize(g->stream));
+      return;
+    }
+    BfsRun r = bfs_setup(g);
     for (uint32_t i = 0; i < k; i++) {
       byo_bfs_stats *st = stats ? &stats[i] : nullptr;
       if (st) std::memset(st, 0, sizeof(*st));
