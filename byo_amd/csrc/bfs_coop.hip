@@ -4,40 +4,55 @@
 // Same traversal as the level-by-level path in bfs.hip, restating BFS.h:47-64
 // over edgeMapData (gbbs/edge_map_data.h:254-307), but every level's
 // direction is decided on the device from counters the previous level left
-// in HBM, so no host round trip happens between levels.  Both frontier
-// representations are kept current at every level -- the queue (sparse
-// vertexSubset) is appended by one atomic per wave in push and in pull, and
-// the bitmap (BitArray) is written by ballot words in pull and set by the
-// CAS winners in push -- so no toSparse / toDense conversion pass exists.
+// in HBM, so no host round trip happens between levels.
 //
-// Per level:  sparse push  = 3 grid barriers (slice sums of frontier
-//             degrees; exclusive scan; edge-balanced push),
-//             dense pull   = 1 barrier,
-//             dense forward = 2 barriers.
-// Counters of the frontier sizes rotate over three slots so that resetting
-// the slot for level L+2 never races with readers of level L or writers of
-// level L+1.
-#include <hip/hip_cooperative_groups.h>
-
+// Frontier representations (vertexSubsetData, gbbs/vertex_subset.h:203-371):
+// the bitmap (BitArray) of the next frontier is produced by every direction
+// (ballot bytes in pull, atomicOr by CAS winners in push/forward); the queue
+// (sparse ids) by push and forward.  A sparse level after a dense one packs
+// the bitmap into a queue first (toSparse, vertex_subset.h:339-354).
+//
+// The kernels are shaped for latency hiding rather than for one thread per
+// vertex:
+//  * sparse push (edgeMapSparse, gbbs/edge_map_blocked.h:18-92): the edges
+//    of the frontier are cut into chunks of 256; the scan pass also records
+//    the first frontier index of every chunk, so a wave loads the (scan,
+//    adjacency base) window of its chunk into LDS with one coalesced load and
+//    resolves each edge's owner by a 6-step LDS search;
+//  * dense pull (edgeMapDense, edge_map_data.h:47-175): 8 lanes per
+//    unvisited vertex read 8 consecutive neighbours at once and test them
+//    against the bitmap, exiting at the first frontier member (the lowest
+//    such neighbour in adjacency order becomes the parent, as the
+//    reference's sequential early-exit scan picks it); a wave owns one
+//    bitmap byte per step, so next-frontier and visited bytes are written
+//    without atomics;
+//  * dense forward (edgeMapDenseForward, :180-247): one wave per frontier
+//    vertex, 64 neighbours per step.
+//
+// Grid barriers per level: push 3 (1 for frontiers of at most 4 vertices
+// per block; +1 when packing), pull 1, forward 1.
+// Frontier counters rotate over three slots so that resetting the slot for
+// level L+2 never races with readers of level L or writers of level L+1.
 #include <algorithm>
 #include <cmath>
 
 #include "primitives.hpp"
-
-namespace cg = cooperative_groups;
 
 namespace byo {
 
 namespace {
 
 struct Ctr {
-  unsigned long long size;  // frontier size == queue cursor
+  unsigned long long size;  // frontier size (queue cursor in push/forward)
   unsigned long long deg;   // sum of out-degrees of the frontier
+  unsigned long long cursor;  // pack cursor of the level using this slot
+  unsigned long long pad;
 };
 
 struct LevelLog {
   unsigned long long fsize, fdeg;
   uint32_t dir, pad;
+  unsigned long long t0;  // wall clock (100 MHz) at the start of the level
 };
 
 struct CoopArgs {
@@ -47,8 +62,9 @@ struct CoopArgs {
   uint32_t *parents;        // parents of source j at parents + j * pstride
   uint64_t pstride;
   uint32_t *q0, *q1;
-  uint64_t *bm0, *bm1, *visited64;
+  uint64_t *bm0, *bm1, *bm2, *visited64;
   uint64_t *scan;
+  uint32_t *cstart;         // first frontier index of each 256-edge chunk
   unsigned long long *bsum;
   Ctr *ctr;                 // 3 slots
   const uint32_t *srcs;
@@ -60,12 +76,15 @@ struct CoopArgs {
   uint64_t thr;
   const float *sel;         // 1379 params + mean[6] + inv_std[6], or null
   uint64_t maxdeg;
+  struct GridBar *bar;
 };
 
 constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
 constexpr int kPerLane = 4;
 constexpr uint64_t kChunk = 64 * kPerLane;
 constexpr int kNP = BYO_MLP_NPARAM;
+constexpr int kG = 8;  // lanes per vertex in the pull
 
 __device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long long *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -133,6 +152,49 @@ __device__ int decide(const CoopArgs &a, bool is_dense, uint64_t fs, uint64_t fd
   return best == 0 ? 0 : (best == 1 ? 1 : 3);
 }
 
+// Grid-wide barrier for a cooperative launch (all blocks co-resident).
+// Arrivals are counted per group of 32 blocks, then once per group on the
+// top counter, so no address sees more than 32 + nblocks/32 atomics; the
+// last arrival bumps the generation word the others poll.  Agent-scope
+// release/acquire fences make every block's writes visible across XCDs.
+struct GridBar {
+  unsigned int top;
+  unsigned int pad0[31];
+  unsigned int gen;  // polled word on a line of its own
+  unsigned int pad1[31];
+  unsigned int group[1024];  // counter of 32-block group g at group[16 g]
+};
+
+__device__ __forceinline__ void grid_barrier(GridBar *b, unsigned int nblocks) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int gen = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned int grp = blockIdx.x >> 5;
+    const unsigned int gsize = min(32u, nblocks - (grp << 5));
+    const unsigned int ngroups = (nblocks + 31) >> 5;
+    bool last = false;
+    const unsigned int a = __hip_atomic_fetch_add(&b->group[grp * 16], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (a == gsize - 1) {
+      // the group's last arrival carries the group's writes forward
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __hip_atomic_store(&b->group[grp * 16], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned int t = __hip_atomic_fetch_add(&b->top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (t == ngroups - 1) {
+        __hip_atomic_store(&b->top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        last = true;
+      }
+    }
+    // poll relaxed: an acquire per poll would invalidate this XCD's L2 and
+    // evict the data the still-working blocks are using
+    if (!last)
+      while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) __builtin_amdgcn_s_sleep(2);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
 // Block-wide sum (all threads get the result).
 __device__ __forceinline__ unsigned long long block_sum(unsigned long long v, unsigned long long *lds) {
   v = wave_sum(v);
@@ -140,62 +202,145 @@ __device__ __forceinline__ unsigned long long block_sum(unsigned long long v, un
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
   __syncthreads();
   unsigned long long s = 0;
-  for (int w = 0; w < kBlock / 64; w++) s += lds[w];
+  for (int w = 0; w < kWaves; w++) s += lds[w];
   return s;
 }
 
+struct PushLds {
+  uint64_t sc[kWaves][64];
+  int64_t base[kWaves][64];
+};
+
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) k_bfs_coop(CoopArgs a) {
-  cg::grid_group grid = cg::this_grid();
   __shared__ unsigned long long lds[8];
   __shared__ MlpLds mlp;
+  __shared__ PushLds win;
   const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
   const uint64_t nb = gridDim.x;
   const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
   const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
   const uint64_t nwords = (a.n + 63) >> 6;
+  const uint64_t nbytes = (a.n + 7) >> 3;
 
   for (uint32_t j = 0; j < a.k; j++) {
     uint32_t *parents = a.parents + (uint64_t)j * a.pstride;
     const uint32_t src = a.srcs[j];
-    for (uint64_t i = global_tid(); i < a.n; i += grid_threads()) parents[i] = kNone;
+    // one pass seeds the BFS: parents[src] = src, the source's bit in the
+    // visited and frontier bitmaps, q[0] = src, counters of level 0.  Both
+    // bitmaps start clear elsewhere: the pull writes only the bytes of real
+    // vertices, so the padding of the last word must already be zero.
+    for (uint64_t i = global_tid(); i < a.n; i += grid_threads()) parents[i] = (i == src) ? src : kNone;
+    const uint64_t sw = src >> 6, sbit = 1ull << (src & 63);
     for (uint64_t i = global_tid(); i < a.nw64; i += grid_threads()) {
-      a.visited64[i] = 0;
-      a.bm0[i] = 0;
+      a.visited64[i] = (i == sw) ? sbit : 0;
+      a.bm0[i] = (i == sw) ? sbit : 0;
+      a.bm1[i] = 0;
+      a.bm2[i] = 0;
     }
-    if (leader)
-      for (int s = 0; s < 3; s++) a.ctr[s] = Ctr{0, 0};
-    grid.sync();
     if (leader) {
-      parents[src] = src;
-      a.visited64[src >> 6] |= 1ull << (src & 63);
-      a.bm0[src >> 6] |= 1ull << (src & 63);
       a.q0[0] = src;
-      a.ctr[0] = Ctr{1, a.off[src + 1] - a.off[src]};
+      a.ctr[0] = Ctr{1, a.off[src + 1] - a.off[src], 0, 0};
+      a.ctr[1] = Ctr{0, 0, 0, 0};
+      a.ctr[2] = Ctr{0, 0, 0, 0};
     }
-    grid.sync();
+    grid_barrier(a.bar, (unsigned)nb);
 
     uint32_t *q = a.q0, *qn = a.q1;
-    uint64_t *fbm = a.bm0, *nbm = a.bm1;
-    bool is_dense = false;
+    uint64_t *const bms[3] = {a.bm0, a.bm1, a.bm2};
+    bool is_dense = false, have_q = true;
     uint64_t visited_deg = 0, reached = 0;
     int L = 0;
     for (;; L++) {
-      const uint64_t fs = ld_relaxed(&a.ctr[L % 3].size);
-      const uint64_t fdeg = ld_relaxed(&a.ctr[L % 3].deg);
+      Ctr *cur = &a.ctr[L % 3];
+      const uint64_t fs = ld_relaxed(&cur->size);
+      const uint64_t fdeg = ld_relaxed(&cur->deg);
       if (fs == 0) break;
       reached += fs;
       const int dir = decide(a, is_dense, fs, fdeg, visited_deg, L, mlp);
-      if (leader && L < 64) a.logs[(uint64_t)j * 64 + L] = LevelLog{fs, fdeg, (uint32_t)dir, 0};
+      if (leader && L < 64) a.logs[(uint64_t)j * 64 + L] = LevelLog{fs, fdeg, (uint32_t)dir, 0, (unsigned long long)wall_clock64()};
       if (dir == 2) {
         visited_deg += fdeg;
         L++;
         break;
       }
       Ctr *nx = &a.ctr[(L + 1) % 3];
-      if (leader) a.ctr[(L + 2) % 3] = Ctr{0, 0};
+      if (leader) a.ctr[(L + 2) % 3] = Ctr{0, 0, 0, 0};
+      // bitmaps rotate like the counters: B[L%3] is this frontier, the next
+      // one is built in B[(L+1)%3], and B[(L+2)%3] -- last level's
+      // frontier, read by nobody now -- is cleared for level L+1
+      uint64_t *const fbm = bms[L % 3], *const nbm = bms[(L + 1) % 3];
+      {
+        uint64_t *clr = bms[(L + 2) % 3];
+        for (uint64_t i = global_tid(); i < a.nw64; i += grid_threads()) clr[i] = 0;
+      }
 
       if (dir == 0) {
-        // (1) slice sums of frontier degrees; clear the next bitmap
+        if (!have_q) {
+          // toSparse: pack the bitmap into the queue (any order)
+          for (uint64_t w0 = wave * 64; w0 < nwords; w0 += nwaves * 64) {
+            const uint64_t w = w0 + lane;
+            uint64_t word = w < nwords ? fbm[w] : 0;
+            const uint32_t c = __popcll(word);
+            const uint32_t incl = wave_incl_scan(c);
+            const uint32_t tot = __shfl(incl, 63, 64);
+            unsigned long long base = 0;
+            if (lane == 0 && tot) base = atomicAdd(&cur->cursor, (unsigned long long)tot);
+            base = __shfl(base, 0, 64);
+            uint64_t pos = base + incl - c;
+            while (word) {
+              const int bp = __builtin_ctzll(word);
+              word &= word - 1;
+              q[pos++] = (uint32_t)(w * 64 + bp);
+            }
+          }
+          grid_barrier(a.bar, (unsigned)nb);
+        }
+        if (fs <= 4 * nb) {
+          // small frontier: one block (fs <= #blocks) or one wave per
+          // frontier vertex, 1 barrier instead of 3
+          const bool per_block = fs <= nb;
+          const uint64_t unit = per_block ? blockIdx.x : wave, nunits = per_block ? nb : nwaves;
+          const int width = per_block ? kBlock : 64;
+          const int lid = per_block ? (int)threadIdx.x : lane;
+          unsigned long long dsum = 0;
+          for (uint64_t i = unit; i < fs; i += nunits) {
+            const uint32_t u = q[i];
+            const uint64_t b = a.off[u], e = a.off[u + 1];
+            for (uint64_t k0 = b; k0 < e; k0 += width) {
+              const uint64_t kk = k0 + lid;
+              bool win_ = false;
+              uint32_t tgt = 0;
+              if (kk < e) {
+                tgt = a.ids[kk];
+                if (!bit_get(a.visited64, tgt) && atomicCAS(&parents[tgt], kNone, u) == kNone) {
+                  win_ = true;
+                  atomicOr(reinterpret_cast<unsigned long long *>(&a.visited64[tgt >> 6]), 1ull << (tgt & 63));
+                  atomicOr(reinterpret_cast<unsigned long long *>(&nbm[tgt >> 6]), 1ull << (tgt & 63));
+                  dsum += a.off[tgt + 1] - a.off[tgt];
+                }
+              }
+              const uint64_t mask = __ballot(win_);
+              if (mask) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(&nx->size, (unsigned long long)__popcll(mask));
+                base = __shfl(base, 0, 64);
+                if (win_) qn[base + mbcnt(mask)] = tgt;
+              }
+            }
+          }
+          dsum = wave_sum(dsum);
+          if (lane == 0 && dsum) atomicAdd(&nx->deg, dsum);
+          grid_barrier(a.bar, (unsigned)nb);
+          is_dense = false;
+          have_q = true;
+          visited_deg += fdeg;
+          uint32_t *tq = q;
+          q = qn;
+          qn = tq;
+          continue;
+        }
+        // (1) slice sums of frontier degrees
         const uint64_t S = (fs + nb - 1) / nb;
         const uint64_t s0 = std::min<uint64_t>(fs, blockIdx.x * S), s1 = std::min<uint64_t>(fs, s0 + S);
         unsigned long long part = 0;
@@ -205,9 +350,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
         }
         part = block_sum(part, lds);
         if (threadIdx.x == 0) a.bsum[blockIdx.x] = part;
-        for (uint64_t i = global_tid(); i < a.nw64; i += grid_threads()) nbm[i] = 0;
-        grid.sync();
-        // (2) exclusive scan of the degrees of the queue
+        grid_barrier(a.bar, (unsigned)nb);
+        // (2) exclusive scan of the queue degrees and the chunk starts
         unsigned long long pre = 0;
         for (uint64_t b = threadIdx.x; b < blockIdx.x; b += kBlock) pre += a.bsum[b];
         pre = block_sum(pre, lds);
@@ -220,166 +364,217 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
           }
           const unsigned long long incl = wave_incl_scan(d);
           __syncthreads();
-          if (lane == 63) lds[threadIdx.x >> 6] = incl;
+          if (lane == 63) lds[wv] = incl;
           __syncthreads();
           unsigned long long wpre = 0, tot = 0;
-          for (int w = 0; w < kBlock / 64; w++) {
-            if (w < (int)(threadIdx.x >> 6)) wpre += lds[w];
+          for (int w = 0; w < kWaves; w++) {
+            if (w < wv) wpre += lds[w];
             tot += lds[w];
           }
-          if (i < s1) a.scan[i] = pre + wpre + incl - d;
+          if (i < s1) {
+            const uint64_t sc = pre + wpre + incl - d;
+            a.scan[i] = sc;
+            // chunks whose first edge falls in [sc, sc + d) start at i
+            for (uint64_t c = (sc + kChunk - 1) / kChunk; c * kChunk < sc + d; c++) a.cstart[c] = (uint32_t)i;
+          }
           pre += tot;
         }
         if (leader) a.scan[fs] = fdeg;
-        grid.sync();
-        // (3) edge-balanced push: chunks of 256 edges per wave
+        grid_barrier(a.bar, (unsigned)nb);
+        // (3) edge-balanced push, 256 edges per wave step
         const uint64_t E = fdeg;
         const uint64_t nch = (E + kChunk - 1) / kChunk;
+        unsigned long long dsum = 0;
         for (uint64_t c = wave; c < nch; c += nwaves) {
           const uint64_t e0 = c * kChunk;
-          uint64_t lo = 0, hi = fs;  // scan[lo] <= e0 < scan[hi]
-          while (hi - lo > 1) {
-            const uint64_t mid = (lo + hi) >> 1;
-            if (a.scan[mid] <= e0) lo = mid; else hi = mid;
+          const uint64_t i0 = a.cstart[c];
+          {
+            const uint64_t idx = i0 + lane;
+            uint64_t sc = ~0ull;
+            int64_t bs = 0;
+            if (idx < fs) {
+              sc = a.scan[idx];
+              bs = (int64_t)a.off[q[idx]] - (int64_t)sc;
+            }
+            win.sc[wv][lane] = sc;
+            win.base[wv][lane] = bs;
           }
-          const uint64_t ib = lo;
+          wave_lds_sync();
+          uint32_t tg[kPerLane];
+          bool wn[kPerLane];
+          uint32_t total = 0;
 #pragma unroll
           for (int t = 0; t < kPerLane; t++) {
             const uint64_t e = e0 + (uint64_t)t * 64 + lane;
-            bool win = false;
+            bool win_ = false;
             uint32_t tgt = 0;
             if (e < E) {
-              uint64_t x = ib, y = fs;
-              while (y - x > 1) {
-                const uint64_t mid = (x + y) >> 1;
-                if (a.scan[mid] <= e) x = mid; else y = mid;
+              int lo = 0, hi = 64;  // sc[lo] <= e (sc[0] <= e0 by construction)
+              while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (win.sc[wv][mid] <= e) lo = mid; else hi = mid;
               }
-              const uint32_t u = q[x];
-              tgt = a.ids[a.off[u] + (e - a.scan[x])];
-              if (!bit_get(a.visited64, tgt) && atomicCAS(&parents[tgt], kNone, u) == kNone) {
-                win = true;
+              uint64_t owner_u;
+              int64_t bs;
+              if (lo == 63 && i0 + 64 < fs && a.scan[i0 + 64] <= e) {
+                // the chunk spans more than 64 frontier vertices
+                uint64_t x = i0 + 64, y = fs;
+                while (y - x > 1) {
+                  const uint64_t mid = (x + y) >> 1;
+                  if (a.scan[mid] <= e) x = mid; else y = mid;
+                }
+                owner_u = q[x];
+                bs = (int64_t)a.off[owner_u] - (int64_t)a.scan[x];
+              } else {
+                owner_u = q[i0 + lo];
+                bs = win.base[wv][lo];
+              }
+              tgt = a.ids[(uint64_t)(bs + (int64_t)e)];
+              if (!bit_get(a.visited64, tgt) && atomicCAS(&parents[tgt], kNone, (uint32_t)owner_u) == kNone) {
+                win_ = true;
                 atomicOr(reinterpret_cast<unsigned long long *>(&a.visited64[tgt >> 6]), 1ull << (tgt & 63));
                 atomicOr(reinterpret_cast<unsigned long long *>(&nbm[tgt >> 6]), 1ull << (tgt & 63));
+                dsum += a.off[tgt + 1] - a.off[tgt];
               }
             }
-            const uint64_t mask = __ballot(win);
-            if (mask) {
-              unsigned long long d = win ? a.off[tgt + 1] - a.off[tgt] : 0;
-              d = wave_sum(d);
-              unsigned long long base = 0;
-              if (lane == 0) {
-                base = atomicAdd(&nx->size, (unsigned long long)__popcll(mask));
-                atomicAdd(&nx->deg, d);
-              }
-              base = __shfl(base, 0, 64);
-              if (win) qn[base + mbcnt(mask)] = tgt;
+            tg[t] = tgt;
+            wn[t] = win_;
+            total += __popcll(__ballot(win_));
+          }
+          if (total) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(&nx->size, (unsigned long long)total);
+            base = __shfl(base, 0, 64);
+#pragma unroll
+            for (int t = 0; t < kPerLane; t++) {
+              const uint64_t mask = __ballot(wn[t]);
+              if (wn[t]) qn[base + mbcnt(mask)] = tg[t];
+              base += __popcll(mask);
             }
           }
+          wave_lds_sync();  // the window is rewritten by the next chunk
         }
-        grid.sync();
+        dsum = wave_sum(dsum);
+        if (lane == 0 && dsum) atomicAdd(&nx->deg, dsum);
+        grid_barrier(a.bar, (unsigned)nb);
         is_dense = false;
-      } else {
-        if (dir == 3) {
-          for (uint64_t i = global_tid(); i < a.nw64; i += grid_threads()) nbm[i] = 0;
-          grid.sync();
-          // dense forward: one wave per frontier word
-          for (uint64_t w = wave; w < nwords; w += nwaves) {
-            uint64_t word = fbm[w];
-            while (word) {
-              const int bp = __builtin_ctzll(word);
-              word &= word - 1;
-              const uint32_t u = (uint32_t)(w * 64 + bp);
-              const uint64_t b = a.off[u], e = a.off[u + 1];
-              for (uint64_t k0 = b; k0 < e; k0 += 64) {
-                const uint64_t kk = k0 + lane;
-                bool win = false;
-                uint32_t t = 0;
-                if (kk < e) {
-                  t = a.ids[kk];
-                  if (!bit_get(a.visited64, t) && atomicCAS(&parents[t], kNone, u) == kNone) {
-                    win = true;
-                    atomicOr(reinterpret_cast<unsigned long long *>(&a.visited64[t >> 6]), 1ull << (t & 63));
-                    atomicOr(reinterpret_cast<unsigned long long *>(&nbm[t >> 6]), 1ull << (t & 63));
-                  }
-                }
-                const uint64_t mask = __ballot(win);
-                if (mask) {
-                  unsigned long long d = wave_sum(win ? (unsigned long long)(a.off[t + 1] - a.off[t]) : 0ull);
-                  unsigned long long base = 0;
-                  if (lane == 0) {
-                    base = atomicAdd(&nx->size, (unsigned long long)__popcll(mask));
-                    atomicAdd(&nx->deg, d);
-                  }
-                  base = __shfl(base, 0, 64);
-                  if (win) qn[base + mbcnt(mask)] = t;
-                }
-              }
+        have_q = true;
+      } else if (dir == 1) {
+        // dense pull: the wave owns one bitmap byte (8 vertices) per step
+        const uint8_t *vis8 = reinterpret_cast<const uint8_t *>(a.visited64);
+        uint8_t *visw8 = reinterpret_cast<uint8_t *>(a.visited64);
+        uint8_t *nbm8 = reinterpret_cast<uint8_t *>(nbm);
+        const int g = lane >> 3, sub = lane & 7;
+        unsigned long long cnt = 0, dsum = 0;
+        for (uint64_t c = wave; c < nbytes; c += nwaves) {
+          const uint8_t vb = vis8[c];
+          const uint64_t v = c * 8 + g;
+          const bool unvis = v < a.n && !((vb >> g) & 1);
+          uint64_t b = 0, e = 0;
+          if (unvis) {
+            b = a.off[v];
+            e = a.off[v + 1];
+          }
+          bool found = false;
+          uint32_t par = 0;
+          for (uint64_t k0 = b;; k0 += kG) {
+            const bool active = unvis && !found && k0 < e;
+            if (!__ballot(active)) break;
+            const uint64_t kk = k0 + sub;
+            uint32_t u = 0;
+            bool hit = false;
+            if (active && kk < e) {
+              u = a.ids[kk];
+              hit = bit_get(fbm, u);
+            }
+            const uint32_t gm = (uint32_t)(__ballot(hit) >> (g * kG)) & 0xFFu;
+            const uint32_t pu = __shfl(u, g * kG + (gm ? __builtin_ctz(gm) : 0), 64);
+            if (active && gm) {
+              found = true;
+              par = pu;
             }
           }
-        } else {
-          // dense pull: a block owns 64 consecutive bitmap words (4096
-          // vertices); its wave i takes words i, i+4, ...
-          const uint64_t nchunks = (nwords + 63) / 64;
-          const int wv = threadIdx.x >> 6;
-          for (uint64_t c = blockIdx.x; c < nchunks; c += nb) {
-            for (uint64_t w = c * 64 + wv; w < std::min<uint64_t>(nwords, c * 64 + 64); w += kBlock / 64) {
-              const uint64_t vis = a.visited64[w];
-              const uint64_t v = w * 64 + lane;
-              bool found = false;
-              uint64_t dv = 0;
-              if (v < a.n && !((vis >> lane) & 1)) {
-                const uint64_t b = a.off[v], e = a.off[v + 1];
-                dv = e - b;
-                for (uint64_t kk = b; kk < e; kk++) {
-                  const uint32_t u = a.ids[kk];
-                  if (bit_get(fbm, u)) {
-                    parents[v] = u;
-                    found = true;
-                    break;
-                  }
+          const uint64_t fmask = __ballot(found && sub == 0);
+          uint32_t byte = 0;
+#pragma unroll
+          for (int x = 0; x < 8; x++) byte |= (uint32_t)((fmask >> (x * kG)) & 1ull) << x;
+          if (found && sub == 0) {
+            parents[v] = par;
+            dsum += e - b;
+          }
+          if (lane == 0) {
+            nbm8[c] = (uint8_t)byte;
+            if (byte) visw8[c] = (uint8_t)(vb | byte);
+          }
+          cnt += __popc(byte);
+        }
+        dsum = wave_sum(dsum);
+        if (lane == 0 && cnt) {
+          atomicAdd(&nx->size, cnt);
+          atomicAdd(&nx->deg, dsum);
+        }
+        grid_barrier(a.bar, (unsigned)nb);
+        is_dense = true;
+        have_q = false;
+      } else {
+        // dense forward: one wave per frontier vertex of each bitmap word
+        for (uint64_t w = wave; w < nwords; w += nwaves) {
+          uint64_t word = fbm[w];
+          while (word) {
+            const int bp = __builtin_ctzll(word);
+            word &= word - 1;
+            const uint32_t u = (uint32_t)(w * 64 + bp);
+            const uint64_t b = a.off[u], e = a.off[u + 1];
+            for (uint64_t k0 = b; k0 < e; k0 += 64) {
+              const uint64_t kk = k0 + lane;
+              bool win_ = false;
+              uint32_t t = 0;
+              if (kk < e) {
+                t = a.ids[kk];
+                if (!bit_get(a.visited64, t) && atomicCAS(&parents[t], kNone, u) == kNone) {
+                  win_ = true;
+                  atomicOr(reinterpret_cast<unsigned long long *>(&a.visited64[t >> 6]), 1ull << (t & 63));
+                  atomicOr(reinterpret_cast<unsigned long long *>(&nbm[t >> 6]), 1ull << (t & 63));
                 }
               }
-              const uint64_t mask = __ballot(found);
-              const unsigned long long d = wave_sum(found ? dv : 0ull);
-              unsigned long long base = 0;
-              if (lane == 0) {
-                nbm[w] = mask;
-                if (mask) {
-                  a.visited64[w] = vis | mask;
+              const uint64_t mask = __ballot(win_);
+              if (mask) {
+                const unsigned long long d = wave_sum(win_ ? (unsigned long long)(a.off[t + 1] - a.off[t]) : 0ull);
+                unsigned long long base = 0;
+                if (lane == 0) {
                   base = atomicAdd(&nx->size, (unsigned long long)__popcll(mask));
                   atomicAdd(&nx->deg, d);
                 }
+                base = __shfl(base, 0, 64);
+                if (win_) qn[base + mbcnt(mask)] = t;
               }
-              base = __shfl(base, 0, 64);
-              if (found) qn[base + mbcnt(mask)] = (uint32_t)v;
             }
           }
         }
-        grid.sync();
+        grid_barrier(a.bar, (unsigned)nb);
         is_dense = true;
+        have_q = true;
       }
       visited_deg += fdeg;
       uint32_t *tq = q;
       q = qn;
       qn = tq;
-      uint64_t *tb = fbm;
-      fbm = nbm;
-      nbm = tb;
     }
     if (leader) {
+      if (L < 64) a.logs[(uint64_t)j * 64 + L].t0 = wall_clock64();
       a.nlevels[j] = (uint32_t)L;
       a.tot[2 * j] = reached;
       a.tot[2 * j + 1] = visited_deg;
     }
-    grid.sync();
+    grid_barrier(a.bar, (unsigned)nb);
   }
 }
 
 }  // namespace
 
-// Runs k BFS in one cooperative launch.  parents_dev: k*n results (or one
-// reused n-buffer when keep_all is false; the last source's tree then
-// remains).  Fills stats[k] when given.
+// Runs k BFS in one cooperative launch.  Results: k*n parents in scratch
+// "bfs_parents_all" (keep_all) or the last source's tree in "bfs_parents".
+// Fills stats[k] when given.
 void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all, byo_bfs_stats *stats,
                   uint64_t maxdeg, double *ms_out) {
   const uint64_t n = g->n;
@@ -396,13 +591,15 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   a.q1 = g->buf("bfs_qn", n * 4 + 64).as<uint32_t>();
   a.bm0 = g->buf("bfs_fbm", a.nw64 * 8).as<uint64_t>();
   a.bm1 = g->buf("bfs_nbm", a.nw64 * 8).as<uint64_t>();
+  a.bm2 = g->buf("bfs_bm2", a.nw64 * 8).as<uint64_t>();
   a.visited64 = g->buf("bfs_visited", a.nw64 * 8).as<uint64_t>();
   a.scan = g->buf("bfs_scan", (n + 1) * 8 + 64).as<uint64_t>();
+  a.cstart = g->buf("bfs_cstart", (g->m / kChunk + 2) * 4 + 64).as<uint32_t>();
   a.ctr = g->buf("bfs_coop_ctr", 3 * sizeof(Ctr) + 64).as<Ctr>();
   a.k = k;
   uint32_t *dsrc = g->buf("bfs_coop_srcs", (size_t)k * 4 + 64).as<uint32_t>();
   a.srcs = dsrc;
-  a.logs = g->buf("bfs_coop_logs", (size_t)k * 64 * sizeof(LevelLog) + 64).as<LevelLog>();
+  a.logs = g->buf("bfs_coop_logs", (size_t)k * 65 * sizeof(LevelLog) + 64).as<LevelLog>();
   a.nlevels = g->buf("bfs_coop_nlev", (size_t)k * 4 + 64).as<uint32_t>();
   a.tot = g->buf("bfs_coop_tot", (size_t)k * 16 + 64).as<unsigned long long>();
   a.policy = g->dir_policy;
@@ -421,6 +618,9 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   }
   const int nblocks = grid_cache;
   a.bsum = g->buf("bfs_coop_bsum", (size_t)nblocks * 8 + 64).as<unsigned long long>();
+  BYO_REQUIRE(nblocks <= 2048, BYO_E_HIP, "grid barrier sized for at most 2048 blocks");
+  a.bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
+  BYO_HIP(hipMemsetAsync(a.bar, 0, sizeof(GridBar), s));
   BYO_HIP(hipMemcpyAsync(dsrc, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s));
   void *args[] = {&a};
   PhaseTimer t(s, true);
@@ -429,6 +629,9 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   const double ms = t.stop();
   if (ms_out) *ms_out = ms;
   if (stats) {
+    int wall_khz = 0;
+    BYO_HIP(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, g->device));
+    const double wall_hz = wall_khz > 0 ? wall_khz * 1e3 : 1e8;
     std::vector<LevelLog> logs((size_t)k * 64);
     std::vector<uint32_t> nlev(k);
     std::vector<unsigned long long> tot(2 * (size_t)k);
@@ -445,6 +648,9 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
         st.fsize[L] = lg.fsize;
         st.fdeg[L] = lg.fdeg;
         st.dir[L] = (uint8_t)lg.dir;
+        // the next record's start (or the end stamp) closes this level
+        if (L + 1 < 64 && L + 1 <= nlev[j])
+          st.level_ms[L] = (float)((double)(logs[(size_t)j * 64 + L + 1].t0 - lg.t0) / wall_hz * 1e3);
       }
       st.reached = (uint32_t)tot[2 * j];
       st.edges_reached = tot[2 * j + 1];

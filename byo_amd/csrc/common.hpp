@@ -222,6 +222,13 @@ __device__ __forceinline__ T wave_incl_scan(T v) {
   }
   return v;
 }
+// Orders this wave's LDS writes before its later LDS reads (and the
+// reverse) for the compiler and the hardware; no cross-wave effect.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
 __device__ __forceinline__ uint64_t grid_threads() {
   return static_cast<uint64_t>(gridDim.x) * blockDim.x;
 }

@@ -14,6 +14,7 @@
 // rank and contribution (2w B/vertex).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "primitives.hpp"
 
@@ -44,47 +45,108 @@ __device__ __forceinline__ Acc<T> combine(Acc<T> a, Acc<T> b) {
   return Acc<T>{s, a.c + b.c - err};
 }
 
-template <class T, int G, bool kCompensated>
-__global__ void __launch_bounds__(256) k_pr_iter(const uint64_t *off, const uint32_t *ids, uint64_t n,
-                                                 const T *contrib_in, const T *p_old, T *p_new, T *contrib_out,
-                                                 double damping, double addc, double *partial) {
+// One pass of the pull over the neighbours of every vertex whose ids lie in
+// one source segment [lo_pos[v], hi_pos[v]).  Adjacency lists are sorted,
+// so a segment is a contiguous sub-range of each list and the gathered
+// contributions of one pass stay inside one window of the rank vector.  A
+// wave takes a tile of 64 consecutive vertices: per-vertex state (segment
+// bounds, the (sum, compensation) carried between passes, ranks) moves with
+// coalesced lane-per-vertex loads and stores, while the gathers run G lanes
+// per vertex over G rounds of 64/G vertices.  The last pass finalises
+// p' = d * sum + (1-d)/n, the next contributions p'/deg and the block's share
+// of the FP64 L1 residual.
+template <class T, int G, bool kComp, bool kFirst, bool kLast>
+__global__ void __launch_bounds__(256) k_pr_seg(const uint64_t *lo_pos, const uint64_t *hi_pos, const uint32_t *ids,
+                                                const uint64_t *off, uint64_t n, const T *contrib_in, T *acc_s,
+                                                T *acc_c, const T *p_old, T *p_new, T *contrib_out, double damping,
+                                                double addc, double *partial) {
+  constexpr int VPR = 64 / G;  // vertices per round
   __shared__ double lds[4];
   const int lane = lane_id();
-  const int sub = lane & (G - 1);
-  const uint64_t group = global_tid() / G, ngroups = grid_threads() / G;
+  const int sub = lane & (G - 1), grp = lane / G;
+  const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
+  const uint64_t ntiles = (n + 63) >> 6;
   double resid = 0.0;
   const T d = (T)damping, t = (T)addc;
-  const uint64_t nrounds = (n + ngroups - 1) / ngroups;
-  for (uint64_t r = 0; r < nrounds; r++) {
-    const uint64_t v = r * ngroups + group;
-    Acc<T> a{0, 0};
-    uint64_t b = 0, e = 0;
-    if (v < n) {
-      b = off[v];
-      e = off[v + 1];
-      for (uint64_t k = b + sub; k < e; k += G) {
-        T x = contrib_in[ids[k]];
-        if (kCompensated) kahan_add(a, x); else a.s += x;
+  for (uint64_t tile = wave; tile < ntiles; tile += nwaves) {
+    const uint64_t v = tile * 64 + lane;
+    const bool valid = v < n;
+    const uint64_t lo = valid ? lo_pos[v] : 0, hi = valid ? hi_pos[v] : 0;
+    Acc<T> mine{0, 0};
+#pragma unroll 2
+    for (int r = 0; r < G; r++) {
+      const int owner = r * VPR + grp;
+      const uint64_t b = __shfl(lo, owner, 64), e = __shfl(hi, owner, 64);
+      Acc<T> a{0, 0};
+      uint64_t k = b + sub;
+      for (; k + G < e; k += 2 * G) {
+        const T x0 = contrib_in[ids[k]];
+        const T x1 = contrib_in[ids[k + G]];
+        if (kComp) {
+          kahan_add(a, x0);
+          kahan_add(a, x1);
+        } else {
+          a.s += x0;
+          a.s += x1;
+        }
+      }
+      if (k < e) {
+        const T x0 = contrib_in[ids[k]];
+        if (kComp) kahan_add(a, x0); else a.s += x0;
+      }
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) {
+        Acc<T> other{__shfl_xor(a.s, o, 64), __shfl_xor(a.c, o, 64)};
+        if (kComp) a = combine(a, other); else a.s += other.s;
+      }
+      // the group leader of vertex r*VPR + i is lane i*G
+      const int rel = lane - r * VPR;
+      const int src = (rel >= 0 && rel < VPR) ? rel * G : 0;
+      const T gs = __shfl(a.s, src, 64), gc = __shfl(a.c, src, 64);
+      if (rel >= 0 && rel < VPR) mine = Acc<T>{gs, gc};
+    }
+    if (valid) {
+      if (!kFirst) {
+        if (kComp) mine = combine(Acc<T>{acc_s[v], acc_c[v]}, mine); else mine.s += acc_s[v];
+      }
+      if (!kLast) {
+        acc_s[v] = mine.s;
+        if (kComp) acc_c[v] = mine.c;
+      } else {
+        const T sum = kComp ? (mine.s - mine.c) : mine.s;
+        const T pv = d * sum + t;
+        const uint64_t deg = off[v + 1] - off[v];
+        p_new[v] = pv;
+        contrib_out[v] = deg ? pv / (T)deg : (T)0;
+        resid += fabs((double)pv - (double)p_old[v]);
       }
     }
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) {
-      Acc<T> other{__shfl_xor(a.s, o, 64), __shfl_xor(a.c, o, 64)};
-      if (kCompensated) a = combine(a, other); else a.s += other.s;
-    }
-    if (v < n && sub == 0) {
-      const T sum = kCompensated ? (a.s - a.c) : a.s;
-      const T pv = d * sum + t;
-      const uint64_t deg = e - b;
-      p_new[v] = pv;
-      contrib_out[v] = deg ? pv / (T)deg : (T)0;
-      resid += fabs((double)pv - (double)p_old[v]);
+  }
+  if (kLast) {
+    resid = wave_sum(resid);
+    if (lane == 0) lds[threadIdx.x >> 6] = resid;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+  }
+}
+
+// pos[(s-1)*n + v] = first index of v's list holding an id >= s*W
+// (s = 1 .. S-1).
+__global__ void k_pr_bounds(const uint64_t *off, const uint32_t *ids, uint64_t n, int S, uint64_t W, uint64_t *pos) {
+  for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
+    uint64_t lo = off[v];
+    const uint64_t end = off[v + 1];
+    for (int s = 1; s < S; s++) {
+      const uint64_t bound = (uint64_t)s * W;
+      uint64_t a = lo, b = end;
+      while (a < b) {
+        const uint64_t mid = (a + b) >> 1;
+        if (ids[mid] < bound) a = mid + 1; else b = mid;
+      }
+      pos[(uint64_t)(s - 1) * n + v] = a;
+      lo = a;
     }
   }
-  resid = wave_sum(resid);
-  if (lane == 0) lds[threadIdx.x >> 6] = resid;
-  __syncthreads();
-  if (threadIdx.x == 0) partial[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
 }
 
 __global__ void k_sum_doubles(const double *partial, int nb, double *out) {
@@ -148,20 +210,39 @@ __global__ void __launch_bounds__(256) k_ppr_iter(const uint64_t *off, const uin
   }
 }
 
-template <class T>
-struct PrState {
-  T *p[2], *c[2];
-};
+template <class T, int G, bool kComp>
+void pr_pass(bool first, bool last, const uint64_t *lo, const uint64_t *hi, const uint32_t *ids, const uint64_t *off,
+             uint64_t n, const T *cin, T *as, T *ac, const T *pold, T *pnew, T *cout, double damping, double addc,
+             double *partial, unsigned grid, hipStream_t s) {
+#define BYO_PR_PASS(F, L) \
+  k_pr_seg<T, G, kComp, F, L><<<grid, 256, 0, s>>>(lo, hi, ids, off, n, cin, as, ac, pold, pnew, cout, damping, addc, partial)
+  if (first && last) BYO_PR_PASS(true, true);
+  else if (first) BYO_PR_PASS(true, false);
+  else if (last) BYO_PR_PASS(false, true);
+  else BYO_PR_PASS(false, false);
+#undef BYO_PR_PASS
+}
 
 template <class T, bool kComp>
-void pr_launch(int G, const uint64_t *off, const uint32_t *ids, uint64_t n, const T *cin, const T *pold, T *pnew,
-               T *cout, double damping, double addc, double *partial, unsigned grid, hipStream_t s) {
+void pr_pass_g(int G, bool first, bool last, const uint64_t *lo, const uint64_t *hi, const uint32_t *ids,
+               const uint64_t *off, uint64_t n, const T *cin, T *as, T *ac, const T *pold, T *pnew, T *cout,
+               double damping, double addc, double *partial, unsigned grid, hipStream_t s) {
   switch (G) {
-    case 4: k_pr_iter<T, 4, kComp><<<grid, 256, 0, s>>>(off, ids, n, cin, pold, pnew, cout, damping, addc, partial); break;
-    case 8: k_pr_iter<T, 8, kComp><<<grid, 256, 0, s>>>(off, ids, n, cin, pold, pnew, cout, damping, addc, partial); break;
-    case 16: k_pr_iter<T, 16, kComp><<<grid, 256, 0, s>>>(off, ids, n, cin, pold, pnew, cout, damping, addc, partial); break;
-    default: k_pr_iter<T, 32, kComp><<<grid, 256, 0, s>>>(off, ids, n, cin, pold, pnew, cout, damping, addc, partial); break;
+    case 4: pr_pass<T, 4, kComp>(first, last, lo, hi, ids, off, n, cin, as, ac, pold, pnew, cout, damping, addc, partial, grid, s); break;
+    case 8: pr_pass<T, 8, kComp>(first, last, lo, hi, ids, off, n, cin, as, ac, pold, pnew, cout, damping, addc, partial, grid, s); break;
+    case 16: pr_pass<T, 16, kComp>(first, last, lo, hi, ids, off, n, cin, as, ac, pold, pnew, cout, damping, addc, partial, grid, s); break;
+    default: pr_pass<T, 32, kComp>(first, last, lo, hi, ids, off, n, cin, as, ac, pold, pnew, cout, damping, addc, partial, grid, s); break;
   }
+}
+
+// Segment width (vertices) so that one segment's contributions take about
+// 2 MiB of L2; BYO_PR_SEG_LOG2 overrides it (tuning).
+template <class T>
+uint64_t segment_width(uint64_t n) {
+  int lg = 40;  // one segment unless tuned
+  if (const char *e = std::getenv("BYO_PR_SEG_LOG2")) lg = std::atoi(e);
+  uint64_t W = 1ull << lg;
+  return W < n ? W : n;
 }
 
 template <class T, bool kComp>
@@ -169,31 +250,41 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
                    int *iters_run, double *device_ms, const char *phase) {
   const uint64_t n = g->n;
   hipStream_t s = g->stream;
-  PrState<T> st;
-  st.p[0] = g->buf("pr_p0", n * sizeof(T) + 64).as<T>();
-  st.p[1] = g->buf("pr_p1", n * sizeof(T) + 64).as<T>();
-  st.c[0] = g->buf("pr_c0", n * sizeof(T) + 64).as<T>();
-  st.c[1] = g->buf("pr_c1", n * sizeof(T) + 64).as<T>();
+  T *p[2], *c[2];
+  p[0] = g->buf("pr_p0", n * sizeof(T) + 64).as<T>();
+  p[1] = g->buf("pr_p1", n * sizeof(T) + 64).as<T>();
+  c[0] = g->buf("pr_c0", n * sizeof(T) + 64).as<T>();
+  c[1] = g->buf("pr_c1", n * sizeof(T) + 64).as<T>();
+  const uint64_t W = segment_width<T>(n);
+  const int S = (int)((n + W - 1) / W);
+  T *acc_s = S > 1 ? g->buf("pr_acc_s", n * sizeof(T) + 64).as<T>() : nullptr;
+  T *acc_c = (S > 1 && kComp) ? g->buf("pr_acc_c", n * sizeof(T) + 64).as<T>() : nullptr;
+  uint64_t *pos = S > 1 ? g->buf("pr_pos", (uint64_t)(S - 1) * n * 8 + 64).as<uint64_t>() : nullptr;
+  // lanes per vertex: about half the per-segment average degree, in [4, 32]
+  const double avg = n ? (double)g->m / (double)n / S : 0.0;
+  int G = 4;
+  while (G < 32 && G < avg / 2) G *= 2;
+  if (const char *e = std::getenv("BYO_PR_LANES")) G = std::atoi(e);
   const unsigned grid = 4096;
   double *partial = g->buf("pr_partial", (grid + 8) * sizeof(double)).as<double>();
   double *l1dev = g->buf("pr_l1", (size_t)(max_iters + 1) * sizeof(double)).as<double>();
   const uint64_t *off = g->off.as<uint64_t>();
   const uint32_t *ids = g->ids.as<uint32_t>();
-  // lanes per vertex: about half the average degree, in [4, 32]
-  const double avg = n ? (double)g->m / (double)n : 0.0;
-  int G = 4;
-  while (G < 32 && G < avg / 2) G *= 2;
   const double addc = (1.0 - damping) * (1.0 / (double)n);
-  k_pr_init<T><<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, n, st.p[0], st.c[0]);
+  PhaseTimer timer(s, true);
+  k_pr_init<T><<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, n, p[0], c[0]);
+  if (S > 1) k_pr_bounds<<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, ids, n, S, W, pos);
   BYO_LAUNCH_CHECK();
   double *hl1 = static_cast<double *>(pinned(g, 64));
   int it = 0, cur = 0;
-  PhaseTimer timer(s, true);
-  double ms_sum = 0.0;
   while (it < max_iters) {
     PhaseTimer it_t(s, g->prof.on);
-    pr_launch<T, kComp>(G, off, ids, n, st.c[cur], st.p[cur], st.p[cur ^ 1], st.c[cur ^ 1], damping, addc, partial,
-                        grid, s);
+    for (int sg = 0; sg < S; sg++) {
+      const uint64_t *lo = sg == 0 ? off : pos + (uint64_t)(sg - 1) * n;
+      const uint64_t *hi = sg == S - 1 ? off + 1 : pos + (uint64_t)sg * n;
+      pr_pass_g<T, kComp>(G, sg == 0, sg == S - 1, lo, hi, ids, off, n, c[cur], acc_s, acc_c, p[cur], p[cur ^ 1],
+                          c[cur ^ 1], damping, addc, partial, grid, s);
+    }
     k_sum_doubles<<<1, 1024, 0, s>>>(partial, grid, l1dev + it);
     BYO_LAUNCH_CHECK();
     if (g->prof.on) g->account(phase, it_t.stop());
@@ -205,15 +296,15 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
       if (*hl1 < eps) break;
     }
   }
-  ms_sum = timer.stop();
-  if (device_ms) *device_ms = ms_sum;
+  const double ms = timer.stop();
+  if (device_ms) *device_ms = ms;
   if (iters_run) *iters_run = it;
   if (l1_log && it) {
     BYO_HIP(hipMemcpyAsync(l1_log, l1dev, it * sizeof(double), hipMemcpyDeviceToHost, s));
   }
   if (p_out) {
     double *pd = g->buf("pr_out_f64", n * sizeof(double) + 64).as<double>();
-    k_to_double<T><<<byo_grid(n, 256, 16384), 256, 0, s>>>(st.p[cur], pd, n);
+    k_to_double<T><<<byo_grid(n, 256, 16384), 256, 0, s>>>(p[cur], pd, n);
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipMemcpyAsync(p_out, pd, n * sizeof(double), hipMemcpyDeviceToHost, s));
   }

@@ -76,14 +76,6 @@ __global__ void k_tc_fill(const uint64_t *off, const uint32_t *ids, uint64_t n, 
 constexpr int kLdsPerWave = 2048;  // staged N+(u) entries per wave (8 KiB)
 constexpr int kTcWaves = 4;
 
-// Orders this wave's LDS writes before its later LDS reads (and the
-// reverse) for the compiler and the hardware.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
 __device__ __forceinline__ bool find_sorted(const uint32_t *a, uint32_t len, uint32_t x) {
   uint32_t lo = 0, hi = len;
   while (lo < hi) {
