@@ -19,13 +19,14 @@
 //    the first frontier index of every chunk, so a wave loads the (scan,
 //    adjacency base) window of its chunk into LDS with one coalesced load and
 //    resolves each edge's owner by a 6-step LDS search;
-//  * dense pull (edgeMapDense, edge_map_data.h:47-175): 8 lanes per
-//    unvisited vertex read 8 consecutive neighbours at once and test them
-//    against the bitmap, exiting at the first frontier member (the lowest
-//    such neighbour in adjacency order becomes the parent, as the
-//    reference's sequential early-exit scan picks it); a wave owns one
-//    bitmap byte per step, so next-frontier and visited bytes are written
-//    without atomics;
+//  * dense pull (edgeMapDense, edge_map_data.h:47-175): a wave owns one
+//    64-vertex bitmap word per step; 8 lanes per unvisited vertex read 8
+//    consecutive neighbours at once and test them against the bitmap,
+//    exiting at the first frontier member (the lowest such neighbour in
+//    adjacency order becomes the parent, as the reference's sequential
+//    early-exit scan picks it); the first 8 neighbours of all 64 vertices are
+//    probed with independent loads, and the next-frontier and visited words
+//    are written whole, without atomics;
 //  * dense forward (edgeMapDenseForward, :180-247): one wave per frontier
 //    vertex, 64 neighbours per step.
 //
@@ -460,53 +461,95 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
         is_dense = false;
         have_q = true;
       } else if (dir == 1) {
-        // dense pull: the wave owns one bitmap byte (8 vertices) per step
-        const uint8_t *vis8 = reinterpret_cast<const uint8_t *>(a.visited64);
-        uint8_t *visw8 = reinterpret_cast<uint8_t *>(a.visited64);
-        uint8_t *nbm8 = reinterpret_cast<uint8_t *>(nbm);
+        // dense pull: a wave owns one 64-vertex bitmap word per step.
+        // Offsets of the 64 vertices are loaded once (lane = vertex); then
+        // every unvisited vertex probes its first 8 neighbours with 8 lanes
+        // (8 vertices x 8 steps, all loads independent); vertices still
+        // unresolved continue 8 neighbours per step, 8 vertices at a time.
         const int g = lane >> 3, sub = lane & 7;
         unsigned long long cnt = 0, dsum = 0;
-        for (uint64_t c = wave; c < nbytes; c += nwaves) {
-          const uint8_t vb = vis8[c];
-          const uint64_t v = c * 8 + g;
-          const bool unvis = v < a.n && !((vb >> g) & 1);
-          uint64_t b = 0, e = 0;
-          if (unvis) {
-            b = a.off[v];
-            e = a.off[v + 1];
+        for (uint64_t w = wave; w < nwords; w += nwaves) {
+          const uint64_t vis = a.visited64[w];
+          const uint64_t vl = w * 64 + lane;
+          uint64_t bl = 0, el = 0;
+          if (vl < a.n) {
+            bl = a.off[vl];
+            el = a.off[vl + 1];
           }
-          bool found = false;
-          uint32_t par = 0;
-          for (uint64_t k0 = b;; k0 += kG) {
-            const bool active = unvis && !found && k0 < e;
-            if (!__ballot(active)) break;
-            const uint64_t kk = k0 + sub;
-            uint32_t u = 0;
-            bool hit = false;
-            if (active && kk < e) {
-              u = a.ids[kk];
-              hit = bit_get(fbm, u);
-            }
-            const uint32_t gm = (uint32_t)(__ballot(hit) >> (g * kG)) & 0xFFu;
-            const uint32_t pu = __shfl(u, g * kG + (gm ? __builtin_ctz(gm) : 0), 64);
-            if (active && gm) {
-              found = true;
-              par = pu;
-            }
-          }
-          const uint64_t fmask = __ballot(found && sub == 0);
-          uint32_t byte = 0;
+          const uint64_t todo = __ballot(vl < a.n && !((vis >> lane) & 1));
+          uint64_t fmask = 0;
+          if (todo) {
+            uint32_t u[8];
+            bool h[8];
 #pragma unroll
-          for (int x = 0; x < 8; x++) byte |= (uint32_t)((fmask >> (x * kG)) & 1ull) << x;
-          if (found && sub == 0) {
-            parents[v] = par;
-            dsum += e - b;
+            for (int st = 0; st < 8; st++) {
+              const int x = st * 8 + g;
+              const uint64_t b = __shfl(bl, x, 64), e = __shfl(el, x, 64);
+              const uint64_t kk = b + sub;
+              h[st] = ((todo >> x) & 1) && kk < e;
+              u[st] = h[st] ? a.ids[kk] : 0u;
+            }
+#pragma unroll
+            for (int st = 0; st < 8; st++) h[st] = h[st] && bit_get(fbm, u[st]);
+#pragma unroll
+            for (int st = 0; st < 8; st++) {
+              const uint32_t gm = (uint32_t)(__ballot(h[st]) >> (g * 8)) & 0xFFu;
+              const uint32_t pu = __shfl(u[st], g * 8 + (gm ? __builtin_ctz(gm) : 0), 64);
+              if (gm && sub == 0) parents[w * 64 + st * 8 + g] = pu;
+              const uint64_t fb = __ballot(gm != 0 && sub == 0);
+              uint64_t byte = 0;
+#pragma unroll
+              for (int t = 0; t < 8; t++) byte |= ((fb >> (t * 8)) & 1ull) << t;
+              fmask |= byte << (st * 8);
+            }
+            // vertices with more than 8 neighbours and no hit yet
+            uint64_t rem = todo & ~fmask & __ballot(el - bl > 8);
+            while (rem) {
+              int x = -1;
+              uint64_t mm = rem;
+              for (int t = 0; t < 8 && mm; t++) {
+                const int bp = __builtin_ctzll(mm);
+                mm &= mm - 1;
+                if (t == g) x = bp;
+              }
+              rem = mm;
+              const bool has = x >= 0;
+              const uint64_t b = __shfl(bl, has ? x : 0, 64) + 8, e = __shfl(el, has ? x : 0, 64);
+              bool found = false;
+              uint32_t par = 0;
+              for (uint64_t k0 = b;; k0 += 8) {
+                const bool active = has && !found && k0 < e;
+                if (!__ballot(active)) break;
+                const uint64_t kk = k0 + sub;
+                uint32_t uu = 0;
+                bool hit = false;
+                if (active && kk < e) {
+                  uu = a.ids[kk];
+                  hit = bit_get(fbm, uu);
+                }
+                const uint32_t gm = (uint32_t)(__ballot(hit) >> (g * 8)) & 0xFFu;
+                const uint32_t pu = __shfl(uu, g * 8 + (gm ? __builtin_ctz(gm) : 0), 64);
+                if (active && gm) {
+                  found = true;
+                  par = pu;
+                }
+              }
+              uint64_t mine = 0;
+              if (found && sub == 0) {
+                parents[w * 64 + x] = par;
+                mine = 1ull << x;
+              }
+#pragma unroll
+              for (int o = 32; o > 0; o >>= 1) mine |= __shfl_xor(mine, o, 64);
+              fmask |= mine;
+            }
           }
           if (lane == 0) {
-            nbm8[c] = (uint8_t)byte;
-            if (byte) visw8[c] = (uint8_t)(vb | byte);
+            nbm[w] = fmask;
+            if (fmask) a.visited64[w] = vis | fmask;
           }
-          cnt += __popc(byte);
+          cnt += __popcll(fmask);
+          if ((fmask >> lane) & 1) dsum += el - bl;
         }
         dsum = wave_sum(dsum);
         if (lane == 0 && cnt) {
