@@ -75,6 +75,9 @@ SIGNATURES = {
     "byo_insert_batch_dev": (_i32, [_vp, _vp, _u64]),
     "byo_remove_batch_dev": (_i32, [_vp, _vp, _u64]),
     "byo_update_rmat_keys": (_i32, [_u64, _u64, _u64, _vp]),
+    "byo_device_alloc": (_i32, [_u64, _P(_vp)]),
+    "byo_device_free": (_i32, [_vp]),
+    "byo_copy_to_device": (_i32, [_vp, _vp, _u64]),
     "byo_set_direction_policy": (_i32, [_vp, _i32, C.c_int64]),
     "byo_bfs": (_i32, [_vp, _u32, _vp, _P(BfsStats)]),
     "byo_bfs_multi": (_i32, [_vp, _vp, _u32, _vp, _vp]),
@@ -132,6 +135,32 @@ def update_rmat_keys(graph_n, count, first=0):
     k = np.empty(max(count, 1), np.uint64)
     check(lib().byo_update_rmat_keys(graph_n, first, count, k.ctypes.data))
     return k[:count]
+
+
+class DeviceBuffer:
+    """A device allocation holding a host array's bytes (for the *_dev
+    batch entry points)."""
+
+    def __init__(self, host_array):
+        a = np.ascontiguousarray(host_array)
+        self.nbytes = a.nbytes
+        self.count = a.size
+        p = C.c_void_p()
+        check(lib().byo_device_alloc(self.nbytes, C.byref(p)))
+        self.ptr = p.value
+        check(lib().byo_copy_to_device(C.c_void_p(self.ptr), a.ctypes.data, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            check(lib().byo_device_free(C.c_void_p(self.ptr)))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and _LIB is not None:
+                _LIB.byo_device_free(C.c_void_p(self.ptr))
+        except Exception:
+            pass
 
 
 def _ptr(a):
@@ -228,6 +257,13 @@ class Graph:
     def remove_batch(self, keys):
         k = as_keys(keys)
         check(lib().byo_remove_batch(self._h, _ptr(k), len(k)))
+
+    def insert_batch_dev(self, dev_ptr, count):
+        """insert_batch with the uint64 keys already in device memory."""
+        check(lib().byo_insert_batch_dev(self._h, C.c_void_p(dev_ptr), count))
+
+    def remove_batch_dev(self, dev_ptr, count):
+        check(lib().byo_remove_batch_dev(self._h, C.c_void_p(dev_ptr), count))
 
     # -- algorithms -------------------------------------------------------
     def set_direction_policy(self, policy, threshold=-1):

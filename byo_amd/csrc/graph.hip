@@ -279,6 +279,28 @@ int byo_csr_create_rmat(int scale, uint64_t edge_factor, double a, double b, dou
   });
 }
 
+int byo_device_alloc(uint64_t bytes, void **dev_ptr) {
+  return guard([&] {
+    BYO_REQUIRE(dev_ptr, BYO_E_INVAL, "dev_ptr is null");
+    require_device();
+    *dev_ptr = nullptr;
+    BYO_HIP(hipMalloc(dev_ptr, bytes ? bytes : 256));
+  });
+}
+
+int byo_device_free(void *dev_ptr) {
+  return guard([&] {
+    if (dev_ptr) BYO_HIP(hipFree(dev_ptr));
+  });
+}
+
+int byo_copy_to_device(void *dev_ptr, const void *host, uint64_t bytes) {
+  return guard([&] {
+    BYO_REQUIRE(dev_ptr && (host || bytes == 0), BYO_E_INVAL, "null argument");
+    if (bytes) BYO_HIP(hipMemcpy(dev_ptr, host, bytes, hipMemcpyHostToDevice));
+  });
+}
+
 int byo_update_rmat_keys(uint64_t graph_n, uint64_t first, uint64_t count, uint64_t *keys) {
   return guard([&] {
     BYO_REQUIRE(keys || count == 0, BYO_E_INVAL, "keys is null");

@@ -111,6 +111,13 @@ int byo_remove_batch(byo_graph_t g, const uint64_t *keys, uint64_t b);
 int byo_insert_batch_dev(byo_graph_t g, const uint64_t *keys_dev, uint64_t b);
 int byo_remove_batch_dev(byo_graph_t g, const uint64_t *keys_dev, uint64_t b);
 
+/* Device buffers for callers that keep update batches resident in HBM
+ * (the *_dev entry points above).  No counterpart in the reference, whose
+ * batches live in host memory. */
+int byo_device_alloc(uint64_t bytes, void **dev_ptr);
+int byo_device_free(void *dev_ptr);
+int byo_copy_to_device(void *dev_ptr, const void *host, uint64_t bytes);
+
 /* Update stream of Batch_insert_runner (benchmarks/run_unweighted.h:781-806,
  * rMat at :730-778), generated on the device: keys[k] = (src<<32)|dst of
  * updates first .. first+count-1 for a graph of graph_n vertices.  The
