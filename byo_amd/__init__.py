@@ -20,7 +20,8 @@ LIB_PATH = os.path.join(_HERE, "libbyo.so")
 
 OK, E_INVAL, E_OOM, E_HIP, E_STATE, E_NODEV = 0, -1, -2, -3, -4, -5
 NONE = 0xFFFFFFFF
-DIR_THRESHOLD, DIR_MLP, DIR_SPARSE, DIR_DENSE = 0, 1, 2, 3
+DIR_THRESHOLD, DIR_MLP, DIR_SPARSE, DIR_DENSE, DIR_SCHEDULE = 0, 1, 2, 3, 4
+DIR_RULE = 255  # schedule entry meaning "use the threshold rule" 
 PR_FP64, PR_FP32 = 0, 1
 MLP_NPARAM = 1379
 
@@ -59,6 +60,8 @@ SIGNATURES = {
     "byo_device_count": (_i32, []),
     "byo_csr_create": (_i32, [_vp, _vp, _u64, _u64, _P(_vp)]),
     "byo_csr_create_rmat": (_i32, [_i32, _u64, _dbl, _dbl, _dbl, _u64, _P(_vp)]),
+    "byo_csr_create_er": (_i32, [_i32, _u64, _u64, _P(_vp)]),
+    "byo_csr_create_grid3d": (_i32, [_u32, _P(_vp)]),
     "byo_cpma_create": (_i32, [_vp, _vp, _u64, _u64, _P(_vp)]),
     "byo_cpma_create_from": (_i32, [_vp, _P(_vp)]),
     "byo_graph_destroy": (_i32, [_vp]),
@@ -79,6 +82,7 @@ SIGNATURES = {
     "byo_device_free": (_i32, [_vp]),
     "byo_copy_to_device": (_i32, [_vp, _vp, _u64]),
     "byo_set_direction_policy": (_i32, [_vp, _i32, C.c_int64]),
+    "byo_set_direction_schedule": (_i32, [_vp, _vp, _u32]),
     "byo_bfs": (_i32, [_vp, _u32, _vp, _P(BfsStats)]),
     "byo_bfs_multi": (_i32, [_vp, _vp, _u32, _vp, _vp]),
     "byo_bfs_levels": (_i32, [_vp, _vp]),
@@ -269,6 +273,11 @@ class Graph:
     def set_direction_policy(self, policy, threshold=-1):
         check(lib().byo_set_direction_policy(self._h, policy, threshold))
 
+    def set_direction_schedule(self, dirs):
+        """Per-level directions (0 push, 1 pull, 3 forward, 255 rule)."""
+        d = np.ascontiguousarray(dirs, np.uint8)
+        check(lib().byo_set_direction_schedule(self._h, _ptr(d), len(d)))
+
     def bfs(self, src, want_parents=True):
         n = self.N()
         par = np.empty(max(n, 1), np.uint32) if want_parents else None
@@ -371,6 +380,18 @@ class CsrGraph(Graph):
     def rmat(cls, scale, edge_factor=16, a=0.5, b=0.1, c=0.1, seed=1):
         h = C.c_void_p()
         check(lib().byo_csr_create_rmat(scale, edge_factor, a, b, c, seed, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def erdos_renyi(cls, scale, num_edges, seed=1):
+        h = C.c_void_p()
+        check(lib().byo_csr_create_er(scale, num_edges, seed, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def grid3d(cls, side):
+        h = C.c_void_p()
+        check(lib().byo_csr_create_grid3d(side, C.byref(h)))
         return cls(h.value)
 
 

@@ -309,6 +309,9 @@ static int choose_direction(byo_graph *g, bool is_dense, uint64_t fs, uint64_t f
   switch (g->dir_policy) {
     case BYO_DIR_SPARSE: return 0;
     case BYO_DIR_DENSE: return 1;
+    case BYO_DIR_SCHEDULE:
+      if (level < 64 && g->dir_sched[level] != 255) return g->dir_sched[level];
+      return rule;
     case BYO_DIR_MLP: {
       if (!g->sel_loaded) return rule;
       float x[6];
@@ -441,9 +444,22 @@ extern "C" {
 int byo_set_direction_policy(byo_graph_t g, int policy, int64_t threshold) {
   return guard([&] {
     BYO_REQUIRE(g, BYO_E_INVAL, "null handle");
-    BYO_REQUIRE(policy >= 0 && policy <= 3, BYO_E_INVAL, "unknown direction policy");
+    BYO_REQUIRE(policy >= 0 && policy <= 4, BYO_E_INVAL, "unknown direction policy");
     g->dir_policy = policy;
     g->dir_threshold = threshold;
+  });
+}
+
+int byo_set_direction_schedule(byo_graph_t g, const uint8_t *dirs, uint32_t len) {
+  return guard([&] {
+    BYO_REQUIRE(g && (dirs || len == 0), BYO_E_INVAL, "null argument");
+    BYO_REQUIRE(len <= 64, BYO_E_INVAL, "schedule longer than 64 levels");
+    for (uint32_t i = 0; i < len; i++)
+      BYO_REQUIRE(dirs[i] == 0 || dirs[i] == 1 || dirs[i] == 3 || dirs[i] == 255, BYO_E_INVAL,
+                  "schedule entries must be 0, 1, 3 or 255");
+    std::memset(g->dir_sched, 255, sizeof(g->dir_sched));
+    if (len) std::memcpy(g->dir_sched, dirs, len);
+    g->dir_policy = BYO_DIR_SCHEDULE;
   });
 }
 

@@ -77,6 +77,7 @@ struct CoopArgs {
   uint64_t thr;
   const float *sel;         // 1379 params + mean[6] + inv_std[6], or null
   uint64_t maxdeg;
+  const uint8_t *sched;     // 64 per-level directions (BYO_DIR_SCHEDULE)
   struct GridBar *bar;
 };
 
@@ -113,6 +114,7 @@ __device__ int decide(const CoopArgs &a, bool is_dense, uint64_t fs, uint64_t fd
   }
   if (a.policy == BYO_DIR_SPARSE) return 0;
   if (a.policy == BYO_DIR_DENSE) return 1;
+  if (a.policy == BYO_DIR_SCHEDULE) return (level < 64 && a.sched[level] != 255) ? a.sched[level] : rule;
   if (a.policy != BYO_DIR_MLP || !a.sel) return rule;
   const float *p = a.sel;
   const int t = threadIdx.x;
@@ -649,6 +651,9 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   a.thr = g->dir_threshold < 0 ? g->m / 20 : (uint64_t)g->dir_threshold;
   a.sel = (g->dir_policy == BYO_DIR_MLP && g->sel_loaded) ? g->sel_params.as<float>() : nullptr;
   a.maxdeg = maxdeg;
+  uint8_t *dsched = g->buf("bfs_coop_sched", 64).as<uint8_t>();
+  BYO_HIP(hipMemcpyAsync(dsched, g->dir_sched, 64, hipMemcpyHostToDevice, s));
+  a.sched = dsched;
 
   static int grid_cache = 0;
   if (grid_cache == 0) {

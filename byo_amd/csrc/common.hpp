@@ -147,6 +147,7 @@ struct byo_graph {
   // Direction policy.
   int dir_policy = BYO_DIR_THRESHOLD;
   int64_t dir_threshold = -1;
+  uint8_t dir_sched[64] = {};  // BYO_DIR_SCHEDULE: per-level directions (255 = rule)
   // Learned selector weights (device, 1379 floats + 12 normalisation).
   byo::DevBuf sel_params;
   std::vector<float> sel_host;  // host copy for the host-side decision

@@ -75,6 +75,33 @@ __global__ void k_rmat_sym_keys(int scale, uint64_t ne, double a, double ab, dou
   }
 }
 
+// Erdos-Renyi G(n, M): edge i joins splitmix64(base + 2i) mod n and
+// splitmix64(base + 2i + 1) mod n; emitted as its two directed keys.
+__global__ void k_er_sym_keys(uint64_t n, uint64_t ne, uint64_t base, uint64_t *keys) {
+  for (uint64_t i = global_tid(); i < ne; i += grid_threads()) {
+    const uint64_t u = splitmix64(base + 2 * i) % n, v = splitmix64(base + 2 * i + 1) % n;
+    const bool loop = (u == v);
+    keys[2 * i] = loop ? ~0ull : (u << 32) | v;
+    keys[2 * i + 1] = loop ? ~0ull : (v << 32) | u;
+  }
+}
+
+// side^3 grid, vertex id x + side*(y + side*z), 6-neighbour: each vertex
+// emits its +x, +y, +z edges in both directions (6 keys, ~0 off the grid).
+__global__ void k_grid3d_sym_keys(uint64_t side, uint64_t *keys) {
+  const uint64_t n = side * side * side;
+  for (uint64_t id = global_tid(); id < n; id += grid_threads()) {
+    const uint64_t x = id % side, y = (id / side) % side, z = id / (side * side);
+    const uint64_t nb[3] = {x + 1 < side ? id + 1 : ~0ull, y + 1 < side ? id + side : ~0ull,
+                            z + 1 < side ? id + side * side : ~0ull};
+    for (int k = 0; k < 3; k++) {
+      const uint64_t t = nb[k];
+      keys[6 * id + 2 * k] = t == ~0ull ? ~0ull : (id << 32) | t;
+      keys[6 * id + 2 * k + 1] = t == ~0ull ? ~0ull : (t << 32) | id;
+    }
+  }
+}
+
 // batch_insert_helpers::hash32 (benchmarks/run_unweighted.h:720-728)
 __device__ __forceinline__ uint32_t ref_hash32(uint32_t a) {
   a = (a + 0x7ed55d16u) + (a << 12);
@@ -330,6 +357,54 @@ int byo_update_rmat_keys(uint64_t graph_n, uint64_t first, uint64_t count, uint6
     k_update_rmat<<<byo_grid(count, 256, 16384), 256>>>(nn, a, first, count, d.as<uint64_t>());
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipMemcpy(keys, d.p, count * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  });
+}
+
+int byo_csr_create_er(int scale, uint64_t num_edges, uint64_t seed, byo_graph_t *out) {
+  return guard([&] {
+    BYO_REQUIRE(out != nullptr, BYO_E_INVAL, "out is null");
+    BYO_REQUIRE(scale >= 1 && scale <= 31, BYO_E_INVAL, "scale must be in [1, 31]");
+    BYO_REQUIRE(2 * num_edges < 0xFFFFFFFFull, BYO_E_INVAL, "too many edges for one sort (2^32)");
+    const uint64_t n = 1ull << scale;
+    byo_graph *g = new_handle(0);
+    try {
+      DevBuf keys(2 * num_edges * 8 + 64), alt(2 * num_edges * 8 + 64);
+      uint64_t z = seed + 0x9E3779B97F4A7C15ull;  // splitmix64(seed) on the host
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      k_er_sym_keys<<<byo_grid(num_edges, 256, 32768), 256, 0, g->stream>>>(n, num_edges, z ^ (z >> 31),
+                                                                             keys.as<uint64_t>());
+      BYO_LAUNCH_CHECK();
+      csr_from_keys(g, keys.as<uint64_t>(), alt.as<uint64_t>(), 2 * num_edges, n);
+      g->scratch.erase("sort_tmp");
+      g->scratch.erase("select_tmp");
+    } catch (...) {
+      byo_graph_destroy(g);
+      throw;
+    }
+    *out = g;
+  });
+}
+
+int byo_csr_create_grid3d(uint32_t side, byo_graph_t *out) {
+  return guard([&] {
+    BYO_REQUIRE(out != nullptr, BYO_E_INVAL, "out is null");
+    BYO_REQUIRE(side >= 2 && side <= 1024, BYO_E_INVAL, "side must be in [2, 1024]");
+    const uint64_t n = (uint64_t)side * side * side;
+    BYO_REQUIRE(6 * n < 0xFFFFFFFFull, BYO_E_INVAL, "grid too large for one sort (2^32 keys)");
+    byo_graph *g = new_handle(0);
+    try {
+      DevBuf keys(6 * n * 8 + 64), alt(6 * n * 8 + 64);
+      k_grid3d_sym_keys<<<byo_grid(n, 256, 32768), 256, 0, g->stream>>>(side, keys.as<uint64_t>());
+      BYO_LAUNCH_CHECK();
+      csr_from_keys(g, keys.as<uint64_t>(), alt.as<uint64_t>(), 6 * n, n);
+      g->scratch.erase("sort_tmp");
+      g->scratch.erase("select_tmp");
+    } catch (...) {
+      byo_graph_destroy(g);
+      throw;
+    }
+    *out = g;
   });
 }
 

@@ -62,6 +62,13 @@ int byo_csr_create(const uint64_t *offsets, const uint32_t *ids, uint64_t n,
 int byo_csr_create_rmat(int scale, uint64_t edge_factor, double a, double b,
                         double c, uint64_t seed, byo_graph_t *out);
 
+/* Held-out graph families for the direction selector, generated on the
+ * device, symmetrised, deduplicated, self loops dropped: Erdos-Renyi with
+ * 2^scale vertices and num_edges uniform random edges (splitmix64 stream of
+ * seed), and the side^3 6-neighbour grid (id = x + side*(y + side*z)). */
+int byo_csr_create_er(int scale, uint64_t num_edges, uint64_t seed, byo_graph_t *out);
+int byo_csr_create_grid3d(uint32_t side, byo_graph_t *out);
+
 /* CPMA-style packed-memory-array graph from host CSR arrays.  Replaces
  * PMA_graph::PMA_graph(v_data, n, m, ...) (benchmarks/run_structures/
  * run_single_pma.cc:75-89): keys (src<<32)|dst. */
@@ -150,8 +157,14 @@ typedef struct {
 #define BYO_DIR_MLP 1       /* learned selector, rule as fallback            */
 #define BYO_DIR_SPARSE 2    /* always sparse push (edgeMap no_dense)         */
 #define BYO_DIR_DENSE 3     /* always dense pull (edgeMap dense_only)        */
+#define BYO_DIR_SCHEDULE 4  /* per-level list set by byo_set_direction_schedule */
 /* threshold < 0 means M/20 (the -1 default of edgeMap). */
 int byo_set_direction_policy(byo_graph_t g, int policy, int64_t threshold);
+/* Explicit per-level directions (0 sparse push, 1 dense pull, 3 dense
+ * forward, 255 = the threshold rule) for levels 0..len-1 (len <= 64), the
+ * rule beyond; selects BYO_DIR_SCHEDULE.  Used to time every direction at
+ * every level when training the learned selector. */
+int byo_set_direction_schedule(byo_graph_t g, const uint8_t *dirs, uint32_t len);
 
 /* BFS from src.  Replaces BFS(G, src) (benchmarks/BFS/NonDeterministicBFS/
  * BFS.h:47-64): parents[n] with parents[src] = src and BYO_NONE for

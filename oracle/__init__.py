@@ -48,6 +48,8 @@ def lib():
             "orc_ref_hash32": (u32, [u32]),
             "orc_rmat_edges": (None, [i32, u64, dbl, dbl, dbl, u64, u64, _u32p, _u32p]),
             "orc_update_rmat": (None, [u64, u64, u64, _u32p, _u32p]),
+            "orc_er_edges": (None, [u64, u64, u64, _u32p, _u32p]),
+            "orc_grid3d_edges": (u64, [u64, _u32p, _u32p]),
             "orc_sort_unique_u64": (u64, [_u64p, u64]),
             "orc_build_sym_csr": (u64, [u64, u64, _u32p, _u32p, _u64p, _u32p]),
             "orc_keys_to_csr": (None, [u64, _u64p, u64, _u64p, _u32p]),
@@ -97,6 +99,22 @@ def rmat_edges(scale, num_edges, a=0.5, b=0.1, c=0.1, seed=1, first=0):
     dst = np.empty(num_edges, np.uint32)
     lib().orc_rmat_edges(scale, num_edges, a, a + b, a + b + c, seed, first, src, dst)
     return src, dst
+
+
+def er_graph(scale, num_edges, seed=1):
+    n = 1 << scale
+    src = np.empty(num_edges, np.uint32)
+    dst = np.empty(num_edges, np.uint32)
+    lib().orc_er_edges(n, num_edges, seed, src, dst)
+    return sym_csr(n, src, dst)
+
+
+def grid3d_graph(side):
+    cap = 3 * side ** 3
+    src = np.empty(cap, np.uint32)
+    dst = np.empty(cap, np.uint32)
+    k = lib().orc_grid3d_edges(side, src, dst)
+    return sym_csr(side ** 3, src[:k], dst[:k])
 
 
 def update_rmat(graph_n, count, first=0):

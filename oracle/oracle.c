@@ -92,6 +92,32 @@ void orc_rmat_edges(int scale, uint64_t num_edges, double a, double ab,
   }
 }
 
+/* Erdos-Renyi edge i: (splitmix64(base+2i) mod n, splitmix64(base+2i+1) mod n),
+ * base = splitmix64(seed).  This framework's generator (no reference
+ * counterpart); pinned by its degree distribution in the tests. */
+void orc_er_edges(uint64_t n, uint64_t num_edges, uint64_t seed, uint32_t *src, uint32_t *dst) {
+  const uint64_t base = orc_splitmix64(seed);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)num_edges; i++) {
+    src[i] = (uint32_t)(orc_splitmix64(base + 2 * (uint64_t)i) % n);
+    dst[i] = (uint32_t)(orc_splitmix64(base + 2 * (uint64_t)i + 1) % n);
+  }
+}
+
+/* side^3 grid, id = x + side*(y + side*z): the +x, +y, +z edges. */
+uint64_t orc_grid3d_edges(uint64_t side, uint32_t *src, uint32_t *dst) {
+  uint64_t k = 0;
+  for (uint64_t z = 0; z < side; z++)
+    for (uint64_t y = 0; y < side; y++)
+      for (uint64_t x = 0; x < side; x++) {
+        uint64_t id = x + side * (y + side * z);
+        if (x + 1 < side) { src[k] = (uint32_t)id; dst[k++] = (uint32_t)(id + 1); }
+        if (y + 1 < side) { src[k] = (uint32_t)id; dst[k++] = (uint32_t)(id + side); }
+        if (z + 1 < side) { src[k] = (uint32_t)id; dst[k++] = (uint32_t)(id + side * side); }
+      }
+  return k;
+}
+
 static int log2_up(uint64_t n) { /* parlay::log2_up: ceil(log2 n) */
   int r = 0;
   while ((1ull << r) < n) r++;

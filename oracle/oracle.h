@@ -35,6 +35,11 @@ void orc_rmat_edges(int scale, uint64_t num_edges, double a, double ab,
                     double abc, uint64_t seed, uint64_t first_edge,
                     uint32_t *src, uint32_t *dst);
 
+/* Erdos-Renyi and 3-D grid edge lists (this framework's held-out graph
+ * families for the direction selector). */
+void orc_er_edges(uint64_t n, uint64_t num_edges, uint64_t seed, uint32_t *src, uint32_t *dst);
+uint64_t orc_grid3d_edges(uint64_t side, uint32_t *src, uint32_t *dst);
+
 /* Reference-exact update generator of Batch_insert_runner
  * (run_unweighted.h:781-806, rMat at :730-778): nn = 2^(ceil(log2 n)-1),
  * seed = (uint32)hash64(0), r = hash32(x) / INT32_MAX. */

@@ -257,3 +257,36 @@ def test_empty_graph(B):
     assert G.N() == 0 and G.M() == 0
     assert G.tc()[0] == 0
     assert G.map_sum()[0] == 0
+
+
+def test_device_er_and_grid_equal_oracle(B):
+    e = O.er_graph(12, 4 * 4096, seed=3)
+    E = B.CsrGraph.erdos_renyi(12, 4 * 4096, seed=3)
+    off, ids = E.export_csr()
+    assert np.array_equal(off, e.offsets) and np.array_equal(ids, e.ids)
+    g = O.grid3d_graph(9)
+    Gd = B.CsrGraph.grid3d(9)
+    off, ids = Gd.export_csr()
+    assert np.array_equal(off, g.offsets) and np.array_equal(ids, g.ids)
+    assert g.m == 2 * 3 * 9 * 9 * 8  # 3 axes x 8 edges per line x 81 lines, both directions
+    # BFS on the grid: level of (x,y,z) from the corner is x+y+z
+    par, st = Gd.bfs(0)
+    lv = Gd.bfs_levels()
+    x = np.arange(g.n) % 9
+    y = (np.arange(g.n) // 9) % 9
+    z = np.arange(g.n) // 81
+    assert np.array_equal(lv, (x + y + z).astype(np.uint32))
+    assert O.check_bfs_tree(g, 0, par, lv) == 0
+
+
+def test_direction_schedule_keeps_levels(B, s16):
+    g, G = s16
+    lv = O.bfs_levels(g, 10)
+    for sched in ([0, 0, 1, 1, 3, 0], [3, 3, 3, 3], [1, 0, 1, 0, 1, 0, 1], [255, 255, 3]):
+        G.set_direction_schedule(sched)
+        par, st = G.bfs(10)
+        assert O.check_bfs_tree(g, 10, par, lv) == 0
+        for i, d in enumerate(sched):
+            if i < st["levels"] and d != 255 and st["dir"][i] != 2:
+                assert st["dir"][i] == d
+    G.set_direction_policy(0)
