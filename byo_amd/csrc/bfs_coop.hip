@@ -299,9 +299,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
           }
           grid_barrier(a.bar, (unsigned)nb);
         }
-        if (fs <= 4 * nb) {
-          // small frontier: one block (fs <= #blocks) or one wave per
-          // frontier vertex, 1 barrier instead of 3
+        if (fs <= 4 * nb || (fs <= 16 * nwaves && fdeg <= 32 * fs)) {
+          // small frontier, or a moderate one of low average degree: one
+          // block (fs <= #blocks) or one wave per frontier vertex, 1 barrier
+          // instead of 3
           const bool per_block = fs <= nb;
           const uint64_t unit = per_block ? blockIdx.x : wave, nunits = per_block ? nb : nwaves;
           const int width = per_block ? kBlock : 64;

@@ -165,11 +165,15 @@ def compare(G, srcs, path=WEIGHTS, repeats=3):
     return res
 
 
+# RMAT scale 18-22 over the skew range: (0.25, 0.25, 0.25) is the uniform
+# (Erdos-Renyi-like) corner, (0.57, 0.19, 0.19) the Graph500 one.
 TRAIN_GRAPHS = [
-    ("rmat", 18, 16, (0.5, 0.1, 0.1)), ("rmat", 19, 16, (0.57, 0.19, 0.19)),
+    ("rmat", 18, 16, (0.5, 0.1, 0.1)), ("rmat", 18, 4, (0.25, 0.25, 0.25)),
+    ("rmat", 19, 16, (0.57, 0.19, 0.19)), ("rmat", 19, 8, (0.25, 0.25, 0.25)),
     ("rmat", 20, 16, (0.5, 0.1, 0.1)), ("rmat", 20, 8, (0.45, 0.15, 0.15)),
-    ("rmat", 21, 16, (0.57, 0.19, 0.19)), ("rmat", 21, 32, (0.5, 0.1, 0.1)),
-    ("rmat", 22, 16, (0.5, 0.1, 0.1)),
+    ("rmat", 20, 4, (0.35, 0.2, 0.2)), ("rmat", 21, 16, (0.57, 0.19, 0.19)),
+    ("rmat", 21, 32, (0.5, 0.1, 0.1)), ("rmat", 21, 8, (0.25, 0.25, 0.25)),
+    ("rmat", 22, 16, (0.5, 0.1, 0.1)), ("rmat", 22, 4, (0.25, 0.25, 0.25)),
 ]
 
 
@@ -178,7 +182,7 @@ def main():
     ap.add_argument("cmd", choices=["train", "eval"])
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--weights", default=WEIGHTS)
-    ap.add_argument("--sources", type=int, default=6)
+    ap.add_argument("--sources", type=int, default=8)
     ap.add_argument("--epochs", type=int, default=3000)
     args = ap.parse_args()
     if args.cmd == "train":
@@ -187,7 +191,7 @@ def main():
         for kind, scale, ef, (a, b, c) in TRAIN_GRAPHS:
             G = CsrGraph.rmat(scale, ef, a, b, c, seed=7 + scale)
             srcs = sources(G.degrees(), args.sources, seed=100 + scale)
-            X, T = collect(G, srcs)
+            X, T = collect(G, srcs, repeats=2)
             Xs.append(X)
             Ts.append(T)
             G.close()
@@ -202,10 +206,17 @@ def main():
             json.dump(w, f)
         print(json.dumps({k: v for k, v in w.items() if k not in ("params", "mean", "inv_std")}), flush=True)
     else:
-        for name, G in (("rmat22", CsrGraph.rmat(22, 16)), ("er23", CsrGraph.erdos_renyi(23, 4 << 23)),
-                        ("grid128", CsrGraph.grid3d(128))):
+        # held-out graphs: RMAT scale 24 ef 32, 256^3 grid, ER n=2^23 degree 8,
+        # and the bench graph (RMAT scale 22 ef 16)
+        for name, make in (("rmat22_ef16", lambda: CsrGraph.rmat(22, 16)),
+                           ("er23_deg8", lambda: CsrGraph.erdos_renyi(23, 4 << 23)),
+                           ("grid256", lambda: CsrGraph.grid3d(256)),
+                           ("rmat24_ef32", lambda: CsrGraph.rmat(24, 32))):
+            G = make()
             srcs = sources(G.degrees(), 4, seed=5)
-            print(name, json.dumps(compare(G, srcs, args.weights)), flush=True)
+            res = compare(G, srcs, args.weights)
+            res["n"], res["m"] = G.N(), G.M()
+            print(name, json.dumps(res), flush=True)
             G.close()
 
 
