@@ -23,6 +23,7 @@ NONE = 0xFFFFFFFF
 DIR_THRESHOLD, DIR_MLP, DIR_SPARSE, DIR_DENSE, DIR_SCHEDULE = 0, 1, 2, 3, 4
 DIR_RULE = 255  # schedule entry meaning "use the threshold rule" 
 PR_FP64, PR_FP32 = 0, 1
+SEL_SINGLE, SEL_BATCH = 0, 1  # selector weights per traversal engine
 MLP_NPARAM = 1379
 
 
@@ -96,6 +97,7 @@ SIGNATURES = {
     "byo_profile_get": (_i32, [_vp, C.c_char_p, _P(_u64), _P(_dbl)]),
     "byo_profile_reset": (_i32, [_vp]),
     "byo_selector_load": (_i32, [_vp, _vp, _vp, _vp]),
+    "byo_selector_load_engine": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "byo_trainer_create": (_i32, [_vp, C.c_float, _P(_vp)]),
     "byo_trainer_destroy": (_i32, [_vp]),
     "byo_trainer_step": (_i32, [_vp, _vp, _vp, _i32, _P(C.c_float)]),
@@ -357,12 +359,14 @@ class Graph:
     def profile_reset(self):
         check(lib().byo_profile_reset(self._h))
 
-    def load_selector(self, params, mean, inv_std):
+    def load_selector(self, params, mean, inv_std, engine=0):
+        """Selector weights for engine SEL_SINGLE (byo_bfs) or SEL_BATCH
+        (the shared multi-source traversal of bfs_multi)."""
         params = np.ascontiguousarray(params, np.float32)
         mean = np.ascontiguousarray(mean, np.float32)
         inv_std = np.ascontiguousarray(inv_std, np.float32)
         assert params.size == MLP_NPARAM and mean.size == 6 and inv_std.size == 6
-        check(lib().byo_selector_load(self._h, _ptr(params), _ptr(mean), _ptr(inv_std)))
+        check(lib().byo_selector_load_engine(self._h, int(engine), _ptr(params), _ptr(mean), _ptr(inv_std)))
 
 
 class CsrGraph(Graph):

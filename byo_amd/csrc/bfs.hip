@@ -27,6 +27,8 @@
 namespace byo {
 
 void mlp_eval_host(const std::vector<float> &p, const float *x, float *logits);  // selector.hip
+void bfs_batch_run(byo_graph *g, const uint32_t *srcs, uint32_t k, byo_bfs_stats *stats, uint64_t maxdeg,
+                   double *ms_out);  // bfs_batch.hip
 void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all, byo_bfs_stats *stats,
                   uint64_t maxdeg, double *ms_out);  // bfs_coop.hip
 
@@ -473,7 +475,37 @@ int byo_bfs_multi(byo_graph_t g, const uint32_t *srcs, uint32_t k, uint32_t *par
     if (k == 0) return;
     const uint64_t mx = (g->dir_policy == BYO_DIR_MLP) ? max_degree(g) : 0;
     const char *mode = std::getenv("BYO_BFS_MODE");
-    if (!(mode && std::strcmp(mode, "levels") == 0)) {
+    const bool levels_mode = mode && std::strcmp(mode, "levels") == 0;
+    const bool seq_mode = mode && std::strcmp(mode, "sequential") == 0;
+    if (k > 1 && !levels_mode && !seq_mode) {
+      // default for several sources: up to 64 at a time share one traversal
+      const uint64_t n = g->n;
+      for (uint32_t c0 = 0; c0 < k; c0 += 64) {
+        const uint32_t kc = std::min<uint32_t>(64, k - c0);
+        double ms = 0;
+        bfs_batch_run(g, srcs + c0, kc, stats ? stats + c0 : nullptr, mx, &ms);
+        if (g->prof.on) {
+          auto &acc = g->prof.acc["bfs"];
+          acc.launches += kc;
+          acc.ms += ms;
+          g->account("bfs_launch", ms);
+        }
+        uint32_t *all = g->scratch["bfs_parents_all"].as<uint32_t>();
+        if (parents)
+          BYO_HIP(hipMemcpyAsync(parents + (uint64_t)c0 * n, all, (uint64_t)kc * n * 4, hipMemcpyDeviceToHost,
+                                 g->stream));
+        if (c0 + kc == k) {
+          uint32_t *last = g->buf("bfs_parents", n * 4 + 64).as<uint32_t>();
+          BYO_HIP(hipMemcpyAsync(last, all + (uint64_t)(kc - 1) * n, n * 4, hipMemcpyDeviceToDevice, g->stream));
+        }
+        BYO_HIP(hipStreamSynchronize(g->stream));
+      }
+      g->scratch["bfs_last_src"].ensure(8);
+      BYO_HIP(hipMemcpyAsync(g->scratch["bfs_last_src"].p, &srcs[k - 1], 4, hipMemcpyHostToDevice, g->stream));
+      BYO_HIP(hipStreamSynchronize(g->stream));
+      return;
+    }
+    if (!levels_mode) {
       // default: every source in one persistent cooperative launch
       const bool keep_all = parents != nullptr && k > 1;
       std::vector<byo_bfs_stats> tmp(stats ? 0 : k);

@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "coop.hpp"
 #include "primitives.hpp"
 
 namespace byo {
@@ -63,7 +64,8 @@ struct CoopArgs {
   uint32_t *parents;        // parents of source j at parents + j * pstride
   uint64_t pstride;
   uint32_t *q0, *q1;
-  uint64_t *bm0, *bm1, *bm2, *visited64;
+  uint64_t *bm;              // three rotating bitmaps of nw64 words
+  uint64_t *visited64;
   uint64_t *scan;
   uint32_t *cstart;         // first frontier index of each 256-edge chunk
   unsigned long long *bsum;
@@ -73,141 +75,17 @@ struct CoopArgs {
   LevelLog *logs;           // k x 64
   uint32_t *nlevels;        // k
   unsigned long long *tot;  // k x {vertices reached, sum of their degrees}
-  int policy;
-  uint64_t thr;
-  const float *sel;         // 1379 params + mean[6] + inv_std[6], or null
-  uint64_t maxdeg;
-  const uint8_t *sched;     // 64 per-level directions (BYO_DIR_SCHEDULE)
-  struct GridBar *bar;
+  DirCtl dc;
+  GridBar *bar;
 };
 
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
+constexpr int kBlock = kCoopBlock;
+constexpr int kWaves = kCoopWaves;
 constexpr int kPerLane = 4;
 constexpr uint64_t kChunk = 64 * kPerLane;
-constexpr int kNP = BYO_MLP_NPARAM;
 constexpr int kG = 8;  // lanes per vertex in the pull
 
-__device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long long *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ __forceinline__ bool bit_get(const uint64_t *bm, uint32_t i) { return (bm[i >> 6] >> (i & 63)) & 1ull; }
-
-struct MlpLds {
-  float x[6], h1[32], h2[32], z[3];
-};
-
-// Direction of one edgeMap call: 0 sparse, 1 dense pull, 2 empty, 3 dense
-// forward (the same decision as choose_direction in bfs.hip).  Called by
-// every thread of the block (the selector MLP runs one neuron per thread,
-// layout as in selector.hip); returns the same value in all threads.
-__device__ int decide(const CoopArgs &a, bool is_dense, uint64_t fs, uint64_t fdeg, uint64_t visited_deg, int level,
-                      MlpLds &s) {
-  int rule;
-  if (is_dense && fs > a.n / 10) {
-    rule = 1;
-  } else if (fdeg == 0) {
-    return 2;
-  } else {
-    rule = (fs + fdeg > a.thr) ? 1 : 0;
-  }
-  if (a.policy == BYO_DIR_SPARSE) return 0;
-  if (a.policy == BYO_DIR_DENSE) return 1;
-  if (a.policy == BYO_DIR_SCHEDULE) return (level < 64 && a.sched[level] != 255) ? a.sched[level] : rule;
-  if (a.policy != BYO_DIR_MLP || !a.sel) return rule;
-  const float *p = a.sel;
-  const int t = threadIdx.x;
-  __syncthreads();
-  if (t == 0) {
-    const double avg = a.n ? (double)a.m / (double)a.n : 0.0;
-    float x[6];
-    x[0] = (float)log2(1.0 + (double)fs);
-    x[1] = (float)log2(1.0 + (double)fdeg);
-    x[2] = (float)log2(1.0 + (double)(a.m > visited_deg ? a.m - visited_deg : 0));
-    x[3] = (float)level;
-    x[4] = (float)log2(1.0 + avg);
-    x[5] = (float)log2(1.0 + (double)a.maxdeg);
-    for (int i = 0; i < 6; i++) s.x[i] = (x[i] - p[kNP + i]) * p[kNP + 6 + i];
-  }
-  __syncthreads();
-  if (t < 32) {
-    float v = p[192 + t];
-    for (int i = 0; i < 6; i++) v += p[t * 6 + i] * s.x[i];
-    s.h1[t] = v > 0.f ? v : 0.f;
-  }
-  __syncthreads();
-  if (t < 32) {
-    float v = p[1248 + t];
-    for (int i = 0; i < 32; i++) v += p[224 + t * 32 + i] * s.h1[i];
-    s.h2[t] = v > 0.f ? v : 0.f;
-  }
-  __syncthreads();
-  if (t < 3) {
-    float v = p[1376 + t];
-    for (int i = 0; i < 32; i++) v += p[1280 + t * 32 + i] * s.h2[i];
-    s.z[t] = v;
-  }
-  __syncthreads();
-  const float z0 = s.z[0], z1 = s.z[1], z2 = s.z[2];
-  if (!isfinite(z0 + z1 + z2)) return rule;
-  const int best = (z1 > z0) ? ((z2 > z1) ? 2 : 1) : ((z2 > z0) ? 2 : 0);
-  return best == 0 ? 0 : (best == 1 ? 1 : 3);
-}
-
-// Grid-wide barrier for a cooperative launch (all blocks co-resident).
-// Arrivals are counted per group of 32 blocks, then once per group on the
-// top counter, so no address sees more than 32 + nblocks/32 atomics; the
-// last arrival bumps the generation word the others poll.  Agent-scope
-// release/acquire fences make every block's writes visible across XCDs.
-struct GridBar {
-  unsigned int top;
-  unsigned int pad0[31];
-  unsigned int gen;  // polled word on a line of its own
-  unsigned int pad1[31];
-  unsigned int group[1024];  // counter of 32-block group g at group[16 g]
-};
-
-__device__ __forceinline__ void grid_barrier(GridBar *b, unsigned int nblocks) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned int gen = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const unsigned int grp = blockIdx.x >> 5;
-    const unsigned int gsize = min(32u, nblocks - (grp << 5));
-    const unsigned int ngroups = (nblocks + 31) >> 5;
-    bool last = false;
-    const unsigned int a = __hip_atomic_fetch_add(&b->group[grp * 16], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (a == gsize - 1) {
-      // the group's last arrival carries the group's writes forward
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      __hip_atomic_store(&b->group[grp * 16], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const unsigned int t = __hip_atomic_fetch_add(&b->top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-      if (t == ngroups - 1) {
-        __hip_atomic_store(&b->top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        last = true;
-      }
-    }
-    // poll relaxed: an acquire per poll would invalidate this XCD's L2 and
-    // evict the data the still-working blocks are using
-    if (!last)
-      while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) __builtin_amdgcn_s_sleep(2);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
-}
-
-// Block-wide sum (all threads get the result).
-__device__ __forceinline__ unsigned long long block_sum(unsigned long long v, unsigned long long *lds) {
-  v = wave_sum(v);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
-  __syncthreads();
-  unsigned long long s = 0;
-  for (int w = 0; w < kWaves; w++) s += lds[w];
-  return s;
-}
 
 struct PushLds {
   uint64_t sc[kWaves][64];
@@ -237,9 +115,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
     const uint64_t sw = src >> 6, sbit = 1ull << (src & 63);
     for (uint64_t i = global_tid(); i < a.nw64; i += grid_threads()) {
       a.visited64[i] = (i == sw) ? sbit : 0;
-      a.bm0[i] = (i == sw) ? sbit : 0;
-      a.bm1[i] = 0;
-      a.bm2[i] = 0;
+      a.bm[i] = (i == sw) ? sbit : 0;
+      a.bm[a.nw64 + i] = 0;
+      a.bm[2 * a.nw64 + i] = 0;
     }
     if (leader) {
       a.q0[0] = src;
@@ -250,7 +128,6 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
     grid_barrier(a.bar, (unsigned)nb);
 
     uint32_t *q = a.q0, *qn = a.q1;
-    uint64_t *const bms[3] = {a.bm0, a.bm1, a.bm2};
     bool is_dense = false, have_q = true;
     uint64_t visited_deg = 0, reached = 0;
     int L = 0;
@@ -260,7 +137,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
       const uint64_t fdeg = ld_relaxed(&cur->deg);
       if (fs == 0) break;
       reached += fs;
-      const int dir = decide(a, is_dense, fs, fdeg, visited_deg, L, mlp);
+      const int dir = decide(a.dc, is_dense, fs, fdeg, visited_deg, L, mlp);
       if (leader && L < 64) a.logs[(uint64_t)j * 64 + L] = LevelLog{fs, fdeg, (uint32_t)dir, 0, (unsigned long long)wall_clock64()};
       if (dir == 2) {
         visited_deg += fdeg;
@@ -272,9 +149,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
       // bitmaps rotate like the counters: B[L%3] is this frontier, the next
       // one is built in B[(L+1)%3], and B[(L+2)%3] -- last level's
       // frontier, read by nobody now -- is cleared for level L+1
-      uint64_t *const fbm = bms[L % 3], *const nbm = bms[(L + 1) % 3];
+      // offsets from one base keep these global (a runtime index into an
+      // array of pointers would make every access a flat one)
+      uint64_t *const fbm = a.bm + (uint64_t)(L % 3) * a.nw64, *const nbm = a.bm + (uint64_t)((L + 1) % 3) * a.nw64;
       {
-        uint64_t *clr = bms[(L + 2) % 3];
+        uint64_t *clr = a.bm + (uint64_t)((L + 2) % 3) * a.nw64;
         for (uint64_t i = global_tid(); i < a.nw64; i += grid_threads()) clr[i] = 0;
       }
 
@@ -490,10 +369,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
               const uint64_t b = __shfl(bl, x, 64), e = __shfl(el, x, 64);
               const uint64_t kk = b + sub;
               h[st] = ((todo >> x) & 1) && kk < e;
-              u[st] = h[st] ? a.ids[kk] : 0u;
+              u[st] = a.ids[h[st] ? kk : 0];  // unconditional: all 8 loads in flight
             }
+            uint64_t fw[8];
 #pragma unroll
-            for (int st = 0; st < 8; st++) h[st] = h[st] && bit_get(fbm, u[st]);
+            for (int st = 0; st < 8; st++) fw[st] = fbm[u[st] >> 6];
+#pragma unroll
+            for (int st = 0; st < 8; st++) h[st] = h[st] && ((fw[st] >> (u[st] & 63)) & 1);
 #pragma unroll
             for (int st = 0; st < 8; st++) {
               const uint32_t gm = (uint32_t)(__ballot(h[st]) >> (g * 8)) & 0xFFu;
@@ -618,6 +500,41 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
 
 }  // namespace
 
+DirCtl dir_ctl(byo_graph *g, uint64_t maxdeg, hipStream_t s, bool batched) {
+  DirCtl d{};
+  d.n = g->n;
+  d.m = g->m;
+  d.policy = g->dir_policy;
+  d.thr = g->dir_threshold < 0 ? g->m / 20 : (uint64_t)g->dir_threshold;
+  if (g->dir_policy == BYO_DIR_MLP) {
+    if (batched && g->sel_loaded_b) d.sel = g->sel_params_b.as<float>();
+    if (!batched && g->sel_loaded) d.sel = g->sel_params.as<float>();
+  }
+  d.maxdeg = maxdeg;
+  uint8_t *dsched = g->buf("bfs_coop_sched", 64).as<uint8_t>();
+  BYO_HIP(hipMemcpyAsync(dsched, g->dir_sched, 64, hipMemcpyHostToDevice, s));
+  d.sched = dsched;
+  return d;
+}
+
+int coop_grid(const void *kernel, int block) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void *>, int> cache;
+  int dev = 0;
+  BYO_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find({dev, kernel});
+  if (it != cache.end()) return it->second;
+  int cus = 0, per = 0;
+  BYO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  BYO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0));
+  BYO_REQUIRE(per >= 1, BYO_E_HIP, "cooperative kernel cannot be resident");
+  const int nb = cus * std::min(per, 4);
+  BYO_REQUIRE(nb <= 2048, BYO_E_HIP, "grid barrier sized for at most 2048 blocks");
+  cache[{dev, kernel}] = nb;
+  return nb;
+}
+
 // Runs k BFS in one cooperative launch.  Results: k*n parents in scratch
 // "bfs_parents_all" (keep_all) or the last source's tree in "bfs_parents".
 // Fills stats[k] when given.
@@ -635,9 +552,7 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   a.parents = g->buf(keep_all ? "bfs_parents_all" : "bfs_parents", (keep_all ? k : 1) * n * 4 + 64).as<uint32_t>();
   a.q0 = g->buf("bfs_q", n * 4 + 64).as<uint32_t>();
   a.q1 = g->buf("bfs_qn", n * 4 + 64).as<uint32_t>();
-  a.bm0 = g->buf("bfs_fbm", a.nw64 * 8).as<uint64_t>();
-  a.bm1 = g->buf("bfs_nbm", a.nw64 * 8).as<uint64_t>();
-  a.bm2 = g->buf("bfs_bm2", a.nw64 * 8).as<uint64_t>();
+  a.bm = g->buf("bfs_bms", 3 * a.nw64 * 8).as<uint64_t>();
   a.visited64 = g->buf("bfs_visited", a.nw64 * 8).as<uint64_t>();
   a.scan = g->buf("bfs_scan", (n + 1) * 8 + 64).as<uint64_t>();
   a.cstart = g->buf("bfs_cstart", (g->m / kChunk + 2) * 4 + 64).as<uint32_t>();
@@ -648,26 +563,9 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   a.logs = g->buf("bfs_coop_logs", (size_t)k * 65 * sizeof(LevelLog) + 64).as<LevelLog>();
   a.nlevels = g->buf("bfs_coop_nlev", (size_t)k * 4 + 64).as<uint32_t>();
   a.tot = g->buf("bfs_coop_tot", (size_t)k * 16 + 64).as<unsigned long long>();
-  a.policy = g->dir_policy;
-  a.thr = g->dir_threshold < 0 ? g->m / 20 : (uint64_t)g->dir_threshold;
-  a.sel = (g->dir_policy == BYO_DIR_MLP && g->sel_loaded) ? g->sel_params.as<float>() : nullptr;
-  a.maxdeg = maxdeg;
-  uint8_t *dsched = g->buf("bfs_coop_sched", 64).as<uint8_t>();
-  BYO_HIP(hipMemcpyAsync(dsched, g->dir_sched, 64, hipMemcpyHostToDevice, s));
-  a.sched = dsched;
-
-  static int grid_cache = 0;
-  if (grid_cache == 0) {
-    int dev = 0, cus = 0, per = 0;
-    BYO_HIP(hipGetDevice(&dev));
-    BYO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    BYO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(k_bfs_coop), kBlock, 0));
-    BYO_REQUIRE(per >= 1, BYO_E_HIP, "cooperative BFS kernel cannot be resident");
-    grid_cache = cus * std::min(per, 4);
-  }
-  const int nblocks = grid_cache;
+  a.dc = dir_ctl(g, maxdeg, s, false);
+  const int nblocks = coop_grid(reinterpret_cast<const void *>(k_bfs_coop), kBlock);
   a.bsum = g->buf("bfs_coop_bsum", (size_t)nblocks * 8 + 64).as<unsigned long long>();
-  BYO_REQUIRE(nblocks <= 2048, BYO_E_HIP, "grid barrier sized for at most 2048 blocks");
   a.bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
   BYO_HIP(hipMemsetAsync(a.bar, 0, sizeof(GridBar), s));
   BYO_HIP(hipMemcpyAsync(dsrc, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s));

@@ -152,6 +152,8 @@ struct byo_graph {
   byo::DevBuf sel_params;
   std::vector<float> sel_host;  // host copy for the host-side decision
   bool sel_loaded = false;
+  byo::DevBuf sel_params_b;  // weights for the shared multi-source traversal
+  bool sel_loaded_b = false;
 
   // Scratch owned by algorithms (grown on demand, kept across calls).
   std::map<std::string, byo::DevBuf> scratch;

@@ -1,0 +1,153 @@
+// coop.hpp -- pieces shared by the persistent cooperative traversal kernels
+// (bfs_coop.hip: one source at a time; bfs_batch.hip: up to 64 sources per
+// traversal): the grid barrier, block reductions and the on-device direction
+// decision (rule of gbbs/edge_map_data.h:257-306 or the learned selector).
+#pragma once
+#include <cmath>
+
+#include "common.hpp"
+
+namespace byo {
+
+constexpr int kCoopBlock = 256;
+constexpr int kCoopWaves = kCoopBlock / 64;
+
+__device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid-wide barrier for a cooperative launch (all blocks co-resident).
+// Arrivals are counted per group of 32 blocks, then once per group on the
+// top counter, so no address sees more than 32 + nblocks/32 atomics; the
+// last arrival bumps the generation word the others poll.  Agent-scope
+// release/acquire fences make every block's writes visible across XCDs.
+struct GridBar {
+  unsigned int top;
+  unsigned int pad0[31];
+  unsigned int gen;  // polled word on a line of its own
+  unsigned int pad1[31];
+  unsigned int group[1024];  // counter of 32-block group g at group[16 g]
+};
+
+__device__ __forceinline__ void grid_barrier(GridBar *b, unsigned int nblocks) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int gen = __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned int grp = blockIdx.x >> 5;
+    const unsigned int gsize = min(32u, nblocks - (grp << 5));
+    const unsigned int ngroups = (nblocks + 31) >> 5;
+    bool last = false;
+    const unsigned int a = __hip_atomic_fetch_add(&b->group[grp * 16], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (a == gsize - 1) {
+      // the group's last arrival carries the group's writes forward
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __hip_atomic_store(&b->group[grp * 16], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned int t = __hip_atomic_fetch_add(&b->top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (t == ngroups - 1) {
+        __hip_atomic_store(&b->top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        last = true;
+      }
+    }
+    // poll relaxed: an acquire per poll would invalidate this XCD's L2 and
+    // evict the data the still-working blocks are using
+    if (!last)
+      while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) __builtin_amdgcn_s_sleep(2);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// Block-wide sum (all threads get the result); lds holds kCoopWaves slots.
+__device__ __forceinline__ unsigned long long block_sum(unsigned long long v, unsigned long long *lds) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  unsigned long long s = 0;
+  for (int w = 0; w < kCoopWaves; w++) s += lds[w];
+  return s;
+}
+
+// Inputs of the per-level direction decision.
+struct DirCtl {
+  uint64_t n, m;
+  int policy;
+  uint64_t thr;
+  const float *sel;      // 1379 params + mean[6] + inv_std[6], or null
+  uint64_t maxdeg;
+  const uint8_t *sched;  // 64 per-level directions (BYO_DIR_SCHEDULE)
+};
+
+struct MlpLds {
+  float x[6], h1[32], h2[32], z[3];
+};
+
+// Direction of one edgeMap call: 0 sparse, 1 dense pull, 2 empty, 3 dense
+// forward (the same decision as choose_direction in bfs.hip).  Called by
+// every thread of the block (the selector MLP runs one neuron per thread,
+// parameter layout as in selector.hip); returns the same value in all
+// threads.
+__device__ inline int decide(const DirCtl &a, bool is_dense, uint64_t fs, uint64_t fdeg, uint64_t visited_deg,
+                             int level, MlpLds &s) {
+  constexpr int kNP = BYO_MLP_NPARAM;
+  int rule;
+  if (is_dense && fs > a.n / 10) {
+    rule = 1;
+  } else if (fdeg == 0) {
+    return 2;
+  } else {
+    rule = (fs + fdeg > a.thr) ? 1 : 0;
+  }
+  if (a.policy == BYO_DIR_SPARSE) return 0;
+  if (a.policy == BYO_DIR_DENSE) return 1;
+  if (a.policy == BYO_DIR_SCHEDULE) return (level < 64 && a.sched[level] != 255) ? a.sched[level] : rule;
+  if (a.policy != BYO_DIR_MLP || !a.sel) return rule;
+  const float *p = a.sel;
+  const int t = threadIdx.x;
+  __syncthreads();
+  if (t == 0) {
+    const double avg = a.n ? (double)a.m / (double)a.n : 0.0;
+    float x[6];
+    x[0] = (float)log2(1.0 + (double)fs);
+    x[1] = (float)log2(1.0 + (double)fdeg);
+    x[2] = (float)log2(1.0 + (double)(a.m > visited_deg ? a.m - visited_deg : 0));
+    x[3] = (float)level;
+    x[4] = (float)log2(1.0 + avg);
+    x[5] = (float)log2(1.0 + (double)a.maxdeg);
+    for (int i = 0; i < 6; i++) s.x[i] = (x[i] - p[kNP + i]) * p[kNP + 6 + i];
+  }
+  __syncthreads();
+  if (t < 32) {
+    float v = p[192 + t];
+    for (int i = 0; i < 6; i++) v += p[t * 6 + i] * s.x[i];
+    s.h1[t] = v > 0.f ? v : 0.f;
+  }
+  __syncthreads();
+  if (t < 32) {
+    float v = p[1248 + t];
+    for (int i = 0; i < 32; i++) v += p[224 + t * 32 + i] * s.h1[i];
+    s.h2[t] = v > 0.f ? v : 0.f;
+  }
+  __syncthreads();
+  if (t < 3) {
+    float v = p[1376 + t];
+    for (int i = 0; i < 32; i++) v += p[1280 + t * 32 + i] * s.h2[i];
+    s.z[t] = v;
+  }
+  __syncthreads();
+  const float z0 = s.z[0], z1 = s.z[1], z2 = s.z[2];
+  if (!isfinite(z0 + z1 + z2)) return rule;
+  const int best = (z1 > z0) ? ((z2 > z1) ? 2 : 1) : ((z2 > z0) ? 2 : 0);
+  return best == 0 ? 0 : (best == 1 ? 1 : 3);
+}
+
+// Direction-decision inputs of a handle (policy, threshold, and the
+// selector weights of the single-source or the batched traversal).
+DirCtl dir_ctl(byo_graph *g, uint64_t maxdeg, hipStream_t s, bool batched);
+
+// Grid size of a cooperative kernel: CUs x min(resident blocks per CU, 4).
+int coop_grid(const void *kernel, int block);
+
+}  // namespace byo

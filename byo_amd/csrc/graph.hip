@@ -418,6 +418,7 @@ int byo_graph_destroy(byo_graph_t g) {
     g->off.release();
     g->ids.release();
     g->sel_params.release();
+    g->sel_params_b.release();
     if (g->pinned) (void)hipHostFree(g->pinned);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;

@@ -177,9 +177,21 @@ void trainer_fwd_bwd(byo_trainer *t, const float *x, const int32_t *y, int B, bo
 
 extern "C" {
 
-int byo_selector_load(byo_graph_t g, const float *params, const float *mean, const float *inv_std) {
+int byo_selector_load_engine(byo_graph_t g, int engine, const float *params, const float *mean,
+                             const float *inv_std) {
   return guard([&] {
     BYO_REQUIRE(g && params && mean && inv_std, BYO_E_INVAL, "null argument");
+    BYO_REQUIRE(engine == BYO_SEL_SINGLE || engine == BYO_SEL_BATCH, BYO_E_INVAL, "unknown traversal engine");
+    if (engine == BYO_SEL_BATCH) {
+      std::vector<float> h(params, params + kNP);
+      h.insert(h.end(), mean, mean + 6);
+      h.insert(h.end(), inv_std, inv_std + 6);
+      bind(g);
+      g->sel_params_b.ensure((kNP + 12) * sizeof(float));
+      BYO_HIP(hipMemcpy(g->sel_params_b.p, h.data(), (kNP + 12) * sizeof(float), hipMemcpyHostToDevice));
+      g->sel_loaded_b = true;
+      return;
+    }
     g->sel_host.assign(params, params + kNP);
     g->sel_host.insert(g->sel_host.end(), mean, mean + 6);
     g->sel_host.insert(g->sel_host.end(), inv_std, inv_std + 6);
@@ -188,6 +200,10 @@ int byo_selector_load(byo_graph_t g, const float *params, const float *mean, con
     BYO_HIP(hipMemcpy(g->sel_params.p, g->sel_host.data(), (kNP + 12) * sizeof(float), hipMemcpyHostToDevice));
     g->sel_loaded = true;
   });
+}
+
+int byo_selector_load(byo_graph_t g, const float *params, const float *mean, const float *inv_std) {
+  return byo_selector_load_engine(g, BYO_SEL_SINGLE, params, mean, inv_std);
 }
 
 int byo_trainer_create(const float *init_params, float lr, byo_trainer_t *out) {

@@ -85,6 +85,19 @@ __device__ __forceinline__ bool find_sorted(const uint32_t *a, uint32_t len, uin
   return lo < len && a[lo] == x;
 }
 
+// The same search over the wave's LDS stage, indexed directly: a pointer
+// that may point to LDS or to global memory would make every probe a flat
+// load.
+template <int N>
+__device__ __forceinline__ bool find_sorted_lds(const uint32_t (&a)[N], uint32_t len, uint32_t x) {
+  uint32_t lo = 0, hi = len;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo < len && a[lo] == x;
+}
+
 __global__ void __launch_bounds__(256) k_tc_count(const uint64_t *dg_off, const uint32_t *dg_ids, uint64_t n,
                                                   unsigned long long *next_u, unsigned long long *total) {
   __shared__ uint32_t stage[kTcWaves][kLdsPerWave];
@@ -106,7 +119,6 @@ __global__ void __launch_bounds__(256) k_tc_count(const uint64_t *dg_off, const 
     const bool in_lds = du <= kLdsPerWave;
     if (in_lds)
       for (uint64_t i = lane; i < du; i += 64) stage[wv][i] = dg_ids[bu + i];
-    const uint32_t *list = in_lds ? stage[wv] : dg_ids + bu;
     // LDS writes by this wave must land before its lanes read them
     wave_lds_sync();
     for (uint64_t c0 = 0; c0 < du; c0 += 64) {
@@ -114,7 +126,7 @@ __global__ void __launch_bounds__(256) k_tc_count(const uint64_t *dg_off, const 
       uint32_t v = 0;
       uint64_t dv = 0, bv = 0;
       if (j < du) {
-        v = in_lds ? stage[wv][j] : dg_ids[bu + j];
+        v = dg_ids[bu + j];
         bv = dg_off[v];
         dv = dg_off[v + 1] - bv;
       }
@@ -140,7 +152,10 @@ __global__ void __launch_bounds__(256) k_tc_count(const uint64_t *dg_off, const 
         const uint64_t obv = __shfl(bv, owner, 64);
         if (x < T) {
           const uint32_t w = dg_ids[obv + (x - startx)];
-          mycount += find_sorted(list, (uint32_t)du, w);
+          if (in_lds)
+            mycount += find_sorted_lds(stage[wv], (uint32_t)du, w);
+          else
+            mycount += find_sorted(dg_ids + bu, (uint32_t)du, w);
         }
       }
       wave_lds_sync();  // pre[] is rewritten by the next chunk

@@ -17,8 +17,14 @@ kernels (byo_trainer_*).  Direction never changes BFS levels, so the learned
 policy is bit-exact whatever it predicts; the rule stays the fallback when no
 weights are loaded.
 
-    python -m byo_amd.selector train --out byo_amd/selector_weights.json
-    python -m byo_amd.selector eval  --weights byo_amd/selector_weights.json
+Each traversal engine has its own network, trained on its own timings: the
+one-source traversal (byo_bfs) and the shared multi-source traversal of
+byo_bfs_multi, whose union frontier makes the pull scan far longer (a vertex
+stops only once every active source is found) and the push cheaper per
+source, so the same frontier statistics call for different directions.
+
+    python -m byo_amd.selector train --engine batch
+    python -m byo_amd.selector eval  --engine batch
 """
 import argparse
 import json
@@ -29,10 +35,13 @@ import time
 
 import numpy as np
 
-from . import (DIR_MLP, DIR_RULE, DIR_THRESHOLD, MLP_NPARAM, CsrGraph, Trainer)
+from . import (DIR_MLP, DIR_RULE, DIR_THRESHOLD, MLP_NPARAM, SEL_BATCH, SEL_SINGLE, CsrGraph, Trainer)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "selector_weights.json")
+WEIGHTS_BATCH = os.path.join(HERE, "selector_weights_batch.json")
+ENGINES = {"single": (SEL_SINGLE, WEIGHTS), "batch": (SEL_BATCH, WEIGHTS_BATCH)}
+BATCH = 8  # sources per shared traversal when collecting and evaluating
 CLASS_TO_DIR = (0, 1, 3)
 DIR_TO_CLASS = {0: 0, 1: 1, 3: 2}
 
@@ -61,20 +70,30 @@ def sources(deg, k, seed):
     return out
 
 
-def collect(G, srcs, repeats=1, max_levels=64):
-    """Per-level samples (features, times of push / pull / forward) of BFS
-    runs from srcs.  Each level is re-run with every direction forced."""
+def _run(G, srcs, engine):
+    """One traversal: a single source (engine "single") or one shared
+    traversal of a batch of sources ("batch"); returns the per-level record
+    (for a batch, the union frontier's)."""
+    if engine == "single":
+        return G.bfs(srcs, want_parents=False)[1]
+    return G.bfs_multi(srcs, want_parents=False)[1][0]
+
+
+def collect(G, units, repeats=1, max_levels=64, engine="single"):
+    """Per-level samples (features, times of push / pull / forward) of
+    traversals over units (sources, or batches of sources for the batched
+    engine).  Each level is re-run with every direction forced."""
     n, m = G.N(), G.M()
     deg = G.degrees()
     maxdeg = int(deg.max()) if n else 0
-    X, T = [], []
-    for s in srcs:
+    X, T, R = [], [], []
+    for u in units:
         G.set_direction_policy(DIR_THRESHOLD)
-        _, st = G.bfs(s, want_parents=False)
-        L = min(st["levels"], max_levels)
+        st = _run(G, u, engine)
+        L = min(max(st["levels"], int(np.count_nonzero(st["fsize"]))), max_levels)
         vdeg = 0
         for lv in range(L):
-            if st["dir"][lv] == 2:
+            if st["fsize"][lv] == 0 or st["dir"][lv] == 2:
                 break
             times = []
             for d in CLASS_TO_DIR:
@@ -82,14 +101,15 @@ def collect(G, srcs, repeats=1, max_levels=64):
                 G.set_direction_schedule(sched)
                 best = float("inf")
                 for _ in range(repeats):
-                    _, s2 = G.bfs(s, want_parents=False)
+                    s2 = _run(G, u, engine)
                     best = min(best, s2["level_ms"][lv])
                 times.append(best)
             X.append(features(st["fsize"][lv], st["fdeg"][lv], vdeg, lv, n, m, maxdeg))
             T.append(times)
+            R.append(DIR_TO_CLASS.get(st["dir"][lv], 0))
             vdeg += st["fdeg"][lv]
     G.set_direction_policy(DIR_THRESHOLD)
-    return np.array(X, np.float32), np.array(T, np.float64)
+    return np.array(X, np.float32), np.array(T, np.float64), np.array(R, np.int32)
 
 
 def init_params(seed=0):
@@ -129,39 +149,66 @@ def train(X, T, epochs=3000, lr=3e-3, seed=0):
             "loss_first": float(losses[0]), "loss_last": float(losses[-1]), "samples": int(len(y))}
 
 
-def load(G, path=WEIGHTS):
-    with open(path) as f:
-        w = json.load(f)
-    G.load_selector(np.array(w["params"], np.float32), np.array(w["mean"], np.float32),
-                    np.array(w["inv_std"], np.float32))
-    return w
+def load(G, path=None, engine=None):
+    """Loads the committed weights of every engine (or one file for one
+    engine) into G; returns the weight dicts by engine name."""
+    out = {}
+    for name, (eid, default) in ENGINES.items():
+        if engine is not None and name != engine:
+            continue
+        p = path if path is not None else default
+        if not os.path.exists(p):
+            continue
+        with open(p) as f:
+            w = json.load(f)
+        G.load_selector(np.array(w["params"], np.float32), np.array(w["mean"], np.float32),
+                        np.array(w["inv_std"], np.float32), engine=eid)
+        out[name] = w
+    return out
 
 
-def compare(G, srcs, path=WEIGHTS, repeats=3):
-    """Device time of k BFS (one launch) under the rule and the learned
-    policy, and whether their levels agree."""
-    load(G, path)
-    res = {}
+def _timed(G, srcs, engine):
+    """Device ms of the traversals of srcs in one call: the batched engine
+    (bfs_multi, shared traversals of up to 64 sources) or the one-source
+    engine (every source in turn, still one persistent launch)."""
+    if engine == "single":
+        os.environ["BYO_BFS_MODE"] = "sequential"
+    try:
+        G.profile(True)
+        G.profile_reset()
+        _, st = G.bfs_multi(srcs, want_parents=False)
+        _, ms = G.profile_get("bfs_launch")
+    finally:
+        if engine == "single":
+            del os.environ["BYO_BFS_MODE"]
+    return ms, st
+
+
+def compare(G, srcs, path=None, repeats=3, engine="batch"):
+    """Device time of the traversals of srcs under the rule and the learned
+    policy, and whether their levels agree (levels of every source are
+    derived from its parent tree on the device)."""
+    load(G, path, engine)
+    res = {"engine": engine}
     lv = {}
     for name, pol in (("threshold", DIR_THRESHOLD), ("learned", DIR_MLP)):
         G.set_direction_policy(pol)
-        G.bfs_multi(srcs, want_parents=False)
+        _timed(G, srcs, engine)
         best = float("inf")
         for _ in range(repeats):
-            G.profile(True)
-            G.profile_reset()
-            _, st = G.bfs_multi(srcs, want_parents=False)
-            _, ms = G.profile_get("bfs_launch")
+            ms, st = _timed(G, srcs, engine)
             best = min(best, ms)
         res[name] = best
-        res[name + "_dirs"] = st[0]["dir"]
+        res[name + "_dirs"] = st[0]["dir"][:max(s["levels"] for s in st)]
         lv[name] = []
         for s in srcs[:2]:
             G.bfs(s, want_parents=False)
             lv[name].append(G.bfs_levels())
+        lv[name].append([(x["levels"], x["reached"], x["edges_reached"]) for x in st])
     G.set_direction_policy(DIR_THRESHOLD)
     res["speedup"] = res["threshold"] / res["learned"]
-    res["levels_identical"] = all(np.array_equal(a, b) for a, b in zip(lv["threshold"], lv["learned"]))
+    res["levels_identical"] = all(np.array_equal(np.asarray(a), np.asarray(b))
+                                  for a, b in zip(lv["threshold"], lv["learned"]))
     return res
 
 
@@ -180,41 +227,50 @@ TRAIN_GRAPHS = [
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("cmd", choices=["train", "eval"])
-    ap.add_argument("--out", default=WEIGHTS)
-    ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--engine", choices=sorted(ENGINES), default="batch")
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--weights", default=None)
     ap.add_argument("--sources", type=int, default=8)
     ap.add_argument("--epochs", type=int, default=3000)
     args = ap.parse_args()
+    eng = args.engine
     if args.cmd == "train":
-        Xs, Ts = [], []
+        out = args.out or ENGINES[eng][1]
+        Xs, Ts, Rs = [], [], []
         t0 = time.time()
         for kind, scale, ef, (a, b, c) in TRAIN_GRAPHS:
             G = CsrGraph.rmat(scale, ef, a, b, c, seed=7 + scale)
-            srcs = sources(G.degrees(), args.sources, seed=100 + scale)
-            X, T = collect(G, srcs, repeats=2)
+            srcs = sources(G.degrees(), args.sources * (BATCH if eng == "batch" else 1), seed=100 + scale)
+            units = srcs if eng == "single" else [srcs[i:i + BATCH] for i in range(0, len(srcs), BATCH)]
+            X, T, R = collect(G, units, repeats=2, engine=eng)
             Xs.append(X)
             Ts.append(T)
+            Rs.append(R)
             G.close()
             print("collected %s-%d ef%d: %d samples (%.0fs)" % (kind, scale, ef, len(X), time.time() - t0),
                   flush=True)
         X = np.concatenate(Xs)
         T = np.concatenate(Ts)
+        R = np.concatenate(Rs)
         w = train(X, T, epochs=args.epochs)
+        w["engine"] = eng
         w["train_graphs"] = [list(map(str, g)) for g in TRAIN_GRAPHS]
         w["label_counts"] = np.bincount(np.argmin(T, axis=1), minlength=3).tolist()
-        with open(args.out, "w") as f:
+        # level time the rule's own choices cost against the per-level best
+        w["rule_time_ratio_vs_best"] = float(T[np.arange(len(R)), R].sum() / T.min(axis=1).sum())
+        with open(out, "w") as f:
             json.dump(w, f)
         print(json.dumps({k: v for k, v in w.items() if k not in ("params", "mean", "inv_std")}), flush=True)
     else:
         # held-out graphs: RMAT scale 24 ef 32, 256^3 grid, ER n=2^23 degree 8,
-        # and the bench graph (RMAT scale 22 ef 16)
+        # and the RMAT scale 22 ef 16 graph
         for name, make in (("rmat22_ef16", lambda: CsrGraph.rmat(22, 16)),
                            ("er23_deg8", lambda: CsrGraph.erdos_renyi(23, 4 << 23)),
                            ("grid256", lambda: CsrGraph.grid3d(256)),
                            ("rmat24_ef32", lambda: CsrGraph.rmat(24, 32))):
             G = make()
-            srcs = sources(G.degrees(), 4, seed=5)
-            res = compare(G, srcs, args.weights)
+            srcs = sources(G.degrees(), BATCH if eng == "batch" else 4, seed=5)
+            res = compare(G, srcs, args.weights, engine=eng)
             res["n"], res["m"] = G.N(), G.M()
             print(name, json.dumps(res), flush=True)
             G.close()

@@ -235,6 +235,15 @@ int byo_profile_reset(byo_graph_t g);
  * row-major) and the 6-feature normalisation (mean[6], inv_std[6]). */
 int byo_selector_load(byo_graph_t g, const float *params, const float *mean,
                       const float *inv_std);
+/* The same for one traversal engine: BYO_SEL_SINGLE is the one-source
+ * traversal (byo_bfs; byo_selector_load loads this one), BYO_SEL_BATCH the
+ * shared multi-source traversal of byo_bfs_multi, whose per-level costs
+ * differ and which is trained on its own timings.  An engine without
+ * weights falls back to the threshold rule under BYO_DIR_MLP. */
+#define BYO_SEL_SINGLE 0
+#define BYO_SEL_BATCH 1
+int byo_selector_load_engine(byo_graph_t g, int engine, const float *params,
+                             const float *mean, const float *inv_std);
 /* One training step on the device: forward, softmax cross-entropy,
  * backward and an Adam update over a minibatch x[B*6], labels[B].  The
  * parameter and moment state live in a device-side trainer object. */

@@ -49,24 +49,30 @@ def test_training_reduces_loss_and_predict(B):
     T.close()
 
 
-def test_learned_policy_is_bit_exact_and_not_slower(B):
-    """The committed selector (trained on the GPU on RMAT scale 18-22 timings)
-    keeps BFS levels bit-exact on graphs it never saw, and its direction
-    choices cost no more device time than the M/20 rule (the gain itself is
-    printed and recorded by bench.py)."""
+@pytest.mark.parametrize("engine", ["single", "batch"])
+def test_learned_policy_is_bit_exact_and_not_slower(B, engine):
+    """The committed selector of each traversal engine (trained on the GPU on
+    RMAT scale 18-22 timings of that engine) keeps BFS levels bit-exact on
+    graphs it never saw, and its direction choices cost no more device time
+    than the M/20 rule (the gain itself is printed and recorded by
+    bench.py)."""
     import os
     from byo_amd import selector as S
-    if not os.path.exists(S.WEIGHTS):
-        pytest.skip("no trained selector weights committed yet")
+    if not os.path.exists(S.ENGINES[engine][1]):
+        pytest.skip("no trained %s-engine selector weights committed" % engine)
     for G, g in ((B.CsrGraph.grid3d(48), O.grid3d_graph(48)), (B.CsrGraph.rmat(16), O.rmat_graph(16))):
         S.load(G)
         G.set_direction_policy(B.DIR_MLP)
-        for s in O.bfs_sources(g, 3, seed=21):
+        srcs = O.bfs_sources(g, 8, seed=21)
+        for s in srcs[:3]:
             par, st = G.bfs(s)
             lv = O.bfs_levels(g, s)
             assert np.array_equal(G.bfs_levels(), lv)
             assert O.check_bfs_tree(g, s, par, lv) == 0
-        res = S.compare(G, O.bfs_sources(g, 4, seed=22))
-        print("selector gain", res["speedup"], res["learned_dirs"], res["threshold_dirs"])
+        par, _ = G.bfs_multi(srcs)
+        for j, s in enumerate(srcs):
+            assert O.check_bfs_tree(g, s, par[j], O.bfs_levels(g, s)) == 0
+        res = S.compare(G, O.bfs_sources(g, 8 if engine == "batch" else 4, seed=22), engine=engine)
+        print("selector gain", engine, res["speedup"], res["learned_dirs"], res["threshold_dirs"])
         assert res["levels_identical"]
         assert res["speedup"] > 0.8
