@@ -77,6 +77,7 @@ struct BatchArgs {
   uint32_t *nlev;           // levels of the whole traversal
   unsigned long long *tot;  // k x {vertices reached, sum of their degrees}
   DirCtl dc;
+  uint64_t *pieces;         // kPieces * nwaves + 1 word bounds
   GridBar *bar;
 };
 
@@ -174,6 +175,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
 
   // logs[64] holds wall-clock marks: start, seeded, traversal end, tallied
   if (leader) a.logs[64].fsize = wall_clock64();
+  pieces_init(a.pieces, a.off, a.n, nwords, nwaves);
   // ---- seed: parents cleared, masks cleared, then every source's bit set
   for (uint32_t j = 0; j < a.k; j++) {
     uint32_t *par = a.parents + (uint64_t)j * a.pstride;
@@ -441,7 +443,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
       // the hit masks gives each bit's first neighbour in adjacency order.
       const int g = lane >> 3, sub = lane & 7;
       unsigned long long cnt = 0;
-      for (uint64_t w = wave; w < nwords; w += nwaves) {
+      for_my_words(a.pieces, wave, nwaves, [&](uint64_t w) {
         const uint64_t vl = w * 64 + lane;
         uint64_t bl = 0, el = 0;
         M s = (M)~(M)0;
@@ -577,14 +579,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
         cnt += __popcll(__ballot(acc != 0));
         if (acc) dsum += el - bl;
         won |= acc;
-      }
+      });
       if (lane == 0 && cnt) atomicAdd(&nx->size, cnt);
       is_dense = true;
       prev_have_q = have_q;
       have_q = false;
     } else {
       // dense forward: one wave per frontier vertex of each 64-vertex word
-      for (uint64_t w = wave; w < nwords; w += nwaves) {
+      for_my_words(a.pieces, wave, nwaves, [&](uint64_t w) {
         const uint64_t vl = w * 64 + lane;
         const M fv = vl < a.n ? fmask[vl] : (M)0;
         uint64_t word = __ballot(fv != 0);
@@ -612,7 +614,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
             }
           }
         }
-      }
+      });
       is_dense = true;
       prev_have_q = have_q;
       have_q = true;
@@ -638,14 +640,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
   // combine lanes in LDS and add their totals with one atomic per source.
   {
     unsigned long long rc = 0, rd = 0;
-    for (uint64_t w = wave; w < nwords; w += nwaves) {
+    for_my_words(a.pieces, wave, nwaves, [&](uint64_t w) {
       const uint64_t vl = w * 64 + lane;
       uint64_t sm = 0, d = 0;
       if (vl < a.n) {
         sm = (uint64_t)seen[vl];
         d = a.off[vl + 1] - a.off[vl];
       }
-      if (!__ballot(sm != 0)) continue;
+      if (!__ballot(sm != 0)) return;
       res[wv][lane] = sm;
       win.sc[wv][lane] = d;
       wave_lds_sync();
@@ -658,7 +660,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
         }
       }
       wave_lds_sync();
-    }
+    });
     if (threadIdx.x < 64) {
       tl[0][threadIdx.x] = 0;
       tl[1][threadIdx.x] = 0;
@@ -727,6 +729,7 @@ void bfs_batch_run(byo_graph *g, const uint32_t *srcs, uint32_t k, byo_bfs_stats
   a.nlev = g->buf("bfs_batch_nlev", 64).as<uint32_t>();
   a.tot = g->buf("bfs_batch_tot", 64 * 16 + 64).as<unsigned long long>();
   a.dc = dir_ctl(g, maxdeg, s, true);
+  a.pieces = g->buf("bfs_pieces", (kPieces * 2048 * 4 + 1) * 8 + 64).as<uint64_t>();
   a.bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
   BYO_HIP(hipMemsetAsync(a.bar, 0, sizeof(GridBar), s));
   BYO_HIP(hipMemcpyAsync(dsrc, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s));

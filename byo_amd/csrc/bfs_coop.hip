@@ -76,6 +76,7 @@ struct CoopArgs {
   uint32_t *nlevels;        // k
   unsigned long long *tot;  // k x {vertices reached, sum of their degrees}
   DirCtl dc;
+  uint64_t *pieces;         // kPieces * nwaves + 1 word bounds
   GridBar *bar;
 };
 
@@ -104,6 +105,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
   const uint64_t nwords = (a.n + 63) >> 6;
   const uint64_t nbytes = (a.n + 7) >> 3;
 
+  pieces_init(a.pieces, a.off, a.n, nwords, nwaves);
   for (uint32_t j = 0; j < a.k; j++) {
     uint32_t *parents = a.parents + (uint64_t)j * a.pstride;
     const uint32_t src = a.srcs[j];
@@ -350,7 +352,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
         // unresolved continue 8 neighbours per step, 8 vertices at a time.
         const int g = lane >> 3, sub = lane & 7;
         unsigned long long cnt = 0, dsum = 0;
-        for (uint64_t w = wave; w < nwords; w += nwaves) {
+        for_my_words(a.pieces, wave, nwaves, [&](uint64_t w) {
           const uint64_t vis = a.visited64[w];
           const uint64_t vl = w * 64 + lane;
           uint64_t bl = 0, el = 0;
@@ -435,7 +437,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
           }
           cnt += __popcll(fmask);
           if ((fmask >> lane) & 1) dsum += el - bl;
-        }
+        });
         dsum = wave_sum(dsum);
         if (lane == 0 && cnt) {
           atomicAdd(&nx->size, cnt);
@@ -446,7 +448,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
         have_q = false;
       } else {
         // dense forward: one wave per frontier vertex of each bitmap word
-        for (uint64_t w = wave; w < nwords; w += nwaves) {
+        for_my_words(a.pieces, wave, nwaves, [&](uint64_t w) {
           uint64_t word = fbm[w];
           while (word) {
             const int bp = __builtin_ctzll(word);
@@ -478,7 +480,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
               }
             }
           }
-        }
+        });
         grid_barrier(a.bar, (unsigned)nb);
         is_dense = true;
         have_q = true;
@@ -564,6 +566,7 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   a.nlevels = g->buf("bfs_coop_nlev", (size_t)k * 4 + 64).as<uint32_t>();
   a.tot = g->buf("bfs_coop_tot", (size_t)k * 16 + 64).as<unsigned long long>();
   a.dc = dir_ctl(g, maxdeg, s, false);
+  a.pieces = g->buf("bfs_pieces", (kPieces * 2048 * 4 + 1) * 8 + 64).as<uint64_t>();
   const int nblocks = coop_grid(reinterpret_cast<const void *>(k_bfs_coop), kBlock);
   a.bsum = g->buf("bfs_coop_bsum", (size_t)nblocks * 8 + 64).as<unsigned long long>();
   a.bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();

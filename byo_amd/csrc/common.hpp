@@ -16,6 +16,16 @@
 namespace byo {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+// Bound on the selector's normalised features, in standard deviations of
+// the training set (the trainer applies the same clamp).
+constexpr float kSelClamp = 3.0f;
+// The learned selector overrides the threshold rule only for frontiers whose
+// features all lie within kSelClamp deviations of the training set and when
+// its logit for another direction beats the rule's by at least kSelMargin.
+constexpr float kSelMargin = 1.0f;
+
+// Selector class (0 push, 1 pull, 2 forward) of a direction 0, 1 or 3.
+__host__ __device__ inline int dir_class(int dir) { return dir == 3 ? 2 : (dir == 1 ? 1 : 0); }
 constexpr int kWave = 64;
 
 // ---------------------------------------------------------------- errors

@@ -3,6 +3,7 @@
 // traversal): the grid barrier, block reductions and the on-device direction
 // decision (rule of gbbs/edge_map_data.h:257-306 or the learned selector).
 #pragma once
+#include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
@@ -70,6 +71,48 @@ __device__ __forceinline__ unsigned long long block_sum(unsigned long long v, un
   return s;
 }
 
+// Share-out of the vertex-driven passes (pull, forward, tallies) over
+// 64-vertex words.  The words are cut into kPieces pieces per wave of equal
+// weight (edges + 8 per vertex, read off the offsets), and wave w takes
+// pieces w, w + nwaves, ..., one from each 1/kPieces of the id space.  Equal
+// weights balance the passes that scan the whole graph; drawing each wave's
+// pieces from every region of the ids averages out the regional skew of the
+// later levels, whose work sits in whatever vertices are still unvisited.
+// (Dealing single words out round-robin lines RMAT's heavy low-id words up
+// on the same few waves; claiming words through atomics costs a round trip
+// per word.)
+constexpr uint64_t kPieces = 8;
+
+// Piece bounds, computed once per launch by the whole grid before its first
+// barrier: bound[i] = first word w with W(w) >= i * W(nwords) / P, where
+// W(w) = off[v] + 8 v, v = min(64 w, n), and P = kPieces * nwaves.
+__device__ inline void pieces_init(uint64_t *bound, const uint64_t *off, uint64_t n, uint64_t nwords,
+                                   uint64_t nwaves) {
+  const uint64_t P = kPieces * nwaves;
+  const uint64_t total = off[n] + 8 * n;
+  for (uint64_t i = global_tid(); i <= P; i += grid_threads()) {
+    const uint64_t target = (uint64_t)(((unsigned __int128)total * i) / P);
+    uint64_t lo = 0, hi = nwords;  // first w in [lo, hi] with W(w) >= target
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      const uint64_t v = std::min<uint64_t>(mid * 64, n);
+      if (off[v] + 8 * v >= target) hi = mid; else lo = mid + 1;
+    }
+    bound[i] = (i == P) ? nwords : lo;
+  }
+}
+
+// Runs body(w) for every word of this wave's pieces; all 64 lanes call it
+// together and body runs wave-uniformly.
+template <class F>
+__device__ __forceinline__ void for_my_words(const uint64_t *bound, uint64_t wave, uint64_t nwaves, F &&body) {
+  for (uint64_t i = 0; i < kPieces; i++) {
+    const uint64_t p = wave + i * nwaves;
+    const uint64_t w1 = bound[p + 1];
+    for (uint64_t w = bound[p]; w < w1; w++) body(w);
+  }
+}
+
 // Inputs of the per-level direction decision.
 struct DirCtl {
   uint64_t n, m;
@@ -82,6 +125,7 @@ struct DirCtl {
 
 struct MlpLds {
   float x[6], h1[32], h2[32], z[3];
+  bool in_range;
 };
 
 // Direction of one edgeMap call: 0 sparse, 1 dense pull, 2 empty, 3 dense
@@ -116,7 +160,13 @@ __device__ inline int decide(const DirCtl &a, bool is_dense, uint64_t fs, uint64
     x[3] = (float)level;
     x[4] = (float)log2(1.0 + avg);
     x[5] = (float)log2(1.0 + (double)a.maxdeg);
-    for (int i = 0; i < 6; i++) s.x[i] = (x[i] - p[kNP + i]) * p[kNP + 6 + i];
+    bool in_range = true;
+    for (int i = 0; i < 6; i++) {
+      const float v = (x[i] - p[kNP + i]) * p[kNP + 6 + i];
+      in_range = in_range && fabsf(v) <= kSelClamp;
+      s.x[i] = fminf(fmaxf(v, -kSelClamp), kSelClamp);
+    }
+    s.in_range = in_range;
   }
   __syncthreads();
   if (t < 32) {
@@ -140,6 +190,9 @@ __device__ inline int decide(const DirCtl &a, bool is_dense, uint64_t fs, uint64
   const float z0 = s.z[0], z1 = s.z[1], z2 = s.z[2];
   if (!isfinite(z0 + z1 + z2)) return rule;
   const int best = (z1 > z0) ? ((z2 > z1) ? 2 : 1) : ((z2 > z0) ? 2 : 0);
+  // the rule stands unless the frontier is in the training range and the
+  // network prefers another direction by at least kSelMargin
+  if (!s.in_range || s.z[best] - s.z[dir_class(rule)] < kSelMargin) return rule;
   return best == 0 ? 0 : (best == 1 ? 1 : 3);
 }
 

@@ -10,6 +10,7 @@
 //
 // Parameter layout (row-major, out x in): W1[32x6] b1[32] W2[32x32] b2[32]
 // W3[3x32] b3[3] = 1379 floats.
+#include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
@@ -20,10 +21,18 @@ constexpr int kNP = BYO_MLP_NPARAM;
 constexpr int oW1 = 0, ob1 = 192, oW2 = 224, ob2 = 1248, oW3 = 1280, ob3 = 1376;
 static_assert(ob3 + 3 == kNP, "layout");
 
-void mlp_eval_host(const std::vector<float> &p, const float *xin, float *z) {
+bool mlp_eval_host(const std::vector<float> &p, const float *xin, float *z) {
   // p holds params followed by mean[6] and inv_std[6]
   float x[6];
-  for (int i = 0; i < 6; i++) x[i] = (xin[i] - p[kNP + i]) * p[kNP + 6 + i];
+  // normalised features are clamped to +-kSelClamp standard deviations, so
+  // graphs unlike the training set cannot drive the network far outside
+  // the region it was fitted on; the return value says none was clamped
+  bool in_range = true;
+  for (int i = 0; i < 6; i++) {
+    const float v = (xin[i] - p[kNP + i]) * p[kNP + 6 + i];
+    in_range = in_range && std::fabs(v) <= kSelClamp;
+    x[i] = std::min(std::max(v, -kSelClamp), kSelClamp);
+  }
   float h1[32], h2[32];
   for (int o = 0; o < 32; o++) {
     float a = p[ob1 + o];
@@ -40,6 +49,7 @@ void mlp_eval_host(const std::vector<float> &p, const float *xin, float *z) {
     for (int i = 0; i < 32; i++) a += p[oW3 + o * 32 + i] * h2[i];
     z[o] = a;
   }
+  return in_range;
 }
 
 namespace {

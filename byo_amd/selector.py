@@ -99,10 +99,15 @@ def collect(G, units, repeats=1, max_levels=64, engine="single"):
             for d in CLASS_TO_DIR:
                 sched = [DIR_RULE] * lv + [d]
                 G.set_direction_schedule(sched)
+                # the label is the time from this level to the end of the
+                # traversal: a direction also changes what the next level
+                # starts from (a pull leaves no queue, so a push after it
+                # packs one first)
                 best = float("inf")
                 for _ in range(repeats):
                     s2 = _run(G, u, engine)
-                    best = min(best, s2["level_ms"][lv])
+                    end = min(max(s2["levels"], int(np.count_nonzero(s2["fsize"]))), 64)
+                    best = min(best, float(np.sum(s2["level_ms"][lv:end])))
                 times.append(best)
             X.append(features(st["fsize"][lv], st["fdeg"][lv], vdeg, lv, n, m, maxdeg))
             T.append(times)
@@ -121,6 +126,13 @@ def init_params(seed=0):
     return p
 
 
+CLAMP = 3.0  # normalised features are clamped to +-CLAMP (kSelClamp on the device)
+
+
+def normalise(X, mean, inv_std):
+    return np.clip((X - mean) * inv_std, -CLAMP, CLAMP).astype(np.float32)
+
+
 def train(X, T, epochs=3000, lr=3e-3, seed=0):
     """Cost-sensitive training: every sample is repeated in proportion to how
     much the wrong choice would cost (relative gap between worst and best
@@ -133,13 +145,13 @@ def train(X, T, epochs=3000, lr=3e-3, seed=0):
     mean = Xr.mean(axis=0)
     std = Xr.std(axis=0)
     inv_std = (1.0 / np.where(std > 1e-6, std, 1.0)).astype(np.float32)
-    Xn = ((Xr - mean) * inv_std).astype(np.float32)
+    Xn = normalise(Xr, mean, inv_std)
     tr = Trainer(init_params(seed), lr=lr)
     losses = []
     for e in range(epochs):
         losses.append(tr.step(Xn, yr))
     params = tr.params()
-    pred = tr.predict(((X - mean) * inv_std).astype(np.float32))
+    pred = tr.predict(normalise(X, mean, inv_std))
     tr.close()
     acc = float((pred == y).mean())
     chosen = T[np.arange(len(y)), pred].sum()
@@ -232,6 +244,7 @@ def main():
     ap.add_argument("--weights", default=None)
     ap.add_argument("--sources", type=int, default=8)
     ap.add_argument("--epochs", type=int, default=3000)
+    ap.add_argument("--dump", default=None, help="also write the samples (X, T, R) to this .npz")
     args = ap.parse_args()
     eng = args.engine
     if args.cmd == "train":
@@ -252,6 +265,8 @@ def main():
         X = np.concatenate(Xs)
         T = np.concatenate(Ts)
         R = np.concatenate(Rs)
+        if args.dump:
+            np.savez(args.dump, X=X, T=T, R=R)
         w = train(X, T, epochs=args.epochs)
         w["engine"] = eng
         w["train_graphs"] = [list(map(str, g)) for g in TRAIN_GRAPHS]
