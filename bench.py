@@ -2,14 +2,18 @@
 
 Workload (one "step"): direction-optimizing BFS from 8 seeded sources over a
 device-resident RMAT scale-22 static CSR (a=.5 b=.1 c=.1, edge factor 16,
-symmetrised, deduplicated, uint32 ids).  `value` is BFS GTEPS over the timed
-steps (Graph500 convention: undirected edges reached / time).  The same JSON
-line carries FP32-Kahan and FP64 PageRank it/s (20 iterations) with their L1
-difference, CC and TC times, CPMA insert/remove edges/s at batch 1e7, the
-roofline of the BFS traversal, and the CPU baseline (the oracle's OpenMP
-direction-optimizing BFS on the host, on a bounded sample of sources).
+symmetrised, deduplicated, uint32 ids), all 8 in one shared traversal (one
+persistent launch) whose per-level direction is chosen by the learned
+selector.  `value` is BFS GTEPS over the timed steps (Graph500 convention:
+undirected edges reached / time).  The same JSON line carries the same
+workload under the M/20 threshold rule (learned vs threshold), the
+one-source-at-a-time engine, FP32-Kahan and FP64 PageRank it/s (20
+iterations) with their L1 difference, CC and TC times, CPMA insert/remove
+edges/s at batch 1e7, the roofline of the BFS launch, and the CPU baseline
+(the oracle's OpenMP direction-optimizing BFS on the host, on a bounded
+sample of sources).
 
-    python bench.py [--steps K] [--warmup W] [--scale 22] [--cpma-scale 22]
+    python bench.py [--steps K] [--warmup W] [--scale 22] [--cpma-scale 24]
 """
 import argparse
 import json
@@ -67,18 +71,19 @@ def cpu_baseline(G, sources, budget_s):
     cores = os.cpu_count()
     done, t_total, edges = 0, 0.0, 0
     O.bfs_diropt(g, sources[0])  # warm (page in)
-    for s in sources:
+    # the sources are traversed one after another, in passes over the list,
+    # until budget_s of CPU time has been spent
+    while t_total < budget_s and done < 1000 * len(sources):
+        s = sources[done % len(sources)]
         t0 = time.perf_counter()
         par, _, _, fd = O.bfs_diropt(g, s)
         dt = time.perf_counter() - t0
         t_total += dt
         edges += int(fd.sum()) // 2
         done += 1
-        if t_total > budget_s:
-            break
     return {"value": edges / t_total / 1e9, "unit": "GTEPS", "cores": cores, "kind": "port",
-            "sample": "oracle OpenMP diropt BFS (M/20 rule) on the same RMAT-%d graph, "
-                      "%d of %d sources, %.2f s" % (int(np.log2(g.n)), done, len(sources), t_total)}
+            "sample": "oracle OpenMP diropt BFS (M/20 rule) on the same RMAT-%d graph: %d traversals "
+                      "(passes over the %d sources), %.1f s" % (int(np.log2(g.n)), done, len(sources), t_total)}
 
 
 def main():
@@ -93,7 +98,7 @@ def main():
     ap.add_argument("--cpma-batch", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -113,32 +118,50 @@ def main():
     print("# graph RMAT-%d ef%d: n=%d m=%d (built on device in %.1fs)" % (args.scale, args.ef, n, m,
                                                                           time.time() - t), file=sys.stderr)
 
-    for _ in range(args.warmup):
-        G.bfs_multi(srcs, want_parents=False)
-    G.profile(True)
-    G.profile_reset()
-    sync()
-    t0 = time.perf_counter()
-    edges = 0
-    bytes_total = 0
-    for _ in range(args.steps):
-        _, stats = G.bfs_multi(srcs, want_parents=False)
-        for st in stats:
-            edges += st["edges_reached"] // 2
-            bytes_total += bfs_bytes(n, st["reached"], st["edges_reached"])
-    sync()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    # the dominant kernel is the persistent BFS launch (all sources of a
-    # step in one k_bfs_coop dispatch), timed with HIP events on its stream
-    nlaunch, launch_ms = G.profile_get("bfs_launch")
+    from byo_amd import selector as S
+    learned = bool(S.load(G).get("batch"))
+    policy = B.DIR_MLP if learned else B.DIR_THRESHOLD
+
+    def timed_bfs(pol, steps, mode=None):
+        """K steps of the 8-source BFS under a direction policy; returns
+        (GTEPS by the wall clock, launches, launch ms, algorithmic bytes,
+        stats of the last step)."""
+        G.set_direction_policy(pol)
+        if mode:
+            os.environ["BYO_BFS_MODE"] = mode
+        try:
+            for _ in range(args.warmup):
+                G.bfs_multi(srcs, want_parents=False)
+            G.profile(True)
+            G.profile_reset()
+            sync()
+            t0 = time.perf_counter()
+            edges = 0
+            nbytes = 0
+            for _ in range(steps):
+                _, stats = G.bfs_multi(srcs, want_parents=False)
+                for st in stats:
+                    edges += st["edges_reached"] // 2
+                    nbytes += bfs_bytes(n, st["reached"], st["edges_reached"])
+            sync()
+            wall = time.perf_counter() - t0
+            nl, ms = G.profile_get("bfs_launch")
+        finally:
+            if mode:
+                del os.environ["BYO_BFS_MODE"]
+            G.set_direction_policy(B.DIR_THRESHOLD)
+        return edges / wall / 1e9, wall, nl, ms, nbytes, stats
+
+    gteps, wall, nlaunch, launch_ms, bytes_total, stats = timed_bfs(policy, args.steps)
+    thr_gteps, _, thr_nl, thr_ms, _, thr_stats = timed_bfs(B.DIR_THRESHOLD, args.steps)
+    seq_gteps = timed_bfs(B.DIR_THRESHOLD, 1, "sequential")[0]
+    # the dominant kernel is the persistent BFS launch (all sources of a step
+    # in one k_bfs_batch dispatch), timed with HIP events on its stream
     avg_ms = launch_ms / max(nlaunch, 1)
-    gteps = edges / wall / 1e9
     avg_bytes = bytes_total / max(nlaunch, 1)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-    dirs = stats[0]["dir"]
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_bfs_coop_rmat22_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "k_bfs_batch_rmat22_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -156,12 +179,22 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic RMAT generated on the device (seed 1)",
-        "config": {"workload": "BFS x%d sources, RMAT scale %d ef %d (a=.5,b=.1,c=.1), symmetric CSR, "
-                               "direction rule M/20" % (args.sources, args.scale, args.ef),
-                   "n": n, "m": m, "sources": srcs, "levels_dirs_src0": dirs},
+        "config": {"workload": "BFS x%d sources in one shared traversal, RMAT scale %d ef %d (a=.5,b=.1,c=.1), "
+                               "symmetric CSR, direction by the %s" % (
+                                   args.sources, args.scale, args.ef,
+                                   "learned selector" if learned else "M/20 rule (no selector weights)"),
+                   "n": n, "m": m, "sources": srcs, "levels_dirs": stats[0]["dir"][:max(s["levels"] for s in stats)]},
+        "learned_vs_threshold": {"learned_gteps": round(gteps, 3), "threshold_gteps": round(thr_gteps, 3),
+                                 "speedup": round(gteps / thr_gteps, 4),
+                                 "launch_ms_learned": round(avg_ms, 4),
+                                 "launch_ms_threshold": round(thr_ms / max(thr_nl, 1), 4),
+                                 "threshold_dirs": thr_stats[0]["dir"][:max(s["levels"] for s in thr_stats)],
+                                 "levels_identical": [(a["levels"], a["reached"], a["edges_reached"]) for a in stats]
+                                 == [(a["levels"], a["reached"], a["edges_reached"]) for a in thr_stats]},
+        "one_source_engine_gteps": round(seq_gteps, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_bfs_coop (one launch = %d BFS)" % len(srcs), "avg_ms": round(avg_ms, 4),
+                     "kernel": "k_bfs_batch<uint8_t> (one launch = %d BFS)" % len(srcs), "avg_ms": round(avg_ms, 4),
                      "algorithmic_bytes": int(avg_bytes)},
     }
 

@@ -428,6 +428,7 @@ __global__ void k_max_deg(const uint64_t *off, uint64_t n, unsigned long long *o
 }
 }  // namespace
 static uint64_t max_degree(byo_graph *g) {
+  if (g->maxdeg_valid) return g->maxdeg;
   auto &b = g->buf("maxdeg", 64);
   BYO_HIP(hipMemsetAsync(b.p, 0, 8, g->stream));
   if (g->n) k_max_deg<<<byo_grid(g->n, 256, 4096), 256, 0, g->stream>>>(g->off.as<uint64_t>(), g->n,
@@ -436,7 +437,9 @@ static uint64_t max_degree(byo_graph *g) {
   uint64_t *h = static_cast<uint64_t *>(pinned(g, 64));
   BYO_HIP(hipMemcpyAsync(h, b.p, 8, hipMemcpyDeviceToHost, g->stream));
   BYO_HIP(hipStreamSynchronize(g->stream));
-  return h[0];
+  g->maxdeg = h[0];
+  g->maxdeg_valid = true;
+  return g->maxdeg;
 }
 }  // namespace byo
 
@@ -501,9 +504,7 @@ int byo_bfs_multi(byo_graph_t g, const uint32_t *srcs, uint32_t k, uint32_t *par
         }
         BYO_HIP(hipStreamSynchronize(g->stream));
       }
-      g->scratch["bfs_last_src"].ensure(8);
-      BYO_HIP(hipMemcpyAsync(g->scratch["bfs_last_src"].p, &srcs[k - 1], 4, hipMemcpyHostToDevice, g->stream));
-      BYO_HIP(hipStreamSynchronize(g->stream));
+      g->last_src = srcs[k - 1];
       return;
     }
     if (!levels_mode) {
@@ -528,9 +529,7 @@ int byo_bfs_multi(byo_graph_t g, const uint32_t *srcs, uint32_t k, uint32_t *par
       } else if (parents) {
         BYO_HIP(hipMemcpyAsync(parents, g->scratch["bfs_parents"].p, n * 4, hipMemcpyDeviceToHost, g->stream));
       }
-      g->scratch["bfs_last_src"].ensure(8);
-      BYO_HIP(hipMemcpyAsync(g->scratch["bfs_last_src"].p, &srcs[k - 1], 4, hipMemcpyHostToDevice, g->stream));
-      BYO_HIP(hipStreamSynchronize(g->stream));
+      g->last_src = srcs[k - 1];
       return;
     }
     BfsRun r = bfs_setup(g);
@@ -548,9 +547,8 @@ int byo_bfs_multi(byo_graph_t g, const uint32_t *srcs, uint32_t k, uint32_t *par
         BYO_HIP(hipStreamSynchronize(g->stream));
       }
     }
-    g->scratch["bfs_last_src"].ensure(8);
-    BYO_HIP(hipMemcpyAsync(g->scratch["bfs_last_src"].p, &srcs[k - 1], 4, hipMemcpyHostToDevice, g->stream));
     BYO_HIP(hipStreamSynchronize(g->stream));
+    g->last_src = srcs[k - 1];
   });
 }
 
@@ -563,10 +561,8 @@ int byo_bfs_levels(byo_graph_t g, uint32_t *levels) {
     BYO_REQUIRE(g && levels, BYO_E_INVAL, "null argument");
     std::lock_guard<std::mutex> lk(g->mu);
     bind(g);
-    auto it = g->scratch.find("bfs_last_src");
-    BYO_REQUIRE(it != g->scratch.end(), BYO_E_STATE, "no BFS has run on this handle");
-    uint32_t src = 0;
-    BYO_HIP(hipMemcpy(&src, it->second.p, 4, hipMemcpyDeviceToHost));
+    BYO_REQUIRE(g->last_src >= 0, BYO_E_STATE, "no BFS has run on this handle");
+    const uint32_t src = (uint32_t)g->last_src;
     const uint64_t n = g->n;
     uint32_t *parents = g->scratch["bfs_parents"].as<uint32_t>();
     uint32_t *anc = g->buf("lv_anc", n * 4 + 64).as<uint32_t>();

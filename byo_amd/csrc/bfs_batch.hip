@@ -721,18 +721,27 @@ void bfs_batch_run(byo_graph *g, const uint32_t *srcs, uint32_t k, byo_bfs_stats
   a.scan = g->buf("bfs_scan", (n + 1) * 8 + 64).as<uint64_t>();
   a.cstart = g->buf("bfs_cstart", (g->m / kChunk + 2) * 4 + 64).as<uint32_t>();
   a.ctr = g->buf("bfs_batch_ctr", 3 * sizeof(BCtr) + 64).as<BCtr>();
-  uint32_t *dsrc = g->buf("bfs_batch_srcs", 64 * 4 + 64).as<uint32_t>();
-  a.srcs = dsrc;
   a.k = k;
-  a.logs = g->buf("bfs_batch_logs", 65 * sizeof(BLog) + 64).as<BLog>();
-  a.lastlev = g->buf("bfs_batch_last", 64 * 4 + 64).as<uint32_t>();
-  a.nlev = g->buf("bfs_batch_nlev", 64).as<uint32_t>();
-  a.tot = g->buf("bfs_batch_tot", 64 * 16 + 64).as<unsigned long long>();
+  // the per-launch results (level logs, last levels, level count, tallies)
+  // sit in one block so they come back with one copy
+  constexpr size_t kLogBytes = 65 * sizeof(BLog), kLastOff = kLogBytes, kNlevOff = kLastOff + 64 * 4,
+                   kTotOff = kNlevOff + 8, kOutBytes = kTotOff + 64 * 16;
+  char *out = g->buf("bfs_batch_out", kOutBytes + 64).as<char>();
+  a.logs = reinterpret_cast<BLog *>(out);
+  a.lastlev = reinterpret_cast<uint32_t *>(out + kLastOff);
+  a.nlev = reinterpret_cast<uint32_t *>(out + kNlevOff);
+  a.tot = reinterpret_cast<unsigned long long *>(out + kTotOff);
   a.dc = dir_ctl(g, maxdeg, s, true);
   a.pieces = g->buf("bfs_pieces", (kPieces * 2048 * 4 + 1) * 8 + 64).as<uint64_t>();
   a.bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
+  // pinned staging: sources at 0, results at 256 (one allocation, sized
+  // before any copy is queued)
+  char *hp = static_cast<char *>(pinned(g, 256 + kOutBytes));
+  std::memcpy(hp, srcs, (size_t)k * 4);
+  uint32_t *dsrc = g->buf("bfs_batch_srcs", 64 * 4 + 64).as<uint32_t>();
+  a.srcs = dsrc;
   BYO_HIP(hipMemsetAsync(a.bar, 0, sizeof(GridBar), s));
-  BYO_HIP(hipMemcpyAsync(dsrc, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s));
+  BYO_HIP(hipMemcpyAsync(dsrc, hp, (size_t)k * 4, hipMemcpyHostToDevice, s));
   double ms;
   if (k <= 8)
     ms = launch<uint8_t>(g, a, s);
@@ -745,15 +754,13 @@ void bfs_batch_run(byo_graph *g, const uint32_t *srcs, uint32_t k, byo_bfs_stats
     int wall_khz = 0;
     BYO_HIP(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, g->device));
     const double wall_hz = wall_khz > 0 ? wall_khz * 1e3 : 1e8;
-    std::vector<BLog> logs(65);
-    std::vector<uint32_t> last(64);
-    uint32_t nlev = 0;
-    std::vector<unsigned long long> tot(2 * (size_t)k);
-    BYO_HIP(hipMemcpyAsync(tot.data(), a.tot, tot.size() * 8, hipMemcpyDeviceToHost, s));
-    BYO_HIP(hipMemcpyAsync(logs.data(), a.logs, logs.size() * sizeof(BLog), hipMemcpyDeviceToHost, s));
-    BYO_HIP(hipMemcpyAsync(last.data(), a.lastlev, 64 * 4, hipMemcpyDeviceToHost, s));
-    BYO_HIP(hipMemcpyAsync(&nlev, a.nlev, 4, hipMemcpyDeviceToHost, s));
+    char *ho = hp + 256;
+    BYO_HIP(hipMemcpyAsync(ho, out, kOutBytes, hipMemcpyDeviceToHost, s));
     BYO_HIP(hipStreamSynchronize(s));
+    const BLog *logs = reinterpret_cast<const BLog *>(ho);
+    const uint32_t *last = reinterpret_cast<const uint32_t *>(ho + kLastOff);
+    const uint32_t nlev = *reinterpret_cast<const uint32_t *>(ho + kNlevOff);
+    const unsigned long long *tot = reinterpret_cast<const unsigned long long *>(ho + kTotOff);
     if (g->prof.on) {
       const BLog &mk = logs[64];
       g->account("bfs_batch_seed", (double)(mk.fdeg - mk.fsize) / wall_hz * 1e3);

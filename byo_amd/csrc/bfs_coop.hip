@@ -514,7 +514,7 @@ DirCtl dir_ctl(byo_graph *g, uint64_t maxdeg, hipStream_t s, bool batched) {
   }
   d.maxdeg = maxdeg;
   uint8_t *dsched = g->buf("bfs_coop_sched", 64).as<uint8_t>();
-  BYO_HIP(hipMemcpyAsync(dsched, g->dir_sched, 64, hipMemcpyHostToDevice, s));
+  if (g->dir_policy == BYO_DIR_SCHEDULE) BYO_HIP(hipMemcpyAsync(dsched, g->dir_sched, 64, hipMemcpyHostToDevice, s));
   d.sched = dsched;
   return d;
 }

@@ -150,6 +150,9 @@ struct byo_graph {
   byo::DevBuf off;  // uint64[n+1]
   byo::DevBuf ids;  // uint32[m]
   bool csr_valid = false;
+  uint64_t maxdeg = 0;        // maximum degree of the current CSR view
+  bool maxdeg_valid = false;  // cleared whenever the CSR view is rebuilt
+  int64_t last_src = -1;      // last source of byo_bfs / byo_bfs_multi
 
   // CPMA state (kind 1); owned by cpma.hip.
   void *cpma = nullptr;

@@ -327,6 +327,7 @@ static void cpma_build(byo_graph *g, const uint64_t *flat, uint64_t m) {
     k_degrees_from_off<<<byo_grid(g->n, 256, 16384), 256, 0, s>>>(g->off.as<uint64_t>(), g->n, c->deg.as<uint32_t>());
   BYO_LAUNCH_CHECK();
   g->csr_valid = true;
+  g->maxdeg_valid = false;
 }
 
 // Compacts the leaves into flat[total] (sorted, unique).
@@ -354,6 +355,7 @@ void cpma_ensure_csr(byo_graph *g) {
   prim::keys_to_csr(flat, c->total, g->n, g->off.as<uint64_t>(), g->ids.as<uint32_t>(), s);
   BYO_HIP(hipStreamSynchronize(s));
   g->csr_valid = true;
+  g->maxdeg_valid = false;
 }
 
 void cpma_destroy(byo_graph *g) {
@@ -446,7 +448,10 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
       BYO_HIP(hipStreamSynchronize(s));
       c->total += hctl->changed;
       g->m = c->total;
-      if (hctl->changed) g->csr_valid = false;
+      if (hctl->changed) {
+        g->csr_valid = false;
+        g->maxdeg_valid = false;
+      }
     } else {
       // global rebalance: flatten, merge with the batch, dedupe, respread
       c->rebalances++;
@@ -473,7 +478,10 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
     BYO_HIP(hipStreamSynchronize(s));
     c->total -= hctl->changed;
     g->m = c->total;
-    if (hctl->changed) g->csr_valid = false;
+    if (hctl->changed) {
+        g->csr_valid = false;
+        g->maxdeg_valid = false;
+      }
     if (c->nleaves > 1 && (double)c->total < kMinDensity * (double)(c->nleaves * kLeaf)) {
       c->rebalances++;
       uint64_t *flat = g->buf("cpma_flat", c->total * 8 + 64).as<uint64_t>();

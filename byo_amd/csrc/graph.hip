@@ -215,6 +215,7 @@ void csr_from_keys(byo_graph *g, uint64_t *keys, uint64_t *alt, uint64_t nk, uin
   prim::keys_to_csr(uniq, m, n, g->off.as<uint64_t>(), g->ids.as<uint32_t>(), s);
   BYO_HIP(hipStreamSynchronize(s));
   g->csr_valid = true;
+  g->maxdeg_valid = false;
 }
 
 void csr_upload(byo_graph *g, const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m) {
@@ -227,6 +228,7 @@ void csr_upload(byo_graph *g, const uint64_t *offsets, const uint32_t *ids, uint
     BYO_HIP(hipMemcpyAsync(g->ids.p, ids, m * sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
   BYO_HIP(hipStreamSynchronize(g->stream));
   g->csr_valid = true;
+  g->maxdeg_valid = false;
 }
 
 void check_csr_input(const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m) {

@@ -133,11 +133,25 @@ def normalise(X, mean, inv_std):
     return np.clip((X - mean) * inv_std, -CLAMP, CLAMP).astype(np.float32)
 
 
-def train(X, T, epochs=3000, lr=3e-3, seed=0):
+RULE_EDGE = 0.05  # another direction must beat the rule's by 5% to become the label
+
+
+def labels(T, R):
+    """Class with the least time from the level to the end of the traversal,
+    where the rule's own choice (R) wins unless another direction beats it
+    by RULE_EDGE: small levels time within noise of each other, and the
+    network should learn to leave the rule only where that pays."""
+    Ta = T.copy()
+    if R is not None:
+        Ta[np.arange(len(R)), R] *= (1.0 - RULE_EDGE)
+    return np.argmin(Ta, axis=1).astype(np.int32)
+
+
+def train(X, T, R=None, epochs=3000, lr=3e-3, seed=0):
     """Cost-sensitive training: every sample is repeated in proportion to how
     much the wrong choice would cost (relative gap between worst and best
     direction), then fit with softmax cross-entropy on the device."""
-    y = np.argmin(T, axis=1).astype(np.int32)
+    y = labels(T, R)
     gap = (T.max(axis=1) - T.min(axis=1)) / np.maximum(T.min(axis=1), 1e-6)
     reps = np.clip(np.round(1 + 3 * np.minimum(gap, 3.0)), 1, 10).astype(int)
     Xr = np.repeat(X, reps, axis=0)
@@ -255,7 +269,7 @@ def main():
             G = CsrGraph.rmat(scale, ef, a, b, c, seed=7 + scale)
             srcs = sources(G.degrees(), args.sources * (BATCH if eng == "batch" else 1), seed=100 + scale)
             units = srcs if eng == "single" else [srcs[i:i + BATCH] for i in range(0, len(srcs), BATCH)]
-            X, T, R = collect(G, units, repeats=2, engine=eng)
+            X, T, R = collect(G, units, repeats=3, engine=eng)
             Xs.append(X)
             Ts.append(T)
             Rs.append(R)
@@ -267,10 +281,10 @@ def main():
         R = np.concatenate(Rs)
         if args.dump:
             np.savez(args.dump, X=X, T=T, R=R)
-        w = train(X, T, epochs=args.epochs)
+        w = train(X, T, R, epochs=args.epochs)
         w["engine"] = eng
         w["train_graphs"] = [list(map(str, g)) for g in TRAIN_GRAPHS]
-        w["label_counts"] = np.bincount(np.argmin(T, axis=1), minlength=3).tolist()
+        w["label_counts"] = np.bincount(labels(T, R), minlength=3).tolist()
         # level time the rule's own choices cost against the per-level best
         w["rule_time_ratio_vs_best"] = float(T[np.arange(len(R)), R].sum() / T.min(axis=1).sum())
         with open(out, "w") as f:
