@@ -96,6 +96,7 @@ SIGNATURES = {
     "byo_profile_enable": (_i32, [_vp, _i32]),
     "byo_profile_get": (_i32, [_vp, C.c_char_p, _P(_u64), _P(_dbl)]),
     "byo_profile_reset": (_i32, [_vp]),
+    "byo_batch_job": (_i32, [_vp, _vp, _u32, _vp, _u32, _dbl, _i32, _vp, _vp, _vp, _P(_dbl)]),
     "byo_selector_load": (_i32, [_vp, _vp, _vp, _vp]),
     "byo_selector_load_engine": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "byo_trainer_create": (_i32, [_vp, C.c_float, _P(_vp)]),
@@ -318,6 +319,22 @@ class Graph:
         ms = _dbl()
         check(lib().byo_ppr_multi(self._h, _ptr(seeds), k, damping, iters, _ptr(p), C.byref(ms)))
         return (p[:n * k].reshape(k, n) if want_p else None), ms.value
+
+    def batch_job(self, bfs_srcs, ppr_seeds, damping=0.85, ppr_iters=30, want_parents=True, want_ppr=True):
+        """k_bfs BFS sources (one shared traversal) and k_ppr personalized
+        PageRank seeds in one persistent launch; returns (parents[k_bfs, n],
+        BFS stats, ppr[k_ppr, n], device ms)."""
+        srcs = np.ascontiguousarray(bfs_srcs, np.uint32)
+        seeds = np.ascontiguousarray(ppr_seeds, np.uint32)
+        n, kb, kp = self.N(), len(srcs), len(seeds)
+        par = np.empty(max(n * kb, 1), np.uint32) if want_parents else None
+        p = np.empty(max(n * kp, 1), np.float32) if want_ppr else None
+        st = (BfsStats * max(kb, 1))()
+        ms = _dbl()
+        check(lib().byo_batch_job(self._h, _ptr(srcs), kb, _ptr(seeds), kp, damping, ppr_iters, _ptr(par),
+                                  C.cast(st, C.c_void_p), _ptr(p), C.byref(ms)))
+        return ((par[:n * kb].reshape(kb, n) if want_parents else None), [st[i].as_dict() for i in range(kb)],
+                (p[:n * kp].reshape(kp, n) if want_ppr else None), ms.value)
 
     def cc(self):
         n = self.N()

@@ -427,7 +427,7 @@ __global__ void k_max_deg(const uint64_t *off, uint64_t n, unsigned long long *o
   if (lane_id() == 0 && mx) atomicMax(out, (unsigned long long)mx);
 }
 }  // namespace
-static uint64_t max_degree(byo_graph *g) {
+uint64_t graph_max_degree(byo_graph *g) {
   if (g->maxdeg_valid) return g->maxdeg;
   auto &b = g->buf("maxdeg", 64);
   BYO_HIP(hipMemsetAsync(b.p, 0, 8, g->stream));
@@ -477,7 +477,7 @@ int byo_bfs_multi(byo_graph_t g, const uint32_t *srcs, uint32_t k, uint32_t *par
     ensure_csr(g);
     for (uint32_t i = 0; i < k; i++) BYO_REQUIRE(srcs[i] < g->n, BYO_E_INVAL, "source out of range");
     if (k == 0) return;
-    const uint64_t mx = (g->dir_policy == BYO_DIR_MLP) ? max_degree(g) : 0;
+    const uint64_t mx = (g->dir_policy == BYO_DIR_MLP) ? graph_max_degree(g) : 0;
     const char *mode = std::getenv("BYO_BFS_MODE");
     const bool levels_mode = mode && std::strcmp(mode, "levels") == 0;
     const bool seq_mode = mode && std::strcmp(mode, "sequential") == 0;

@@ -193,6 +193,8 @@ void require_device();
 void bind(byo_graph *g);
 byo_graph *new_handle(int kind);
 void *pinned(byo_graph *g, size_t bytes);
+// Maximum degree of the current CSR view (cached until it is rebuilt).
+uint64_t graph_max_degree(byo_graph *g);
 // CSR construction (graph.hip).
 void csr_from_keys(byo_graph *g, uint64_t *keys, uint64_t *alt, uint64_t nk, uint64_t n);
 void csr_upload(byo_graph *g, const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m);

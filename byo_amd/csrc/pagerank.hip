@@ -16,34 +16,11 @@
 #include <cmath>
 #include <cstdlib>
 
+#include "kahan.hpp"
 #include "primitives.hpp"
 
 namespace byo {
 namespace {
-
-template <class T>
-struct Acc {
-  T s, c;
-};
-
-// Kahan step (per lane).
-template <class T>
-__device__ __forceinline__ void kahan_add(Acc<T> &a, T x) {
-  T y = x - a.c;
-  T t = a.s + y;
-  a.c = (t - a.s) - y;
-  a.s = t;
-}
-
-// Compensated combine of two partial sums (TwoSum on the heads).
-template <class T>
-__device__ __forceinline__ Acc<T> combine(Acc<T> a, Acc<T> b) {
-  T s = a.s + b.s;
-  T bp = s - a.s;
-  T err = (a.s - (s - bp)) + (b.s - bp);
-  // Kahan carries the negated error in c (value = s - c)
-  return Acc<T>{s, a.c + b.c - err};
-}
 
 // One pass of the pull over the neighbours of every vertex whose ids lie in
 // one source segment [lo_pos[v], hi_pos[v]).  Adjacency lists are sorted,
@@ -178,38 +155,6 @@ __global__ void k_to_double(const T *in, double *out, uint64_t n) {
   for (uint64_t v = global_tid(); v < n; v += grid_threads()) out[v] = (double)in[v];
 }
 
-// ---------------------------------------------- personalized PageRank, k seeds
-// Vertex-major state p[v*K + j]: one wave per vertex, lane j = seed j, so
-// every edge gathers K consecutive floats (one 128 B line for K = 32).
-__global__ void k_ppr_init(const uint64_t *off, uint64_t n, const uint32_t *seeds, int K, float *p, float *contrib) {
-  for (uint64_t i = global_tid(); i < n * (uint64_t)K; i += grid_threads()) {
-    const uint64_t v = i / K;
-    const int j = (int)(i % K);
-    const uint64_t deg = off[v + 1] - off[v];
-    const float x = (seeds[j] == v) ? 1.0f : 0.0f;
-    p[i] = x;
-    contrib[i] = deg ? x / (float)deg : 0.0f;
-  }
-}
-
-__global__ void __launch_bounds__(256) k_ppr_iter(const uint64_t *off, const uint32_t *ids, uint64_t n,
-                                                  const uint32_t *seeds, int K, const float *contrib_in,
-                                                  float *p_new, float *contrib_out, float damping) {
-  const int lane = lane_id();
-  const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
-  const uint32_t my_seed = lane < K ? seeds[lane] : kNone;
-  for (uint64_t v = wave; v < n; v += nwaves) {
-    const uint64_t b = off[v], e = off[v + 1];
-    Acc<float> a{0.f, 0.f};
-    if (lane < K) {
-      for (uint64_t k = b; k < e; k++) kahan_add(a, contrib_in[(uint64_t)ids[k] * K + lane]);
-      const float pv = damping * (a.s - a.c) + (my_seed == v ? 1.0f - damping : 0.0f);
-      p_new[v * K + lane] = pv;
-      contrib_out[v * K + lane] = (e > b) ? pv / (float)(e - b) : 0.0f;
-    }
-  }
-}
-
 template <class T, int G, bool kComp>
 void pr_pass(bool first, bool last, const uint64_t *lo, const uint64_t *hi, const uint32_t *ids, const uint64_t *off,
              uint64_t n, const T *cin, T *as, T *ac, const T *pold, T *pnew, T *cout, double damping, double addc,
@@ -336,50 +281,6 @@ int byo_pagerank(byo_graph_t g, double damping, int max_iters, double eps, int p
       pagerank_impl<float, true>(g, damping, max_iters, eps, p, l1_log, iters_run, device_ms, "pr_fp32");
     else
       pagerank_impl<double, false>(g, damping, max_iters, eps, p, l1_log, iters_run, device_ms, "pr_fp64");
-  });
-}
-
-int byo_ppr_multi(byo_graph_t g, const uint32_t *seeds, uint32_t k, double damping, int iters, float *p,
-                  double *device_ms) {
-  return guard([&] {
-    BYO_REQUIRE(g && (seeds || k == 0), BYO_E_INVAL, "null argument");
-    BYO_REQUIRE(k <= 64, BYO_E_INVAL, "at most 64 seeds per call");
-    BYO_REQUIRE(iters >= 0, BYO_E_INVAL, "iters < 0");
-    std::lock_guard<std::mutex> lk(g->mu);
-    bind(g);
-    ensure_csr(g);
-    for (uint32_t j = 0; j < k; j++) BYO_REQUIRE(seeds[j] < g->n, BYO_E_INVAL, "seed out of range");
-    if (k == 0 || g->n == 0) return;
-    const uint64_t n = g->n;
-    hipStream_t s = g->stream;
-    const uint64_t sz = n * k * sizeof(float) + 64;
-    float *pa = g->buf("ppr_p", sz).as<float>();
-    float *c0 = g->buf("ppr_c0", sz).as<float>();
-    float *c1 = g->buf("ppr_c1", sz).as<float>();
-    uint32_t *dseeds = g->buf("ppr_seeds", 64 * 4).as<uint32_t>();
-    BYO_HIP(hipMemcpyAsync(dseeds, seeds, k * 4, hipMemcpyHostToDevice, s));
-    const uint64_t *off = g->off.as<uint64_t>();
-    const uint32_t *ids = g->ids.as<uint32_t>();
-    k_ppr_init<<<byo_grid(n * k, 256, 16384), 256, 0, s>>>(off, n, dseeds, (int)k, pa, c0);
-    BYO_LAUNCH_CHECK();
-    PhaseTimer t(s, true);
-    float *cin = c0, *cout = c1;
-    for (int it = 0; it < iters; it++) {
-      k_ppr_iter<<<8192, 256, 0, s>>>(off, ids, n, dseeds, (int)k, cin, pa, cout, (float)damping);
-      std::swap(cin, cout);
-    }
-    BYO_LAUNCH_CHECK();
-    double ms = t.stop();
-    if (g->prof.on) g->account("ppr", ms);
-    if (device_ms) *device_ms = ms;
-    if (p) {
-      std::vector<float> h(n * k);
-      BYO_HIP(hipMemcpyAsync(h.data(), pa, n * k * sizeof(float), hipMemcpyDeviceToHost, s));
-      BYO_HIP(hipStreamSynchronize(s));
-      for (uint64_t v = 0; v < n; v++)
-        for (uint32_t j = 0; j < k; j++) p[(uint64_t)j * n + v] = h[v * k + j];
-    }
-    BYO_HIP(hipStreamSynchronize(s));
   });
 }
 

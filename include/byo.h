@@ -198,6 +198,18 @@ int byo_pagerank(byo_graph_t g, double damping, int max_iters, double eps,
  * (1-damping)[i == seed].  p[k*n] as FP32 (may be NULL). */
 int byo_ppr_multi(byo_graph_t g, const uint32_t *seeds, uint32_t k,
                   double damping, int iters, float *p, double *device_ms);
+/* Batched job in ONE persistent cooperative launch: k_bfs <= 64 BFS sources
+ * sharing one traversal (as byo_bfs_multi) and then k_ppr <= 64
+ * personalized-PageRank seeds for ppr_iters iterations (as byo_ppr_multi),
+ * over the same graph.  No single reference call covers this: it batches the
+ * per-source BFS(G, src) of BFS.h:47-64 and a seeded PageRank_edgeMap
+ * (PageRank.h:80-123) the way run_all (run_unweighted.h:930-1034) would run
+ * them one after another.  parents[k_bfs*n], stats[k_bfs], ppr[k_ppr*n]
+ * (FP32) and device_ms may be NULL. */
+int byo_batch_job(byo_graph_t g, const uint32_t *bfs_srcs, uint32_t k_bfs,
+                  const uint32_t *ppr_seeds, uint32_t k_ppr, double damping,
+                  int ppr_iters, uint32_t *parents, byo_bfs_stats *stats,
+                  float *ppr, double *device_ms);
 
 /* Connected components.  Replaces SimpleUnionAsync(G) (benchmarks/
  * Connectivity/SimpleUnionAsync/Connectivity.h:122-136): labels[n] with
