@@ -96,6 +96,7 @@ SIGNATURES = {
     "byo_profile_enable": (_i32, [_vp, _i32]),
     "byo_profile_get": (_i32, [_vp, C.c_char_p, _P(_u64), _P(_dbl)]),
     "byo_profile_reset": (_i32, [_vp]),
+    "byo_insert_batches_dev": (_i32, [_vp, _vp, _vp, _u32, _P(_u32), _P(_dbl)]),
     "byo_batch_job": (_i32, [_vp, _vp, _u32, _vp, _u32, _dbl, _i32, _vp, _vp, _vp, _P(_dbl)]),
     "byo_selector_load": (_i32, [_vp, _vp, _vp, _vp]),
     "byo_selector_load_engine": (_i32, [_vp, _i32, _vp, _vp, _vp]),
@@ -271,6 +272,17 @@ class Graph:
 
     def remove_batch_dev(self, dev_ptr, count):
         check(lib().byo_remove_batch_dev(self._h, C.c_void_p(dev_ptr), count))
+
+    def insert_batches_dev(self, dev_ptr, offsets):
+        """Batches keys[offsets[i]:offsets[i+1]] (device memory) inserted in
+        order as one captured HIP graph; returns (graph launches, device ms)."""
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        nb = len(offsets) - 1
+        graphs = _u32()
+        ms = _dbl()
+        check(lib().byo_insert_batches_dev(self._h, C.c_void_p(dev_ptr), _ptr(offsets), nb, C.byref(graphs),
+                                           C.byref(ms)))
+        return graphs.value, ms.value
 
     # -- algorithms -------------------------------------------------------
     def set_direction_policy(self, policy, threshold=-1):

@@ -84,8 +84,12 @@ __global__ void k_degrees_from_off(const uint64_t *off, uint64_t n, uint32_t *de
   for (uint64_t v = global_tid(); v < n; v += grid_threads()) deg[v] = (uint32_t)(off[v + 1] - off[v]);
 }
 
-// leaf index of each batch key: last l with sep[l] <= key
-__global__ void k_leaf_of(const uint64_t *keys, uint64_t b, const uint64_t *sep, uint64_t nleaves, uint32_t *leaf) {
+// leaf index of each batch key: last l with sep[l] <= key.  The unique key
+// count *bu is read on the device (keys at and past it are skipped), so the
+// same launch serves a batch whose size is known only after deduplication.
+__global__ void k_leaf_of(const uint64_t *keys, const unsigned long long *bu, const uint64_t *sep, uint64_t nleaves,
+                          uint32_t *leaf) {
+  const uint64_t b = *bu;
   for (uint64_t i = global_tid(); i < b; i += grid_threads()) {
     const uint64_t k = keys[i];
     uint64_t lo = 0, hi = nleaves;  // sep[lo] <= k < sep[hi] (sep[nleaves] = ~0 > any key)
@@ -97,14 +101,28 @@ __global__ void k_leaf_of(const uint64_t *keys, uint64_t b, const uint64_t *sep,
   }
 }
 
-__global__ void k_seg_flags(const uint32_t *leaf, uint64_t b, uint32_t *flag) {
-  for (uint64_t i = global_tid(); i < b; i += grid_threads()) flag[i] = (i == 0 || leaf[i] != leaf[i - 1]) ? 1u : 0u;
+// Segment heads over the first *bu keys; the flags up to cap are written
+// (zero past *bu) so a scan over cap entries counts the segments.
+__global__ void k_seg_flags(const uint32_t *leaf, const unsigned long long *bu, uint64_t cap, uint32_t *flag) {
+  const uint64_t b = *bu;
+  for (uint64_t i = global_tid(); i < cap; i += grid_threads())
+    flag[i] = (i < b && (i == 0 || leaf[i] != leaf[i - 1])) ? 1u : 0u;
 }
 
-__global__ void k_seg_starts(const uint32_t *flag, const uint32_t *pos, uint64_t b, uint32_t *starts) {
-  for (uint64_t i = global_tid(); i < b; i += grid_threads())
+__global__ void k_seg_starts(const uint32_t *flag, const uint32_t *pos, uint64_t cap, uint32_t *starts) {
+  for (uint64_t i = global_tid(); i < cap; i += grid_threads())
     if (flag[i]) starts[pos[i]] = (uint32_t)i;
 }
+
+// Device-side counts of one update: unique keys *bu (ctl->pad[0]) and
+// segments *nseg (the scan total of the flags).  halt, when given, is the
+// captured-sequence stop flag: a set flag turns the CPMA-writing kernels of
+// this and every later batch into no-ops.
+struct UpdDev {
+  const unsigned long long *bu;
+  const uint32_t *nseg;
+  const uint32_t *halt;
+};
 
 __device__ __forceinline__ bool in_sorted(const uint64_t *a, uint64_t len, uint64_t x) {
   uint64_t lo = 0, hi = len;
@@ -125,10 +143,12 @@ __device__ __forceinline__ uint32_t lower_rank(const uint64_t *a, uint32_t len, 
 }
 
 // Pass 1 of an insert: count the segment keys absent from their leaf.
-__global__ void __launch_bounds__(256) k_ins_count(const uint64_t *keys, uint64_t b, const uint32_t *starts,
-                                                   uint64_t nseg, const uint32_t *leaf_of, const uint64_t *slots,
+__global__ void __launch_bounds__(256) k_ins_count(const uint64_t *keys, UpdDev d, const uint32_t *starts,
+                                                   const uint32_t *leaf_of, const uint64_t *slots,
                                                    const uint32_t *cnt, uint32_t *newc, UpdCtl *ctl) {
   __shared__ uint32_t acc;
+  if (d.halt && *d.halt) return;
+  const uint64_t b = *d.bu, nseg = *d.nseg;
   for (uint64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
     const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
     const uint32_t l = leaf_of[s];
@@ -150,14 +170,16 @@ __global__ void __launch_bounds__(256) k_ins_count(const uint64_t *keys, uint64_
 }
 
 // Pass 2 of an insert (no leaf overflows): merge in LDS and write back.
-__global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, uint64_t b, const uint32_t *starts,
-                                                   uint64_t nseg, const uint32_t *leaf_of, uint64_t *slots,
-                                                   uint32_t *cnt, uint32_t *deg, UpdCtl *ctl) {
+__global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, UpdDev d, const uint32_t *starts,
+                                                   const uint32_t *leaf_of, uint64_t *slots, uint32_t *cnt,
+                                                   uint32_t *deg, UpdCtl *ctl) {
   __shared__ uint64_t old_keys[kLeaf];
   __shared__ uint64_t add[kLeaf];
   __shared__ uint32_t wcnt[4];
   __shared__ uint32_t nadd;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if ((d.halt && *d.halt) || ctl->overflow) return;
+  const uint64_t b = *d.bu, nseg = *d.nseg;
   for (uint64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
     const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
     const uint32_t l = leaf_of[s];
@@ -205,12 +227,14 @@ __global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, uint64_
 }
 
 // Removal: each touched leaf keeps the keys absent from its segment.
-__global__ void __launch_bounds__(256) k_remove(const uint64_t *keys, uint64_t b, const uint32_t *starts,
-                                                uint64_t nseg, const uint32_t *leaf_of, uint64_t *slots,
-                                                uint32_t *cnt, uint32_t *deg, UpdCtl *ctl) {
+__global__ void __launch_bounds__(256) k_remove(const uint64_t *keys, UpdDev d, const uint32_t *starts,
+                                                const uint32_t *leaf_of, uint64_t *slots, uint32_t *cnt,
+                                                uint32_t *deg, UpdCtl *ctl) {
   __shared__ uint64_t kept[kLeaf];
   __shared__ uint32_t wcnt[4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (d.halt && *d.halt) return;
+  const uint64_t b = *d.bu, nseg = *d.nseg;
   for (uint64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
     const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
     const uint32_t l = leaf_of[s];
@@ -370,53 +394,65 @@ uint64_t cpma_bytes(byo_graph *g) {
   return c->nleaves * kLeaf * 8 + c->nleaves * 4 + (c->nleaves + 1) * 8 + g->n * 4;
 }
 
-// Sorts + dedupes keys_in[b] (device) into a batch buffer; returns it and
-// the unique count.
-static const uint64_t *prepare_batch(byo_graph *g, const uint64_t *keys_in, uint64_t b, uint64_t *bu, UpdCtl *ctl,
-                                     UpdCtl *hctl) {
+namespace {
+__global__ void k_zero_ctl(UpdCtl *ctl, uint64_t count) {
+  for (uint64_t i = global_tid(); i < count; i++) ctl[i] = UpdCtl{0, 0, 0, {0, 0, 0, 0, 0}};
+}
+__global__ void k_zero_u32(uint32_t *p) { *p = 0; }
+// Stops a captured sequence at batch `index` (1-based) when its keys hold an
+// id >= n or one of its leaves would overflow.
+__global__ void k_gate(const UpdCtl *ctl, uint32_t *halt, uint32_t index) {
+  if (*halt == 0 && (ctl->overflow || ctl->bad)) *halt = index;
+}
+}  // namespace
+
+// Buffers of one update for batches of up to cap keys.
+struct UpdBufs {
+  uint64_t *k0, *k1;
+  void *sort_tmp, *sel_tmp, *scan_tmp;
+  uint32_t *leaf, *flag, *pos, *starts, *newc;
+};
+
+static UpdBufs upd_bufs(byo_graph *g, uint64_t cap) {
+  UpdBufs u;
+  u.k0 = g->buf("upd_k0", cap * 8 + 64).as<uint64_t>();
+  u.k1 = g->buf("upd_k1", cap * 8 + 64).as<uint64_t>();
+  u.sort_tmp = g->buf("upd_sort_tmp", prim::radix_tmp_bytes(cap)).p;
+  u.sel_tmp = g->buf("upd_sel_tmp", prim::select_tmp_bytes(cap) + 256).p;
+  u.scan_tmp = g->buf("upd_scan_tmp", prim::scan_tmp_bytes(cap + 1)).p;
+  u.leaf = g->buf("upd_leaf", cap * 4 + 64).as<uint32_t>();
+  u.flag = g->buf("upd_flag", cap * 4 + 64).as<uint32_t>();
+  u.pos = g->buf("upd_pos", (cap + 1) * 4 + 64).as<uint32_t>();
+  u.starts = g->buf("upd_starts", cap * 4 + 64).as<uint32_t>();
+  u.newc = g->buf("upd_newc", cap * 4 + 64).as<uint32_t>();
+  return u;
+}
+
+// Device-only part of an update (no host round trip, so it can be captured):
+// ids checked (ctl->bad), keys_in[b] radix sorted and deduplicated (count in
+// ctl->pad[0]), each unique key's leaf found and the batch cut into one
+// segment per touched leaf (count at u.pos[b]).  ctl must be zero.
+static const uint64_t *sort_and_segment(byo_graph *g, const uint64_t *keys_in, uint64_t b, UpdCtl *ctl,
+                                        const UpdBufs &u) {
+  Cpma *c = state(g);
   hipStream_t s = g->stream;
-  uint64_t *k0 = g->buf("upd_k0", b * 8 + 64).as<uint64_t>();
-  uint64_t *k1 = g->buf("upd_k1", b * 8 + 64).as<uint64_t>();
-  BYO_HIP(hipMemsetAsync(ctl, 0, sizeof(UpdCtl), s));
   k_check_ids<<<byo_grid(b, 256, 8192), 256, 0, s>>>(keys_in, b, g->n, ctl);
-  BYO_HIP(hipMemcpyAsync(k0, keys_in, b * 8, hipMemcpyDeviceToDevice, s));
+  BYO_HIP(hipMemcpyAsync(u.k0, keys_in, b * 8, hipMemcpyDeviceToDevice, s));
   int idbits = 1;
   while ((1ull << idbits) < g->n) idbits++;
   int shifts[16];
   const int np = prim::edge_key_shifts(idbits, shifts);
-  void *rtmp = g->buf("upd_sort_tmp", prim::radix_tmp_bytes(b)).p;
-  uint64_t *sorted = prim::radix_sort_u64(k0, k1, b, shifts, np, rtmp, s);
-  uint64_t *uniq = (sorted == k0) ? k1 : k0;
-  void *stmp = g->buf("upd_sel_tmp", prim::select_tmp_bytes(b) + 256).p;
-  prim::unique_u64(sorted, uniq, b, kEmpty, reinterpret_cast<uint64_t *>(&ctl->pad[0]), stmp, s);
-  BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
-  BYO_HIP(hipStreamSynchronize(s));
-  BYO_REQUIRE(hctl->bad == 0, BYO_E_INVAL, "batch holds a vertex id >= n");
-  *bu = hctl->pad[0];
-  return uniq;
-}
-
-// Segments of the sorted batch per leaf; returns the segment count.
-static uint64_t segment_batch(byo_graph *g, const uint64_t *keys, uint64_t b, uint32_t **leaf_of, uint32_t **starts) {
-  Cpma *c = state(g);
-  hipStream_t s = g->stream;
-  uint32_t *lf = g->buf("upd_leaf", b * 4 + 64).as<uint32_t>();
-  uint32_t *flag = g->buf("upd_flag", b * 4 + 64).as<uint32_t>();
-  uint32_t *pos = g->buf("upd_pos", (b + 1) * 4 + 64).as<uint32_t>();
-  uint32_t *st = g->buf("upd_starts", b * 4 + 64).as<uint32_t>();
-  void *tmp = g->buf("upd_scan_tmp", prim::scan_tmp_bytes(b + 1)).p;
+  uint64_t *sorted = prim::radix_sort_u64(u.k0, u.k1, b, shifts, np, u.sort_tmp, s);
+  uint64_t *uniq = (sorted == u.k0) ? u.k1 : u.k0;
+  const unsigned long long *bu = &ctl->pad[0];
+  prim::unique_u64(sorted, uniq, b, kEmpty, reinterpret_cast<uint64_t *>(&ctl->pad[0]), u.sel_tmp, s);
   const unsigned grid = byo_grid(b, 256, 8192);
-  k_leaf_of<<<grid, 256, 0, s>>>(keys, b, c->sep.as<uint64_t>(), c->nleaves, lf);
-  k_seg_flags<<<grid, 256, 0, s>>>(lf, b, flag);
-  prim::exclusive_scan_u32(flag, pos, b, true, tmp, s);
-  k_seg_starts<<<grid, 256, 0, s>>>(flag, pos, b, st);
+  k_leaf_of<<<grid, 256, 0, s>>>(uniq, bu, c->sep.as<uint64_t>(), c->nleaves, u.leaf);
+  k_seg_flags<<<grid, 256, 0, s>>>(u.leaf, bu, b, u.flag);
+  prim::exclusive_scan_u32(u.flag, u.pos, b, true, u.scan_tmp, s);
+  k_seg_starts<<<grid, 256, 0, s>>>(u.flag, u.pos, b, u.starts);
   BYO_LAUNCH_CHECK();
-  uint32_t *h = static_cast<uint32_t *>(pinned(g, 64));
-  BYO_HIP(hipMemcpyAsync(h, pos + b, 4, hipMemcpyDeviceToHost, s));
-  BYO_HIP(hipStreamSynchronize(s));
-  *leaf_of = lf;
-  *starts = st;
-  return h[0];
+  return uniq;
 }
 
 static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool insert) {
@@ -426,22 +462,24 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
   UpdCtl *ctl = g->buf("upd_ctl", sizeof(UpdCtl)).as<UpdCtl>();
   UpdCtl *hctl = static_cast<UpdCtl *>(pinned(g, sizeof(UpdCtl)));
   PhaseTimer t(s, g->prof.on);
-  uint64_t bu = 0;
-  const uint64_t *batch = prepare_batch(g, keys_dev, b, &bu, ctl, hctl);
+  const UpdBufs u = upd_bufs(g, b);
+  k_zero_ctl<<<1, 1, 0, s>>>(ctl, 1);
+  const uint64_t *batch = sort_and_segment(g, keys_dev, b, ctl, u);
+  BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
+  BYO_HIP(hipStreamSynchronize(s));
+  BYO_REQUIRE(hctl->bad == 0, BYO_E_INVAL, "batch holds a vertex id >= n");
+  const uint64_t bu = hctl->pad[0];
   if (bu == 0) return;
-  uint32_t *leaf_of = nullptr, *starts = nullptr;
-  const uint64_t nseg = segment_batch(g, batch, bu, &leaf_of, &starts);
-  BYO_HIP(hipMemsetAsync(ctl, 0, sizeof(UpdCtl), s));
-  const unsigned grid = (unsigned)std::min<uint64_t>(nseg, 16384);
+  const UpdDev d{&ctl->pad[0], u.pos + b, nullptr};
+  const unsigned grid = (unsigned)std::min<uint64_t>(bu, 16384);
   if (insert) {
-    uint32_t *newc = g->buf("upd_newc", nseg * 4 + 64).as<uint32_t>();
-    k_ins_count<<<grid, 256, 0, s>>>(batch, bu, starts, nseg, leaf_of, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
-                                     newc, ctl);
+    k_ins_count<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+                                     u.newc, ctl);
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
     BYO_HIP(hipStreamSynchronize(s));
     if (!hctl->overflow) {
-      k_ins_merge<<<grid, 256, 0, s>>>(batch, bu, starts, nseg, leaf_of, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+      k_ins_merge<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
                                        c->deg.as<uint32_t>(), ctl);
       BYO_LAUNCH_CHECK();
       BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
@@ -471,7 +509,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
       BYO_HIP(hipStreamSynchronize(s));
     }
   } else {
-    k_remove<<<grid, 256, 0, s>>>(batch, bu, starts, nseg, leaf_of, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+    k_remove<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
                                   c->deg.as<uint32_t>(), ctl);
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
@@ -479,9 +517,9 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
     c->total -= hctl->changed;
     g->m = c->total;
     if (hctl->changed) {
-        g->csr_valid = false;
-        g->maxdeg_valid = false;
-      }
+      g->csr_valid = false;
+      g->maxdeg_valid = false;
+    }
     if (c->nleaves > 1 && (double)c->total < kMinDensity * (double)(c->nleaves * kLeaf)) {
       c->rebalances++;
       uint64_t *flat = g->buf("cpma_flat", c->total * 8 + 64).as<uint64_t>();
@@ -491,6 +529,78 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
     }
   }
   if (g->prof.on) g->account(insert ? "cpma_insert" : "cpma_remove", t.stop());
+}
+
+// Insert batches [first, nb) as one captured HIP graph; returns the 1-based
+// index of the batch that stopped the sequence (0: all applied) and adds the
+// graph's launch time to *ms.
+static uint32_t insert_graph(byo_graph *g, const uint64_t *keys, const uint64_t *offs, uint32_t first, uint32_t nb,
+                             const UpdBufs &u, UpdCtl *ctls, uint32_t *halt, double *ms) {
+  Cpma *c = state(g);
+  hipStream_t s = g->stream;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  BYO_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+  try {
+    k_zero_u32<<<1, 1, 0, s>>>(halt);
+    k_zero_ctl<<<1, 1, 0, s>>>(ctls + first, nb - first);
+    for (uint32_t i = first; i < nb; i++) {
+      const uint64_t b = offs[i + 1] - offs[i];
+      if (b == 0) continue;
+      UpdCtl *ctl = ctls + i;
+      const uint64_t *batch = sort_and_segment(g, keys + offs[i], b, ctl, u);
+      const UpdDev d{&ctl->pad[0], u.pos + b, halt};
+      const unsigned grid = (unsigned)std::min<uint64_t>(b, 16384);
+      k_ins_count<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+                                       u.newc, ctl);
+      k_gate<<<1, 1, 0, s>>>(ctl, halt, i + 1);
+      k_ins_merge<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+                                       c->deg.as<uint32_t>(), ctl);
+    }
+    BYO_LAUNCH_CHECK();
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  BYO_HIP(hipStreamEndCapture(s, &graph));
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    BYO_HIP(ie);
+  }
+  double t_ms = 0;
+  {
+    PhaseTimer t(s, true);
+    const hipError_t le = hipGraphLaunch(exec, s);
+    if (le != hipSuccess) {
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      BYO_HIP(le);
+    }
+    t_ms = t.stop();
+  }
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  *ms += t_ms;
+  // read back the stop index and the per-batch counts
+  const size_t nbytes = (size_t)(nb - first) * sizeof(UpdCtl);
+  char *h = static_cast<char *>(pinned(g, 64 + nbytes));
+  BYO_HIP(hipMemcpyAsync(h, halt, 4, hipMemcpyDeviceToHost, s));
+  BYO_HIP(hipMemcpyAsync(h + 64, ctls + first, nbytes, hipMemcpyDeviceToHost, s));
+  BYO_HIP(hipStreamSynchronize(s));
+  const uint32_t stop = *reinterpret_cast<uint32_t *>(h);
+  const UpdCtl *hc = reinterpret_cast<const UpdCtl *>(h + 64);
+  const uint32_t applied_end = stop ? stop - 1 : nb;
+  uint64_t changed = 0;
+  for (uint32_t i = first; i < applied_end; i++) changed += hc[i - first].changed;
+  c->total += changed;
+  g->m = c->total;
+  if (changed) {
+    g->csr_valid = false;
+    g->maxdeg_valid = false;
+  }
+  return stop;
 }
 
 }  // namespace byo
@@ -549,6 +659,46 @@ int byo_cpma_create_from(byo_graph_t src, byo_graph_t *out) {
       throw;
     }
     *out = g;
+  });
+}
+
+int byo_insert_batches_dev(byo_graph_t g, const uint64_t *keys_dev, const uint64_t *offsets, uint32_t nbatches,
+                           uint32_t *graphs, double *device_ms) {
+  return guard([&] {
+    BYO_REQUIRE(g && offsets && (keys_dev || nbatches == 0), BYO_E_INVAL, "null argument");
+    BYO_REQUIRE(g->kind == 1, BYO_E_STATE, "insert_batch needs a CPMA container");
+    uint64_t cap = 0;
+    for (uint32_t i = 0; i < nbatches; i++) {
+      BYO_REQUIRE(offsets[i + 1] >= offsets[i], BYO_E_INVAL, "batch offsets must not decrease");
+      cap = std::max<uint64_t>(cap, offsets[i + 1] - offsets[i]);
+    }
+    std::lock_guard<std::mutex> lk(g->mu);
+    bind(g);
+    if (graphs) *graphs = 0;
+    if (device_ms) *device_ms = 0;
+    if (cap == 0) return;
+    // every buffer the captured kernels touch exists before capture starts
+    const UpdBufs u = upd_bufs(g, cap);
+    UpdCtl *ctls = g->buf("upd_ctls", (size_t)nbatches * sizeof(UpdCtl) + 64).as<UpdCtl>();
+    uint32_t *halt = g->buf("upd_halt", 64).as<uint32_t>();
+    pinned(g, 64 + (size_t)nbatches * sizeof(UpdCtl));
+    double ms = 0;
+    uint32_t first = 0, launches = 0;
+    while (first < nbatches) {
+      const uint32_t stop = insert_graph(g, keys_dev, offsets, first, nbatches, u, ctls, halt, &ms);
+      launches++;
+      if (stop == 0) break;
+      // the stopping batch goes through the host-driven path (global
+      // rebalance, or the error for an id >= n); the rest follows in a new graph
+      const uint32_t h = stop - 1;
+      PhaseTimer t(g->stream, true);
+      cpma_update(g, keys_dev + offsets[h], offsets[h + 1] - offsets[h], true);
+      ms += t.stop();
+      first = h + 1;
+    }
+    if (graphs) *graphs = launches;
+    if (device_ms) *device_ms = ms;
+    if (g->prof.on) g->account("cpma_insert_graph", ms);
   });
 }
 

@@ -117,6 +117,22 @@ int byo_remove_batch(byo_graph_t g, const uint64_t *keys, uint64_t b);
 /* The same with the batch already in device memory (keys_dev, b entries). */
 int byo_insert_batch_dev(byo_graph_t g, const uint64_t *keys_dev, uint64_t b);
 int byo_remove_batch_dev(byo_graph_t g, const uint64_t *keys_dev, uint64_t b);
+/* nbatches insert batches held back to back in device memory (batch i is
+ * keys_dev[offsets[i] .. offsets[i+1]), offsets on the host), applied in
+ * order as ONE captured HIP graph with no host round trip between batches:
+ * each batch is sorted, deduplicated, segmented and merged into its leaves
+ * on the device.  A batch that would overflow a leaf, or holds an id >= n,
+ * stops the graph's updates from that batch on; the host then applies it as
+ * byo_insert_batch_dev does (global rebalance, or BYO_E_INVAL with the
+ * earlier batches applied) and continues with a new graph.  The result
+ * equals byo_insert_batch_dev over the batches in order.  *graphs (may be
+ * NULL) receives the number of graph launches and *device_ms (may be NULL)
+ * the device time of the updates, graph capture excluded.  No single
+ * reference call: it runs Graph::insert_batch (gbbs/graph.h:3071-3095) over a
+ * stream of batches. */
+int byo_insert_batches_dev(byo_graph_t g, const uint64_t *keys_dev,
+                           const uint64_t *offsets, uint32_t nbatches,
+                           uint32_t *graphs, double *device_ms);
 
 /* Device buffers for callers that keep update batches resident in HBM
  * (the *_dev entry points above).  No counterpart in the reference, whose

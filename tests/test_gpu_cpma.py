@@ -132,3 +132,64 @@ def test_algorithms_after_updates(B):
     assert np.abs(p - p_ref).sum() < 1e-12
     s, _ = G.map_sum()
     assert s == O.map_sum(h)
+
+
+# ---- several batches as one captured HIP graph (byo_insert_batches_dev) ----
+
+def _graph_insert(B, G, batches):
+    keys = np.concatenate(batches)
+    offs = np.concatenate([[0], np.cumsum([len(b) for b in batches])]).astype(np.uint64)
+    d = B.DeviceBuffer(keys)
+    try:
+        return G.insert_batches_dev(d.ptr, offs)
+    finally:
+        d.free()
+
+
+def test_captured_batches_equal_sequential(B):
+    """Uneven batches in one graph give the edge set, M and degrees of the
+    same batches inserted one call at a time."""
+    g = O.rmat_graph(14)
+    sizes = [10, 1000, 3, 20000, 500, 7, 4096]
+    batches, first = [], 0
+    for b in sizes:
+        batches.append(upd_keys(g.n, b, first))
+        first += b
+    G1 = B.CpmaGraph.from_csr(g.offsets, g.ids)
+    graphs, ms = _graph_insert(B, G1, batches)
+    print("graphs", graphs, "ms", ms, flush=True)
+    assert graphs >= 1 and ms > 0
+    G2 = B.CpmaGraph.from_csr(g.offsets, g.ids)
+    for b in batches:
+        G2.insert_batch(b)
+    E = O.graph_keys(g)
+    for b in batches:
+        E = np.union1d(E, b)
+    k1, _ = keys_of(G1)
+    k2, _ = keys_of(G2)
+    assert np.array_equal(k1, E) and np.array_equal(k2, E)
+    assert G1.M() == len(E) == G2.M()
+    assert np.array_equal(G1.degrees(), G2.degrees())
+
+
+def test_captured_batches_overflow_and_bad_ids(B):
+    """A batch that overflows a leaf stops the graph, goes through the host
+    rebalance, and the rest follows in a new graph; a batch with an id >= n
+    raises after the batches before it were applied."""
+    g = O.rmat_graph(12)
+    hub = (np.uint64(5) << np.uint64(32)) | np.arange(g.n, dtype=np.uint64)  # one vertex gains n edges
+    batches = [upd_keys(g.n, 100, 0), hub, upd_keys(g.n, 300, 100)]
+    G = B.CpmaGraph.from_csr(g.offsets, g.ids)
+    graphs, _ = _graph_insert(B, G, batches)
+    assert graphs >= 2
+    E = O.graph_keys(g)
+    for b in batches:
+        E = np.union1d(E, b)
+    assert np.array_equal(keys_of(G)[0], E)
+    bad = np.array([(np.uint64(g.n) << np.uint64(32)) | np.uint64(1)], np.uint64)
+    later = upd_keys(g.n, 50, 5000)
+    ok = upd_keys(g.n, 40, 7000)
+    with pytest.raises(B.ByoError):
+        _graph_insert(B, G, [ok, bad, later])
+    E2 = np.union1d(E, ok)
+    assert np.array_equal(keys_of(G)[0], E2)
