@@ -86,6 +86,46 @@ def cpu_baseline(G, sources, budget_s):
                       "(passes over the %d sources), %.1f s" % (int(np.log2(g.n)), done, len(sources), t_total)}
 
 
+def batched_job(B, args):
+    """The batched persistent workload: on RMAT scale job_scale ef 16, 64 BFS
+    sources (one shared traversal) and 32 personalized-PageRank seeds (FP32,
+    damping 0.85, 30 iterations) in one persistent launch; then 16 insert
+    batches of log-uniform sizes 1e3..1e6 (seeded) applied in order as
+    captured HIP graphs on a CPMA built from the same graph."""
+    J = B.CsrGraph.rmat(args.job_scale, 16)
+    knobs = J.apply_tuning()
+    n = J.N()
+    deg = J.degrees()
+    srcs = pick_sources(deg, 64, seed=2)
+    seeds = pick_sources(deg, 32, seed=3)
+    J.batch_job(srcs, seeds, 0.85, 30, want_parents=False, want_ppr=False)
+    J.profile(True)
+    J.profile_reset()
+    _, st, _, ms = J.batch_job(srcs, seeds, 0.85, 30, want_parents=False, want_ppr=False)
+    bfs_ms = sum(J.profile_get(k)[1] for k in ("bfs_batch_seed", "bfs_batch_levels", "bfs_batch_tally"))
+    edges = sum(s["edges_reached"] // 2 for s in st)
+    res = {"graph": "RMAT-%d ef16 (n=%d, m=%d)" % (args.job_scale, n, J.M()), "tuning": knobs,
+           "bfs_sources": 64, "ppr_seeds": 32,
+           "ppr_iters": 30, "launch_ms": round(ms, 3), "bfs_part_ms": round(bfs_ms, 3),
+           "ppr_part_ms": round(ms - bfs_ms, 3), "bfs_gteps_in_launch": round(edges / (bfs_ms * 1e-3) / 1e9, 2),
+           "ppr_ms_per_iteration": round((ms - bfs_ms) / 30, 3)}
+    rng = np.random.default_rng(16)
+    sizes = [int(10 ** x) for x in rng.uniform(3, 6, 16)]
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    keys = B.update_rmat_keys(n, int(offs[-1]))
+    C = B.CpmaGraph.from_graph(J)
+    J.close()
+    dkeys = B.DeviceBuffer(keys)
+    m0 = C.M()
+    graphs, ins_ms = C.insert_batches_dev(dkeys.ptr, offs)
+    res.update({"insert_batches": 16, "insert_sizes": sizes, "insert_keys": int(offs[-1]),
+                "edges_added": int(C.M() - m0), "insert_graph_launches": graphs, "insert_ms": round(ins_ms, 3),
+                "insert_edges_s": round(float(offs[-1]) / (ins_ms * 1e-3), 1)})
+    dkeys.free()
+    C.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +136,7 @@ def main():
     ap.add_argument("--sources", type=int, default=8)
     ap.add_argument("--cpma-scale", type=int, default=24)
     ap.add_argument("--cpma-batch", type=int, default=10_000_000)
+    ap.add_argument("--job-scale", type=int, default=23, help="scale of the batched job graph (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
@@ -118,6 +159,7 @@ def main():
     print("# graph RMAT-%d ef%d: n=%d m=%d (built on device in %.1fs)" % (args.scale, args.ef, n, m,
                                                                           time.time() - t), file=sys.stderr)
 
+    knobs = G.apply_tuning()  # autotuned block/tile sizes for this graph size, if stored
     from byo_amd import selector as S
     learned = bool(S.load(G).get("batch"))
     policy = B.DIR_MLP if learned else B.DIR_THRESHOLD
@@ -128,7 +170,7 @@ def main():
         stats of the last step)."""
         G.set_direction_policy(pol)
         if mode:
-            os.environ["BYO_BFS_MODE"] = mode
+            G.set_tuning("bfs_engine", mode)
         try:
             for _ in range(args.warmup):
                 G.bfs_multi(srcs, want_parents=False)
@@ -148,13 +190,13 @@ def main():
             nl, ms = G.profile_get("bfs_launch")
         finally:
             if mode:
-                del os.environ["BYO_BFS_MODE"]
+                G.set_tuning("bfs_engine", 0)
             G.set_direction_policy(B.DIR_THRESHOLD)
         return edges / wall / 1e9, wall, nl, ms, nbytes, stats
 
     gteps, wall, nlaunch, launch_ms, bytes_total, stats = timed_bfs(policy, args.steps)
     thr_gteps, _, thr_nl, thr_ms, _, thr_stats = timed_bfs(B.DIR_THRESHOLD, args.steps)
-    seq_gteps = timed_bfs(B.DIR_THRESHOLD, 1, "sequential")[0]
+    seq_gteps = timed_bfs(B.DIR_THRESHOLD, 1, 1)[0]  # bfs_engine 1: one source at a time
     # the dominant kernel is the persistent BFS launch (all sources of a step
     # in one k_bfs_batch dispatch), timed with HIP events on its stream
     avg_ms = launch_ms / max(nlaunch, 1)
@@ -183,7 +225,8 @@ def main():
                                "symmetric CSR, direction by the %s" % (
                                    args.sources, args.scale, args.ef,
                                    "learned selector" if learned else "M/20 rule (no selector weights)"),
-                   "n": n, "m": m, "sources": srcs, "levels_dirs": stats[0]["dir"][:max(s["levels"] for s in stats)]},
+                   "n": n, "m": m, "sources": srcs, "levels_dirs": stats[0]["dir"][:max(s["levels"] for s in stats)],
+                   "tuning": knobs},
         "learned_vs_threshold": {"learned_gteps": round(gteps, 3), "threshold_gteps": round(thr_gteps, 3),
                                  "speedup": round(gteps / thr_gteps, 4),
                                  "launch_ms_learned": round(avg_ms, 4),
@@ -247,8 +290,10 @@ def main():
         ex["cpma_insert_edges_s"] = round(args.cpma_batch / np.mean(ins_t), 1)
         ex["cpma_remove_edges_s"] = round(args.cpma_batch / np.mean(rem_t), 1)
         ex["cpma_scale"] = args.cpma_scale
-        out["extras"] = ex
         C.close()
+        if args.job_scale:
+            ex["batched_job"] = batched_job(B, args)
+        out["extras"] = ex
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(G, srcs, args.cpu_budget)

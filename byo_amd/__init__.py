@@ -97,6 +97,8 @@ SIGNATURES = {
     "byo_profile_get": (_i32, [_vp, C.c_char_p, _P(_u64), _P(_dbl)]),
     "byo_profile_reset": (_i32, [_vp]),
     "byo_insert_batches_dev": (_i32, [_vp, _vp, _vp, _u32, _P(_u32), _P(_dbl)]),
+    "byo_set_tuning": (_i32, [_vp, C.c_char_p, C.c_int64]),
+    "byo_get_tuning": (_i32, [_vp, C.c_char_p, _P(C.c_int64)]),
     "byo_batch_job": (_i32, [_vp, _vp, _u32, _vp, _u32, _dbl, _i32, _vp, _vp, _vp, _P(_dbl)]),
     "byo_selector_load": (_i32, [_vp, _vp, _vp, _vp]),
     "byo_selector_load_engine": (_i32, [_vp, _i32, _vp, _vp, _vp]),
@@ -331,6 +333,23 @@ class Graph:
         ms = _dbl()
         check(lib().byo_ppr_multi(self._h, _ptr(seeds), k, damping, iters, _ptr(p), C.byref(ms)))
         return (p[:n * k].reshape(k, n) if want_p else None), ms.value
+
+    def set_tuning(self, key, value):
+        check(lib().byo_set_tuning(self._h, key.encode(), int(value)))
+
+    def get_tuning(self, key):
+        v = C.c_int64()
+        check(lib().byo_get_tuning(self._h, key.encode(), C.byref(v)))
+        return v.value
+
+    def apply_tuning(self, path=None):
+        """Applies the autotuned knobs stored for the closest graph size
+        (byo_amd/tuning.json); returns the knobs applied."""
+        from . import autotune
+        knobs = autotune.lookup(self.N(), self.M(), path)
+        for k, v in knobs.items():
+            self.set_tuning(k, v)
+        return knobs
 
     def batch_job(self, bfs_srcs, ppr_seeds, damping=0.85, ppr_iters=30, want_parents=True, want_ppr=True):
         """k_bfs BFS sources (one shared traversal) and k_ppr personalized

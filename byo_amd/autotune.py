@@ -1,0 +1,122 @@
+"""Autotuner of the block and tile sizes (the tuning knobs of byo_set_tuning).
+
+For one graph it times every knob's candidates on the device -- the other
+knobs held at their best so far (one coordinate-descent pass) -- keeps the
+fastest, and stores the winners per graph size in byo_amd/tuning.json.
+Graph.apply_tuning() later applies the entry of the closest size.  What each
+knob is timed on:
+
+    coop_blocks_per_cu, pieces   8-source BFS in one shared traversal (launch ms)
+    ppr_blocks_per_cu            32-seed personalized PageRank, 5 iterations
+    pr_lanes, pr_grid            FP32 PageRank, 5 iterations
+
+    python -m byo_amd.autotune --scale 22 --scale 23
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TUNING = os.path.join(HERE, "tuning.json")
+
+CANDIDATES = {
+    "coop_blocks_per_cu": [2, 3, 4],
+    "pieces": [4, 8, 16],
+    "ppr_blocks_per_cu": [4, 8],
+    "pr_lanes": [4, 8, 16, 32],
+    "pr_grid": [2048, 4096, 8192, 16384],
+}
+WORKLOAD = {"coop_blocks_per_cu": "bfs", "pieces": "bfs", "ppr_blocks_per_cu": "ppr", "pr_lanes": "pr",
+            "pr_grid": "pr"}
+
+
+def _sources(deg, k, seed):
+    from .selector import sources
+    return sources(deg, k, seed)
+
+
+def measure(G, workload, srcs, seeds, repeats=3):
+    """Best-of-repeats device ms of one workload."""
+    from . import PR_FP32
+    best = float("inf")
+    for _ in range(repeats):
+        if workload == "bfs":
+            G.profile(True)
+            G.profile_reset()
+            G.bfs_multi(srcs, want_parents=False)
+            ms = G.profile_get("bfs_launch")[1]
+        elif workload == "ppr":
+            ms = G.ppr(seeds, 0.85, 5, want_p=False)[1]
+        else:
+            ms = G.pagerank(0.85, 5, 0.0, PR_FP32, want_p=False)[2]
+        best = min(best, ms)
+    return best
+
+
+def tune(G, label=""):
+    """Coordinate descent over CANDIDATES; returns (knobs, times)."""
+    deg = G.degrees()
+    srcs = _sources(deg, 8, 11)
+    seeds = _sources(deg, 32, 12)
+    knobs, times = {}, {}
+    for key, cands in CANDIDATES.items():
+        wl = WORKLOAD[key]
+        measure(G, wl, srcs, seeds, 1)  # warm
+        res = {}
+        for v in cands:
+            G.set_tuning(key, v)
+            res[v] = measure(G, wl, srcs, seeds)
+        best = min(res, key=res.get)
+        G.set_tuning(key, best)
+        knobs[key] = best
+        times[key] = {str(v): round(t, 4) for v, t in res.items()}
+        print("%s %s: %s -> %s" % (label, key, times[key], best), flush=True)
+    return knobs, times
+
+
+def load(path=None):
+    p = path or TUNING
+    if not os.path.exists(p):
+        return {"entries": []}
+    with open(p) as f:
+        return json.load(f)
+
+
+def lookup(n, m, path=None):
+    """Knobs of the stored entry closest in log2 of the edge count (and of the
+    vertex count on ties); {} when nothing is stored."""
+    entries = load(path)["entries"]
+    if not entries or m <= 0:
+        return {}
+    def dist(e):
+        return (abs(math.log2(max(e["m"], 1)) - math.log2(m)), abs(math.log2(max(e["n"], 1)) - math.log2(max(n, 1))))
+    return dict(min(entries, key=dist)["knobs"])
+
+
+def main():
+    from . import CsrGraph
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scale", type=int, action="append", default=None)
+    ap.add_argument("--ef", type=int, default=16)
+    ap.add_argument("--out", default=TUNING)
+    args = ap.parse_args()
+    db = load(args.out)
+    for scale in args.scale or [22]:
+        t0 = time.time()
+        G = CsrGraph.rmat(scale, args.ef)
+        label = "rmat%d_ef%d" % (scale, args.ef)
+        knobs, times = tune(G, label)
+        entry = {"graph": label, "n": G.N(), "m": G.M(), "knobs": knobs, "times_ms": times}
+        db["entries"] = [e for e in db["entries"] if e["graph"] != label] + [entry]
+        G.close()
+        print("%s tuned in %.0fs" % (label, time.time() - t0), flush=True)
+    with open(args.out, "w") as f:
+        json.dump(db, f, indent=1)
+    print(json.dumps(db), flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

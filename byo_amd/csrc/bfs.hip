@@ -478,9 +478,9 @@ int byo_bfs_multi(byo_graph_t g, const uint32_t *srcs, uint32_t k, uint32_t *par
     for (uint32_t i = 0; i < k; i++) BYO_REQUIRE(srcs[i] < g->n, BYO_E_INVAL, "source out of range");
     if (k == 0) return;
     const uint64_t mx = (g->dir_policy == BYO_DIR_MLP) ? graph_max_degree(g) : 0;
-    const char *mode = std::getenv("BYO_BFS_MODE");
-    const bool levels_mode = mode && std::strcmp(mode, "levels") == 0;
-    const bool seq_mode = mode && std::strcmp(mode, "sequential") == 0;
+    const int64_t engine = g->tuning("bfs_engine");
+    const bool levels_mode = engine == 2;
+    const bool seq_mode = engine == 1;
     if (k > 1 && !levels_mode && !seq_mode) {
       // default for several sources: up to 64 at a time share one traversal
       const uint64_t n = g->n;

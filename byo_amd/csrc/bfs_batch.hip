@@ -41,7 +41,7 @@ void prepare(byo_graph *g, const uint32_t *srcs, uint32_t k, uint64_t maxdeg, Ba
   a.nlev = reinterpret_cast<uint32_t *>(out + kNlevOff);
   a.tot = reinterpret_cast<unsigned long long *>(out + kTotOff);
   a.dc = dir_ctl(g, maxdeg, s, true);
-  a.pieces = g->buf("bfs_pieces", (kPieces * 2048 * 4 + 1) * 8 + 64).as<uint64_t>();
+  a.pieces = make_pieces(g);
   a.bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
   a.bsum = g->buf("bfs_batch_bsum", (size_t)nblocks * 8 + 64).as<unsigned long long>();
   a.seen = g->buf("bfs_batch_seen", a.nm * mask_bytes + 64).p;
@@ -97,7 +97,7 @@ namespace {
 template <class M>
 double launch(byo_graph *g, const uint32_t *srcs, uint32_t k, uint64_t maxdeg, BatchArgs &a) {
   const void *kern = reinterpret_cast<const void *>(k_bfs_batch<M>);
-  const int nblocks = coop_grid(kern, kBlock);
+  const int nblocks = coop_grid(kern, kBlock, (int)g->tuning("coop_blocks_per_cu"));
   prepare(g, srcs, k, maxdeg, a, (int)sizeof(M), nblocks);
   void *args[] = {&a};
   PhaseTimer t(g->stream, true);

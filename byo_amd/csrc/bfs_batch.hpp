@@ -78,7 +78,7 @@ struct BatchArgs {
   uint32_t *nlev;           // levels of the whole traversal
   unsigned long long *tot;  // k x {vertices reached, sum of their degrees}
   DirCtl dc;
-  uint64_t *pieces;         // kPieces * nwaves + 1 word bounds
+  Pieces pieces;            // weight-balanced word pieces of the grid
   GridBar *bar;
 };
 

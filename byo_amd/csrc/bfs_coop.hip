@@ -76,7 +76,7 @@ struct CoopArgs {
   uint32_t *nlevels;        // k
   unsigned long long *tot;  // k x {vertices reached, sum of their degrees}
   DirCtl dc;
-  uint64_t *pieces;         // kPieces * nwaves + 1 word bounds
+  Pieces pieces;            // weight-balanced word pieces of the grid
   GridBar *bar;
 };
 
@@ -519,22 +519,34 @@ DirCtl dir_ctl(byo_graph *g, uint64_t maxdeg, hipStream_t s, bool batched) {
   return d;
 }
 
-int coop_grid(const void *kernel, int block) {
+int coop_grid(const void *kernel, int block, int cap) {
   static std::mutex mu;
-  static std::map<std::pair<int, const void *>, int> cache;
+  static std::map<std::pair<int, const void *>, std::pair<int, int>> cache;  // (CUs, resident blocks per CU)
   int dev = 0;
   BYO_HIP(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(mu);
-  auto it = cache.find({dev, kernel});
-  if (it != cache.end()) return it->second;
-  int cus = 0, per = 0;
-  BYO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  BYO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0));
-  BYO_REQUIRE(per >= 1, BYO_E_HIP, "cooperative kernel cannot be resident");
-  const int nb = cus * std::min(per, 4);
+  std::pair<int, int> cp;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find({dev, kernel});
+    if (it == cache.end()) {
+      int cus = 0, per = 0;
+      BYO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      BYO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0));
+      BYO_REQUIRE(per >= 1, BYO_E_HIP, "cooperative kernel cannot be resident");
+      it = cache.emplace(std::make_pair(dev, kernel), std::make_pair(cus, per)).first;
+    }
+    cp = it->second;
+  }
+  const int nb = cp.first * std::max(1, std::min(cp.second, cap));
   BYO_REQUIRE(nb <= 2048, BYO_E_HIP, "grid barrier sized for at most 2048 blocks");
-  cache[{dev, kernel}] = nb;
   return nb;
+}
+
+Pieces make_pieces(byo_graph *g) {
+  Pieces pc;
+  pc.per_wave = (uint32_t)g->tuning("pieces");
+  pc.bound = g->buf("bfs_pieces", ((uint64_t)kMaxPieces * 2048 * 4 + 1) * 8 + 64).as<uint64_t>();
+  return pc;
 }
 
 // Runs k BFS in one cooperative launch.  Results: k*n parents in scratch
@@ -566,8 +578,8 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   a.nlevels = g->buf("bfs_coop_nlev", (size_t)k * 4 + 64).as<uint32_t>();
   a.tot = g->buf("bfs_coop_tot", (size_t)k * 16 + 64).as<unsigned long long>();
   a.dc = dir_ctl(g, maxdeg, s, false);
-  a.pieces = g->buf("bfs_pieces", (kPieces * 2048 * 4 + 1) * 8 + 64).as<uint64_t>();
-  const int nblocks = coop_grid(reinterpret_cast<const void *>(k_bfs_coop), kBlock);
+  a.pieces = make_pieces(g);
+  const int nblocks = coop_grid(reinterpret_cast<const void *>(k_bfs_coop), kBlock, (int)g->tuning("coop_blocks_per_cu"));
   a.bsum = g->buf("bfs_coop_bsum", (size_t)nblocks * 8 + 64).as<unsigned long long>();
   a.bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
   BYO_HIP(hipMemsetAsync(a.bar, 0, sizeof(GridBar), s));

@@ -139,6 +139,30 @@ struct PhaseTimer {
 }  // namespace byo
 
 // ------------------------------------------------------------ the handle
+namespace byo {
+// Tuning knobs of a handle: block and tile sizes the autotuner
+// (byo_amd/autotune.py) picks per graph and sets through byo_set_tuning.
+struct TuneSpec {
+  const char *key;
+  int64_t dflt, lo, hi;
+  const char *what;
+};
+inline const std::vector<TuneSpec> &tune_specs() {
+  static const std::vector<TuneSpec> specs = {
+      {"coop_blocks_per_cu", 4, 1, 4, "blocks per CU of the persistent traversal launches"},
+      {"ppr_blocks_per_cu", 4, 1, 8, "blocks per CU of the persistent personalized PageRank launch"},
+      {"pieces", 8, 1, 16, "weight-balanced word pieces per wave in vertex-driven passes"},
+      {"pr_lanes", 0, 0, 32, "PageRank lanes per vertex (4, 8, 16, 32; 0: from the average degree)"},
+      {"pr_grid", 4096, 256, 65536, "PageRank pull grid (blocks of 256)"},
+      {"pr_seg_log2", 40, 10, 40, "log2 of the PageRank source-segment width (40: one segment)"},
+      {"bfs_engine", 0, 0, 2,
+       "BFS of several sources: 0 one shared traversal per 64, 1 one persistent launch taking them in turn, "
+       "2 host-driven level loop"},
+  };
+  return specs;
+}
+}  // namespace byo
+
 struct byo_graph {
   int kind = 0;  // 0 CSR, 1 CPMA
   int device = 0;
@@ -178,6 +202,15 @@ struct byo_graph {
   // Pinned host staging for small readbacks.
   void *pinned = nullptr;
   size_t pinned_bytes = 0;
+
+  std::map<std::string, int64_t> tune;  // knobs set through byo_set_tuning
+  int64_t tuning(const char *key) const {
+    auto it = tune.find(key);
+    if (it != tune.end()) return it->second;
+    for (const auto &t : byo::tune_specs())
+      if (std::strcmp(t.key, key) == 0) return t.dflt;
+    return 0;
+  }
 
   byo::Profiler prof;
   void account(const char *phase, double ms) {

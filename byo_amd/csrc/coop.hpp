@@ -81,14 +81,21 @@ __device__ __forceinline__ unsigned long long block_sum(unsigned long long v, un
 // (Dealing single words out round-robin lines RMAT's heavy low-id words up
 // on the same few waves; claiming words through atomics costs a round trip
 // per word.)
-constexpr uint64_t kPieces = 8;
+// Pieces per wave is a tuning knob ("pieces", 8 by default, at most
+// kMaxPieces).
+constexpr uint32_t kMaxPieces = 16;
+struct Pieces {
+  uint64_t *bound;  // per_wave * nwaves + 1 word bounds
+  uint32_t per_wave;
+};
 
 // Piece bounds, computed once per launch by the whole grid before its first
 // barrier: bound[i] = first word w with W(w) >= i * W(nwords) / P, where
-// W(w) = off[v] + 8 v, v = min(64 w, n), and P = kPieces * nwaves.
-__device__ inline void pieces_init(uint64_t *bound, const uint64_t *off, uint64_t n, uint64_t nwords,
+// W(w) = off[v] + 8 v, v = min(64 w, n), and P = per_wave * nwaves.
+__device__ inline void pieces_init(const Pieces &pc, const uint64_t *off, uint64_t n, uint64_t nwords,
                                    uint64_t nwaves) {
-  const uint64_t P = kPieces * nwaves;
+  uint64_t *bound = pc.bound;
+  const uint64_t P = (uint64_t)pc.per_wave * nwaves;
   const uint64_t total = off[n] + 8 * n;
   for (uint64_t i = global_tid(); i <= P; i += grid_threads()) {
     const uint64_t target = (uint64_t)(((unsigned __int128)total * i) / P);
@@ -105,11 +112,11 @@ __device__ inline void pieces_init(uint64_t *bound, const uint64_t *off, uint64_
 // Runs body(w) for every word of this wave's pieces; all 64 lanes call it
 // together and body runs wave-uniformly.
 template <class F>
-__device__ __forceinline__ void for_my_words(const uint64_t *bound, uint64_t wave, uint64_t nwaves, F &&body) {
-  for (uint64_t i = 0; i < kPieces; i++) {
+__device__ __forceinline__ void for_my_words(const Pieces &pc, uint64_t wave, uint64_t nwaves, F &&body) {
+  for (uint64_t i = 0; i < pc.per_wave; i++) {
     const uint64_t p = wave + i * nwaves;
-    const uint64_t w1 = bound[p + 1];
-    for (uint64_t w = bound[p]; w < w1; w++) body(w);
+    const uint64_t w1 = pc.bound[p + 1];
+    for (uint64_t w = pc.bound[p]; w < w1; w++) body(w);
   }
 }
 
@@ -200,7 +207,11 @@ __device__ inline int decide(const DirCtl &a, bool is_dense, uint64_t fs, uint64
 // selector weights of the single-source or the batched traversal).
 DirCtl dir_ctl(byo_graph *g, uint64_t maxdeg, hipStream_t s, bool batched);
 
-// Grid size of a cooperative kernel: CUs x min(resident blocks per CU, 4).
-int coop_grid(const void *kernel, int block);
+// Grid size of a cooperative kernel: CUs x min(resident blocks per CU, cap)
+// (cap from the handle's tuning, at most 8; at most 2048 blocks).
+int coop_grid(const void *kernel, int block, int cap);
+
+// Word pieces of a cooperative launch (buffer and the "pieces" knob).
+Pieces make_pieces(byo_graph *g);
 
 }  // namespace byo

@@ -548,6 +548,36 @@ int byo_map_sum(byo_graph_t g, uint64_t *sum, double *device_ms) {
   });
 }
 
+int byo_set_tuning(byo_graph_t g, const char *key, int64_t value) {
+  return guard([&] {
+    BYO_REQUIRE(g && key, BYO_E_INVAL, "null argument");
+    std::lock_guard<std::mutex> lk(g->mu);
+    for (const auto &t : tune_specs()) {
+      if (std::strcmp(t.key, key) != 0) continue;
+      BYO_REQUIRE(value >= t.lo && value <= t.hi, BYO_E_INVAL, std::string("tuning value out of range for ") + key);
+      if (std::strcmp(key, "pr_lanes") == 0)
+        BYO_REQUIRE(value == 0 || value == 4 || value == 8 || value == 16 || value == 32, BYO_E_INVAL,
+                    "pr_lanes must be 0, 4, 8, 16 or 32");
+      g->tune[key] = value;
+      return;
+    }
+    throw Error(BYO_E_INVAL, std::string("unknown tuning knob ") + key);
+  });
+}
+
+int byo_get_tuning(byo_graph_t g, const char *key, int64_t *value) {
+  return guard([&] {
+    BYO_REQUIRE(g && key && value, BYO_E_INVAL, "null argument");
+    std::lock_guard<std::mutex> lk(g->mu);
+    for (const auto &t : tune_specs())
+      if (std::strcmp(t.key, key) == 0) {
+        *value = g->tuning(key);
+        return;
+      }
+    throw Error(BYO_E_INVAL, std::string("unknown tuning knob ") + key);
+  });
+}
+
 int byo_profile_enable(byo_graph_t g, int on) {
   return guard([&] {
     BYO_REQUIRE(g, BYO_E_INVAL, "null handle");

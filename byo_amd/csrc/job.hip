@@ -32,7 +32,7 @@ struct PprArgs {
   float damping;
   float *p;       // n x K values of the last iteration
   float *c0, *c1;  // n x K contributions p/deg, alternating
-  uint64_t *pieces;
+  Pieces pieces;
 };
 
 // Every thread of the grid.  pieces must already hold the word pieces of
@@ -122,7 +122,7 @@ PprArgs ppr_args(byo_graph *g, const uint32_t *seeds, uint32_t k, double damping
   uint32_t *dseeds = g->buf("ppr_seeds", 64 * 4 + 64).as<uint32_t>();
   if (k) BYO_HIP(hipMemcpyAsync(dseeds, seeds, k * 4, hipMemcpyHostToDevice, g->stream));
   q.seeds = dseeds;
-  q.pieces = g->buf("bfs_pieces", (kPieces * 2048 * 4 + 1) * 8 + 64).as<uint64_t>();
+  q.pieces = make_pieces(g);
   return q;
 }
 
@@ -140,7 +140,7 @@ template <class M>
 double job_launch(byo_graph *g, const uint32_t *srcs, uint32_t kb, uint64_t maxdeg, batch::BatchArgs &a,
                   PprArgs &q) {
   const void *kern = reinterpret_cast<const void *>(k_job<M>);
-  const int nblocks = coop_grid(kern, kCoopBlock);
+  const int nblocks = coop_grid(kern, kCoopBlock, (int)g->tuning("coop_blocks_per_cu"));
   batch::prepare(g, srcs, std::max<uint32_t>(kb, 1), maxdeg, a, (int)sizeof(M), nblocks);
   a.k = kb;
   void *args[] = {&a, &q};
@@ -169,7 +169,7 @@ int byo_ppr_multi(byo_graph_t g, const uint32_t *seeds, uint32_t k, double dampi
     if (k == 0 || g->n == 0) return;
     PprArgs q = ppr_args(g, seeds, k, damping, iters);
     const void *kern = reinterpret_cast<const void *>(k_ppr_persist);
-    const int nblocks = coop_grid(kern, kCoopBlock);
+    const int nblocks = coop_grid(kern, kCoopBlock, (int)g->tuning("ppr_blocks_per_cu"));
     GridBar *bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
     BYO_HIP(hipMemsetAsync(bar, 0, sizeof(GridBar), g->stream));
     void *args[] = {&q, &bar};

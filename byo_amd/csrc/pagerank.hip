@@ -180,12 +180,11 @@ void pr_pass_g(int G, bool first, bool last, const uint64_t *lo, const uint64_t 
   }
 }
 
-// Segment width (vertices) so that one segment's contributions take about
-// 2 MiB of L2; BYO_PR_SEG_LOG2 overrides it (tuning).
+// Source-segment width in vertices (tuning knob pr_seg_log2; one segment
+// by default).
 template <class T>
-uint64_t segment_width(uint64_t n) {
-  int lg = 40;  // one segment unless tuned
-  if (const char *e = std::getenv("BYO_PR_SEG_LOG2")) lg = std::atoi(e);
+uint64_t segment_width(byo_graph *g, uint64_t n) {
+  const int lg = (int)g->tuning("pr_seg_log2");
   uint64_t W = 1ull << lg;
   return W < n ? W : n;
 }
@@ -200,7 +199,7 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   p[1] = g->buf("pr_p1", n * sizeof(T) + 64).as<T>();
   c[0] = g->buf("pr_c0", n * sizeof(T) + 64).as<T>();
   c[1] = g->buf("pr_c1", n * sizeof(T) + 64).as<T>();
-  const uint64_t W = segment_width<T>(n);
+  const uint64_t W = segment_width<T>(g, n);
   const int S = (int)((n + W - 1) / W);
   T *acc_s = S > 1 ? g->buf("pr_acc_s", n * sizeof(T) + 64).as<T>() : nullptr;
   T *acc_c = (S > 1 && kComp) ? g->buf("pr_acc_c", n * sizeof(T) + 64).as<T>() : nullptr;
@@ -209,8 +208,8 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   const double avg = n ? (double)g->m / (double)n / S : 0.0;
   int G = 4;
   while (G < 32 && G < avg / 2) G *= 2;
-  if (const char *e = std::getenv("BYO_PR_LANES")) G = std::atoi(e);
-  const unsigned grid = 4096;
+  if (g->tuning("pr_lanes")) G = (int)g->tuning("pr_lanes");
+  const unsigned grid = (unsigned)g->tuning("pr_grid");
   double *partial = g->buf("pr_partial", (grid + 8) * sizeof(double)).as<double>();
   double *l1dev = g->buf("pr_l1", (size_t)(max_iters + 1) * sizeof(double)).as<double>();
   const uint64_t *off = g->off.as<uint64_t>();

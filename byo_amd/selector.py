@@ -197,16 +197,14 @@ def _timed(G, srcs, engine):
     """Device ms of the traversals of srcs in one call: the batched engine
     (bfs_multi, shared traversals of up to 64 sources) or the one-source
     engine (every source in turn, still one persistent launch)."""
-    if engine == "single":
-        os.environ["BYO_BFS_MODE"] = "sequential"
+    G.set_tuning("bfs_engine", 1 if engine == "single" else 0)
     try:
         G.profile(True)
         G.profile_reset()
         _, st = G.bfs_multi(srcs, want_parents=False)
         _, ms = G.profile_get("bfs_launch")
     finally:
-        if engine == "single":
-            del os.environ["BYO_BFS_MODE"]
+        G.set_tuning("bfs_engine", 0)
     return ms, st
 
 

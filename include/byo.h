@@ -259,6 +259,23 @@ int byo_profile_reset(byo_graph_t g);
 /* Learned direction selector (MLP 6->32->32->3, FP32)                     */
 /* ---------------------------------------------------------------------- */
 #define BYO_MLP_NPARAM 1379
+/* Tuning knobs of a handle -- block and tile sizes picked per graph by the
+ * project's autotuner (byo_amd/autotune.py), which times candidates on the
+ * device and stores the winners in byo_amd/tuning.json.  No counterpart in
+ * the reference (its block sizes are compile-time constants).  Keys:
+ *   coop_blocks_per_cu  1..4  blocks per CU of the persistent traversals (4)
+ *   ppr_blocks_per_cu   1..8  blocks per CU of persistent PPR (4)
+ *   pieces              1..16 word pieces per wave in vertex passes (8)
+ *   pr_lanes            0,4,8,16,32 PageRank lanes per vertex (0: auto)
+ *   pr_grid             256..65536 PageRank pull grid (4096)
+ *   pr_seg_log2         10..40 PageRank source-segment width (40: one)
+ *   bfs_engine          0..2 several-source BFS: 0 shared traversal per 64
+ *                       sources, 1 one persistent launch taking them in
+ *                       turn, 2 host-driven level loop (0)
+ * Unknown keys and out-of-range values give BYO_E_INVAL. */
+int byo_set_tuning(byo_graph_t g, const char *key, int64_t value);
+int byo_get_tuning(byo_graph_t g, const char *key, int64_t *value);
+
 /* Loads selector weights (layout W1[32x6] b1 W2[32x32] b2 W3[3x32] b3,
  * row-major) and the 6-feature normalisation (mean[6], inv_std[6]). */
 int byo_selector_load(byo_graph_t g, const float *params, const float *mean,

@@ -118,11 +118,11 @@ def test_batch_matches_sequential_engine(B, s16):
     g, G = s16
     srcs = O.bfs_sources(g, 8, seed=17)
     _, st_b = G.bfs_multi(srcs, want_parents=False)
-    os.environ["BYO_BFS_MODE"] = "sequential"
+    G.set_tuning("bfs_engine", 1)
     try:
         par_s, st_s = G.bfs_multi(srcs)
     finally:
-        del os.environ["BYO_BFS_MODE"]
+        G.set_tuning("bfs_engine", 0)
     for j, s in enumerate(srcs):
         assert st_b[j]["reached"] == st_s[j]["reached"]
         assert st_b[j]["edges_reached"] == st_s[j]["edges_reached"]

@@ -1,0 +1,84 @@
+"""Every tuning knob value the autotuner may pick keeps results exact: BFS
+trees valid with the oracle's levels for every several-source engine,
+PageRank within 1e-6 L1 of the oracle's FP64 vector, personalized PageRank
+within 1e-6 L1 of the oracle's; bad keys and values are rejected."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def B():
+    import byo_amd
+    assert byo_amd.device_count() > 0, "no GPU visible"
+    return byo_amd
+
+
+@pytest.fixture(scope="module")
+def s15(B):
+    g = O.rmat_graph(15)
+    return g, B.CsrGraph.from_csr(g.offsets, g.ids)
+
+
+def check_bfs(g, G, srcs):
+    par, _ = G.bfs_multi(srcs)
+    for j, s in enumerate(srcs):
+        assert O.check_bfs_tree(g, s, par[j], O.bfs_levels(g, s)) == 0
+
+
+@pytest.mark.parametrize("key,values", [("coop_blocks_per_cu", [1, 2, 3, 4]), ("pieces", [1, 4, 16]),
+                                        ("bfs_engine", [0, 1, 2])])
+def test_bfs_knobs(B, s15, key, values):
+    g, G = s15
+    srcs = O.bfs_sources(g, 6, seed=4)
+    try:
+        for v in values:
+            print(key, v, flush=True)
+            G.set_tuning(key, v)
+            assert G.get_tuning(key) == v
+            check_bfs(g, G, srcs)
+            par, _ = G.bfs(srcs[0])
+            assert O.check_bfs_tree(g, srcs[0], par, O.bfs_levels(g, srcs[0])) == 0
+    finally:
+        G.set_tuning(key, {"coop_blocks_per_cu": 4, "pieces": 8, "bfs_engine": 0}[key])
+
+
+@pytest.mark.parametrize("key,values", [("pr_lanes", [4, 8, 16, 32, 0]), ("pr_grid", [256, 2048, 16384]),
+                                        ("pr_seg_log2", [12, 40])])
+def test_pagerank_knobs(B, s15, key, values):
+    g, G = s15
+    ref = O.pagerank(g, 0.85, 20)
+    try:
+        for v in values:
+            print(key, v, flush=True)
+            G.set_tuning(key, v)
+            p, _, _ = G.pagerank(0.85, 20, 0.0, B.PR_FP32)
+            assert np.abs(p - ref).sum() <= 1e-6
+    finally:
+        G.set_tuning(key, {"pr_lanes": 0, "pr_grid": 4096, "pr_seg_log2": 40}[key])
+
+
+def test_ppr_blocks_knob(B, s15):
+    g, G = s15
+    seeds = O.bfs_sources(g, 4, seed=8)
+    base, _ = G.ppr(seeds, 0.85, 15)
+    for v in (1, 8):
+        G.set_tuning("ppr_blocks_per_cu", v)
+        p, _ = G.ppr(seeds, 0.85, 15)
+        assert np.array_equal(p, base)  # one lane sums each list in order: identical
+    G.set_tuning("ppr_blocks_per_cu", 4)
+    for j, s in enumerate(seeds):
+        assert np.abs(base[j] - O.ppr(g, s, 0.85, 15)).sum() <= 1e-6
+
+
+def test_bad_knobs(B, s15):
+    _, G = s15
+    with pytest.raises(B.ByoError):
+        G.set_tuning("no_such_knob", 1)
+    with pytest.raises(B.ByoError):
+        G.set_tuning("pieces", 0)
+    with pytest.raises(B.ByoError):
+        G.set_tuning("pr_lanes", 6)
