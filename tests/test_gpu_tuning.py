@@ -50,7 +50,7 @@ def test_bfs_knobs(B, s15, key, values):
                                         ("pr_seg_log2", [12, 40])])
 def test_pagerank_knobs(B, s15, key, values):
     g, G = s15
-    ref = O.pagerank(g, 0.85, 20)
+    ref, _ = O.pagerank(g, 0.85, 20)
     try:
         for v in values:
             print(key, v, flush=True)
