@@ -10,7 +10,7 @@
 // seeds are kept vertex-major (p[v*K + j]), so every edge gathers the K
 // contributions of its source as one contiguous run; W = K rounded up to a
 // power of two lanes serve one vertex, 64/W vertices per wave step, and
-// each lane sums its seed's contributions in adjacency order (four
+// each lane sums its seed's contributions in adjacency order (eight
 // neighbour ids and gathers in flight per step, the adds in order), so the
 // result is the same as one thread summing the list.  The vertices of a
 // sweep are shared out by the traversal's weight-balanced word pieces;
@@ -20,6 +20,10 @@
 
 namespace byo {
 namespace {
+
+// Neighbours whose ids and contributions are loaded before any is added
+// (the adds stay in adjacency order).
+constexpr int kPprUnroll = 8;
 
 struct PprArgs {
   const uint64_t *off;
@@ -65,14 +69,15 @@ __device__ __forceinline__ void ppr_body(const PprArgs &q, GridBar *bar) {
           const uint64_t b = q.off[v], e = q.off[v + 1];
           Acc<float> a{0.f, 0.f};
           uint64_t k = b;
-          for (; k + 4 <= e; k += 4) {
-            const uint32_t u0 = q.ids[k], u1 = q.ids[k + 1], u2 = q.ids[k + 2], u3 = q.ids[k + 3];
-            const float f0 = cin[(uint64_t)u0 * K + sub], f1 = cin[(uint64_t)u1 * K + sub];
-            const float f2 = cin[(uint64_t)u2 * K + sub], f3 = cin[(uint64_t)u3 * K + sub];
-            kahan_add(a, f0);
-            kahan_add(a, f1);
-            kahan_add(a, f2);
-            kahan_add(a, f3);
+          for (; k + kPprUnroll <= e; k += kPprUnroll) {
+            uint32_t u[kPprUnroll];
+            float f[kPprUnroll];
+#pragma unroll
+            for (int t = 0; t < kPprUnroll; t++) u[t] = q.ids[k + t];
+#pragma unroll
+            for (int t = 0; t < kPprUnroll; t++) f[t] = cin[(uint64_t)u[t] * K + sub];
+#pragma unroll
+            for (int t = 0; t < kPprUnroll; t++) kahan_add(a, f[t]);
           }
           for (; k < e; k++) kahan_add(a, cin[(uint64_t)q.ids[k] * K + sub]);
           const float pv = q.damping * (a.s - a.c) + (my_seed == v ? teleport : 0.0f);
@@ -140,7 +145,10 @@ template <class M>
 double job_launch(byo_graph *g, const uint32_t *srcs, uint32_t kb, uint64_t maxdeg, batch::BatchArgs &a,
                   PprArgs &q) {
   const void *kern = reinterpret_cast<const void *>(k_job<M>);
-  const int nblocks = coop_grid(kern, kCoopBlock, (int)g->tuning("coop_blocks_per_cu"));
+  // the PageRank sweeps dominate a job: it runs at the larger of the two
+  // tuned residencies (the traversal body caps it at its own occupancy)
+  const int cap = (int)std::max(g->tuning("coop_blocks_per_cu"), g->tuning("ppr_blocks_per_cu"));
+  const int nblocks = coop_grid(kern, kCoopBlock, cap);
   batch::prepare(g, srcs, std::max<uint32_t>(kb, 1), maxdeg, a, (int)sizeof(M), nblocks);
   a.k = kb;
   void *args[] = {&a, &q};
