@@ -290,6 +290,25 @@ def main():
         ex["cpma_insert_edges_s"] = round(args.cpma_batch / np.mean(ins_t), 1)
         ex["cpma_remove_edges_s"] = round(args.cpma_batch / np.mean(rem_t), 1)
         ex["cpma_scale"] = args.cpma_scale
+        # Batch_insert_runner's sweep (run_unweighted.h:781-893): b = 10 ..
+        # 1e7 in powers of 10, the same update stream prefix for every size,
+        # rounds + 1 with round 0 untimed, insert then remove the batch
+        sweep = {}
+        b = 10
+        while b <= args.cpma_batch:
+            ti, tr = [], []
+            for r in range(4):
+                a = time.perf_counter()
+                C.insert_batch_dev(dkeys.ptr, b)
+                m1 = time.perf_counter()
+                C.remove_batch_dev(dkeys.ptr, b)
+                m2 = time.perf_counter()
+                if r:
+                    ti.append(m1 - a)
+                    tr.append(m2 - m1)
+            sweep[str(b)] = {"insert_edges_s": round(b / np.mean(ti), 1), "remove_edges_s": round(b / np.mean(tr), 1)}
+            b *= 10
+        ex["cpma_sweep"] = sweep
         C.close()
         if args.job_scale:
             ex["batched_job"] = batched_job(B, args)

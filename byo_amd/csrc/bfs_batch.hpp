@@ -86,6 +86,9 @@ constexpr int kBlock = kCoopBlock;
 constexpr int kWaves = kCoopWaves;
 constexpr int kPerLane = 4;
 constexpr uint64_t kChunk = 64 * kPerLane;
+// loads per lane of one pull window (the pull's remaining edges are
+// latency-bound per wave, so a window keeps this many in flight)
+constexpr int kPullPerLane = 8;
 
 // Per-vertex source masks.  One-byte masks are updated through the 32-bit
 // word that holds them.
@@ -501,7 +504,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
           // the remaining edges of vertices with bits still uncovered are
           // scanned wave-wide: the word's unfinished vertices lay their
           // edges end to end (exclusive prefix of remaining degrees in LDS,
-          // owner found by a 6-step search) and every window of 256 edges
+          // owner found by a 6-step search) and every window of 512 edges
           // issues its 4 loads per lane before using any of them.  Each
           // window starts at the first vertex that still has bits to find;
           // bits found are merged into the vertex's LDS mask by atomicOr, and
@@ -525,22 +528,22 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
               // ids[0]) so that all of a window's loads are in flight at once
               // the 4 owner searches advance in lockstep so each step's LDS
               // reads are issued together
-              uint32_t uu[kPerLane];
-              int xx[kPerLane];
-              M hm[kPerLane];
+              uint32_t uu[kPullPerLane];
+              int xx[kPullPerLane];
+              M hm[kPullPerLane];
 #pragma unroll
-              for (int t = 0; t < kPerLane; t++) xx[t] = 0;
+              for (int t = 0; t < kPullPerLane; t++) xx[t] = 0;
 #pragma unroll
               for (int st = 32; st > 0; st >>= 1) {
-                uint64_t sv[kPerLane];
+                uint64_t sv[kPullPerLane];
 #pragma unroll
-                for (int t = 0; t < kPerLane; t++) sv[t] = win.sc[wv][xx[t] + st];
+                for (int t = 0; t < kPullPerLane; t++) sv[t] = win.sc[wv][xx[t] + st];
 #pragma unroll
-                for (int t = 0; t < kPerLane; t++)
+                for (int t = 0; t < kPullPerLane; t++)
                   xx[t] += (sv[t] <= t0 + (uint64_t)t * 64 + lane) ? st : 0;
               }
 #pragma unroll
-              for (int t = 0; t < kPerLane; t++) {
+              for (int t = 0; t < kPullPerLane; t++) {
                 const uint64_t idx = t0 + (uint64_t)t * 64 + lane;
                 const int x = xx[t];
                 const M Ux = (M)win.f[wv][x], ax = (M)res[wv][x];
@@ -549,13 +552,13 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
                 hm[t] = ok ? (M)(Ux & (M)~ax) : (M)0;
                 uu[t] = a.ids[ok ? (uint64_t)(bs + (int64_t)idx) : 0];
               }
-              M fv[kPerLane];
+              M fv[kPullPerLane];
 #pragma unroll
-              for (int t = 0; t < kPerLane; t++) fv[t] = fmask[uu[t]];
+              for (int t = 0; t < kPullPerLane; t++) fv[t] = fmask[uu[t]];
 #pragma unroll
-              for (int t = 0; t < kPerLane; t++) hm[t] &= fv[t];
+              for (int t = 0; t < kPullPerLane; t++) hm[t] &= fv[t];
 #pragma unroll
-              for (int t = 0; t < kPerLane; t++) {
+              for (int t = 0; t < kPullPerLane; t++) {
                 if (hm[t]) {
                   atomicOr(reinterpret_cast<unsigned long long *>(&res[wv][xx[t]]), (unsigned long long)hm[t]);
                   M nw = hm[t];
@@ -567,7 +570,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
                 }
               }
               wave_lds_sync();
-              t0 += 64 * kPerLane;
+              t0 += 64 * kPullPerLane;
             }
           }
           wave_lds_sync();

@@ -172,17 +172,30 @@ __global__ void __launch_bounds__(256) k_ins_count(const uint64_t *keys, UpdDev 
 // Pass 2 of an insert (no leaf overflows): merge in LDS and write back.
 __global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, UpdDev d, const uint32_t *starts,
                                                    const uint32_t *leaf_of, uint64_t *slots, uint32_t *cnt,
-                                                   uint32_t *deg, UpdCtl *ctl) {
+                                                   uint32_t *deg, UpdCtl *ctl, const uint32_t *cand,
+                                                   uint64_t nleaves) {
   __shared__ uint64_t old_keys[kLeaf];
   __shared__ uint64_t add[kLeaf];
   __shared__ uint32_t wcnt[4];
   __shared__ uint32_t nadd;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if ((d.halt && *d.halt) || ctl->overflow) return;
+  // stop on a halted sequence or a global rebalance; the leaves of rebuilt
+  // windows (candidate windows, when a leaf overflowed) are skipped
+  if ((d.halt && *d.halt) || ctl->pad[2] || ctl->bad) return;
+  const bool windows = ctl->overflow != 0;
   const uint64_t b = *d.bu, nseg = *d.nseg;
   for (uint64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
     const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
     const uint32_t l = leaf_of[s];
+    if (windows) {
+      bool inw = false;
+      for (int k = 1; k < 40 && (1ull << (k - 1)) < nleaves; k++)
+        if (cand[l & ~((1ull << k) - 1)] >= (uint32_t)k) {
+          inw = true;
+          break;
+        }
+      if (inw) continue;
+    }
     uint64_t *leaf = slots + (uint64_t)l * kLeaf;
     const uint32_t c = cnt[l];
     for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) old_keys[i] = leaf[i];
@@ -233,7 +246,7 @@ __global__ void __launch_bounds__(256) k_remove(const uint64_t *keys, UpdDev d, 
   __shared__ uint64_t kept[kLeaf];
   __shared__ uint32_t wcnt[4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (d.halt && *d.halt) return;
+  if ((d.halt && *d.halt) || ctl->bad) return;
   const uint64_t b = *d.bu, nseg = *d.nseg;
   for (uint64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
     const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
@@ -266,6 +279,399 @@ __global__ void __launch_bounds__(256) k_remove(const uint64_t *keys, UpdDev d, 
       if (c > run) atomicAdd(&ctl->changed, (unsigned long long)(c - run));
     }
     __syncthreads();
+  }
+}
+
+// ------------------------------------------- windowed rebalance (insert)
+// A leaf that would overflow is fixed locally, as a packed-memory array
+// does: the smallest aligned window of 2^k leaves (k >= 1) whose keys plus
+// new keys fit at kWinDensity fill is rebuilt -- its old keys and the
+// batch keys that fall in it merged and spread evenly over its leaves --
+// and only when the whole array is too dense (ctl->pad[2]) does the update
+// fall back to a global rebalance.  Windows nest (aligned), so the roots
+// are the candidate windows no larger candidate contains; the other
+// segments merge in their leaves as before.
+constexpr double kWinDensity = 0.75;
+
+struct WinArgs {
+  const uint64_t *keys;  // sorted unique batch
+  UpdDev d;
+  const uint32_t *starts, *leaf_of, *newc;
+  uint64_t *slots;
+  uint32_t *cnt;
+  uint64_t *sep;
+  uint32_t *deg;
+  uint64_t nleaves;
+  uint64_t *dem;        // nleaves: keys + new keys per leaf
+  uint64_t *P;          // nleaves + 1: exclusive prefix of dem
+  uint32_t *cand;       // nleaves: largest candidate window level based at the leaf
+  uint64_t *roots;      // (base << 8) | level of each root window
+  uint64_t *stage;      // old keys of a window, at its base * kLeaf
+  uint64_t *stage_new;  // new keys of all windows, in batch order
+  UpdCtl *ctl;
+  // grid-wide rebuild
+  uint64_t *cnt64;      // nleaves: counts widened for the scan
+  uint64_t *pc;         // nleaves + 1: exclusive prefix of the counts
+  uint32_t *nflag;      // cap: batch key is new and in a window
+  uint32_t *npos;       // cap + 1: exclusive prefix of nflag
+  uint64_t *rinfo;      // nleaves x 4 per root: base, end, first new key, new-key count
+  uint64_t *rsize;      // nleaves: merged size of root r (0 past the roots)
+  uint64_t *rpre;       // nleaves + 1: exclusive prefix of rsize
+  uint64_t cap;         // batch capacity
+};
+
+// Root window (base, level) containing leaf l, or level 0: the largest
+// candidate window that holds it.
+__device__ __forceinline__ uint32_t root_of(const WinArgs &w, uint64_t l, uint64_t *base) {
+  int kmax = 1;
+  while (kmax < 40 && (1ull << kmax) < w.nleaves) kmax++;
+  for (int k = kmax; k >= 1; k--) {
+    const uint64_t b = l & ~((1ull << k) - 1);
+    const uint32_t c = w.cand[b];
+    if (c >= (uint32_t)k) {
+      *base = b;
+      return c;
+    }
+  }
+  return 0;
+}
+
+__device__ __forceinline__ bool win_skip(const WinArgs &w) {
+  return (w.d.halt && *w.d.halt) || !w.ctl->overflow || w.ctl->bad;
+}
+
+// True when leaf l lies in a candidate window (hence in a root window).
+__device__ __forceinline__ bool in_window(const WinArgs &w, uint64_t l) {
+  for (int k = 1; k < 40 && (1ull << (k - 1)) < w.nleaves; k++) {
+    const uint64_t b = l & ~((1ull << k) - 1);
+    if (w.cand[b] >= (uint32_t)k) return true;
+  }
+  return false;
+}
+
+__global__ void k_win_dem(WinArgs w) {
+  if (win_skip(w)) return;
+  for (uint64_t l = global_tid(); l < w.nleaves; l += grid_threads()) {
+    w.dem[l] = w.cnt[l];
+    w.cand[l] = 0;
+  }
+}
+
+__global__ void k_win_add(WinArgs w) {
+  if (win_skip(w)) return;
+  const uint64_t nseg = *w.d.nseg;
+  for (uint64_t j = global_tid(); j < nseg; j += grid_threads()) {
+    const uint32_t l = w.leaf_of[w.starts[j]];
+    w.dem[l] = (uint64_t)w.cnt[l] + w.newc[j];
+  }
+}
+
+__global__ void k_win_pick(WinArgs w) {
+  if (win_skip(w)) return;
+  const uint64_t nseg = *w.d.nseg;
+  for (uint64_t j = global_tid(); j < nseg; j += grid_threads()) {
+    const uint64_t l = w.leaf_of[w.starts[j]];
+    if ((uint64_t)w.cnt[l] + w.newc[j] <= kLeaf) continue;
+    int k = 1;
+    for (;; k++) {
+      const uint64_t size = 1ull << k;
+      const uint64_t b = l & ~(size - 1);
+      const uint64_t e = std::min<uint64_t>(b + size, w.nleaves);
+      if ((double)(w.P[e] - w.P[b]) <= kWinDensity * (double)((e - b) * kLeaf)) {
+        atomicMax(&w.cand[b], (uint32_t)k);
+        break;
+      }
+      if (b == 0 && e == w.nleaves) {  // the whole array is too dense
+        atomicOr(&w.ctl->pad[2], 1ull);
+        break;
+      }
+    }
+  }
+}
+
+__global__ void k_win_roots(WinArgs w) {
+  if (win_skip(w) || w.ctl->pad[2]) return;
+  for (uint64_t b = global_tid(); b < w.nleaves; b += grid_threads()) {
+    const uint32_t k = w.cand[b];
+    if (!k) continue;
+    bool contained = false;
+    for (int k2 = (int)k + 1; k2 < 40 && (1ull << (k2 - 1)) < w.nleaves; k2++) {
+      const uint64_t b2 = b & ~((1ull << k2) - 1);
+      if (w.cand[b2] >= (uint32_t)k2) {
+        contained = true;
+        break;
+      }
+    }
+    if (!contained) w.roots[atomicAdd(&w.ctl->pad[3], 1ull)] = (b << 8) | k;
+  }
+}
+
+// Rebuild of the root windows by the whole grid, in kernels:
+//  w1  counts widened (then scanned into pc);
+//  w2  every leaf of a window copies its keys to the window's staging run;
+//  w3  a batch key is new when it is in a window and its own leaf lacks it
+//      (then scanned into npos; w4 compacts them, in batch order, and counts
+//      them into the degrees);
+//  w5  per root: its old and new key ranges and merged size (scanned into
+//      rpre);
+//  w6  merged positions of all windows, 64 per thread, one merge-path search
+//      per run: leaf i of a window of W leaves and T keys takes positions
+//      [i T / W, (i+1) T / W);
+//  w7  counts and empty tails of the window leaves.
+__device__ __forceinline__ bool win_live(const WinArgs &w) { return !win_skip(w) && !w.ctl->pad[2]; }
+
+__global__ void k_win_w1(WinArgs w) {
+  if (!win_live(w)) return;
+  for (uint64_t l = global_tid(); l < w.nleaves; l += grid_threads()) w.cnt64[l] = w.cnt[l];
+}
+
+__global__ void k_win_w2(WinArgs w) {
+  if (!win_live(w)) return;
+  const int lane = lane_id();
+  const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
+  for (uint64_t l = wave; l < w.nleaves; l += nwaves) {
+    uint64_t b = 0;
+    if (!root_of(w, l, &b)) continue;
+    const uint32_t c = w.cnt[l];
+    uint64_t *dst = w.stage + b * kLeaf + (w.pc[l] - w.pc[b]);
+    for (uint32_t i = lane; i < c; i += 64) dst[i] = w.slots[l * kLeaf + i];
+  }
+}
+
+__global__ void k_win_w3(WinArgs w) {
+  if (!win_live(w)) return;
+  const uint64_t bu = *w.d.bu;
+  for (uint64_t i = global_tid(); i < w.cap; i += grid_threads()) {
+    uint32_t f = 0;
+    if (i < bu) {
+      const uint32_t l = w.leaf_of[i];
+      uint64_t b = 0;
+      if (root_of(w, l, &b)) f = !in_sorted(w.slots + (uint64_t)l * kLeaf, w.cnt[l], w.keys[i]);
+    }
+    w.nflag[i] = f;
+  }
+}
+
+__global__ void k_win_w4(WinArgs w) {
+  if (!win_live(w)) return;
+  const uint64_t bu = *w.d.bu;
+  for (uint64_t i = global_tid(); i < bu; i += grid_threads())
+    if (w.nflag[i]) {
+      const uint64_t key = w.keys[i];
+      w.stage_new[w.npos[i]] = key;
+      atomicAdd(&w.deg[key >> 32], 1u);
+    }
+}
+
+__global__ void k_win_w5(WinArgs w) {
+  if (!win_live(w)) return;
+  const uint64_t nroots = w.ctl->pad[3];
+  const uint64_t bu = *w.d.bu, nseg = *w.d.nseg;
+  for (uint64_t r = global_tid(); r < w.nleaves; r += grid_threads()) {
+    if (r >= nroots) {
+      w.rsize[r] = 0;
+      continue;
+    }
+    const uint64_t b = w.roots[r] >> 8;
+    const uint64_t e = std::min<uint64_t>(b + (1ull << (w.roots[r] & 0xFF)), w.nleaves);
+    uint64_t jlo = 0, jhi = nseg;  // first segment with leaf >= b
+    while (jlo < jhi) {
+      const uint64_t mid = (jlo + jhi) >> 1;
+      if (w.leaf_of[w.starts[mid]] < b) jlo = mid + 1; else jhi = mid;
+    }
+    uint64_t j2 = jlo, j3 = nseg;  // first segment with leaf >= e
+    while (j2 < j3) {
+      const uint64_t mid = (j2 + j3) >> 1;
+      if (w.leaf_of[w.starts[mid]] < e) j2 = mid + 1; else j3 = mid;
+    }
+    const uint64_t ks = jlo < nseg ? w.starts[jlo] : bu, ke = j2 < nseg ? w.starts[j2] : bu;
+    const uint64_t n0 = w.npos[ks], nn = w.npos[ke] - n0;
+    w.rinfo[4 * r] = b;
+    w.rinfo[4 * r + 1] = e;
+    w.rinfo[4 * r + 2] = n0;
+    w.rinfo[4 * r + 3] = nn;
+    w.rsize[r] = (w.pc[e] - w.pc[b]) + nn;
+    w.cnt64[b] = (w.pc[e] - w.pc[b]) + nn;  // merged size by root base (cnt64 is free after w1's scan)
+    if (nn) atomicAdd(&w.ctl->changed, (unsigned long long)nn);
+  }
+}
+
+__global__ void k_win_w6(WinArgs w) {
+  if (!win_live(w)) return;
+  const uint64_t nroots = w.ctl->pad[3];
+  if (!nroots) return;
+  const uint64_t total = w.rpre[nroots];
+  constexpr uint64_t kRun = 64;
+  for (uint64_t p0 = global_tid() * kRun; p0 < total; p0 += grid_threads() * kRun) {
+    uint64_t p = p0;
+    const uint64_t pend = std::min<uint64_t>(p0 + kRun, total);
+    while (p < pend) {
+      uint64_t lo = 0, hi = nroots;  // root r with rpre[r] <= p < rpre[r + 1]
+      while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (w.rpre[mid] <= p) lo = mid; else hi = mid;
+      }
+      const uint64_t r = lo;
+      const uint64_t b = w.rinfo[4 * r], e = w.rinfo[4 * r + 1];
+      const uint64_t W = e - b;
+      const uint64_t T_old = w.pc[e] - w.pc[b], T_new = w.rinfo[4 * r + 3];
+      const uint64_t T = T_old + T_new;
+      const uint64_t *old = w.stage + b * kLeaf;
+      const uint64_t *nw = w.stage_new + w.rinfo[4 * r + 2];
+      const uint64_t q0 = p - w.rpre[r];
+      const uint64_t q1 = std::min<uint64_t>(T, q0 + (pend - p));
+      uint64_t x = q0 > T_new ? q0 - T_new : 0, y = q0 < T_old ? q0 : T_old;
+      while (x < y) {
+        const uint64_t mid = (x + y) >> 1;
+        if (old[mid] < nw[q0 - mid - 1]) x = mid + 1; else y = mid;
+      }
+      uint64_t a = x, c = q0 - x;
+      uint64_t i = (uint64_t)(((unsigned __int128)q0 * W) / T);
+      while (i + 1 < W && (uint64_t)(((unsigned __int128)(i + 1) * T) / W) <= q0) i++;
+      while (i > 0 && (uint64_t)(((unsigned __int128)i * T) / W) > q0) i--;
+      uint64_t si = (uint64_t)(((unsigned __int128)i * T) / W);
+      uint64_t snext = (uint64_t)(((unsigned __int128)(i + 1) * T) / W);
+      for (uint64_t q = q0; q < q1; q++) {
+        uint64_t key;
+        if (c >= T_new || (a < T_old && old[a] < nw[c])) key = old[a++];
+        else key = nw[c++];
+        while (q >= snext) {
+          i++;
+          si = snext;
+          snext = (uint64_t)(((unsigned __int128)(i + 1) * T) / W);
+        }
+        w.slots[(b + i) * kLeaf + (q - si)] = key;
+        if (q == si && i > 0) w.sep[b + i] = key;
+      }
+      p += q1 - q0;
+    }
+  }
+}
+
+__global__ void k_win_w7(WinArgs w) {
+  if (!win_live(w)) return;
+  const int lane = lane_id();
+  const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
+  for (uint64_t l = wave; l < w.nleaves; l += nwaves) {
+    uint64_t b = 0;
+    const uint32_t k = root_of(w, l, &b);
+    if (!k) continue;
+    // the root's merged size: old keys from pc, new keys from npos over the
+    // window's batch range (recomputed from the segment search in w5's
+    // record, found by the root's base)
+    const uint64_t e = std::min<uint64_t>(b + (1ull << k), w.nleaves);
+    const uint64_t W = e - b;
+    const uint64_t T = w.cnt64[b];  // the root's merged size (w5)
+    const uint64_t i = l - b;
+    const uint64_t si = (uint64_t)(((unsigned __int128)i * T) / W);
+    const uint64_t ei = (uint64_t)(((unsigned __int128)(i + 1) * T) / W);
+    for (uint64_t x = (ei - si) + lane; x < kLeaf; x += 64) w.slots[l * kLeaf + x] = kEmpty;
+    if (lane == 0) w.cnt[l] = (uint32_t)(ei - si);
+  }
+}
+
+// Small batches (b <= kSmallBatch): one workgroup checks the ids, sorts the
+// keys in LDS (bitonic), deduplicates them and cuts the segments -- the work
+// of ~40 launches of the general path (radix passes, scans) in one.
+constexpr uint32_t kSmallBatch = 4096;
+constexpr int kSmallThreads = 1024;
+
+__device__ __forceinline__ uint32_t block_excl_scan4k(uint32_t *v, uint32_t *part, uint32_t *total) {
+  // v[4096] in LDS, 4 consecutive entries per thread; returns nothing, v
+  // becomes its exclusive prefix and *total the sum
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  uint32_t a0 = v[4 * t], a1 = v[4 * t + 1], a2 = v[4 * t + 2], a3 = v[4 * t + 3];
+  const uint32_t sum = a0 + a1 + a2 + a3;
+  const uint32_t incl = wave_incl_scan(sum);
+  if (lane == 63) part[wv] = incl;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+  for (int x = 0; x < kSmallThreads / 64; x++) {
+    if (x < wv) before += part[x];
+    tot += part[x];
+  }
+  uint32_t p = before + incl - sum;
+  v[4 * t] = p;
+  p += a0;
+  v[4 * t + 1] = p;
+  p += a1;
+  v[4 * t + 2] = p;
+  p += a2;
+  v[4 * t + 3] = p;
+  if (t == 0) *total = tot;
+  __syncthreads();
+  return tot;
+}
+
+__global__ void __launch_bounds__(kSmallThreads) k_small_batch(const uint64_t *keys_in, uint64_t b, uint64_t n,
+                                                                const uint64_t *sep, uint64_t nleaves,
+                                                                uint64_t *uniq, uint32_t *leaf_of,
+                                                                uint32_t *starts, uint32_t *pos_total,
+                                                                UpdCtl *ctl) {
+  __shared__ uint64_t k[kSmallBatch];
+  __shared__ uint32_t f[kSmallBatch];
+  __shared__ uint32_t lf[kSmallBatch];
+  __shared__ uint32_t part[kSmallThreads / 64];
+  __shared__ uint32_t tot;
+  const int t = threadIdx.x;
+  for (uint32_t i = t; i < kSmallBatch; i += kSmallThreads) {
+    uint64_t v = kEmpty;
+    if (i < b) {
+      v = keys_in[i];
+      if ((v >> 32) >= n || (v & 0xFFFFFFFFull) >= n) atomicOr(&ctl->bad, 1ull);
+    }
+    k[i] = v;
+  }
+  __syncthreads();
+  uint32_t N = 64;
+  while (N < b) N <<= 1;
+  for (uint32_t size = 2; size <= N; size <<= 1)
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t i = t; i < N; i += kSmallThreads) {
+        const uint32_t j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          const uint64_t x = k[i], y = k[j];
+          if ((x > y) == up) {
+            k[i] = y;
+            k[j] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  // unique keys (the empty key, padding or a bad id, is dropped)
+  for (uint32_t i = t; i < kSmallBatch; i += kSmallThreads)
+    f[i] = (i < N && k[i] != kEmpty && (i == 0 || k[i] != k[i - 1])) ? 1u : 0u;
+  __syncthreads();
+  for (uint32_t i = t; i < kSmallBatch; i += kSmallThreads) lf[i] = f[i];
+  __syncthreads();
+  const uint32_t bu = block_excl_scan4k(f, part, &tot);
+  for (uint32_t i = t; i < N; i += kSmallThreads)
+    if (lf[i]) uniq[f[i]] = k[i];
+  __syncthreads();
+  // leaves of the unique keys, then one segment per run of equal leaves
+  for (uint32_t j = t; j < bu; j += kSmallThreads) {
+    const uint64_t key = uniq[j];
+    uint64_t lo = 0, hi = nleaves;
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (sep[mid] <= key) lo = mid; else hi = mid;
+    }
+    lf[j] = (uint32_t)lo;
+    leaf_of[j] = (uint32_t)lo;
+  }
+  __syncthreads();
+  for (uint32_t j = t; j < kSmallBatch; j += kSmallThreads) f[j] = (j < bu && (j == 0 || lf[j] != lf[j - 1])) ? 1u : 0u;
+  __syncthreads();
+  for (uint32_t j = t; j < bu; j += kSmallThreads) k[j] = f[j];  // keep the flags
+  __syncthreads();
+  const uint32_t nseg = block_excl_scan4k(f, part, &tot);
+  for (uint32_t j = t; j < bu; j += kSmallThreads)
+    if (k[j]) starts[f[j]] = j;
+  if (t == 0) {
+    ctl->pad[0] = bu;
+    *pos_total = nseg;
   }
 }
 
@@ -400,9 +806,9 @@ __global__ void k_zero_ctl(UpdCtl *ctl, uint64_t count) {
 }
 __global__ void k_zero_u32(uint32_t *p) { *p = 0; }
 // Stops a captured sequence at batch `index` (1-based) when its keys hold an
-// id >= n or one of its leaves would overflow.
+// id >= n or the array is too dense for a windowed rebalance.
 __global__ void k_gate(const UpdCtl *ctl, uint32_t *halt, uint32_t index) {
-  if (*halt == 0 && (ctl->overflow || ctl->bad)) *halt = index;
+  if (*halt == 0 && (ctl->pad[2] || ctl->bad)) *halt = index;
 }
 }  // namespace
 
@@ -436,6 +842,13 @@ static const uint64_t *sort_and_segment(byo_graph *g, const uint64_t *keys_in, u
                                         const UpdBufs &u) {
   Cpma *c = state(g);
   hipStream_t s = g->stream;
+  if (b <= kSmallBatch) {
+    // u.pos + b holds the segment count, as after the general path's scan
+    k_small_batch<<<1, kSmallThreads, 0, s>>>(keys_in, b, g->n, c->sep.as<uint64_t>(), c->nleaves, u.k1, u.leaf,
+                                              u.starts, u.pos + b, ctl);
+    BYO_LAUNCH_CHECK();
+    return u.k1;
+  }
   k_check_ids<<<byo_grid(b, 256, 8192), 256, 0, s>>>(keys_in, b, g->n, ctl);
   BYO_HIP(hipMemcpyAsync(u.k0, keys_in, b * 8, hipMemcpyDeviceToDevice, s));
   int idbits = 1;
@@ -455,6 +868,74 @@ static const uint64_t *sort_and_segment(byo_graph *g, const uint64_t *keys_in, u
   return uniq;
 }
 
+// Buffers and arguments of the windowed rebalance for batches of up to cap
+// keys (the staging area is as large as the leaves).
+static WinArgs win_args(byo_graph *g, const uint64_t *batch, const UpdDev &d, const UpdBufs &u, UpdCtl *ctl,
+                        uint64_t cap) {
+  Cpma *c = state(g);
+  const uint64_t nl = c->nleaves;
+  WinArgs w{};
+  w.keys = batch;
+  w.d = d;
+  w.starts = u.starts;
+  w.leaf_of = u.leaf;
+  w.newc = u.newc;
+  w.slots = c->slots.as<uint64_t>();
+  w.cnt = c->cnt.as<uint32_t>();
+  w.sep = c->sep.as<uint64_t>();
+  w.deg = c->deg.as<uint32_t>();
+  w.nleaves = nl;
+  w.dem = g->buf("win_dem", (nl + 1) * 8 + 64).as<uint64_t>();
+  w.P = g->buf("win_prefix", (nl + 2) * 8 + 64).as<uint64_t>();
+  w.cand = g->buf("win_cand", nl * 4 + 64).as<uint32_t>();
+  w.roots = g->buf("win_roots", nl * 8 + 64).as<uint64_t>();
+  w.stage = g->buf("win_stage", nl * kLeaf * 8 + 64).as<uint64_t>();
+  w.stage_new = g->buf("win_stage_new", cap * 8 + 64).as<uint64_t>();
+  w.ctl = ctl;
+  w.cnt64 = g->buf("win_cnt64", (nl + 1) * 8 + 64).as<uint64_t>();
+  w.pc = g->buf("win_pc", (nl + 2) * 8 + 64).as<uint64_t>();
+  w.nflag = g->buf("win_nflag", (cap + 1) * 4 + 64).as<uint32_t>();
+  w.npos = g->buf("win_npos", (cap + 2) * 4 + 64).as<uint32_t>();
+  w.rinfo = g->buf("win_rinfo", nl * 32 + 64).as<uint64_t>();
+  w.rsize = g->buf("win_rsize", (nl + 1) * 8 + 64).as<uint64_t>();
+  w.rpre = g->buf("win_rpre", (nl + 2) * 8 + 64).as<uint64_t>();
+  w.cap = cap;
+  g->buf("win_scan_tmp", prim::scan_tmp_bytes(nl + 1));
+  g->buf("win_scan_tmp2", prim::scan_tmp_bytes(cap + 1));
+  return w;
+}
+
+// Candidate windows and their roots (then, in window_rebuild, the rebuilt
+// windows); every kernel returns at once unless a leaf overflowed.
+static void window_pick(byo_graph *g, const WinArgs &w, uint64_t cap) {
+  hipStream_t s = g->stream;
+  const unsigned gl = byo_grid(w.nleaves, 256, 16384), gs = byo_grid(cap, 256, 16384);
+  k_win_dem<<<gl, 256, 0, s>>>(w);
+  k_win_add<<<gs, 256, 0, s>>>(w);
+  prim::exclusive_scan_u64(w.dem, w.P, w.nleaves, true, g->scratch["win_scan_tmp"].p, s);
+  k_win_pick<<<gs, 256, 0, s>>>(w);
+  k_win_roots<<<gl, 256, 0, s>>>(w);
+  BYO_LAUNCH_CHECK();
+}
+
+static void window_rebuild(byo_graph *g, const WinArgs &w) {
+  hipStream_t s = g->stream;
+  void *t1 = g->scratch["win_scan_tmp"].p, *t2 = g->scratch["win_scan_tmp2"].p;
+  const unsigned gl = byo_grid(w.nleaves, 256, 16384), gw = byo_grid(w.nleaves * 64, 256, 16384),
+                 gc = byo_grid(w.cap, 256, 16384);
+  k_win_w1<<<gl, 256, 0, s>>>(w);
+  prim::exclusive_scan_u64(w.cnt64, w.pc, w.nleaves, true, t1, s);
+  k_win_w2<<<gw, 256, 0, s>>>(w);
+  k_win_w3<<<gc, 256, 0, s>>>(w);
+  prim::exclusive_scan_u32(w.nflag, w.npos, w.cap, true, t2, s);
+  k_win_w4<<<gc, 256, 0, s>>>(w);
+  k_win_w5<<<gl, 256, 0, s>>>(w);
+  prim::exclusive_scan_u64(w.rsize, w.rpre, w.nleaves, true, t1, s);
+  k_win_w6<<<4096, 256, 0, s>>>(w);
+  k_win_w7<<<gw, 256, 0, s>>>(w);
+  BYO_LAUNCH_CHECK();
+}
+
 static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool insert) {
   Cpma *c = state(g);
   hipStream_t s = g->stream;
@@ -465,25 +946,34 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
   const UpdBufs u = upd_bufs(g, b);
   k_zero_ctl<<<1, 1, 0, s>>>(ctl, 1);
   const uint64_t *batch = sort_and_segment(g, keys_dev, b, ctl, u);
-  BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
-  BYO_HIP(hipStreamSynchronize(s));
-  BYO_REQUIRE(hctl->bad == 0, BYO_E_INVAL, "batch holds a vertex id >= n");
-  const uint64_t bu = hctl->pad[0];
-  if (bu == 0) return;
+  // counts stay on the device; the first read-back also reports a bad id
+  // (no kernel writes the container when ctl->bad is set)
   const UpdDev d{&ctl->pad[0], u.pos + b, nullptr};
-  const unsigned grid = (unsigned)std::min<uint64_t>(bu, 16384);
+  const unsigned grid = (unsigned)std::min<uint64_t>(b, 16384);
   if (insert) {
     k_ins_count<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
                                      u.newc, ctl);
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
     BYO_HIP(hipStreamSynchronize(s));
-    if (!hctl->overflow) {
-      k_ins_merge<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
-                                       c->deg.as<uint32_t>(), ctl);
-      BYO_LAUNCH_CHECK();
-      BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
-      BYO_HIP(hipStreamSynchronize(s));
+    BYO_REQUIRE(hctl->bad == 0, BYO_E_INVAL, "batch holds a vertex id >= n");
+    const uint64_t bu = hctl->pad[0];
+    if (bu == 0) return;
+    const uint32_t *cand = nullptr;
+    if (hctl->overflow) {
+      // leaves that would overflow: rebuild their windows on the device
+      const WinArgs w = win_args(g, batch, d, u, ctl, b);
+      window_pick(g, w, b);
+      window_rebuild(g, w);
+      cand = w.cand;
+    }
+    k_ins_merge<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+                                     c->deg.as<uint32_t>(), ctl, cand, c->nleaves);
+    BYO_LAUNCH_CHECK();
+    BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
+    BYO_HIP(hipStreamSynchronize(s));
+    if (!hctl->pad[2]) {
+      if (hctl->overflow) c->rebalances++;
       c->total += hctl->changed;
       g->m = c->total;
       if (hctl->changed) {
@@ -514,6 +1004,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
     BYO_HIP(hipStreamSynchronize(s));
+    BYO_REQUIRE(hctl->bad == 0, BYO_E_INVAL, "batch holds a vertex id >= n");
     c->total -= hctl->changed;
     g->m = c->total;
     if (hctl->changed) {
@@ -540,6 +1031,11 @@ static uint32_t insert_graph(byo_graph *g, const uint64_t *keys, const uint64_t 
   hipStream_t s = g->stream;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  uint64_t cap = 0;
+  for (uint32_t i = first; i < nb; i++) cap = std::max<uint64_t>(cap, offs[i + 1] - offs[i]);
+  // the windowed-rebalance buffers for the current leaf count exist before
+  // capture (nothing is allocated while capturing)
+  (void)win_args(g, nullptr, UpdDev{}, u, ctls, cap);
   BYO_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
   try {
     k_zero_u32<<<1, 1, 0, s>>>(halt);
@@ -553,9 +1049,12 @@ static uint32_t insert_graph(byo_graph *g, const uint64_t *keys, const uint64_t 
       const unsigned grid = (unsigned)std::min<uint64_t>(b, 16384);
       k_ins_count<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
                                        u.newc, ctl);
+      const WinArgs w = win_args(g, batch, d, u, ctl, cap);
+      window_pick(g, w, b);
       k_gate<<<1, 1, 0, s>>>(ctl, halt, i + 1);
+      window_rebuild(g, w);
       k_ins_merge<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
-                                       c->deg.as<uint32_t>(), ctl);
+                                       c->deg.as<uint32_t>(), ctl, w.cand, c->nleaves);
     }
     BYO_LAUNCH_CHECK();
   } catch (...) {

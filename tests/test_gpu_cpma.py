@@ -146,6 +146,27 @@ def _graph_insert(B, G, batches):
         d.free()
 
 
+def test_windowed_rebalance_many_hot_leaves(B):
+    """Batches that overflow many leaves at once (RMAT-skewed hot spots and
+    one vertex gaining every edge) keep the exact edge set and degrees, one
+    call at a time and in a captured sequence."""
+    g = O.rmat_graph(13)
+    E = O.graph_keys(g)
+    G = B.CpmaGraph.from_csr(g.offsets, g.ids)
+    for i, b in enumerate([upd_keys(g.n, 30000, 0), upd_keys(g.n, 60000, 30000),
+                           (np.uint64(3) << np.uint64(32)) | np.arange(g.n, dtype=np.uint64)]):
+        G.insert_batch(b)
+        E = np.union1d(E, b)
+        k, _ = keys_of(G)
+        assert np.array_equal(k, E), i
+        assert np.array_equal(G.degrees(), np.bincount((E >> np.uint64(32)).astype(np.int64), minlength=g.n))
+    # then removals over rebuilt windows
+    rem = upd_keys(g.n, 60000, 30000)
+    G.remove_batch(rem)
+    E = np.setdiff1d(E, rem)
+    assert np.array_equal(keys_of(G)[0], E)
+
+
 def test_captured_batches_equal_sequential(B):
     """Uneven batches in one graph give the edge set, M and degrees of the
     same batches inserted one call at a time."""
@@ -173,16 +194,27 @@ def test_captured_batches_equal_sequential(B):
 
 
 def test_captured_batches_overflow_and_bad_ids(B):
-    """A batch that overflows a leaf stops the graph, goes through the host
-    rebalance, and the rest follows in a new graph; a batch with an id >= n
-    raises after the batches before it were applied."""
+    """Leaves that would overflow are rebuilt in windows inside the graph; a
+    batch that makes the whole array too dense stops the graph, goes through
+    the host's global rebalance, and the rest follows in a new graph; a batch
+    with an id >= n raises after the batches before it were applied."""
     g = O.rmat_graph(12)
     hub = (np.uint64(5) << np.uint64(32)) | np.arange(g.n, dtype=np.uint64)  # one vertex gains n edges
-    batches = [upd_keys(g.n, 100, 0), hub, upd_keys(g.n, 300, 100)]
     G = B.CpmaGraph.from_csr(g.offsets, g.ids)
+    batches = [upd_keys(g.n, 100, 0), hub, upd_keys(g.n, 300, 100)]
+    graphs, _ = _graph_insert(B, G, batches)
+    assert graphs == 1  # windowed rebalance, no host fallback
+    E = O.graph_keys(g)
+    for b in batches:
+        E = np.union1d(E, b)
+    assert np.array_equal(keys_of(G)[0], E)
+    assert np.array_equal(G.degrees(), np.bincount((E >> np.uint64(32)).astype(np.int64), minlength=g.n))
+    # grow the array by far more than its slack in one batch: global rebalance
+    dense = ((np.arange(64, dtype=np.uint64)[:, None] << np.uint64(32)) |
+             np.arange(g.n, dtype=np.uint64)[None, :]).ravel()
+    batches = [upd_keys(g.n, 50, 900), dense, upd_keys(g.n, 70, 1200)]
     graphs, _ = _graph_insert(B, G, batches)
     assert graphs >= 2
-    E = O.graph_keys(g)
     for b in batches:
         E = np.union1d(E, b)
     assert np.array_equal(keys_of(G)[0], E)
