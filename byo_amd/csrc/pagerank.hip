@@ -155,6 +155,35 @@ __global__ void k_to_double(const T *in, double *out, uint64_t n) {
   for (uint64_t v = global_tid(); v < n; v += grid_threads()) out[v] = (double)in[v];
 }
 
+// Rank of vertex v from the relabelled vector (perm: old id -> new id).
+template <class T>
+__global__ void k_unpermute(const T *in, const uint32_t *perm, double *out, uint64_t n) {
+  for (uint64_t v = global_tid(); v < n; v += grid_threads()) out[v] = (double)in[perm[v]];
+}
+
+// ------------------------------------------------ degree-ordered relabelling
+// Optional layout for the pull (tuning knob pr_relabel): vertices renumbered
+// by descending degree (ties by id), so the contributions the gathers hit
+// most sit together at low ids -- the hot part of the vector stays in L2.
+// Built once per graph on the device: a radix sort of (maxdeg - deg, v)
+// gives the order, every edge (u, v) becomes (perm u, perm v), a radix sort
+// of those keys and keys_to_csr give the relabelled CSR.
+__global__ void k_rl_vkeys(const uint64_t *off, uint64_t n, uint64_t maxdeg, uint64_t *keys) {
+  for (uint64_t v = global_tid(); v < n; v += grid_threads()) keys[v] = ((maxdeg - (off[v + 1] - off[v])) << 32) | v;
+}
+__global__ void k_rl_perm(const uint64_t *sorted, uint64_t n, uint32_t *perm) {
+  for (uint64_t i = global_tid(); i < n; i += grid_threads()) perm[sorted[i] & 0xFFFFFFFFull] = (uint32_t)i;
+}
+__global__ void k_rl_ekeys(const uint64_t *off, const uint32_t *ids, uint64_t n, const uint32_t *perm,
+                           uint64_t *keys) {
+  const int lane = lane_id();
+  const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
+  for (uint64_t u = wave; u < n; u += nwaves) {
+    const uint64_t pu = (uint64_t)perm[u] << 32;
+    for (uint64_t k = off[u] + lane; k < off[u + 1]; k += 64) keys[k] = pu | perm[ids[k]];
+  }
+}
+
 template <class T, int G, bool kComp>
 void pr_pass(bool first, bool last, const uint64_t *lo, const uint64_t *hi, const uint32_t *ids, const uint64_t *off,
              uint64_t n, const T *cin, T *as, T *ac, const T *pold, T *pnew, T *cout, double damping, double addc,
@@ -189,11 +218,51 @@ uint64_t segment_width(byo_graph *g, uint64_t n) {
   return W < n ? W : n;
 }
 
+struct PrView {
+  const uint64_t *off;
+  const uint32_t *ids;
+  const uint32_t *perm;  // old id -> new id, or null for the graph as it is
+};
+
+PrView pr_view(byo_graph *g) {
+  if (!g->tuning("pr_relabel") || g->n == 0) return PrView{g->off.as<uint64_t>(), g->ids.as<uint32_t>(), nullptr};
+  const uint64_t n = g->n, m = g->m;
+  uint64_t *off2 = g->buf("pr_rl_off", (n + 1) * 8 + 64).as<uint64_t>();
+  uint32_t *ids2 = g->buf("pr_rl_ids", m * 4 + 64).as<uint32_t>();
+  uint32_t *perm = g->buf("pr_rl_perm", n * 4 + 64).as<uint32_t>();
+  if (!g->pr_rl_valid) {
+    hipStream_t s = g->stream;
+    PhaseTimer t(s, g->prof.on);
+    const uint64_t maxdeg = graph_max_degree(g);
+    const uint64_t cap = std::max<uint64_t>(n, m);
+    DevBuf k0(cap * 8 + 64), k1(cap * 8 + 64), tmp(prim::radix_tmp_bytes(cap));
+    int idbits = 1, vbits = 1;
+    while ((1ull << idbits) < n) idbits++;
+    while ((1ull << vbits) < std::max<uint64_t>(n, maxdeg + 1)) vbits++;
+    int shifts[16], vshifts[16];
+    const int np = prim::edge_key_shifts(idbits, shifts);
+    const int nvp = prim::edge_key_shifts(vbits, vshifts);
+    k_rl_vkeys<<<byo_grid(n, 256, 16384), 256, 0, s>>>(g->off.as<uint64_t>(), n, maxdeg, k0.as<uint64_t>());
+    uint64_t *sv = prim::radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), n, vshifts, nvp, tmp.p, s);
+    k_rl_perm<<<byo_grid(n, 256, 16384), 256, 0, s>>>(sv, n, perm);
+    k_rl_ekeys<<<byo_grid(n * 64, 256, 16384), 256, 0, s>>>(g->off.as<uint64_t>(), g->ids.as<uint32_t>(), n, perm,
+                                                             k0.as<uint64_t>());
+    uint64_t *se = prim::radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), m, shifts, np, tmp.p, s);
+    prim::keys_to_csr(se, m, n, off2, ids2, s);
+    BYO_LAUNCH_CHECK();
+    BYO_HIP(hipStreamSynchronize(s));
+    if (g->prof.on) g->account("pr_relabel", t.stop());
+    g->pr_rl_valid = true;
+  }
+  return PrView{off2, ids2, perm};
+}
+
 template <class T, bool kComp>
 void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, double *p_out, double *l1_log,
                    int *iters_run, double *device_ms, const char *phase) {
   const uint64_t n = g->n;
   hipStream_t s = g->stream;
+  const PrView view = pr_view(g);
   T *p[2], *c[2];
   p[0] = g->buf("pr_p0", n * sizeof(T) + 64).as<T>();
   p[1] = g->buf("pr_p1", n * sizeof(T) + 64).as<T>();
@@ -212,8 +281,8 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   const unsigned grid = (unsigned)g->tuning("pr_grid");
   double *partial = g->buf("pr_partial", (grid + 8) * sizeof(double)).as<double>();
   double *l1dev = g->buf("pr_l1", (size_t)(max_iters + 1) * sizeof(double)).as<double>();
-  const uint64_t *off = g->off.as<uint64_t>();
-  const uint32_t *ids = g->ids.as<uint32_t>();
+  const uint64_t *off = view.off;
+  const uint32_t *ids = view.ids;
   const double addc = (1.0 - damping) * (1.0 / (double)n);
   PhaseTimer timer(s, true);
   k_pr_init<T><<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, n, p[0], c[0]);
@@ -248,7 +317,10 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   }
   if (p_out) {
     double *pd = g->buf("pr_out_f64", n * sizeof(double) + 64).as<double>();
-    k_to_double<T><<<byo_grid(n, 256, 16384), 256, 0, s>>>(p[cur], pd, n);
+    if (view.perm)
+      k_unpermute<T><<<byo_grid(n, 256, 16384), 256, 0, s>>>(p[cur], view.perm, pd, n);
+    else
+      k_to_double<T><<<byo_grid(n, 256, 16384), 256, 0, s>>>(p[cur], pd, n);
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipMemcpyAsync(p_out, pd, n * sizeof(double), hipMemcpyDeviceToHost, s));
   }

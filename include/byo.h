@@ -269,6 +269,10 @@ int byo_profile_reset(byo_graph_t g);
  *   pr_lanes            0,4,8,16,32 PageRank lanes per vertex (0: auto)
  *   pr_grid             256..65536 PageRank pull grid (4096)
  *   pr_seg_log2         10..40 PageRank source-segment width (40: one)
+ *   pr_relabel          0/1 PageRank on a copy of the graph relabelled by
+ *                       descending degree, built on the first call after
+ *                       each change of the graph; ranks return in the
+ *                       original ids (0)
  *   bfs_engine          0..2 several-source BFS: 0 shared traversal per 64
  *                       sources, 1 one persistent launch taking them in
  *                       turn, 2 host-driven level loop (0)

@@ -61,6 +61,43 @@ def test_pagerank_knobs(B, s15, key, values):
         G.set_tuning(key, {"pr_lanes": 0, "pr_grid": 4096, "pr_seg_log2": 40}[key])
 
 
+@pytest.mark.parametrize("precision", ["PR_FP32", "PR_FP64"])
+def test_pagerank_relabel(B, s15, precision):
+    """PageRank on the degree-relabelled copy, mapped back to the original ids."""
+    g, G = s15
+    ref, _ = O.pagerank(g, 0.85, 20)
+    G.set_tuning("pr_relabel", 1)
+    try:
+        p, _, _ = G.pagerank(0.85, 20, 0.0, getattr(B, precision))
+        p2, _, _ = G.pagerank(0.85, 20, 0.0, getattr(B, precision))  # cached copy
+    finally:
+        G.set_tuning("pr_relabel", 0)
+    assert np.abs(p - ref).sum() <= 1e-6
+    assert np.array_equal(p, p2)
+
+
+def test_pagerank_relabel_isolated_and_updates(B):
+    """Isolated vertices and ties in degree; the relabelled copy follows the
+    graph through insert and remove batches."""
+    g = O.rmat_graph(12)
+    G = B.CpmaGraph.from_csr(g.offsets, g.ids)
+    G.set_tuning("pr_relabel", 1)
+    s, d = O.update_rmat(g.n, 5000, 3)
+    batch = (s.astype(np.uint64) << np.uint64(32)) | d.astype(np.uint64)
+    for step in ("base", "insert", "remove"):
+        if step == "insert":
+            G.insert_batch(batch)
+        elif step == "remove":
+            G.remove_batch(batch[: len(batch) // 2])
+        off, ids = G.export_csr()
+        h = O.Graph(off, ids)
+        ref, _ = O.pagerank(h, 0.85, 15)
+        p, _, _ = G.pagerank(0.85, 15, 0.0, B.PR_FP32)
+        print(step, h.m, flush=True)
+        assert np.abs(p - ref).sum() <= 1e-6, step
+    G.close()
+
+
 def test_ppr_blocks_knob(B, s15):
     g, G = s15
     seeds = O.bfs_sources(g, 4, seed=8)
