@@ -248,7 +248,7 @@ def main():
         G.profile_reset()
         p32, l1_32, ms32 = G.pagerank(0.85, 20, 0.0, B.PR_FP32)  # builds the relabelled copy if knob on
         ex["pr_relabel"] = relabel
-        ex["pr_relabel_build_ms"] = round(G.profile_get("pr_relabel")[1], 3)
+        ex["pr_relabel_build_ms"] = round(G.profile_get("relabel")[1], 3)
         G.profile(False)
         p32, l1_32, ms32 = G.pagerank(0.85, 20, 0.0, B.PR_FP32)
         p64, l1_64, ms64 = G.pagerank(0.85, 20, 0.0, B.PR_FP64)

@@ -28,7 +28,7 @@ CANDIDATES = {
     "pieces": [4, 8, 16],
     "ppr_blocks_per_cu": [4, 8],
     "pr_relabel": [0, 1],
-    "pr_lanes": [4, 8, 16, 32],
+    "pr_lanes": [0, 4, 8, 16, 32],
     "pr_grid": [4096, 8192, 16384, 32768, 65536],
 }
 WORKLOAD = {"coop_blocks_per_cu": "bfs", "pieces": "bfs", "ppr_blocks_per_cu": "ppr", "pr_lanes": "pr",

@@ -152,7 +152,7 @@ inline const std::vector<TuneSpec> &tune_specs() {
       {"coop_blocks_per_cu", 4, 1, 4, "blocks per CU of the persistent traversal launches"},
       {"ppr_blocks_per_cu", 4, 1, 8, "blocks per CU of the persistent personalized PageRank launch"},
       {"pieces", 8, 1, 16, "weight-balanced word pieces per wave in vertex-driven passes"},
-      {"pr_lanes", 0, 0, 32, "PageRank lanes per vertex (4, 8, 16, 32; 0: from the average degree)"},
+      {"pr_lanes", 0, 0, 32, "PageRank lanes per vertex (4, 8, 16, 32; 0: per tile from its largest degree)"},
       {"pr_grid", 4096, 256, 65536, "PageRank pull grid (blocks of 256)"},
       {"pr_seg_log2", 40, 10, 40, "log2 of the PageRank source-segment width (40: one segment)"},
       {"pr_relabel", 0, 0, 1, "PageRank on a copy relabelled by descending degree (built once per graph)"},
@@ -177,7 +177,7 @@ struct byo_graph {
   bool csr_valid = false;
   uint64_t maxdeg = 0;        // maximum degree of the current CSR view
   bool maxdeg_valid = false;  // cleared whenever the CSR view is rebuilt
-  bool pr_rl_valid = false;   // degree-relabelled PageRank copy is current
+  bool rl_valid = false;      // degree-relabelled copy (relabel_view) is current
   int64_t last_src = -1;      // last source of byo_bfs / byo_bfs_multi
 
   // CPMA state (kind 1); owned by cpma.hip.
@@ -230,6 +230,16 @@ byo_graph *new_handle(int kind);
 void *pinned(byo_graph *g, size_t bytes);
 // Maximum degree of the current CSR view (cached until it is rebuilt).
 uint64_t graph_max_degree(byo_graph *g);
+// Copy of the current CSR view with vertices renumbered by descending
+// degree (relabel.hip, used by the PageRank pull): perm[old] = new.  Built
+// on the first call after each change of the graph; the CSR view must be
+// current.
+struct RelabelView {
+  const uint64_t *off;
+  const uint32_t *ids;
+  const uint32_t *perm;  // null when the graph is used as it is
+};
+RelabelView relabel_view(byo_graph *g);
 // CSR construction (graph.hip).
 void csr_from_keys(byo_graph *g, uint64_t *keys, uint64_t *alt, uint64_t nk, uint64_t n);
 void csr_upload(byo_graph *g, const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m);

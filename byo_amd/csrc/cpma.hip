@@ -758,7 +758,7 @@ static void cpma_build(byo_graph *g, const uint64_t *flat, uint64_t m) {
   BYO_LAUNCH_CHECK();
   g->csr_valid = true;
   g->maxdeg_valid = false;
-  g->pr_rl_valid = false;
+  g->rl_valid = false;
 }
 
 // Compacts the leaves into flat[total] (sorted, unique).
@@ -787,7 +787,7 @@ void cpma_ensure_csr(byo_graph *g) {
   BYO_HIP(hipStreamSynchronize(s));
   g->csr_valid = true;
   g->maxdeg_valid = false;
-  g->pr_rl_valid = false;
+  g->rl_valid = false;
 }
 
 void cpma_destroy(byo_graph *g) {
@@ -981,7 +981,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
       if (hctl->changed) {
         g->csr_valid = false;
         g->maxdeg_valid = false;
-        g->pr_rl_valid = false;
+        g->rl_valid = false;
       }
     } else {
       // global rebalance: flatten, merge with the batch, dedupe, respread
@@ -1013,7 +1013,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
     if (hctl->changed) {
       g->csr_valid = false;
       g->maxdeg_valid = false;
-      g->pr_rl_valid = false;
+      g->rl_valid = false;
     }
     if (c->nleaves > 1 && (double)c->total < kMinDensity * (double)(c->nleaves * kLeaf)) {
       c->rebalances++;
@@ -1102,7 +1102,7 @@ static uint32_t insert_graph(byo_graph *g, const uint64_t *keys, const uint64_t 
   if (changed) {
     g->csr_valid = false;
     g->maxdeg_valid = false;
-    g->pr_rl_valid = false;
+    g->rl_valid = false;
   }
   return stop;
 }

@@ -216,7 +216,7 @@ void csr_from_keys(byo_graph *g, uint64_t *keys, uint64_t *alt, uint64_t nk, uin
   BYO_HIP(hipStreamSynchronize(s));
   g->csr_valid = true;
   g->maxdeg_valid = false;
-  g->pr_rl_valid = false;
+  g->rl_valid = false;
 }
 
 void csr_upload(byo_graph *g, const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m) {
@@ -230,7 +230,7 @@ void csr_upload(byo_graph *g, const uint64_t *offsets, const uint32_t *ids, uint
   BYO_HIP(hipStreamSynchronize(g->stream));
   g->csr_valid = true;
   g->maxdeg_valid = false;
-  g->pr_rl_valid = false;
+  g->rl_valid = false;
 }
 
 void check_csr_input(const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m) {

@@ -107,6 +107,99 @@ __global__ void __launch_bounds__(256) k_pr_seg(const uint64_t *lo_pos, const ui
   }
 }
 
+// Pull over whole lists (one source segment) with the lanes per vertex
+// picked per tile: a wave takes 64 consecutive vertices and gives each
+// G = the smallest power of two with G * kPrUnroll >= the tile's largest
+// degree (at most 64) lanes, so a tile of leaves runs one vertex per lane
+// and a tile of hubs one vertex per wave.  Every lane keeps kPrUnroll
+// gathers in flight: the ids of a step are loaded first (clamped to the
+// vertex's first edge past its end, so the loads need no branch), then the
+// contributions, then the Kahan adds of the ones in range.  Tiles of
+// similar degrees -- vertices numbered by degree, as the relabelled copy
+// is -- waste the fewest lanes.
+constexpr int kPrUnroll = 8;
+
+template <class T, int G, bool kComp>
+__device__ __forceinline__ Acc<T> pr_tile(uint64_t lo, uint64_t hi, const uint32_t *ids, const T *contrib_in,
+                                          int lane) {
+  constexpr int VPR = 64 / G;  // vertices per round
+  const int sub = lane & (G - 1), grp = lane / G;
+  Acc<T> mine{0, 0};
+  for (int r = 0; r < G; r++) {
+    const int owner = r * VPR + grp;
+    const uint64_t b = G == 1 ? lo : __shfl(lo, owner, 64);
+    const uint64_t e = G == 1 ? hi : __shfl(hi, owner, 64);
+    Acc<T> a{0, 0};
+    for (uint64_t k = b + sub; k < e; k += (uint64_t)kPrUnroll * G) {
+      uint32_t u[kPrUnroll];
+      T x[kPrUnroll];
+#pragma unroll
+      for (int j = 0; j < kPrUnroll; j++) {
+        const uint64_t kk = k + (uint64_t)j * G;
+        u[j] = ids[kk < e ? kk : b];
+      }
+#pragma unroll
+      for (int j = 0; j < kPrUnroll; j++) x[j] = contrib_in[u[j]];
+#pragma unroll
+      for (int j = 0; j < kPrUnroll; j++) {
+        if (k + (uint64_t)j * G < e) {
+          if (kComp) kahan_add(a, x[j]); else a.s += x[j];
+        }
+      }
+    }
+    if (G == 1) return a;
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+      Acc<T> other{__shfl_xor(a.s, o, 64), __shfl_xor(a.c, o, 64)};
+      if (kComp) a = combine(a, other); else a.s += other.s;
+    }
+    // the group leader of vertex r*VPR + i is lane i*G
+    const int rel = lane - r * VPR;
+    const int src = (rel >= 0 && rel < VPR) ? rel * G : 0;
+    const T gs = __shfl(a.s, src, 64), gc = __shfl(a.c, src, 64);
+    if (rel >= 0 && rel < VPR) mine = Acc<T>{gs, gc};
+  }
+  return mine;
+}
+
+template <class T, bool kComp>
+__global__ void __launch_bounds__(256) k_pr_tiles(const uint64_t *off, const uint32_t *ids, uint64_t n,
+                                                  const T *contrib_in, const T *p_old, T *p_new, T *contrib_out,
+                                                  double damping, double addc, double *partial) {
+  __shared__ double lds[4];
+  const int lane = lane_id();
+  const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
+  const uint64_t ntiles = (n + 63) >> 6;
+  double resid = 0.0;
+  const T d = (T)damping, t = (T)addc;
+  for (uint64_t tile = wave; tile < ntiles; tile += nwaves) {
+    const uint64_t v = tile * 64 + lane;
+    const bool valid = v < n;
+    const uint64_t lo = off[valid ? v : n], hi = off[valid ? v + 1 : n];
+    const uint32_t dmax = wave_max((uint32_t)(hi - lo));
+    Acc<T> mine;
+    if (dmax <= kPrUnroll) mine = pr_tile<T, 1, kComp>(lo, hi, ids, contrib_in, lane);
+    else if (dmax <= 2 * kPrUnroll) mine = pr_tile<T, 2, kComp>(lo, hi, ids, contrib_in, lane);
+    else if (dmax <= 4 * kPrUnroll) mine = pr_tile<T, 4, kComp>(lo, hi, ids, contrib_in, lane);
+    else if (dmax <= 8 * kPrUnroll) mine = pr_tile<T, 8, kComp>(lo, hi, ids, contrib_in, lane);
+    else if (dmax <= 16 * kPrUnroll) mine = pr_tile<T, 16, kComp>(lo, hi, ids, contrib_in, lane);
+    else if (dmax <= 32 * kPrUnroll) mine = pr_tile<T, 32, kComp>(lo, hi, ids, contrib_in, lane);
+    else mine = pr_tile<T, 64, kComp>(lo, hi, ids, contrib_in, lane);
+    if (valid) {
+      const T sum = kComp ? (mine.s - mine.c) : mine.s;
+      const T pv = d * sum + t;
+      const uint64_t deg = hi - lo;
+      p_new[v] = pv;
+      contrib_out[v] = deg ? pv / (T)deg : (T)0;
+      resid += fabs((double)pv - (double)p_old[v]);
+    }
+  }
+  resid = wave_sum(resid);
+  if (lane == 0) lds[threadIdx.x >> 6] = resid;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+}
+
 // pos[(s-1)*n + v] = first index of v's list holding an id >= s*W
 // (s = 1 .. S-1).
 __global__ void k_pr_bounds(const uint64_t *off, const uint32_t *ids, uint64_t n, int S, uint64_t W, uint64_t *pos) {
@@ -161,29 +254,6 @@ __global__ void k_unpermute(const T *in, const uint32_t *perm, double *out, uint
   for (uint64_t v = global_tid(); v < n; v += grid_threads()) out[v] = (double)in[perm[v]];
 }
 
-// ------------------------------------------------ degree-ordered relabelling
-// Optional layout for the pull (tuning knob pr_relabel): vertices renumbered
-// by descending degree (ties by id), so the contributions the gathers hit
-// most sit together at low ids -- the hot part of the vector stays in L2.
-// Built once per graph on the device: a radix sort of (maxdeg - deg, v)
-// gives the order, every edge (u, v) becomes (perm u, perm v), a radix sort
-// of those keys and keys_to_csr give the relabelled CSR.
-__global__ void k_rl_vkeys(const uint64_t *off, uint64_t n, uint64_t maxdeg, uint64_t *keys) {
-  for (uint64_t v = global_tid(); v < n; v += grid_threads()) keys[v] = ((maxdeg - (off[v + 1] - off[v])) << 32) | v;
-}
-__global__ void k_rl_perm(const uint64_t *sorted, uint64_t n, uint32_t *perm) {
-  for (uint64_t i = global_tid(); i < n; i += grid_threads()) perm[sorted[i] & 0xFFFFFFFFull] = (uint32_t)i;
-}
-__global__ void k_rl_ekeys(const uint64_t *off, const uint32_t *ids, uint64_t n, const uint32_t *perm,
-                           uint64_t *keys) {
-  const int lane = lane_id();
-  const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
-  for (uint64_t u = wave; u < n; u += nwaves) {
-    const uint64_t pu = (uint64_t)perm[u] << 32;
-    for (uint64_t k = off[u] + lane; k < off[u + 1]; k += 64) keys[k] = pu | perm[ids[k]];
-  }
-}
-
 template <class T, int G, bool kComp>
 void pr_pass(bool first, bool last, const uint64_t *lo, const uint64_t *hi, const uint32_t *ids, const uint64_t *off,
              uint64_t n, const T *cin, T *as, T *ac, const T *pold, T *pnew, T *cout, double damping, double addc,
@@ -218,43 +288,13 @@ uint64_t segment_width(byo_graph *g, uint64_t n) {
   return W < n ? W : n;
 }
 
-struct PrView {
-  const uint64_t *off;
-  const uint32_t *ids;
-  const uint32_t *perm;  // old id -> new id, or null for the graph as it is
-};
-
-PrView pr_view(byo_graph *g) {
-  if (!g->tuning("pr_relabel") || g->n == 0) return PrView{g->off.as<uint64_t>(), g->ids.as<uint32_t>(), nullptr};
-  const uint64_t n = g->n, m = g->m;
-  uint64_t *off2 = g->buf("pr_rl_off", (n + 1) * 8 + 64).as<uint64_t>();
-  uint32_t *ids2 = g->buf("pr_rl_ids", m * 4 + 64).as<uint32_t>();
-  uint32_t *perm = g->buf("pr_rl_perm", n * 4 + 64).as<uint32_t>();
-  if (!g->pr_rl_valid) {
-    hipStream_t s = g->stream;
-    PhaseTimer t(s, g->prof.on);
-    const uint64_t maxdeg = graph_max_degree(g);
-    const uint64_t cap = std::max<uint64_t>(n, m);
-    DevBuf k0(cap * 8 + 64), k1(cap * 8 + 64), tmp(prim::radix_tmp_bytes(cap));
-    int idbits = 1, vbits = 1;
-    while ((1ull << idbits) < n) idbits++;
-    while ((1ull << vbits) < std::max<uint64_t>(n, maxdeg + 1)) vbits++;
-    int shifts[16], vshifts[16];
-    const int np = prim::edge_key_shifts(idbits, shifts);
-    const int nvp = prim::edge_key_shifts(vbits, vshifts);
-    k_rl_vkeys<<<byo_grid(n, 256, 16384), 256, 0, s>>>(g->off.as<uint64_t>(), n, maxdeg, k0.as<uint64_t>());
-    uint64_t *sv = prim::radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), n, vshifts, nvp, tmp.p, s);
-    k_rl_perm<<<byo_grid(n, 256, 16384), 256, 0, s>>>(sv, n, perm);
-    k_rl_ekeys<<<byo_grid(n * 64, 256, 16384), 256, 0, s>>>(g->off.as<uint64_t>(), g->ids.as<uint32_t>(), n, perm,
-                                                             k0.as<uint64_t>());
-    uint64_t *se = prim::radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), m, shifts, np, tmp.p, s);
-    prim::keys_to_csr(se, m, n, off2, ids2, s);
-    BYO_LAUNCH_CHECK();
-    BYO_HIP(hipStreamSynchronize(s));
-    if (g->prof.on) g->account("pr_relabel", t.stop());
-    g->pr_rl_valid = true;
-  }
-  return PrView{off2, ids2, perm};
+// The graph the pull runs on: as it is, or the degree-relabelled copy
+// (tuning knob pr_relabel), whose contributions the gathers hit most sit
+// together at low ids, where they stay in L2.
+RelabelView pr_view(byo_graph *g) {
+  if (!g->tuning("pr_relabel") || g->n == 0)
+    return RelabelView{g->off.as<uint64_t>(), g->ids.as<uint32_t>(), nullptr};
+  return relabel_view(g);
 }
 
 template <class T, bool kComp>
@@ -262,7 +302,7 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
                    int *iters_run, double *device_ms, const char *phase) {
   const uint64_t n = g->n;
   hipStream_t s = g->stream;
-  const PrView view = pr_view(g);
+  const RelabelView view = pr_view(g);
   T *p[2], *c[2];
   p[0] = g->buf("pr_p0", n * sizeof(T) + 64).as<T>();
   p[1] = g->buf("pr_p1", n * sizeof(T) + 64).as<T>();
@@ -273,7 +313,10 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   T *acc_s = S > 1 ? g->buf("pr_acc_s", n * sizeof(T) + 64).as<T>() : nullptr;
   T *acc_c = (S > 1 && kComp) ? g->buf("pr_acc_c", n * sizeof(T) + 64).as<T>() : nullptr;
   uint64_t *pos = S > 1 ? g->buf("pr_pos", (uint64_t)(S - 1) * n * 8 + 64).as<uint64_t>() : nullptr;
-  // lanes per vertex: about half the per-segment average degree, in [4, 32]
+  // lanes per vertex: picked per tile with one segment and pr_lanes 0;
+  // otherwise pr_lanes, or about half the per-segment average degree, in
+  // [4, 32]
+  const bool per_tile = S == 1 && !g->tuning("pr_lanes");
   const double avg = n ? (double)g->m / (double)n / S : 0.0;
   int G = 4;
   while (G < 32 && G < avg / 2) G *= 2;
@@ -292,11 +335,16 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   int it = 0, cur = 0;
   while (it < max_iters) {
     PhaseTimer it_t(s, g->prof.on);
-    for (int sg = 0; sg < S; sg++) {
-      const uint64_t *lo = sg == 0 ? off : pos + (uint64_t)(sg - 1) * n;
-      const uint64_t *hi = sg == S - 1 ? off + 1 : pos + (uint64_t)sg * n;
-      pr_pass_g<T, kComp>(G, sg == 0, sg == S - 1, lo, hi, ids, off, n, c[cur], acc_s, acc_c, p[cur], p[cur ^ 1],
-                          c[cur ^ 1], damping, addc, partial, grid, s);
+    if (per_tile) {
+      k_pr_tiles<T, kComp><<<grid, 256, 0, s>>>(off, ids, n, c[cur], p[cur], p[cur ^ 1], c[cur ^ 1], damping, addc,
+                                                partial);
+    } else {
+      for (int sg = 0; sg < S; sg++) {
+        const uint64_t *lo = sg == 0 ? off : pos + (uint64_t)(sg - 1) * n;
+        const uint64_t *hi = sg == S - 1 ? off + 1 : pos + (uint64_t)sg * n;
+        pr_pass_g<T, kComp>(G, sg == 0, sg == S - 1, lo, hi, ids, off, n, c[cur], acc_s, acc_c, p[cur],
+                            p[cur ^ 1], c[cur ^ 1], damping, addc, partial, grid, s);
+      }
     }
     k_sum_doubles<<<1, 1024, 0, s>>>(partial, grid, l1dev + it);
     BYO_LAUNCH_CHECK();

@@ -266,7 +266,8 @@ int byo_profile_reset(byo_graph_t g);
  *   coop_blocks_per_cu  1..4  blocks per CU of the persistent traversals (4)
  *   ppr_blocks_per_cu   1..8  blocks per CU of persistent PPR (4)
  *   pieces              1..16 word pieces per wave in vertex passes (8)
- *   pr_lanes            0,4,8,16,32 PageRank lanes per vertex (0: auto)
+ *   pr_lanes            0,4,8,16,32 PageRank lanes per vertex (0: per
+ *                       tile of 64 vertices from its largest degree)
  *   pr_grid             256..65536 PageRank pull grid (4096)
  *   pr_seg_log2         10..40 PageRank source-segment width (40: one)
  *   pr_relabel          0/1 PageRank on a copy of the graph relabelled by

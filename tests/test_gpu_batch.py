@@ -130,6 +130,26 @@ def test_batch_matches_sequential_engine(B, s16):
         assert O.check_bfs_tree(g, s, par_s[j], O.bfs_levels(g, s)) == 0
 
 
+def test_batch_follows_updates(B):
+    """Batched BFS on a CPMA graph after symmetric insert and remove batches."""
+    g = O.rmat_graph(12)
+    G = B.CpmaGraph.from_csr(g.offsets, g.ids)
+    s, d = O.update_rmat(g.n, 4000, 9)
+    one = (s.astype(np.uint64) << np.uint64(32)) | d.astype(np.uint64)
+    rev = (d.astype(np.uint64) << np.uint64(32)) | s.astype(np.uint64)
+    batch = np.concatenate([one, rev])  # symmetric: the pull reads out-lists as in-lists
+    for step in ("base", "insert", "remove"):
+        if step == "insert":
+            G.insert_batch(batch)
+        elif step == "remove":
+            G.remove_batch(np.concatenate([one[: len(one) // 2], rev[: len(rev) // 2]]))
+        off, ids = G.export_csr()
+        h = O.Graph(off, ids)
+        print(step, h.m, flush=True)
+        check_batch(h, G, O.bfs_sources(h, 9, seed=3))
+    G.close()
+
+
 def test_batch_learned_policy(B, s16):
     from byo_amd import selector as S
     if not os.path.exists(S.WEIGHTS):

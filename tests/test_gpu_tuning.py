@@ -83,12 +83,14 @@ def test_pagerank_relabel_isolated_and_updates(B):
     G = B.CpmaGraph.from_csr(g.offsets, g.ids)
     G.set_tuning("pr_relabel", 1)
     s, d = O.update_rmat(g.n, 5000, 3)
-    batch = (s.astype(np.uint64) << np.uint64(32)) | d.astype(np.uint64)
+    one = (s.astype(np.uint64) << np.uint64(32)) | d.astype(np.uint64)
+    rev = (d.astype(np.uint64) << np.uint64(32)) | s.astype(np.uint64)
+    batch = np.concatenate([one, rev])  # symmetric: the pull reads out-lists as in-lists
     for step in ("base", "insert", "remove"):
         if step == "insert":
             G.insert_batch(batch)
         elif step == "remove":
-            G.remove_batch(batch[: len(batch) // 2])
+            G.remove_batch(np.concatenate([one[: len(one) // 2], rev[: len(rev) // 2]]))
         off, ids = G.export_csr()
         h = O.Graph(off, ids)
         ref, _ = O.pagerank(h, 0.85, 15)
