@@ -8,8 +8,8 @@ knob is timed on:
 
     coop_blocks_per_cu, pieces   8-source BFS in one shared traversal (launch ms)
     ppr_blocks_per_cu            32-seed personalized PageRank, 5 iterations
-    pr_relabel, pr_lanes,        FP32 PageRank, 5 iterations (the one-time
-    pr_grid                      relabelling is built before the timing)
+    pr_relabel, pr_xcd,          FP32 PageRank, 5 iterations (the one-time
+    pr_lanes, pr_grid            layouts are built before the timing)
 
     python -m byo_amd.autotune --scale 22 --scale 23
 """
@@ -28,11 +28,12 @@ CANDIDATES = {
     "pieces": [4, 8, 16],
     "ppr_blocks_per_cu": [4, 8],
     "pr_relabel": [0, 1],
+    "pr_xcd": [0, 1],
     "pr_lanes": [0, 4, 8, 16, 32],
     "pr_grid": [4096, 8192, 16384, 32768, 65536],
 }
 WORKLOAD = {"coop_blocks_per_cu": "bfs", "pieces": "bfs", "ppr_blocks_per_cu": "ppr", "pr_lanes": "pr",
-            "pr_grid": "pr", "pr_relabel": "pr"}
+            "pr_grid": "pr", "pr_relabel": "pr", "pr_xcd": "pr"}
 
 
 def _sources(deg, k, seed):

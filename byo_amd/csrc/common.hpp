@@ -156,6 +156,7 @@ inline const std::vector<TuneSpec> &tune_specs() {
       {"pr_grid", 4096, 256, 65536, "PageRank pull grid (blocks of 256)"},
       {"pr_seg_log2", 40, 10, 40, "log2 of the PageRank source-segment width (40: one segment)"},
       {"pr_relabel", 0, 0, 1, "PageRank on a copy relabelled by descending degree (built once per graph)"},
+      {"pr_xcd", 0, 0, 1, "PageRank pull split into one source slice per XCD (built once per graph)"},
       {"bfs_engine", 0, 0, 2,
        "BFS of several sources: 0 one shared traversal per 64, 1 one persistent launch taking them in turn, "
        "2 host-driven level loop"},
@@ -178,6 +179,8 @@ struct byo_graph {
   uint64_t maxdeg = 0;        // maximum degree of the current CSR view
   bool maxdeg_valid = false;  // cleared whenever the CSR view is rebuilt
   bool rl_valid = false;      // degree-relabelled copy (relabel_view) is current
+  bool xs_valid = false;      // PageRank per-XCD slices are current (for layout xs_layout)
+  int xs_layout = 0;
   int64_t last_src = -1;      // last source of byo_bfs / byo_bfs_multi
 
   // CPMA state (kind 1); owned by cpma.hip.

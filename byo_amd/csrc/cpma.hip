@@ -759,6 +759,7 @@ static void cpma_build(byo_graph *g, const uint64_t *flat, uint64_t m) {
   g->csr_valid = true;
   g->maxdeg_valid = false;
   g->rl_valid = false;
+  g->xs_valid = false;
 }
 
 // Compacts the leaves into flat[total] (sorted, unique).
@@ -788,6 +789,7 @@ void cpma_ensure_csr(byo_graph *g) {
   g->csr_valid = true;
   g->maxdeg_valid = false;
   g->rl_valid = false;
+  g->xs_valid = false;
 }
 
 void cpma_destroy(byo_graph *g) {
@@ -982,6 +984,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
         g->csr_valid = false;
         g->maxdeg_valid = false;
         g->rl_valid = false;
+        g->xs_valid = false;
       }
     } else {
       // global rebalance: flatten, merge with the batch, dedupe, respread
@@ -1014,6 +1017,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
       g->csr_valid = false;
       g->maxdeg_valid = false;
       g->rl_valid = false;
+      g->xs_valid = false;
     }
     if (c->nleaves > 1 && (double)c->total < kMinDensity * (double)(c->nleaves * kLeaf)) {
       c->rebalances++;
@@ -1103,6 +1107,7 @@ static uint32_t insert_graph(byo_graph *g, const uint64_t *keys, const uint64_t 
     g->csr_valid = false;
     g->maxdeg_valid = false;
     g->rl_valid = false;
+    g->xs_valid = false;
   }
   return stop;
 }

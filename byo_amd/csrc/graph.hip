@@ -217,6 +217,7 @@ void csr_from_keys(byo_graph *g, uint64_t *keys, uint64_t *alt, uint64_t nk, uin
   g->csr_valid = true;
   g->maxdeg_valid = false;
   g->rl_valid = false;
+  g->xs_valid = false;
 }
 
 void csr_upload(byo_graph *g, const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m) {
@@ -231,6 +232,7 @@ void csr_upload(byo_graph *g, const uint64_t *offsets, const uint32_t *ids, uint
   g->csr_valid = true;
   g->maxdeg_valid = false;
   g->rl_valid = false;
+  g->xs_valid = false;
 }
 
 void check_csr_input(const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m) {

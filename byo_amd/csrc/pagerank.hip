@@ -200,6 +200,134 @@ __global__ void __launch_bounds__(256) k_pr_tiles(const uint64_t *off, const uin
   if (threadIdx.x == 0) partial[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
 }
 
+// ------------------------------------------------ pull split over the XCDs
+// (tuning knob pr_xcd).  Workgroups are dealt to the chip's 8 XCDs
+// round-robin, so workgroup b runs on XCD b % 8; XCD x sums, for every
+// vertex, only the neighbours in source slice x -- a range of ids holding
+// 1/8 of the edges -- so the contributions it gathers form one slice of the
+// vector, sized to stay in that XCD's own L2 instead of the whole vector
+// competing for every L2.  The slices' neighbour lists are stored apart
+// (slice x of vertex v at so[x * (n+1) + v] .. so[x * (n+1) + v + 1] of
+// ids_s), each XCD writes its (sum, compensation) per vertex, and a second
+// kernel combines the 8 partials in slice order (deterministic) and
+// finalises the ranks.
+constexpr int kXcds = 8;
+
+template <class T, bool kComp>
+__global__ void __launch_bounds__(256) k_pr_xcd(const uint32_t *so, const uint32_t *ids_s, uint64_t n,
+                                                const T *contrib_in, T *part_s, T *part_c) {
+  const int lane = lane_id();
+  const uint32_t x = blockIdx.x % kXcds;
+  const uint64_t wave = (uint64_t)(blockIdx.x / kXcds) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint64_t nwaves = (uint64_t)(gridDim.x / kXcds) * (blockDim.x >> 6);
+  const uint64_t ntiles = (n + 63) >> 6;
+  const uint32_t *sox = so + (uint64_t)x * (n + 1);
+  for (uint64_t tile = wave; tile < ntiles; tile += nwaves) {
+    const uint64_t v = tile * 64 + lane;
+    const bool valid = v < n;
+    const uint64_t lo = sox[valid ? v : n], hi = sox[valid ? v + 1 : n];
+    const uint32_t dmax = wave_max((uint32_t)(hi - lo));
+    Acc<T> mine;
+    if (dmax <= kPrUnroll) mine = pr_tile<T, 1, kComp>(lo, hi, ids_s, contrib_in, lane);
+    else if (dmax <= 2 * kPrUnroll) mine = pr_tile<T, 2, kComp>(lo, hi, ids_s, contrib_in, lane);
+    else if (dmax <= 4 * kPrUnroll) mine = pr_tile<T, 4, kComp>(lo, hi, ids_s, contrib_in, lane);
+    else if (dmax <= 8 * kPrUnroll) mine = pr_tile<T, 8, kComp>(lo, hi, ids_s, contrib_in, lane);
+    else if (dmax <= 16 * kPrUnroll) mine = pr_tile<T, 16, kComp>(lo, hi, ids_s, contrib_in, lane);
+    else if (dmax <= 32 * kPrUnroll) mine = pr_tile<T, 32, kComp>(lo, hi, ids_s, contrib_in, lane);
+    else mine = pr_tile<T, 64, kComp>(lo, hi, ids_s, contrib_in, lane);
+    if (valid) {
+      part_s[(uint64_t)x * n + v] = mine.s;
+      if (kComp) part_c[(uint64_t)x * n + v] = mine.c;
+    }
+  }
+}
+
+template <class T, bool kComp>
+__global__ void __launch_bounds__(256) k_pr_xcd_combine(const T *part_s, const T *part_c, const uint64_t *off,
+                                                        uint64_t n, const T *p_old, T *p_new, T *contrib_out,
+                                                        double damping, double addc, double *partial) {
+  __shared__ double lds[4];
+  const T d = (T)damping, t = (T)addc;
+  double resid = 0.0;
+  for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
+    Acc<T> a{part_s[v], kComp ? part_c[v] : (T)0};
+#pragma unroll
+    for (int x = 1; x < kXcds; x++) {
+      if (kComp) a = combine(a, Acc<T>{part_s[(uint64_t)x * n + v], part_c[(uint64_t)x * n + v]});
+      else a.s += part_s[(uint64_t)x * n + v];
+    }
+    const T sum = kComp ? (a.s - a.c) : a.s;
+    const T pv = d * sum + t;
+    const uint64_t deg = off[v + 1] - off[v];
+    p_new[v] = pv;
+    contrib_out[v] = deg ? pv / (T)deg : (T)0;
+    resid += fabs((double)pv - (double)p_old[v]);
+  }
+  resid = wave_sum(resid);
+  if (lane_id() == 0) lds[threadIdx.x >> 6] = resid;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+}
+
+// Slice bounds: bnd[x] = first vertex v with off[v] >= x * m / 8.
+__global__ void k_xs_bounds(const uint64_t *off, uint64_t n, uint32_t *bnd) {
+  const int x = threadIdx.x;
+  if (x > kXcds) return;
+  const uint64_t target = off[n] / kXcds * x + (off[n] % kXcds) * x / kXcds;
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (off[mid] >= target) hi = mid; else lo = mid + 1;
+  }
+  bnd[x] = x == kXcds ? (uint32_t)n : (uint32_t)lo;
+}
+
+struct XcdBounds {
+  uint32_t b[kXcds + 1];
+};
+
+// First index of v's sorted list [lo, hi) holding an id >= bound.
+__device__ __forceinline__ uint64_t list_lower_bound(const uint32_t *ids, uint64_t lo, uint64_t hi, uint32_t bound) {
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (ids[mid] < bound) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// cnt[x * (n+1) + v] = neighbours of v in slice x (0 at v = n).
+__global__ void k_xs_count(const uint64_t *off, const uint32_t *ids, uint64_t n, XcdBounds bd, uint32_t *cnt) {
+  for (uint64_t v = global_tid(); v <= n; v += grid_threads()) {
+    if (v == n) {
+      for (int x = 0; x < kXcds; x++) cnt[(uint64_t)x * (n + 1) + n] = 0;
+      continue;
+    }
+    uint64_t a = off[v];
+    const uint64_t e = off[v + 1];
+    for (int x = 0; x < kXcds; x++) {
+      const uint64_t b = x == kXcds - 1 ? e : list_lower_bound(ids, a, e, bd.b[x + 1]);
+      cnt[(uint64_t)x * (n + 1) + v] = (uint32_t)(b - a);
+      a = b;
+    }
+  }
+}
+
+__global__ void k_xs_scatter(const uint64_t *off, const uint32_t *ids, uint64_t n, XcdBounds bd, const uint32_t *so,
+                             uint32_t *ids_s) {
+  for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
+    int x = 0;
+    uint64_t pos = so[v];
+    for (uint64_t e = off[v]; e < off[v + 1]; e++) {
+      const uint32_t u = ids[e];
+      while (x < kXcds - 1 && u >= bd.b[x + 1]) {
+        x++;
+        pos = so[(uint64_t)x * (n + 1) + v];
+      }
+      ids_s[pos++] = u;
+    }
+  }
+}
+
 // pos[(s-1)*n + v] = first index of v's list holding an id >= s*W
 // (s = 1 .. S-1).
 __global__ void k_pr_bounds(const uint64_t *off, const uint32_t *ids, uint64_t n, int S, uint64_t W, uint64_t *pos) {
@@ -297,6 +425,39 @@ RelabelView pr_view(byo_graph *g) {
   return relabel_view(g);
 }
 
+// Per-slice neighbour lists of the pull's graph view for k_pr_xcd, built
+// once per graph and layout (relabelled or not).
+struct XcdSlices {
+  const uint32_t *so, *ids_s;
+};
+
+XcdSlices xcd_slices(byo_graph *g, const RelabelView &view) {
+  const uint64_t n = g->n, m = g->m;
+  uint32_t *so = g->buf("pr_xs_so", (uint64_t)kXcds * (n + 1) * 4 + 64).as<uint32_t>();
+  uint32_t *ids_s = g->buf("pr_xs_ids", m * 4 + 64).as<uint32_t>();
+  const int layout = view.perm ? 1 : 0;
+  if (!g->xs_valid || g->xs_layout != layout) {
+    hipStream_t s = g->stream;
+    PhaseTimer t(s, g->prof.on);
+    uint32_t *dbnd = g->buf("pr_xs_bnd", 64 * 4).as<uint32_t>();
+    k_xs_bounds<<<1, 64, 0, s>>>(view.off, n, dbnd);
+    XcdBounds bd{};
+    BYO_HIP(hipMemcpyAsync(bd.b, dbnd, sizeof(bd.b), hipMemcpyDeviceToHost, s));
+    BYO_HIP(hipStreamSynchronize(s));
+    const uint64_t cnt_n = (uint64_t)kXcds * (n + 1);
+    DevBuf cnt(cnt_n * 4 + 64), tmp(prim::scan_tmp_bytes(cnt_n));
+    k_xs_count<<<byo_grid(n + 1, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd, cnt.as<uint32_t>());
+    prim::exclusive_scan_u32(cnt.as<uint32_t>(), so, cnt_n, false, tmp.p, s);
+    k_xs_scatter<<<byo_grid(n, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd, so, ids_s);
+    BYO_LAUNCH_CHECK();
+    BYO_HIP(hipStreamSynchronize(s));
+    if (g->prof.on) g->account("pr_xcd_slices", t.stop());
+    g->xs_valid = true;
+    g->xs_layout = layout;
+  }
+  return XcdSlices{so, ids_s};
+}
+
 template <class T, bool kComp>
 void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, double *p_out, double *l1_log,
                    int *iters_run, double *device_ms, const char *phase) {
@@ -317,12 +478,22 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   // otherwise pr_lanes, or about half the per-segment average degree, in
   // [4, 32]
   const bool per_tile = S == 1 && !g->tuning("pr_lanes");
+  // the XCD split needs the tile pull and 32-bit edge positions
+  const bool per_xcd = per_tile && g->tuning("pr_xcd") && g->m < (1ull << 32) && n > 0;
   const double avg = n ? (double)g->m / (double)n / S : 0.0;
   int G = 4;
   while (G < 32 && G < avg / 2) G *= 2;
   if (g->tuning("pr_lanes")) G = (int)g->tuning("pr_lanes");
   const unsigned grid = (unsigned)g->tuning("pr_grid");
-  double *partial = g->buf("pr_partial", (grid + 8) * sizeof(double)).as<double>();
+  const unsigned cgrid = (unsigned)byo_grid(n, 256, 4096);
+  double *partial = g->buf("pr_partial", (std::max(grid, cgrid) + 8) * sizeof(double)).as<double>();
+  XcdSlices xs{};
+  T *part_s = nullptr, *part_c = nullptr;
+  if (per_xcd) {
+    xs = xcd_slices(g, view);
+    part_s = g->buf("pr_xs_part_s", (uint64_t)kXcds * n * sizeof(T) + 64).as<T>();
+    if (kComp) part_c = g->buf("pr_xs_part_c", (uint64_t)kXcds * n * sizeof(T) + 64).as<T>();
+  }
   double *l1dev = g->buf("pr_l1", (size_t)(max_iters + 1) * sizeof(double)).as<double>();
   const uint64_t *off = view.off;
   const uint32_t *ids = view.ids;
@@ -335,7 +506,13 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   int it = 0, cur = 0;
   while (it < max_iters) {
     PhaseTimer it_t(s, g->prof.on);
-    if (per_tile) {
+    unsigned nparts = grid;
+    if (per_xcd) {
+      k_pr_xcd<T, kComp><<<grid / kXcds * kXcds, 256, 0, s>>>(xs.so, xs.ids_s, n, c[cur], part_s, part_c);
+      k_pr_xcd_combine<T, kComp><<<cgrid, 256, 0, s>>>(part_s, part_c, off, n, p[cur], p[cur ^ 1], c[cur ^ 1],
+                                                       damping, addc, partial);
+      nparts = cgrid;
+    } else if (per_tile) {
       k_pr_tiles<T, kComp><<<grid, 256, 0, s>>>(off, ids, n, c[cur], p[cur], p[cur ^ 1], c[cur ^ 1], damping, addc,
                                                 partial);
     } else {
@@ -346,7 +523,7 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
                             p[cur ^ 1], c[cur ^ 1], damping, addc, partial, grid, s);
       }
     }
-    k_sum_doubles<<<1, 1024, 0, s>>>(partial, grid, l1dev + it);
+    k_sum_doubles<<<1, 1024, 0, s>>>(partial, nparts, l1dev + it);
     BYO_LAUNCH_CHECK();
     if (g->prof.on) g->account(phase, it_t.stop());
     cur ^= 1;

@@ -274,6 +274,11 @@ int byo_profile_reset(byo_graph_t g);
  *                       descending degree, built on the first call after
  *                       each change of the graph; ranks return in the
  *                       original ids (0)
+ *   pr_xcd              0/1 PageRank pull split into 8 source slices of
+ *                       equal edge count, slice x summed by the workgroups
+ *                       that run on XCD x (blockIdx % 8), partial sums
+ *                       combined in slice order; slices built on the first
+ *                       call after each change of the graph (0)
  *   bfs_engine          0..2 several-source BFS: 0 shared traversal per 64
  *                       sources, 1 one persistent launch taking them in
  *                       turn, 2 host-driven level loop (0)

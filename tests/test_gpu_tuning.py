@@ -62,26 +62,35 @@ def test_pagerank_knobs(B, s15, key, values):
 
 
 @pytest.mark.parametrize("precision", ["PR_FP32", "PR_FP64"])
-def test_pagerank_relabel(B, s15, precision):
-    """PageRank on the degree-relabelled copy, mapped back to the original ids."""
+@pytest.mark.parametrize("relabel,xcd", [(1, 0), (0, 1), (1, 1)])
+def test_pagerank_layouts(B, s15, precision, relabel, xcd):
+    """PageRank on the degree-relabelled copy (ranks mapped back to the
+    original ids) and with the pull split into per-XCD source slices."""
     g, G = s15
     ref, _ = O.pagerank(g, 0.85, 20)
-    G.set_tuning("pr_relabel", 1)
+    G.set_tuning("pr_relabel", relabel)
+    G.set_tuning("pr_xcd", xcd)
     try:
         p, _, _ = G.pagerank(0.85, 20, 0.0, getattr(B, precision))
-        p2, _, _ = G.pagerank(0.85, 20, 0.0, getattr(B, precision))  # cached copy
+        p2, _, _ = G.pagerank(0.85, 20, 0.0, getattr(B, precision))  # cached layout
+        G.set_tuning("pr_grid", 300)  # not a multiple of the 8 XCDs
+        p3, _, _ = G.pagerank(0.85, 20, 0.0, getattr(B, precision))
     finally:
         G.set_tuning("pr_relabel", 0)
+        G.set_tuning("pr_xcd", 0)
+        G.set_tuning("pr_grid", 4096)
     assert np.abs(p - ref).sum() <= 1e-6
     assert np.array_equal(p, p2)
+    assert np.abs(p3 - ref).sum() <= 1e-6
 
 
-def test_pagerank_relabel_isolated_and_updates(B):
-    """Isolated vertices and ties in degree; the relabelled copy follows the
-    graph through insert and remove batches."""
+def test_pagerank_layouts_isolated_and_updates(B):
+    """Isolated vertices and ties in degree; the relabelled copy and the XCD
+    slices follow the graph through insert and remove batches."""
     g = O.rmat_graph(12)
     G = B.CpmaGraph.from_csr(g.offsets, g.ids)
     G.set_tuning("pr_relabel", 1)
+    G.set_tuning("pr_xcd", 1)
     s, d = O.update_rmat(g.n, 5000, 3)
     one = (s.astype(np.uint64) << np.uint64(32)) | d.astype(np.uint64)
     rev = (d.astype(np.uint64) << np.uint64(32)) | s.astype(np.uint64)
