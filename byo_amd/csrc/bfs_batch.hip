@@ -41,7 +41,7 @@ void prepare(byo_graph *g, const uint32_t *srcs, uint32_t k, uint64_t maxdeg, Ba
   a.nlev = reinterpret_cast<uint32_t *>(out + kNlevOff);
   a.tot = reinterpret_cast<unsigned long long *>(out + kTotOff);
   a.dc = dir_ctl(g, maxdeg, s, true);
-  a.pieces = make_pieces(g);
+  a.pieces = make_pieces(g, nblocks);
   a.bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
   a.bsum = g->buf("bfs_batch_bsum", (size_t)nblocks * 8 + 64).as<unsigned long long>();
   a.seen = g->buf("bfs_batch_seen", a.nm * mask_bytes + 64).p;
@@ -102,6 +102,7 @@ double launch(byo_graph *g, const uint32_t *srcs, uint32_t k, uint64_t maxdeg, B
   void *args[] = {&a};
   PhaseTimer t(g->stream, true);
   BYO_HIP(hipLaunchCooperativeKernel(kern, dim3(nblocks), dim3(kBlock), args, 0, g->stream));
+  pieces_launched(g, a.pieces, nblocks);
   return t.stop();
 }
 }  // namespace

@@ -181,11 +181,9 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
   // logs[64] holds wall-clock marks: start, seeded, traversal end, tallied
   if (leader) a.logs[64].fsize = wall_clock64();
   pieces_init(a.pieces, a.off, a.n, nwords, nwaves);
-  // ---- seed: parents cleared, masks cleared, then every source's bit set
-  for (uint32_t j = 0; j < a.k; j++) {
-    uint32_t *par = a.parents + (uint64_t)j * a.pstride;
-    for (uint64_t i = global_tid(); i < a.n; i += grid_threads()) par[i] = kNone;
-  }
+  // ---- seed: masks cleared, then every source's bit set.  (Parents are
+  // not cleared: the tally pass writes kNone for every source that did not
+  // reach a vertex, the traversal writes all the others.)
   {
     const uint64_t nvec = a.nm * sizeof(M) / 16;
     uint4 *s4 = reinterpret_cast<uint4 *>(seen);
@@ -640,10 +638,47 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
     a.logs[64].active = t;
   }
 
-  // ---- per-source tallies: vertices reached and the sum of their degrees.
-  // Lane j counts source j over each word's 64 masks staged in LDS; blocks
-  // combine lanes in LDS and add their totals with one atomic per source.
-  {
+  // ---- per-source tallies: vertices reached and the sum of their degrees,
+  // and kNone as the parent of every (source, vertex) pair not reached.
+  // Up to 8 sources: every lane counts its own vertices for all sources in
+  // registers, and the wave sums them once at the end.  More: lane j counts
+  // source j over each word's 64 masks staged in LDS.  Blocks combine lanes
+  // in LDS and add their totals with one atomic per source.
+  if constexpr (sizeof(M) == 1) {
+    unsigned long long rc[8] = {}, rd[8] = {};
+    for_my_words(a.pieces, wave, nwaves, [&](uint64_t w) {
+      const uint64_t vl = w * 64 + lane;
+      if (vl < a.n) {
+        const uint32_t sm = (uint32_t)seen[vl];
+        const unsigned long long d = a.off[vl + 1] - a.off[vl];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const unsigned long long bit = (sm >> j) & 1;
+          rc[j] += bit;
+          rd[j] += bit * d;
+          if (!bit && (uint32_t)j < a.k) a.parents[(uint64_t)j * a.pstride + vl] = kNone;
+        }
+      }
+    });
+    if (threadIdx.x < 64) {
+      tl[0][threadIdx.x] = 0;
+      tl[1][threadIdx.x] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const unsigned long long c = wave_sum(rc[j]), d = wave_sum(rd[j]);
+      if (lane == 0 && c) {
+        atomicAdd(&tl[0][j], c);
+        atomicAdd(&tl[1][j], d);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < a.k && tl[0][threadIdx.x]) {
+      atomicAdd(&a.tot[2 * threadIdx.x], tl[0][threadIdx.x]);
+      atomicAdd(&a.tot[2 * threadIdx.x + 1], tl[1][threadIdx.x]);
+    }
+  } else {
     unsigned long long rc = 0, rd = 0;
     for_my_words(a.pieces, wave, nwaves, [&](uint64_t w) {
       const uint64_t vl = w * 64 + lane;
@@ -651,6 +686,8 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
       if (vl < a.n) {
         sm = (uint64_t)seen[vl];
         d = a.off[vl + 1] - a.off[vl];
+        for (uint32_t j = 0; j < a.k; j++)
+          if (!((sm >> j) & 1)) a.parents[(uint64_t)j * a.pstride + vl] = kNone;
       }
       if (!__ballot(sm != 0)) return;
       res[wv][lane] = sm;

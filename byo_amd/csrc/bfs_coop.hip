@@ -542,11 +542,18 @@ int coop_grid(const void *kernel, int block, int cap) {
   return nb;
 }
 
-Pieces make_pieces(byo_graph *g) {
+Pieces make_pieces(byo_graph *g, int nblocks) {
   Pieces pc;
   pc.per_wave = (uint32_t)g->tuning("pieces");
   pc.bound = g->buf("bfs_pieces", ((uint64_t)kMaxPieces * 2048 * 4 + 1) * 8 + 64).as<uint64_t>();
+  pc.ready = g->pieces_valid && g->pieces_nblocks == nblocks && g->pieces_per_wave == pc.per_wave;
   return pc;
+}
+
+void pieces_launched(byo_graph *g, const Pieces &pc, int nblocks) {
+  g->pieces_valid = true;
+  g->pieces_nblocks = nblocks;
+  g->pieces_per_wave = pc.per_wave;
 }
 
 // Runs k BFS in one cooperative launch.  Results: k*n parents in scratch
@@ -578,8 +585,8 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   a.nlevels = g->buf("bfs_coop_nlev", (size_t)k * 4 + 64).as<uint32_t>();
   a.tot = g->buf("bfs_coop_tot", (size_t)k * 16 + 64).as<unsigned long long>();
   a.dc = dir_ctl(g, maxdeg, s, false);
-  a.pieces = make_pieces(g);
   const int nblocks = coop_grid(reinterpret_cast<const void *>(k_bfs_coop), kBlock, (int)g->tuning("coop_blocks_per_cu"));
+  a.pieces = make_pieces(g, nblocks);
   a.bsum = g->buf("bfs_coop_bsum", (size_t)nblocks * 8 + 64).as<unsigned long long>();
   a.bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
   BYO_HIP(hipMemsetAsync(a.bar, 0, sizeof(GridBar), s));
@@ -588,6 +595,7 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   PhaseTimer t(s, true);
   BYO_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_bfs_coop), dim3(nblocks), dim3(kBlock), args, 0,
                                      s));
+  pieces_launched(g, a.pieces, nblocks);
   const double ms = t.stop();
   if (ms_out) *ms_out = ms;
   if (stats) {

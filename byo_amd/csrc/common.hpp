@@ -179,6 +179,9 @@ struct byo_graph {
   uint64_t maxdeg = 0;        // maximum degree of the current CSR view
   bool maxdeg_valid = false;  // cleared whenever the CSR view is rebuilt
   bool rl_valid = false;      // degree-relabelled copy (relabel_view) is current
+  bool pieces_valid = false;  // "bfs_pieces" holds the word pieces of this grid:
+  int pieces_nblocks = 0;
+  uint32_t pieces_per_wave = 0;
   bool xs_valid = false;      // PageRank per-XCD slices are current (for layout xs_layout)
   int xs_layout = 0;
   int64_t last_src = -1;      // last source of byo_bfs / byo_bfs_multi

@@ -87,6 +87,7 @@ constexpr uint32_t kMaxPieces = 16;
 struct Pieces {
   uint64_t *bound;  // per_wave * nwaves + 1 word bounds
   uint32_t per_wave;
+  uint32_t ready;   // bounds already hold this graph's pieces for this grid
 };
 
 // Piece bounds, computed once per launch by the whole grid before its first
@@ -94,6 +95,7 @@ struct Pieces {
 // W(w) = off[v] + 8 v, v = min(64 w, n), and P = per_wave * nwaves.
 __device__ inline void pieces_init(const Pieces &pc, const uint64_t *off, uint64_t n, uint64_t nwords,
                                    uint64_t nwaves) {
+  if (pc.ready) return;
   uint64_t *bound = pc.bound;
   const uint64_t P = (uint64_t)pc.per_wave * nwaves;
   const uint64_t total = off[n] + 8 * n;
@@ -211,7 +213,13 @@ DirCtl dir_ctl(byo_graph *g, uint64_t maxdeg, hipStream_t s, bool batched);
 // (cap from the handle's tuning, at most 8; at most 2048 blocks).
 int coop_grid(const void *kernel, int block, int cap);
 
-// Word pieces of a cooperative launch (buffer and the "pieces" knob).
-Pieces make_pieces(byo_graph *g);
+// Word pieces of a cooperative launch of nblocks blocks (buffer and the
+// "pieces" knob).  The bounds depend only on the graph and the grid, so they
+// are computed by the first launch after either changes and reused by the
+// next ones (ready set); a launcher reports a queued launch with
+// pieces_launched.
+Pieces make_pieces(byo_graph *g, int nblocks);
+// Records that a launch computing pc's bounds has been queued.
+void pieces_launched(byo_graph *g, const Pieces &pc, int nblocks);
 
 }  // namespace byo

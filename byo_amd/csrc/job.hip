@@ -127,7 +127,6 @@ PprArgs ppr_args(byo_graph *g, const uint32_t *seeds, uint32_t k, double damping
   uint32_t *dseeds = g->buf("ppr_seeds", 64 * 4 + 64).as<uint32_t>();
   if (k) BYO_HIP(hipMemcpyAsync(dseeds, seeds, k * 4, hipMemcpyHostToDevice, g->stream));
   q.seeds = dseeds;
-  q.pieces = make_pieces(g);
   return q;
 }
 
@@ -151,9 +150,11 @@ double job_launch(byo_graph *g, const uint32_t *srcs, uint32_t kb, uint64_t maxd
   const int nblocks = coop_grid(kern, kCoopBlock, cap);
   batch::prepare(g, srcs, std::max<uint32_t>(kb, 1), maxdeg, a, (int)sizeof(M), nblocks);
   a.k = kb;
+  q.pieces = a.pieces;  // one set of pieces for both parts
   void *args[] = {&a, &q};
   PhaseTimer t(g->stream, true);
   BYO_HIP(hipLaunchCooperativeKernel(kern, dim3(nblocks), dim3(kCoopBlock), args, 0, g->stream));
+  pieces_launched(g, a.pieces, nblocks);
   return t.stop();
 }
 
@@ -178,11 +179,13 @@ int byo_ppr_multi(byo_graph_t g, const uint32_t *seeds, uint32_t k, double dampi
     PprArgs q = ppr_args(g, seeds, k, damping, iters);
     const void *kern = reinterpret_cast<const void *>(k_ppr_persist);
     const int nblocks = coop_grid(kern, kCoopBlock, (int)g->tuning("ppr_blocks_per_cu"));
+    q.pieces = make_pieces(g, nblocks);
     GridBar *bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
     BYO_HIP(hipMemsetAsync(bar, 0, sizeof(GridBar), g->stream));
     void *args[] = {&q, &bar};
     PhaseTimer t(g->stream, true);
     BYO_HIP(hipLaunchCooperativeKernel(kern, dim3(nblocks), dim3(kCoopBlock), args, 0, g->stream));
+    pieces_launched(g, q.pieces, nblocks);
     const double ms = t.stop();
     if (g->prof.on) g->account("ppr", ms);
     if (device_ms) *device_ms = ms;
