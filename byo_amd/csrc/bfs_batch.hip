@@ -6,7 +6,7 @@ namespace byo {
 namespace batch {
 
 template <class M>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) k_bfs_batch(BatchArgs a) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_bfs_batch(BatchArgs a) {
   bfs_batch_body<M>(a);
 }
 
