@@ -74,6 +74,9 @@ SIGNATURES = {
     "byo_neighbors": (_i32, [_vp, _u32, _vp, _u64, _P(_u64)]),
     "byo_export_csr": (_i32, [_vp, _vp, _vp]),
     "byo_memory_size": (_i32, [_vp, _P(_u64)]),
+    "byo_adjacency_info": (_i32, [C.c_char_p, _i32, _P(_u64), _P(_u64)]),
+    "byo_adjacency_read": (_i32, [C.c_char_p, _i32, _u64, _u64, _vp, _vp]),
+    "byo_adjacency_write": (_i32, [C.c_char_p, _i32, _vp, _vp, _u64, _u64]),
     "byo_insert_batch": (_i32, [_vp, _vp, _u64]),
     "byo_remove_batch": (_i32, [_vp, _vp, _u64]),
     "byo_insert_batch_dev": (_i32, [_vp, _vp, _u64]),
@@ -133,6 +136,25 @@ def check(rc):
     if rc != OK:
         raise ByoError(rc, lib().byo_last_error().decode())
     return rc
+
+
+def read_adjacency(path, binary=False):
+    """(offsets, ids) of an AdjacencyGraph file (gbbs/graph_io.cc:35-100);
+    host only."""
+    p = os.fsencode(path)
+    n, m = _u64(), _u64()
+    check(lib().byo_adjacency_info(p, int(binary), C.byref(n), C.byref(m)))
+    off = np.empty(n.value + 1, np.uint64)
+    ids = np.empty(max(m.value, 1), np.uint32)
+    check(lib().byo_adjacency_read(p, int(binary), n.value, m.value, _ptr(off), _ptr(ids)))
+    return off, ids[:m.value]
+
+
+def write_adjacency(path, offsets, ids, binary=False):
+    """Writes a host CSR as an AdjacencyGraph file; host only."""
+    off = np.ascontiguousarray(offsets, np.uint64)
+    ids = np.ascontiguousarray(ids, np.uint32)
+    check(lib().byo_adjacency_write(os.fsencode(path), int(binary), _ptr(off), _ptr(ids), len(off) - 1, len(ids)))
 
 
 def device_count():
@@ -252,6 +274,12 @@ class Graph:
         ids = np.empty(max(m, 1), np.uint32)
         check(lib().byo_export_csr(self._h, _ptr(off), _ptr(ids)))
         return off, ids[:m]
+
+    def save(self, path, binary=False):
+        """Writes the graph as an AdjacencyGraph file (text, or the -b binary
+        layout)."""
+        off, ids = self.export_csr()
+        write_adjacency(path, off, ids, binary)
 
     def get_memory_size(self):
         b = _u64()
@@ -429,6 +457,11 @@ class CsrGraph(Graph):
         return cls(h.value)
 
     @classmethod
+    def load(cls, path, binary=False):
+        """CSR from an AdjacencyGraph file (text, or the -b binary layout)."""
+        return cls.from_csr(*read_adjacency(path, binary))
+
+    @classmethod
     def rmat(cls, scale, edge_factor=16, a=0.5, b=0.1, c=0.1, seed=1):
         h = C.c_void_p()
         check(lib().byo_csr_create_rmat(scale, edge_factor, a, b, c, seed, C.byref(h)))
@@ -449,6 +482,10 @@ class CsrGraph(Graph):
 
 class CpmaGraph(Graph):
     """CPMA-style packed-memory-array graph (run_single_pma.cc:29-140)."""
+
+    @classmethod
+    def load(cls, path, binary=False):
+        return cls.from_csr(*read_adjacency(path, binary))
 
     @classmethod
     def from_csr(cls, offsets, ids):

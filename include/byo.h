@@ -103,6 +103,22 @@ int byo_export_csr(byo_graph_t g, uint64_t *offsets, uint32_t *ids);
  * run_single_pma.cc:126-131). */
 int byo_memory_size(byo_graph_t g, uint64_t *bytes);
 
+/* AdjacencyGraph files, host only (no device needed).  Text: the tokens
+ * "AdjacencyGraph" n m, n offsets, m ids, as parse_unweighted_graph reads
+ * them (gbbs/graph_io.cc:35-100) and write_graph_to_file writes them
+ * (gbbs/graph_io.h:634-687), one number per line.  Binary (binary != 0):
+ * the -b layout of graph_io.cc:86-97 with -DLONG -- int64 {n, m, 4m},
+ * uint64 offsets[n+1], uint32 ids[m].  byo_adjacency_info reads n and m
+ * from the header; byo_adjacency_read fills caller arrays of n+1 and m
+ * entries (n and m must match the header) and checks the CSR (monotone
+ * offsets, ids < n); malformed files give BYO_E_INVAL.  byo_adjacency_write
+ * writes a host CSR (e.g. from byo_export_csr). */
+int byo_adjacency_info(const char *path, int binary, uint64_t *n, uint64_t *m);
+int byo_adjacency_read(const char *path, int binary, uint64_t n, uint64_t m,
+                       uint64_t *offsets, uint32_t *ids);
+int byo_adjacency_write(const char *path, int binary, const uint64_t *offsets,
+                        const uint32_t *ids, uint64_t n, uint64_t m);
+
 /* Batch updates.  Replace Graph::insert_batch / remove_batch
  * (gbbs/graph.h:3071-3124) over PMA_graph::insert_sorted_batch /
  * remove_sorted_batch (run_single_pma.cc:63-70).  keys[b] use the

@@ -86,6 +86,34 @@ def test_fixture_graphs(B, golden, name):
         assert np.array_equal(G.neighbors(v), g.neighbors(v))
 
 
+@pytest.mark.parametrize("binary", [False, True])
+def test_load_and_save_files(B, golden, tmp_path, binary):
+    """Containers loaded from the reference's AdjacencyGraph inputs give the
+    golden results; a device graph saved and loaded again is the same."""
+    name = "rMatGraph_J_5_100"
+    src = os.path.join(INPUTS, name)
+    if binary:
+        p = str(tmp_path / "g.bin")
+        B.write_adjacency(p, *B.read_adjacency(src), binary=True)
+    else:
+        p = src
+    f = golden[name]
+    for cls in (B.CsrGraph, B.CpmaGraph):
+        G = cls.load(p, binary)
+        assert G.N() == f["n"] and G.M() == f["m"]
+        assert G.tc()[0] == f["triangles"]
+        assert list(G.cc()[0]) == f["cc_labels"]
+        G.close()
+    R = B.CsrGraph.rmat(12, 8)
+    q = str(tmp_path / "r.out")
+    R.save(q, binary)
+    R2 = B.CsrGraph.load(q, binary)
+    a, b = R.export_csr(), R2.export_csr()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    R.close()
+    R2.close()
+
+
 def test_device_rmat_equals_oracle_generator(B, golden):
     for scale, ef in [(8, 4), (12, 16), (16, 16)]:
         g = O.rmat_graph(scale, ef)
