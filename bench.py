@@ -276,6 +276,15 @@ def main():
         s, map_ms = G.map_sum()
         s, map_ms = G.map_sum()
         ex["map_GBs"] = round((8 * (n + 1) + 4 * m) / (map_ms * 1e-3) / 1e9, 1)
+        # Map_withRemote_<w>: one gather of a w-byte element per edge, the
+        # runner's one-hot input; G gathers/s for each element width
+        mr = {}
+        for w in (0, 1, 2, 4, 8, 16, 32, 64):
+            one = np.zeros(max(n * w, (n + 7) // 8), np.uint8)
+            one[0] = 1
+            G.map_remote(one, w)
+            mr[str(w)] = round(m / (min(G.map_remote(one, w)[1] for _ in range(3)) * 1e-3) / 1e9, 1)
+        ex["map_remote_Gedges_s"] = mr
         # CPMA batch updates at batch size cpma_batch on RMAT-cpma_scale
         if args.cpma_scale == args.scale:
             H = G

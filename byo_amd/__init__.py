@@ -96,6 +96,7 @@ SIGNATURES = {
     "byo_cc_labelprop": (_i32, [_vp, _vp, _P(_u32), _P(_dbl)]),
     "byo_tc": (_i32, [_vp, _P(_u64), _P(_dbl)]),
     "byo_map_sum": (_i32, [_vp, _P(_u64), _P(_dbl)]),
+    "byo_map_remote": (_i32, [_vp, _vp, _i32, _P(_u64), _P(_dbl)]),
     "byo_profile_enable": (_i32, [_vp, _i32]),
     "byo_profile_get": (_i32, [_vp, C.c_char_p, _P(_u64), _P(_dbl)]),
     "byo_profile_reset": (_i32, [_vp]),
@@ -415,6 +416,21 @@ class Graph:
         ms = _dbl()
         check(lib().byo_tc(self._h, C.byref(c), C.byref(ms)))
         return c.value, ms.value
+
+    def map_remote(self, remote, read_size):
+        """Map_withRemote_<read_size>: sum over edges of remote[dst]; remote
+        is a byte buffer of n elements of read_size bytes (0: packed bits,
+        little bit order).  Returns (sum, device ms)."""
+        buf = np.ascontiguousarray(remote).view(np.uint8).reshape(-1)
+        if buf.size == 0:
+            buf = np.zeros(1, np.uint8)
+        n = self.N()
+        need = (n + 7) // 8 if read_size == 0 else n * read_size
+        if buf.size < need:
+            raise ValueError("remote holds %d bytes, %d needed" % (buf.size, need))
+        s, ms = _u64(), _dbl()
+        check(lib().byo_map_remote(self._h, _ptr(buf), int(read_size), C.byref(s), C.byref(ms)))
+        return s.value, ms.value
 
     def map_sum(self):
         s = _u64()

@@ -159,6 +159,47 @@ __global__ void k_map_sum(const uint32_t *ids, uint64_t m, uint64_t *partial) {
   }
 }
 
+// Test_Map_Remote (benchmarks/Map/RemoteRead/Map.h:22-34): the sum over all
+// edges of remote[dst], remote holding n elements of W bytes (W = 0: one bit
+// per vertex, bit v at byte v/8, bit v%8).  Elements wider than 8 bytes are
+// read as the uint64 at their start, the value of the reference's
+// wide_int<size_t, W>; their stride still spreads the gathers over W*n
+// bytes.  Every thread gathers 8 elements at a time from 8 coalesced ids.
+template <int W>
+__device__ __forceinline__ uint64_t remote_at(const uint8_t *r, uint32_t d) {
+  if constexpr (W == 0) return (r[d >> 3] >> (d & 7)) & 1;
+  else if constexpr (W == 1) return r[d];
+  else if constexpr (W == 2) return reinterpret_cast<const uint16_t *>(r)[d];
+  else if constexpr (W == 4) return reinterpret_cast<const uint32_t *>(r)[d];
+  else return *reinterpret_cast<const uint64_t *>(r + (uint64_t)d * W);
+}
+
+template <int W>
+__global__ void k_map_remote(const uint32_t *ids, uint64_t m, const uint8_t *remote, uint64_t *partial) {
+  __shared__ uint64_t lds[16];
+  uint64_t acc = 0;
+  const uint64_t n8 = m / 8;
+  const uint4 *v4 = reinterpret_cast<const uint4 *>(ids);
+  for (uint64_t i = global_tid(); i < n8; i += grid_threads()) {
+    const uint4 a = v4[2 * i], b = v4[2 * i + 1];
+    const uint32_t d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint64_t x[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) x[j] = remote_at<W>(remote, d[j]);
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc += x[j];
+  }
+  for (uint64_t i = n8 * 8 + global_tid(); i < m; i += grid_threads()) acc += remote_at<W>(remote, ids[i]);
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t s = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += lds[w];
+    partial[blockIdx.x] = s;
+  }
+}
+
 __global__ void k_sum_partials(const uint64_t *partial, int nb, uint64_t *out) {
   __shared__ uint64_t lds[16];
   uint64_t acc = 0;
@@ -550,6 +591,43 @@ int byo_map_sum(byo_graph_t g, uint64_t *sum, double *device_ms) {
     if (device_ms) *device_ms = ms;
     BYO_HIP(hipMemcpyAsync(sum, part.as<uint64_t>() + nb, sizeof(uint64_t), hipMemcpyDeviceToHost,
                            g->stream));
+    BYO_HIP(hipStreamSynchronize(g->stream));
+  });
+}
+
+int byo_map_remote(byo_graph_t g, const void *remote, int read_size, uint64_t *sum, double *device_ms) {
+  return guard([&] {
+    BYO_REQUIRE(g && remote && sum, BYO_E_INVAL, "null argument");
+    BYO_REQUIRE(read_size == 0 || read_size == 1 || read_size == 2 || read_size == 4 || read_size == 8 ||
+                    read_size == 16 || read_size == 32 || read_size == 64,
+                BYO_E_INVAL, "read_size must be 0, 1, 2, 4, 8, 16, 32 or 64");
+    std::lock_guard<std::mutex> lk(g->mu);
+    bind(g);
+    ensure_csr(g);
+    const uint64_t bytes = read_size == 0 ? (g->n + 7) / 8 : g->n * (uint64_t)read_size;
+    uint8_t *dr = g->buf("map_remote", bytes + 64).as<uint8_t>();
+    BYO_HIP(hipMemcpyAsync(dr, remote, bytes, hipMemcpyHostToDevice, g->stream));
+    const int nb = 4096;
+    auto &part = g->buf("map_partial", (nb + 1) * sizeof(uint64_t));
+    const uint32_t *ids = g->ids.as<uint32_t>();
+    uint64_t *pp = part.as<uint64_t>();
+    PhaseTimer t(g->stream, true);
+    switch (read_size) {
+      case 0: k_map_remote<0><<<nb, 256, 0, g->stream>>>(ids, g->m, dr, pp); break;
+      case 1: k_map_remote<1><<<nb, 256, 0, g->stream>>>(ids, g->m, dr, pp); break;
+      case 2: k_map_remote<2><<<nb, 256, 0, g->stream>>>(ids, g->m, dr, pp); break;
+      case 4: k_map_remote<4><<<nb, 256, 0, g->stream>>>(ids, g->m, dr, pp); break;
+      case 8: k_map_remote<8><<<nb, 256, 0, g->stream>>>(ids, g->m, dr, pp); break;
+      case 16: k_map_remote<16><<<nb, 256, 0, g->stream>>>(ids, g->m, dr, pp); break;
+      case 32: k_map_remote<32><<<nb, 256, 0, g->stream>>>(ids, g->m, dr, pp); break;
+      default: k_map_remote<64><<<nb, 256, 0, g->stream>>>(ids, g->m, dr, pp); break;
+    }
+    k_sum_partials<<<1, 1024, 0, g->stream>>>(pp, nb, pp + nb);
+    BYO_LAUNCH_CHECK();
+    const double ms = t.stop();
+    if (g->prof.on) g->account("map_remote", ms);
+    if (device_ms) *device_ms = ms;
+    BYO_HIP(hipMemcpyAsync(sum, pp + nb, sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream));
     BYO_HIP(hipStreamSynchronize(g->stream));
   });
 }

@@ -260,6 +260,15 @@ int byo_tc(byo_graph_t g, uint64_t *count, double *device_ms);
 
 /* Test_Map (benchmarks/Map/Base/Map.h:8-18): sum of dst over all edges. */
 int byo_map_sum(byo_graph_t g, uint64_t *sum, double *device_ms);
+/* Test_Map_Remote (benchmarks/Map/RemoteRead/Map.h:22-34, run as
+ * Map_withRemote_<read_size> by run_unweighted.h:50-125): the sum over all
+ * edges of remote[dst].  remote is a host array of n elements of read_size
+ * bytes (0, 1, 2, 4, 8, 16, 32 or 64; 0 = one bit per vertex, bit v at byte
+ * v/8, bit v%8; elements wider than 8 bytes count the little-endian uint64
+ * at their start, as wide_int<size_t, read_size> does).  It is copied to
+ * the device first; device_ms covers the edge scan only. */
+int byo_map_remote(byo_graph_t g, const void *remote, int read_size, uint64_t *sum,
+                   double *device_ms);
 
 /* ---------------------------------------------------------------------- */
 /* Profiling: HIP events around each launch of the named device phases.    */

@@ -234,6 +234,26 @@ def map_sum(g):
     return int(lib().orc_map_sum(g.m, g.ids))
 
 
+def remote_values(remote, read_size, n):
+    """Element values of a Map_withRemote array: n elements of read_size
+    bytes (0: bits, little bit order; wider than 8: the uint64 at each
+    element's start, the value of wide_int<size_t, W>)."""
+    r = np.ascontiguousarray(remote).view(np.uint8).reshape(-1)
+    if read_size == 0:
+        return np.unpackbits(r, bitorder="little")[:n].astype(np.uint64)
+    if read_size <= 8:
+        dt = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[read_size]
+        return r[:n * read_size].view(dt).astype(np.uint64)
+    return r[:n * read_size].view(np.uint64).reshape(n, read_size // 8)[:, 0].copy()
+
+
+def map_remote(g, remote, read_size):
+    """Test_Map_Remote (benchmarks/Map/RemoteRead/Map.h:22-34): sum over all
+    edges of remote[dst], modulo 2^64 like a size_t."""
+    vals = remote_values(remote, read_size, g.n)
+    return int(np.sum(vals[g.ids.astype(np.int64)], dtype=np.uint64)) if g.m else 0
+
+
 def mlp_forward_backward(params, x, labels):
     params = np.ascontiguousarray(params, np.float32)
     x = np.ascontiguousarray(x, np.float32)

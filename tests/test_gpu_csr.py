@@ -266,6 +266,32 @@ def test_map_sum(B, s16, golden):
     assert G.map_sum()[0] == O.map_sum(g) == golden["rmat_s16"]["map_sum"]
 
 
+@pytest.mark.parametrize("rs", [0, 1, 2, 4, 8, 16, 32, 64])
+def test_map_remote(B, s16, rs):
+    """Map_withRemote_<rs>: random element values (full width, so 64-bit
+    sums wrap as size_t does), the reference runner's one-hot input, and
+    noise in the padding of wide elements, against the oracle."""
+    g, G = s16
+    n = g.n
+    rng = np.random.default_rng(rs)
+    if rs == 0:
+        arr = rng.integers(0, 256, (n + 7) // 8, dtype=np.uint8)
+    elif rs <= 8:
+        arr = rng.integers(0, 2 ** 63, n, dtype=np.uint64).view(np.uint8)[:n * rs].copy()
+    else:
+        arr = rng.integers(0, 2 ** 63, (n, rs // 8), dtype=np.uint64)
+    s, ms = G.map_remote(arr, rs)
+    assert s == O.map_remote(g, arr, rs) and ms > 0
+    deg = g.degrees()
+    v = int(np.argmax(deg))
+    one = np.zeros(max(n * rs, (n + 7) // 8), np.uint8)
+    if rs == 0:
+        one[v >> 3] = 1 << (v & 7)
+    else:
+        one[v * rs] = 1
+    assert G.map_remote(one, rs)[0] == int(deg[v])
+
+
 def test_errors_are_reported(B, s16):
     g, G = s16
     with pytest.raises(B.ByoError) as e:

@@ -367,3 +367,32 @@ def test_mlp_gradients_finite_difference():
         lm, _, _ = O.mlp_forward_backward(Pm.astype(np.float32), x, y)
         fd = (lp - lm) / (float(np.float32(Pp[k])) - float(np.float32(Pm[k])))
         assert abs(fd - g[k]) < 2e-3 + 2e-2 * abs(g[k]), (k, fd, g[k])
+
+
+def _remote_array(vals, rs, n, pad=0):
+    """n values as a Map_withRemote array of read_size rs (pad fills the
+    bytes of wide elements after their value)."""
+    if rs == 0:
+        return np.packbits((vals & 1).astype(np.uint8), bitorder="little")
+    if rs <= 8:
+        return vals.astype({1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[rs])
+    a = np.full((n, rs // 8), pad, np.uint64)
+    a[:, 0] = vals
+    return a
+
+
+@pytest.mark.parametrize("rs", [0, 1, 2, 4, 8, 16, 32, 64])
+def test_map_remote_closed_forms(rs):
+    """Map_Remote_runner's input -- one vertex set to 1 -- sums to that
+    vertex's in-degree (its degree, the graph being symmetric); all ones sum
+    to m; value v at vertex v sums to Test_Map's dst sum."""
+    g = O.rmat_graph(10)
+    n, deg = g.n, g.degrees()
+    for v in (0, int(np.argmax(deg)), n - 1):
+        one = np.zeros(n, np.uint64)
+        one[v] = 1
+        assert O.map_remote(g, _remote_array(one, rs, n, pad=7), rs) == int(deg[v])
+    assert O.map_remote(g, _remote_array(np.ones(n, np.uint64), rs, n, pad=3), rs) == g.m
+    if rs >= 2:
+        ident = np.arange(n, dtype=np.uint64)
+        assert O.map_remote(g, _remote_array(ident, rs, n, pad=1), rs) == O.map_sum(g)
