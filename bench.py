@@ -270,6 +270,9 @@ def main():
         lab, cc_ms = G.cc()
         ex["cc_ms"] = round(cc_ms, 3)
         ex["cc_components"] = int(len(np.unique(lab)))
+        G.bc(srcs[0])
+        _, bc_ms = G.bc(srcs[0])
+        ex["bc_ms"] = round(bc_ms, 3)  # single-source betweenness (BFS + both passes)
         tri, tc_ms = G.tc()
         ex["tc_ms"] = round(tc_ms, 3)
         ex["triangles"] = int(tri)

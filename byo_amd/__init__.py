@@ -95,6 +95,7 @@ SIGNATURES = {
     "byo_cc": (_i32, [_vp, _vp, _P(_dbl)]),
     "byo_cc_labelprop": (_i32, [_vp, _vp, _P(_u32), _P(_dbl)]),
     "byo_tc": (_i32, [_vp, _P(_u64), _P(_dbl)]),
+    "byo_bc": (_i32, [_vp, _u32, _vp, _P(_dbl)]),
     "byo_map_sum": (_i32, [_vp, _P(_u64), _P(_dbl)]),
     "byo_map_remote": (_i32, [_vp, _vp, _i32, _P(_u64), _P(_dbl)]),
     "byo_profile_enable": (_i32, [_vp, _i32]),
@@ -416,6 +417,15 @@ class Graph:
         ms = _dbl()
         check(lib().byo_tc(self._h, C.byref(c), C.byref(ms)))
         return c.value, ms.value
+
+    def bc(self, src):
+        """Single-source betweenness dependencies (SSBetweennessCentrality_BFS);
+        returns (float64[n], device ms), NaN where src does not reach."""
+        n = self.N()
+        out = np.empty(max(n, 1), np.float64)
+        ms = _dbl()
+        check(lib().byo_bc(self._h, int(src), _ptr(out), C.byref(ms)))
+        return out[:n], ms.value
 
     def map_remote(self, remote, read_size):
         """Map_withRemote_<read_size>: sum over edges of remote[dst]; remote

@@ -262,6 +262,31 @@ __global__ void k_lv_jump(const uint32_t *anc, const uint32_t *dep, uint32_t *an
 
 }  // namespace
 
+// Per-vertex levels of the tree in "bfs_parents" (the last BFS, rooted at
+// g->last_src) by pointer jumping: every vertex doubles the distance to its
+// known ancestor each pass, so ceil(log2 n) + 1 passes reach the root.
+const uint32_t *bfs_levels_dev(byo_graph *g) {
+  const uint32_t src = (uint32_t)g->last_src;
+  const uint64_t n = g->n;
+  uint32_t *parents = g->scratch["bfs_parents"].as<uint32_t>();
+  uint32_t *anc = g->buf("lv_anc", n * 4 + 64).as<uint32_t>();
+  uint32_t *dep = g->buf("lv_dep", n * 4 + 64).as<uint32_t>();
+  uint32_t *anc2 = g->buf("lv_anc2", n * 4 + 64).as<uint32_t>();
+  uint32_t *dep2 = g->buf("lv_dep2", n * 4 + 64).as<uint32_t>();
+  hipStream_t s = g->stream;
+  unsigned grid = byo_grid(n, 256, 16384);
+  k_lv_init<<<grid, 256, 0, s>>>(parents, n, src, anc, dep);
+  int jumps = 1;
+  while ((1ull << jumps) < n + 1) jumps++;
+  for (int j = 0; j < jumps + 1; j++) {
+    k_lv_jump<<<grid, 256, 0, s>>>(anc, dep, anc2, dep2, n, src);
+    std::swap(anc, anc2);
+    std::swap(dep, dep2);
+  }
+  BYO_LAUNCH_CHECK();
+  return dep;
+}
+
 // Host-side state of one BFS on a handle.
 struct BfsRun {
   byo_graph *g;
@@ -562,27 +587,9 @@ int byo_bfs_levels(byo_graph_t g, uint32_t *levels) {
     std::lock_guard<std::mutex> lk(g->mu);
     bind(g);
     BYO_REQUIRE(g->last_src >= 0, BYO_E_STATE, "no BFS has run on this handle");
-    const uint32_t src = (uint32_t)g->last_src;
-    const uint64_t n = g->n;
-    uint32_t *parents = g->scratch["bfs_parents"].as<uint32_t>();
-    uint32_t *anc = g->buf("lv_anc", n * 4 + 64).as<uint32_t>();
-    uint32_t *dep = g->buf("lv_dep", n * 4 + 64).as<uint32_t>();
-    uint32_t *anc2 = g->buf("lv_anc2", n * 4 + 64).as<uint32_t>();
-    uint32_t *dep2 = g->buf("lv_dep2", n * 4 + 64).as<uint32_t>();
-    hipStream_t s = g->stream;
-    unsigned grid = byo_grid(n, 256, 16384);
-    k_lv_init<<<grid, 256, 0, s>>>(parents, n, src, anc, dep);
-    // depth <= n, so ceil(log2 n) + 1 jumps always reach the root
-    int jumps = 1;
-    while ((1ull << jumps) < n + 1) jumps++;
-    for (int j = 0; j < jumps + 1; j++) {
-      k_lv_jump<<<grid, 256, 0, s>>>(anc, dep, anc2, dep2, n, src);
-      std::swap(anc, anc2);
-      std::swap(dep, dep2);
-    }
-    BYO_LAUNCH_CHECK();
-    BYO_HIP(hipMemcpyAsync(levels, dep, n * 4, hipMemcpyDeviceToHost, s));
-    BYO_HIP(hipStreamSynchronize(s));
+    const uint32_t *dep = bfs_levels_dev(g);
+    BYO_HIP(hipMemcpyAsync(levels, dep, g->n * 4, hipMemcpyDeviceToHost, g->stream));
+    BYO_HIP(hipStreamSynchronize(g->stream));
   });
 }
 

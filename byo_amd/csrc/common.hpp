@@ -236,6 +236,9 @@ byo_graph *new_handle(int kind);
 void *pinned(byo_graph *g, size_t bytes);
 // Maximum degree of the current CSR view (cached until it is rebuilt).
 uint64_t graph_max_degree(byo_graph *g);
+// Device levels (n entries, kNone when unreached) of the last BFS tree left
+// in "bfs_parents" (bfs.hip); valid until the next call.
+const uint32_t *bfs_levels_dev(byo_graph *g);
 // Copy of the current CSR view with vertices renumbered by descending
 // degree (relabel.hip, used by the PageRank pull): perm[old] = new.  Built
 // on the first call after each change of the graph; the CSR view must be

@@ -258,6 +258,14 @@ int byo_cc_labelprop(byo_graph_t g, uint32_t *labels, uint32_t *rounds,
  * intersections.  CSR only (the reference never runs it on the CPMA). */
 int byo_tc(byo_graph_t g, uint64_t *count, double *device_ms);
 
+/* Single-source betweenness centrality, bc_bfs::SSBetweennessCentrality_BFS
+ * (benchmarks/SSBetweenessCentrality/Brandes/SSBetweennessCentrality.h:
+ * 332-454), the variant run_all runs: out[v] = the dependency of src on v
+ * in FP64 (sigma(v) * Dep(v) - 1; for src itself the vertices reached - 1),
+ * NaN for vertices src does not reach, as in the reference.  device_ms
+ * covers the BFS, both passes and the final formula. */
+int byo_bc(byo_graph_t g, uint32_t src, double *out, double *device_ms);
+
 /* Test_Map (benchmarks/Map/Base/Map.h:8-18): sum of dst over all edges. */
 int byo_map_sum(byo_graph_t g, uint64_t *sum, double *device_ms);
 /* Test_Map_Remote (benchmarks/Map/RemoteRead/Map.h:22-34, run as

@@ -66,6 +66,7 @@ def lib():
             "orc_tc_brute": (u64, [u64, _u64p, _u32p]),
             "orc_orient": (None, [u64, _u64p, _u32p, _u64p, _u32p]),
             "orc_map_sum": (u64, [u64, _u32p]),
+            "orc_bc": (None, [u64, _u64p, _u32p, u32, _f64p]),
             "orc_mlp_forward_backward": (dbl, [_f32p, _f32p, _i32p, i32, _f32p, C.c_void_p]),
         }
         for name, (res, args) in sig.items():
@@ -232,6 +233,58 @@ def orient(g):
 
 def map_sum(g):
     return int(lib().orc_map_sum(g.m, g.ids))
+
+
+def bc(g, src):
+    """Single-source betweenness dependencies, bc_bfs::
+    SSBetweennessCentrality_BFS (SSBetweennessCentrality.h:332-454), FP64;
+    NaN at unreached vertices."""
+    out = np.empty(max(g.n, 1), np.float64)
+    lib().orc_bc(g.n, g.offsets, g.ids, src, out)
+    return out[:g.n]
+
+
+def bc_brute(g, src):
+    """Textbook single-source dependency by brute force (small graphs):
+    delta(v) = sum over t != src, v of sigma(src, v) sigma(v, t) / sigma(src, t)
+    for v on a shortest src-t path; the source's own value, which the
+    level recursion gives as reached - 1, is set to that."""
+    n = g.n
+    INF = -1
+
+    def bfs_counts(s):
+        dist = np.full(n, INF, np.int64)
+        sig = np.zeros(n, np.float64)
+        dist[s], sig[s] = 0, 1.0
+        q = [s]
+        for u in q:
+            for v in g.ids[g.offsets[u]:g.offsets[u + 1]]:
+                v = int(v)
+                if dist[v] == INF:
+                    dist[v] = dist[u] + 1
+                    q.append(v)
+                if dist[v] == dist[u] + 1:
+                    sig[v] += sig[u]
+        return dist, sig
+
+    ds, ss = bfs_counts(src)
+    out = np.full(n, np.nan)
+    reached = ds != INF
+    for v in range(n):
+        if not reached[v]:
+            continue
+        if v == src:
+            out[v] = float(reached.sum() - 1)
+            continue
+        dv, sv = bfs_counts(v)
+        acc = 0.0
+        for t in range(n):
+            if t in (src, v) or not reached[t]:
+                continue
+            if dv[t] != INF and ds[v] + dv[t] == ds[t]:
+                acc += ss[v] * sv[t] / ss[t]
+        out[v] = acc
+    return out
 
 
 def remote_values(remote, read_size, n):

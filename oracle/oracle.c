@@ -20,6 +20,9 @@
  *   orc_tc            : orc_tc_brute (independent triple enumeration) and
  *       inputs/ facts (107, 2, 0 triangles).
  *   orc_mlp_*         : central finite differences of the loss.
+ *   orc_bc            : brute-force dependencies from all-pairs shortest-path
+ *       counts (textbook definition) on small graphs; star and path closed
+ *       forms; the source's own value = reached - 1.
  */
 #include "oracle.h"
 
@@ -676,6 +679,62 @@ uint64_t orc_tc_brute(uint64_t n, const uint64_t *off, const uint32_t *ids) {
 /* ===================================================================== */
 /* Map                                                                   */
 /* ===================================================================== */
+
+/* bc_bfs::SSBetweennessCentrality_BFS (benchmarks/SSBetweenessCentrality/
+ * Brandes/SSBetweennessCentrality.h:332-454), level-synchronous, FP64:
+ * forward, NumPaths[u] = sum of NumPaths over neighbours in earlier levels
+ * (Visited == 2); backward from the deepest level, Dep[u] = sum of Dep over
+ * neighbours in deeper levels, then Dep[u] += 1/NumPaths[u]; result
+ * (Dep - 1/NumPaths) / (1/NumPaths) -- NaN for unreached vertices, exactly
+ * as the reference's arithmetic gives.  Sums run in adjacency order. */
+void orc_bc(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t src, double *out) {
+  uint32_t *lv = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+  double *np = (double *)calloc(n ? n : 1, sizeof(double));
+  double *dep = (double *)calloc(n ? n : 1, sizeof(double));
+  uint32_t *order = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+  orc_bfs_levels(n, off, ids, src, lv);
+  /* vertices in BFS order (levels ascending) */
+  uint64_t cnt = 0;
+  if (n) {
+    uint64_t head = 0;
+    order[cnt++] = src;
+    uint8_t *seen = (uint8_t *)calloc(n, 1);
+    seen[src] = 1;
+    while (head < cnt) {
+      const uint32_t u = order[head++];
+      for (uint64_t e = off[u]; e < off[u + 1]; e++)
+        if (!seen[ids[e]]) {
+          seen[ids[e]] = 1;
+          order[cnt++] = ids[e];
+        }
+    }
+    free(seen);
+    np[src] = 1.0;
+  }
+  for (uint64_t i = 1; i < cnt; i++) {
+    const uint32_t u = order[i];
+    double s = 0.0;
+    for (uint64_t e = off[u]; e < off[u + 1]; e++)
+      if (lv[ids[e]] < lv[u]) s += np[ids[e]];
+    np[u] = s;
+  }
+  for (uint64_t i = 0; i < n; i++) np[i] = 1.0 / np[i];
+  /* deepest level first; within a level any order (sums read deeper only) */
+  for (uint64_t i = cnt; i-- > 0;) {
+    const uint32_t u = order[i];
+    double s = 0.0;
+    for (uint64_t e = off[u]; e < off[u + 1]; e++) {
+      const uint32_t d = lv[ids[e]];
+      if (d != ORC_NONE && d > lv[u]) s += dep[ids[e]];
+    }
+    dep[u] = s + np[u];
+  }
+  for (uint64_t i = 0; i < n; i++) out[i] = (dep[i] - np[i]) / np[i];
+  free(lv);
+  free(np);
+  free(dep);
+  free(order);
+}
 
 uint64_t orc_map_sum(uint64_t m, const uint32_t *ids) {
   uint64_t s = 0;

@@ -1,6 +1,8 @@
 """Pins the CPU oracle against the reference's own tests and fixtures and
 against independent mathematics (brute force, closed forms, scipy).  No GPU.
 """
+import os
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -396,3 +398,40 @@ def test_map_remote_closed_forms(rs):
     if rs >= 2:
         ident = np.arange(n, dtype=np.uint64)
         assert O.map_remote(g, _remote_array(ident, rs, n, pad=1), rs) == O.map_sum(g)
+
+
+def _bc_close(a, b):
+    nan = np.isnan(a)
+    assert np.array_equal(nan, np.isnan(b))
+    assert np.allclose(a[~nan], b[~nan], rtol=1e-9, atol=1e-9)
+
+
+def test_bc_closed_forms():
+    """Star: from a leaf, the centre carries every other leaf (n - 2), the
+    leaves nothing; from the centre, the centre = reached - 1.  Path 0-1-..-k
+    from 0: vertex i carries the k - i vertices beyond it.  Unreached: NaN."""
+    k = 9
+    star = O.sym_csr(k + 1, np.zeros(k, np.uint32), np.arange(1, k + 1, dtype=np.uint32))
+    b = O.bc(star, 3)
+    assert b[0] == k - 1 and b[3] == k and all(b[i] == 0 for i in range(1, k + 1) if i != 3)
+    b = O.bc(star, 0)
+    assert b[0] == k and all(b[1:] == 0)
+    path = O.sym_csr(k + 2, np.arange(k, dtype=np.uint32), np.arange(1, k + 1, dtype=np.uint32))
+    b = O.bc(path, 0)
+    assert list(b[:k + 1]) == [float(k)] + [float(k - i) for i in range(1, k + 1)]
+    assert np.isnan(b[k + 1])  # the isolated vertex
+
+
+@pytest.mark.parametrize("which", ["rmat", "grid", "er", "triangles"])
+def test_bc_matches_brute_force(which):
+    if which == "rmat":
+        g = O.rmat_graph(7, 4)
+    elif which == "grid":
+        g = O.grid3d_graph(4)
+    elif which == "er":
+        g = O.er_graph(6, 90, seed=3)
+    else:
+        g = O.read_adjacency_graph(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "inputs",
+                                                "triangles.txt"))
+    for src in (0, g.n // 2):
+        _bc_close(O.bc(g, src), O.bc_brute(g, src))
