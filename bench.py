@@ -273,6 +273,11 @@ def main():
         G.bc(srcs[0])
         _, bc_ms = G.bc(srcs[0])
         ex["bc_ms"] = round(bc_ms, 3)  # single-source betweenness (BFS + both passes)
+        G.kcore()
+        _, kc_kmax, kc_rounds, kc_ms = G.kcore()
+        ex["kcore_ms"] = round(kc_ms, 3)
+        ex["kcore_kmax"] = int(kc_kmax)
+        ex["kcore_rounds"] = int(kc_rounds)
         tri, tc_ms = G.tc()
         ex["tc_ms"] = round(tc_ms, 3)
         ex["triangles"] = int(tri)

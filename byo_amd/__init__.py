@@ -96,6 +96,7 @@ SIGNATURES = {
     "byo_cc_labelprop": (_i32, [_vp, _vp, _P(_u32), _P(_dbl)]),
     "byo_tc": (_i32, [_vp, _P(_u64), _P(_dbl)]),
     "byo_bc": (_i32, [_vp, _u32, _vp, _P(_dbl)]),
+    "byo_kcore": (_i32, [_vp, _vp, _P(_u32), _P(_u64), _P(_dbl)]),
     "byo_map_sum": (_i32, [_vp, _P(_u64), _P(_dbl)]),
     "byo_map_remote": (_i32, [_vp, _vp, _i32, _P(_u64), _P(_dbl)]),
     "byo_profile_enable": (_i32, [_vp, _i32]),
@@ -417,6 +418,15 @@ class Graph:
         ms = _dbl()
         check(lib().byo_tc(self._h, C.byref(c), C.byref(ms)))
         return c.value, ms.value
+
+    def kcore(self):
+        """Coreness of every vertex (KCore); returns (uint32[n], k_max,
+        rounds, device ms)."""
+        n = self.N()
+        core = np.empty(max(n, 1), np.uint32)
+        kmax, rounds, ms = _u32(), _u64(), _dbl()
+        check(lib().byo_kcore(self._h, _ptr(core), C.byref(kmax), C.byref(rounds), C.byref(ms)))
+        return core[:n], kmax.value, rounds.value, ms.value
 
     def bc(self, src):
         """Single-source betweenness dependencies (SSBetweennessCentrality_BFS);

@@ -283,6 +283,15 @@ __device__ __forceinline__ T wave_max(T v) {
   }
   return v;
 }
+template <class T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    T w = __shfl_xor(v, o, 64);
+    v = w < v ? w : v;
+  }
+  return v;
+}
 // Inclusive prefix sum across the wave.
 template <class T>
 __device__ __forceinline__ T wave_incl_scan(T v) {

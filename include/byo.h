@@ -266,6 +266,13 @@ int byo_tc(byo_graph_t g, uint64_t *count, double *device_ms);
  * covers the BFS, both passes and the final formula. */
 int byo_bc(byo_graph_t g, uint32_t src, double *out, double *device_ms);
 
+/* Coreness of every vertex, KCore (benchmarks/KCore/JulienneDBS17/
+ * KCore.h:8-56, run_all's "KCore"): core[n]; *k_max the largest coreness
+ * (the reference prints it), *rounds the peeling rounds run (either may be
+ * null).  Symmetric graphs, as in the reference. */
+int byo_kcore(byo_graph_t g, uint32_t *core, uint32_t *k_max, uint64_t *rounds,
+              double *device_ms);
+
 /* Test_Map (benchmarks/Map/Base/Map.h:8-18): sum of dst over all edges. */
 int byo_map_sum(byo_graph_t g, uint64_t *sum, double *device_ms);
 /* Test_Map_Remote (benchmarks/Map/RemoteRead/Map.h:22-34, run as

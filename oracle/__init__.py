@@ -67,6 +67,7 @@ def lib():
             "orc_orient": (None, [u64, _u64p, _u32p, _u64p, _u32p]),
             "orc_map_sum": (u64, [u64, _u32p]),
             "orc_bc": (None, [u64, _u64p, _u32p, u32, _f64p]),
+            "orc_kcore": (None, [u64, _u64p, _u32p, _u32p]),
             "orc_mlp_forward_backward": (dbl, [_f32p, _f32p, _i32p, i32, _f32p, C.c_void_p]),
         }
         for name, (res, args) in sig.items():
@@ -233,6 +234,37 @@ def orient(g):
 
 def map_sum(g):
     return int(lib().orc_map_sum(g.m, g.ids))
+
+
+def kcore(g):
+    """Coreness of every vertex (KCore, KCore.h:8-56)."""
+    out = np.empty(max(g.n, 1), np.uint32)
+    lib().orc_kcore(g.n, g.offsets, g.ids, out)
+    return out[:g.n]
+
+
+def kcore_brute(g):
+    """Coreness by the definition: for k = 1, 2, ... peel vertices of degree
+    < k until none is left; a vertex's coreness is the last k whose core
+    still holds it (small graphs)."""
+    n = g.n
+    adj = [set(int(x) for x in g.ids[g.offsets[v]:g.offsets[v + 1]]) for v in range(n)]
+    core = np.zeros(n, np.uint32)
+    k = 1
+    while True:
+        alive = set(range(n))
+        changed = True
+        while changed:
+            changed = False
+            for v in list(alive):
+                if len(adj[v] & alive) < k:
+                    alive.discard(v)
+                    changed = True
+        if not alive:
+            return core
+        for v in alive:
+            core[v] = k
+        k += 1
 
 
 def bc(g, src):

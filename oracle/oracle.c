@@ -20,6 +20,8 @@
  *   orc_tc            : orc_tc_brute (independent triple enumeration) and
  *       inputs/ facts (107, 2, 0 triangles).
  *   orc_mlp_*         : central finite differences of the loss.
+ *   orc_kcore         : test_kcore.cc vectors; brute force of the definition
+ *       (largest k whose k-core, by repeated removal, holds the vertex).
  *   orc_bc            : brute-force dependencies from all-pairs shortest-path
  *       counts (textbook definition) on small graphs; star and path closed
  *       forms; the source's own value = reached - 1.
@@ -734,6 +736,62 @@ void orc_bc(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t src, 
   free(np);
   free(dep);
   free(order);
+}
+
+/* KCore (benchmarks/KCore/JulienneDBS17/KCore.h:8-56) returns every
+ * vertex's coreness: the bucket k at which peeling removes it, degrees
+ * floored at the current k.  Restated as the sequential bucket peel of
+ * Batagelj and Zaversnik (vertices in order of current degree, each
+ * removal decrementing its higher-degree neighbours). */
+void orc_kcore(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t *core) {
+  if (n == 0) return;
+  uint32_t md = 0;
+  uint32_t *deg = (uint32_t *)malloc(n * sizeof(uint32_t));
+  for (uint64_t v = 0; v < n; v++) {
+    deg[v] = (uint32_t)(off[v + 1] - off[v]);
+    if (deg[v] > md) md = deg[v];
+  }
+  uint64_t *bin = (uint64_t *)calloc((size_t)md + 2, sizeof(uint64_t));
+  uint32_t *vert = (uint32_t *)malloc(n * sizeof(uint32_t));
+  uint64_t *pos = (uint64_t *)malloc(n * sizeof(uint64_t));
+  for (uint64_t v = 0; v < n; v++) bin[deg[v]]++;
+  uint64_t start = 0;
+  for (uint32_t d = 0; d <= md; d++) {
+    const uint64_t c = bin[d];
+    bin[d] = start;
+    start += c;
+  }
+  for (uint64_t v = 0; v < n; v++) {
+    pos[v] = bin[deg[v]];
+    vert[pos[v]] = (uint32_t)v;
+    bin[deg[v]]++;
+  }
+  for (uint32_t d = md; d >= 1; d--) bin[d] = bin[d - 1];
+  bin[0] = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    const uint32_t v = vert[i];
+    for (uint64_t e = off[v]; e < off[v + 1]; e++) {
+      const uint32_t u = ids[e];
+      if (deg[u] > deg[v]) {
+        const uint32_t du = deg[u];
+        const uint64_t pu = pos[u], pw = bin[du];
+        const uint32_t w = vert[pw];
+        if (u != w) {
+          pos[u] = pw;
+          vert[pu] = w;
+          pos[w] = pu;
+          vert[pw] = u;
+        }
+        bin[du]++;
+        deg[u]--;
+      }
+    }
+  }
+  for (uint64_t v = 0; v < n; v++) core[v] = deg[v];
+  free(deg);
+  free(bin);
+  free(vert);
+  free(pos);
 }
 
 uint64_t orc_map_sum(uint64_t m, const uint32_t *ids) {

@@ -122,6 +122,7 @@ void orc_orient(uint64_t n, const uint64_t *off, const uint32_t *ids,
 /* ---- Map ------------------------------------------------------------- */
 /* Test_Map (Map/Base/Map.h:8-18): sum of dst over all CSR entries. */
 void orc_bc(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t src, double *out);
+void orc_kcore(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t *core);
 uint64_t orc_map_sum(uint64_t m, const uint32_t *ids);
 
 /* ---- Learned selector: MLP 6->32->32->3 reference math --------------- */

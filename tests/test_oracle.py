@@ -435,3 +435,25 @@ def test_bc_matches_brute_force(which):
                                                 "triangles.txt"))
     for src in (0, g.n // 2):
         _bc_close(O.bc(g, src), O.bc_brute(g, src))
+
+
+def test_kcore_reference_unit_cases():
+    """test_kcore.cc: the edgeless graph and the 8-vertex BasicUsage graph."""
+    assert list(O.kcore(O.sym_csr(3, np.zeros(0, np.uint32), np.zeros(0, np.uint32)))) == [0, 0, 0]
+    e = np.array([[0, 1], [2, 3], [3, 4], [3, 5], [4, 5], [5, 6]], np.uint32)
+    g = O.sym_csr(8, e[:, 0], e[:, 1])
+    assert list(O.kcore(g)) == [1, 1, 1, 2, 2, 2, 1, 0]
+
+
+@pytest.mark.parametrize("which", ["rmat", "grid", "er", "inputs"])
+def test_kcore_matches_definition(which):
+    if which == "rmat":
+        g = O.rmat_graph(8, 8)
+    elif which == "grid":
+        g = O.grid3d_graph(5)
+    elif which == "er":
+        g = O.er_graph(7, 400, seed=5)
+    else:
+        g = O.read_adjacency_graph(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "inputs",
+                                                "rMatGraph_J_5_100"))
+    assert np.array_equal(O.kcore(g), O.kcore_brute(g))
