@@ -273,6 +273,10 @@ def main():
         G.bc(srcs[0])
         _, bc_ms = G.bc(srcs[0])
         ex["bc_ms"] = round(bc_ms, 3)  # single-source betweenness (BFS + both passes)
+        G.pagerank_delta(want_p=False)
+        _, pd_rounds, pd_ms = G.pagerank_delta(want_p=False)  # run_all's -delta settings
+        ex["pr_delta_ms"] = round(pd_ms, 3)
+        ex["pr_delta_rounds"] = int(pd_rounds)
         G.kcore()
         _, kc_kmax, kc_rounds, kc_ms = G.kcore()
         ex["kcore_ms"] = round(kc_ms, 3)

@@ -97,6 +97,7 @@ SIGNATURES = {
     "byo_tc": (_i32, [_vp, _P(_u64), _P(_dbl)]),
     "byo_bc": (_i32, [_vp, _u32, _vp, _P(_dbl)]),
     "byo_kcore": (_i32, [_vp, _vp, _P(_u32), _P(_u64), _P(_dbl)]),
+    "byo_pagerank_delta": (_i32, [_vp, _dbl, _dbl, _i32, _vp, _P(_i32), _P(_dbl)]),
     "byo_map_sum": (_i32, [_vp, _P(_u64), _P(_dbl)]),
     "byo_map_remote": (_i32, [_vp, _vp, _i32, _P(_u64), _P(_dbl)]),
     "byo_profile_enable": (_i32, [_vp, _i32]),
@@ -418,6 +419,14 @@ class Graph:
         ms = _dbl()
         check(lib().byo_tc(self._h, C.byref(c), C.byref(ms)))
         return c.value, ms.value
+
+    def pagerank_delta(self, eps=1e-4, local_eps=0.01, max_iters=20, want_p=True):
+        """PageRankDelta (PageRank.h:335-396); returns (p, rounds, device ms)."""
+        n = self.N()
+        p = np.empty(max(n, 1), np.float64) if want_p else None
+        r, ms = _i32(), _dbl()
+        check(lib().byo_pagerank_delta(self._h, eps, local_eps, int(max_iters), _ptr(p), C.byref(r), C.byref(ms)))
+        return (p[:n] if want_p else None), r.value, ms.value
 
     def kcore(self):
         """Coreness of every vertex (KCore); returns (uint32[n], k_max,

@@ -266,6 +266,15 @@ int byo_tc(byo_graph_t g, uint64_t *count, double *device_ms);
  * covers the BFS, both passes and the final formula. */
 int byo_bc(byo_graph_t g, uint32_t src, double *out, double *device_ms);
 
+/* delta::PageRankDelta (benchmarks/PageRank/PageRank.h:335-396), run_all's
+ * PageRank with -delta: FP64, damping 0.85; stops when the sum of |delta|
+ * falls below eps or after max_iters rounds (*rounds_run).  p[n] may be
+ * null.  The reference's semantics are kept, including the first round's
+ * signed test (only positive deltas stay active), so the result is not the
+ * PageRank vector of byo_pagerank. */
+int byo_pagerank_delta(byo_graph_t g, double eps, double local_eps, int max_iters,
+                       double *p, int *rounds_run, double *device_ms);
+
 /* Coreness of every vertex, KCore (benchmarks/KCore/JulienneDBS17/
  * KCore.h:8-56, run_all's "KCore"): core[n]; *k_max the largest coreness
  * (the reference prints it), *rounds the peeling rounds run (either may be

@@ -68,6 +68,7 @@ def lib():
             "orc_map_sum": (u64, [u64, _u32p]),
             "orc_bc": (None, [u64, _u64p, _u32p, u32, _f64p]),
             "orc_kcore": (None, [u64, _u64p, _u32p, _u32p]),
+            "orc_pagerank_delta": (i32, [u64, _u64p, _u32p, dbl, dbl, i32, _f64p]),
             "orc_mlp_forward_backward": (dbl, [_f32p, _f32p, _i32p, i32, _f32p, C.c_void_p]),
         }
         for name, (res, args) in sig.items():
@@ -234,6 +235,13 @@ def orient(g):
 
 def map_sum(g):
     return int(lib().orc_map_sum(g.m, g.ids))
+
+
+def pagerank_delta(g, eps=1e-4, local_eps=0.01, max_iters=20):
+    """delta::PageRankDelta (PageRank.h:335-396); returns (p, rounds)."""
+    p = np.empty(max(g.n, 1), np.float64)
+    r = lib().orc_pagerank_delta(g.n, g.offsets, g.ids, eps, local_eps, max_iters, p)
+    return p[:g.n], r
 
 
 def kcore(g):

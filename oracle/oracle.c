@@ -682,6 +682,65 @@ uint64_t orc_tc_brute(uint64_t n, const uint64_t *off, const uint32_t *ids) {
 /* Map                                                                   */
 /* ===================================================================== */
 
+/* delta::PageRankDelta (benchmarks/PageRank/PageRank.h:335-396), FP64.
+ * Every round both branches of sparse_or_dense (:230-269) give
+ * nghSum[v] = sum of Delta[s].delta_over_degree over v's neighbours s in
+ * the frontier (sums here in adjacency order); the first round's filter
+ * (:271-297) adds damping * nghSum + (1-d)/n to p and keeps
+ * delta = that - 1/n, active when delta > local_eps * p; later rounds
+ * (:305-327) take delta = damping * nghSum for every vertex and add it to p
+ * only where |delta| > local_eps * p, which is also the next frontier.
+ * Stops when sum |delta| < eps or after max_iters rounds; returns the
+ * rounds run.  Isolated vertices divide by degree 0, as the reference does
+ * (their quotient is never read). */
+int orc_pagerank_delta(uint64_t n, const uint64_t *off, const uint32_t *ids, double eps, double local_eps,
+                       int max_iters, double *p) {
+  const double damping = 0.85, one_over_n = 1.0 / (double)n;
+  double *dod = (double *)malloc((n ? n : 1) * sizeof(double));
+  double *ngh = (double *)calloc(n ? n : 1, sizeof(double));
+  uint8_t *front = (uint8_t *)malloc(n ? n : 1);
+  uint8_t *act = (uint8_t *)malloc(n ? n : 1);
+  for (uint64_t i = 0; i < n; i++) {
+    p[i] = 0.0;
+    dod[i] = one_over_n / (double)(off[i + 1] - off[i]);
+    front[i] = 1;
+  }
+  int round = 0, ran = 0;
+  while (round++ < max_iters) {
+    ran = round;
+    for (uint64_t v = 0; v < n; v++) {
+      double sum = 0.0;
+      for (uint64_t e = off[v]; e < off[v + 1]; e++)
+        if (front[ids[e]]) sum += dod[ids[e]];
+      ngh[v] = sum;
+    }
+    double l1 = 0.0;
+    for (uint64_t i = 0; i < n; i++) {
+      const double deg = (double)(off[i + 1] - off[i]);
+      double nd;
+      if (round == 1) {
+        const double pre = damping * ngh[i] + (1.0 - damping) * one_over_n;
+        p[i] += pre;
+        nd = pre - one_over_n;
+        act[i] = nd > local_eps * p[i];
+      } else {
+        nd = ngh[i] * damping;
+        act[i] = fabs(nd) > local_eps * p[i];
+        if (act[i]) p[i] += nd;
+      }
+      dod[i] = nd / deg;
+      l1 += fabs(nd);
+    }
+    if (l1 < eps) break;
+    memcpy(front, act, n);
+  }
+  free(dod);
+  free(ngh);
+  free(front);
+  free(act);
+  return ran;
+}
+
 /* bc_bfs::SSBetweennessCentrality_BFS (benchmarks/SSBetweenessCentrality/
  * Brandes/SSBetweennessCentrality.h:332-454), level-synchronous, FP64:
  * forward, NumPaths[u] = sum of NumPaths over neighbours in earlier levels

@@ -123,6 +123,8 @@ void orc_orient(uint64_t n, const uint64_t *off, const uint32_t *ids,
 /* Test_Map (Map/Base/Map.h:8-18): sum of dst over all CSR entries. */
 void orc_bc(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t src, double *out);
 void orc_kcore(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t *core);
+int orc_pagerank_delta(uint64_t n, const uint64_t *off, const uint32_t *ids, double eps, double local_eps,
+                       int max_iters, double *p);
 uint64_t orc_map_sum(uint64_t m, const uint32_t *ids);
 
 /* ---- Learned selector: MLP 6->32->32->3 reference math --------------- */

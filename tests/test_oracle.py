@@ -457,3 +457,28 @@ def test_kcore_matches_definition(which):
         g = O.read_adjacency_graph(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "inputs",
                                                 "rMatGraph_J_5_100"))
     assert np.array_equal(O.kcore(g), O.kcore_brute(g))
+
+
+def test_pagerank_delta_pins():
+    """PageRankDelta (PageRank.h:335-396).  One round is the first Jacobi
+    iterate; with every vertex active (local eps -inf) r rounds are r Jacobi
+    iterations (each round adds d * A * (p_k - p_{k-1})); and two rounds on
+    the path 0-1-2 worked by hand, where the first round's signed test
+    (delta > local_eps * p, :289) keeps only vertex 1 active."""
+    g = O.rmat_graph(11)
+    p, r = O.pagerank_delta(g, 0.0, 0.01, 1)
+    assert r == 1 and np.abs(p - O.pagerank(g, 0.85, 1)[0]).sum() < 1e-12
+    for iters in (2, 5, 20):
+        p, r = O.pagerank_delta(g, 0.0, -1e300, iters)
+        assert r == iters and np.abs(p - O.pagerank(g, 0.85, iters)[0]).sum() < 1e-10
+    path = O.sym_csr(3, np.array([0, 1], np.uint32), np.array([1, 2], np.uint32))
+    p, r = O.pagerank_delta(path, 0.0, 0.01, 2)
+    d, third = 0.85, 1.0 / 3.0
+    pre0 = d * (third / 2) + (1 - d) * third
+    pre1 = d * (2 * third) + (1 - d) * third
+    dod1 = (pre1 - third) / 2
+    assert r == 2
+    assert np.allclose(p, [pre0 + d * dod1, pre1, pre0 + d * dod1], rtol=0, atol=1e-15)
+    # stops once sum |delta| < eps
+    p, r = O.pagerank_delta(g, 1e-4, 0.01, 100)
+    assert 1 < r < 100
