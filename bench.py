@@ -277,6 +277,10 @@ def main():
         _, pd_rounds, pd_ms = G.pagerank_delta(want_p=False)  # run_all's -delta settings
         ex["pr_delta_ms"] = round(pd_ms, 3)
         ex["pr_delta_rounds"] = int(pd_rounds)
+        G.ldd(0.2)
+        ldd_cl, ldd_rounds, ldd_ms = G.ldd(0.2)
+        ex["ldd_ms"] = round(ldd_ms, 3)
+        ex["ldd_clusters"] = int(len(np.unique(ldd_cl)))
         G.kcore()
         _, kc_kmax, kc_rounds, kc_ms = G.kcore()
         ex["kcore_ms"] = round(kc_ms, 3)

@@ -97,6 +97,7 @@ SIGNATURES = {
     "byo_tc": (_i32, [_vp, _P(_u64), _P(_dbl)]),
     "byo_bc": (_i32, [_vp, _u32, _vp, _P(_dbl)]),
     "byo_kcore": (_i32, [_vp, _vp, _P(_u32), _P(_u64), _P(_dbl)]),
+    "byo_ldd": (_i32, [_vp, _dbl, _vp, _P(_u32), _P(_dbl)]),
     "byo_pagerank_delta": (_i32, [_vp, _dbl, _dbl, _i32, _vp, _P(_i32), _P(_dbl)]),
     "byo_map_sum": (_i32, [_vp, _P(_u64), _P(_dbl)]),
     "byo_map_remote": (_i32, [_vp, _vp, _i32, _P(_u64), _P(_dbl)]),
@@ -427,6 +428,15 @@ class Graph:
         r, ms = _i32(), _dbl()
         check(lib().byo_pagerank_delta(self._h, eps, local_eps, int(max_iters), _ptr(p), C.byref(r), C.byref(ms)))
         return (p[:n] if want_p else None), r.value, ms.value
+
+    def ldd(self, beta=0.2):
+        """Low-diameter decomposition (LDD, permute false); returns
+        (cluster centre per vertex, rounds, device ms)."""
+        n = self.N()
+        cl = np.empty(max(n, 1), np.uint32)
+        r, ms = _u32(), _dbl()
+        check(lib().byo_ldd(self._h, float(beta), _ptr(cl), C.byref(r), C.byref(ms)))
+        return cl[:n], r.value, ms.value
 
     def kcore(self):
         """Coreness of every vertex (KCore); returns (uint32[n], k_max,

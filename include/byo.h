@@ -275,6 +275,16 @@ int byo_bc(byo_graph_t g, uint32_t src, double *out, double *device_ms);
 int byo_pagerank_delta(byo_graph_t g, double eps, double local_eps, int max_iters,
                        double *p, int *rounds_run, double *device_ms);
 
+/* Low-diameter decomposition, LDD (benchmarks/LowDiameterDecomposition/
+ * MPX13/LowDiameterDecomposition.h:98-186) without permutation, as run_all
+ * runs it (beta 0.2): cluster[v] = the id of v's cluster centre.  Vertices
+ * become centres in id order at rounds set by generate_shifts (:12-25);
+ * a vertex reached from several clusters in one round joins the smallest
+ * cluster id (one of the outcomes of the reference's CAS race), so the
+ * result is deterministic.  *rounds_run may be null. */
+int byo_ldd(byo_graph_t g, double beta, uint32_t *cluster, uint32_t *rounds_run,
+            double *device_ms);
+
 /* Coreness of every vertex, KCore (benchmarks/KCore/JulienneDBS17/
  * KCore.h:8-56, run_all's "KCore"): core[n]; *k_max the largest coreness
  * (the reference prints it), *rounds the peeling rounds run (either may be

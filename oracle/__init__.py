@@ -69,6 +69,7 @@ def lib():
             "orc_bc": (None, [u64, _u64p, _u32p, u32, _f64p]),
             "orc_kcore": (None, [u64, _u64p, _u32p, _u32p]),
             "orc_pagerank_delta": (i32, [u64, _u64p, _u32p, dbl, dbl, i32, _f64p]),
+            "orc_ldd": (None, [u64, _u64p, _u32p, dbl, _u32p, _u32p]),
             "orc_mlp_forward_backward": (dbl, [_f32p, _f32p, _i32p, i32, _f32p, C.c_void_p]),
         }
         for name, (res, args) in sig.items():
@@ -235,6 +236,25 @@ def orient(g):
 
 def map_sum(g):
     return int(lib().orc_map_sum(g.m, g.ids))
+
+
+def ldd(g, beta=0.2):
+    """LDD without permutation (LowDiameterDecomposition.h:98-186), the
+    smallest cluster id winning ties; returns (cluster, round)."""
+    cl = np.empty(max(g.n, 1), np.uint32)
+    rd = np.empty(max(g.n, 1), np.uint32)
+    lib().orc_ldd(g.n, g.offsets, g.ids, beta, cl, rd)
+    return cl[:g.n], rd[:g.n]
+
+
+def ldd_start_round(n, beta=0.2):
+    """Round at which vertex v becomes eligible as a centre: the r with
+    shifts[r] <= v < shifts[r+1] (generate_shifts, :12-25)."""
+    import math
+    last = int(min(n + 1, 2 + math.ceil(math.log(n) / beta)))
+    sizes = [math.floor(math.exp(i * beta)) for i in range(last)]
+    bounds = np.cumsum([0] + sizes)
+    return np.searchsorted(bounds, np.arange(n), side="right") - 1
 
 
 def pagerank_delta(g, eps=1e-4, local_eps=0.01, max_iters=20):

@@ -682,6 +682,68 @@ uint64_t orc_tc_brute(uint64_t n, const uint64_t *off, const uint32_t *ids) {
 /* Map                                                                   */
 /* ===================================================================== */
 
+/* LDD with permute = false (LowDiameterDecomposition.h:98-186 and
+ * ldd_utils::generate_shifts :12-25; run_all's beta 0.2, ldd_permute
+ * false): round r first makes every still unclaimed vertex in
+ * [shifts[r], min(shifts[r+1], n)) a centre of its own cluster, then one
+ * BFS step from the frontier (last round's claims and the new centres)
+ * claims the unclaimed neighbours.  A vertex reached from several clusters
+ * in one step joins the smallest cluster id -- one of the outcomes the
+ * reference's CAS race allows (its pull would pick the first frontier
+ * neighbour in adjacency order).  cluster[v] = the centre's id; round[v] =
+ * the round whose frontier holds v. */
+void orc_ldd(uint64_t n, const uint64_t *off, const uint32_t *ids, double beta, uint32_t *cluster, uint32_t *round) {
+  if (n == 0) return;
+  const uint64_t last = (uint64_t)fmin((double)(n + 1), 2.0 + ceil(log((double)n) / beta));
+  uint64_t *shifts = (uint64_t *)calloc(last + 1, sizeof(uint64_t));
+  for (uint64_t i = 0; i < last; i++) shifts[i] = (uint64_t)floor(exp((double)i * beta));
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i <= last; i++) {
+    const uint64_t x = shifts[i];
+    shifts[i] = acc;
+    acc += x;
+  }
+  uint32_t *fr = (uint32_t *)malloc(n * sizeof(uint32_t));
+  uint32_t *nx = (uint32_t *)malloc(n * sizeof(uint32_t));
+  for (uint64_t v = 0; v < n; v++) {
+    cluster[v] = ORC_NONE;
+    round[v] = ORC_NONE;
+  }
+  uint64_t nf = 0, visited = 0;
+  for (uint64_t r = 0; visited < n; r++) {
+    const uint64_t start = shifts[r], end = shifts[r + 1] < n ? shifts[r + 1] : n;
+    for (uint64_t v = start; v < end; v++)
+      if (cluster[v] == ORC_NONE) {
+        cluster[v] = (uint32_t)v;
+        round[v] = (uint32_t)r;
+        fr[nf++] = (uint32_t)v;
+      }
+    visited += nf;
+    if (visited >= n) break;
+    uint64_t nn = 0;
+    for (uint64_t i = 0; i < nf; i++) {
+      const uint32_t u = fr[i];
+      for (uint64_t e = off[u]; e < off[u + 1]; e++) {
+        const uint32_t v = ids[e];
+        if (round[v] == ORC_NONE) {
+          round[v] = (uint32_t)(r + 1);
+          cluster[v] = cluster[u];
+          nx[nn++] = v;
+        } else if (round[v] == r + 1 && cluster[u] < cluster[v]) {
+          cluster[v] = cluster[u];
+        }
+      }
+    }
+    uint32_t *t = fr;
+    fr = nx;
+    nx = t;
+    nf = nn;
+  }
+  free(shifts);
+  free(fr);
+  free(nx);
+}
+
 /* delta::PageRankDelta (benchmarks/PageRank/PageRank.h:335-396), FP64.
  * Every round both branches of sparse_or_dense (:230-269) give
  * nghSum[v] = sum of Delta[s].delta_over_degree over v's neighbours s in

@@ -125,6 +125,7 @@ void orc_bc(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t src, 
 void orc_kcore(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t *core);
 int orc_pagerank_delta(uint64_t n, const uint64_t *off, const uint32_t *ids, double eps, double local_eps,
                        int max_iters, double *p);
+void orc_ldd(uint64_t n, const uint64_t *off, const uint32_t *ids, double beta, uint32_t *cluster, uint32_t *round);
 uint64_t orc_map_sum(uint64_t m, const uint32_t *ids);
 
 /* ---- Learned selector: MLP 6->32->32->3 reference math --------------- */

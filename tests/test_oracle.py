@@ -482,3 +482,46 @@ def test_pagerank_delta_pins():
     # stops once sum |delta| < eps
     p, r = O.pagerank_delta(g, 1e-4, 0.01, 100)
     assert 1 < r < 100
+
+
+@pytest.mark.parametrize("which", ["rmat", "grid", "er", "inputs"])
+def test_ldd_properties(which):
+    """LDD (MPX13, permute false, beta 0.2): centres label themselves at the
+    round they become eligible and were not reached earlier; every other
+    vertex is reached one round after its nearest frontier neighbour, no
+    later than its own eligibility round, and joins the smallest cluster
+    among the neighbours one round ahead of it; clusters are connected."""
+    if which == "rmat":
+        g = O.rmat_graph(11)
+    elif which == "grid":
+        g = O.grid3d_graph(9)
+    elif which == "er":
+        g = O.er_graph(10, 1500, seed=4)
+    else:
+        g = O.read_adjacency_graph(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "inputs",
+                                                "rMatGraph_J_5_100"))
+    cl, rd = O.ldd(g)
+    n = g.n
+    s = O.ldd_start_round(n)
+    assert (cl < n).all() and (rd != O.NONE).all()
+    for v in range(n):
+        nb = g.ids[g.offsets[v]:g.offsets[v + 1]].astype(np.int64)
+        near = rd[nb].min() + 1 if len(nb) else 1 << 40
+        if cl[v] == v:
+            assert rd[v] == s[v] and near > rd[v]
+        else:
+            assert rd[v] == near and s[v] >= rd[v]
+            ahead = nb[rd[nb] == rd[v] - 1]
+            assert cl[v] == cl[ahead].min()
+    # clusters are connected: from each centre, a walk inside the cluster reaches all of it
+    for c in np.unique(cl):
+        members = set(np.nonzero(cl == c)[0].tolist())
+        seen, st = {int(c)}, [int(c)]
+        while st:
+            u = st.pop()
+            for w in g.ids[g.offsets[u]:g.offsets[u + 1]]:
+                w = int(w)
+                if w in members and w not in seen:
+                    seen.add(w)
+                    st.append(w)
+        assert seen == members
