@@ -14,7 +14,8 @@
 //            vertices, NaN for unreached ones (0 - inf) / inf, as the
 //            reference's arithmetic gives.
 //
-// Vertices are bucketed by level (counting sort), and each level is one
+// Vertices are bucketed by level (counting sort through per-block LDS
+// histograms), and each level is one
 // pass over its bucket's adjacency lists.  A vertex's level and value share
 // a 16-byte record, so every edge costs one gather; a wave takes 64 bucket
 // entries and picks the lanes per vertex from their largest degree, with 8
@@ -39,19 +40,60 @@ struct alignas(16) BcState {
 
 constexpr int kBcUnroll = 8;
 
-__global__ void k_bc_init(const uint32_t *lev, uint64_t n, uint32_t src, BcState *st, uint32_t *cnt) {
+// Levels are few (tens on small-world graphs) and every reached vertex
+// counts into one, so counting and bucketing go through a per-block LDS
+// histogram: one global atomic per level per block instead of one per
+// vertex on a handful of addresses.  Graphs deeper than kBcLdsLevels levels
+// (long paths, grids) use global atomics, spread over many levels anyway.
+constexpr uint32_t kBcLdsLevels = 4096;
+constexpr int kBcBlock = 256;
+
+__global__ void __launch_bounds__(kBcBlock) k_bc_init(const uint32_t *lev, uint64_t n, uint32_t src, uint32_t R,
+                                                      BcState *st, uint32_t *cnt) {
+  __shared__ uint32_t h[kBcLdsLevels];
+  const bool local = R <= kBcLdsLevels;
+  if (local)
+    for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) h[i] = 0;
+  __syncthreads();
   for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
     const uint32_t l = lev[v];
     st[v] = BcState{v == src ? 1.0 : 0.0, l, 0};
-    if (l != kNone) atomicAdd(&cnt[l], 1u);
+    if (l != kNone) {
+      if (local) atomicAdd(&h[l], 1u);
+      else atomicAdd(&cnt[l], 1u);
+    }
   }
+  __syncthreads();
+  if (local)
+    for (uint32_t i = threadIdx.x; i < R; i += blockDim.x)
+      if (h[i]) atomicAdd(&cnt[i], h[i]);
 }
 
-// Bucket fill: pos[l] starts at the bucket's offset (exclusive scan).
-__global__ void k_bc_bucket(const uint32_t *lev, uint64_t n, uint32_t *pos, uint32_t *order) {
-  for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
-    const uint32_t l = lev[v];
-    if (l != kNone) order[atomicAdd(&pos[l], 1u)] = (uint32_t)v;
+// Bucket fill: pos[l] starts at the bucket's offset (exclusive scan).  Each
+// block takes kBcBlock vertices at a time, ranks them per level in LDS,
+// reserves one range per level, then writes.
+__global__ void __launch_bounds__(kBcBlock) k_bc_bucket(const uint32_t *lev, uint64_t n, uint32_t R, uint32_t *pos,
+                                                        uint32_t *order) {
+  __shared__ uint32_t h[kBcLdsLevels];
+  if (R > kBcLdsLevels) {
+    for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
+      const uint32_t l = lev[v];
+      if (l != kNone) order[atomicAdd(&pos[l], 1u)] = (uint32_t)v;
+    }
+    return;
+  }
+  for (uint64_t c0 = (uint64_t)blockIdx.x * kBcBlock; c0 < n; c0 += (uint64_t)gridDim.x * kBcBlock) {
+    for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const uint64_t v = c0 + threadIdx.x;
+    const uint32_t l = v < n ? lev[v] : kNone;
+    const uint32_t rank = l != kNone ? atomicAdd(&h[l], 1u) : 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < R; i += blockDim.x)
+      if (h[i]) h[i] = atomicAdd(&pos[i], h[i]);  // count -> this block's base
+    __syncthreads();
+    if (l != kNone) order[h[l] + rank] = (uint32_t)v;
+    __syncthreads();
   }
 }
 
@@ -164,11 +206,12 @@ int byo_bc(byo_graph_t g, uint32_t src, double *out, double *device_ms) {
     DevBuf tmp(prim::scan_tmp_bytes(R + 1) + 64);
     const unsigned grid = byo_grid(n, 256, 16384);
     prim::fill_u32(cnt, 0, R + 1, s);
-    k_bc_init<<<grid, 256, 0, s>>>(lev, n, src, st, cnt);
+    const unsigned hgrid = (unsigned)byo_grid(n, kBcBlock, 2048);
+    k_bc_init<<<hgrid, kBcBlock, 0, s>>>(lev, n, src, R, st, cnt);
     prim::exclusive_scan_u32(cnt, pos, R + 1, false, tmp.as<uint32_t>(), s);
     std::vector<uint32_t> start(R + 1);
     BYO_HIP(hipMemcpyAsync(start.data(), pos, (R + 1) * 4, hipMemcpyDeviceToHost, s));
-    k_bc_bucket<<<grid, 256, 0, s>>>(lev, n, pos, order);
+    k_bc_bucket<<<hgrid, kBcBlock, 0, s>>>(lev, n, R, pos, order);
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipStreamSynchronize(s));
     // bucket l holds order[start[l] .. start[l+1]); start[R] = vertices reached
