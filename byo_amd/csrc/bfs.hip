@@ -26,7 +26,9 @@
 
 namespace byo {
 
-bool mlp_eval_host(const std::vector<float> &p, const float *x, float *logits);  // selector.hip
+// logits of the selector for features x; true when every normalised
+// feature lies within range deviations of the training set (selector.hip)
+bool mlp_eval_host(const std::vector<float> &p, const float *x, float *logits, float range);
 void bfs_batch_run(byo_graph *g, const uint32_t *srcs, uint32_t k, byo_bfs_stats *stats, uint64_t maxdeg,
                    double *ms_out);  // bfs_batch.hip
 void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all, byo_bfs_stats *stats,
@@ -350,12 +352,14 @@ static int choose_direction(byo_graph *g, bool is_dense, uint64_t fs, uint64_t f
       x[4] = (float)std::log2(1.0 + avg);
       x[5] = (float)std::log2(1.0 + (double)max_deg);
       float logits[3];
-      const bool in_range = mlp_eval_host(g->sel_host, x, logits);
+      const bool in_range =
+          mlp_eval_host(g->sel_host, x, logits, std::min(kSelClamp, (float)g->tuning("sel_range_x100") / 100.0f));
       int best = 0;
       for (int k = 1; k < 3; k++)
         if (logits[k] > logits[best]) best = k;
       if (!std::isfinite(logits[0] + logits[1] + logits[2])) return rule;  // fallback
-      if (!in_range || logits[best] - logits[dir_class(rule)] < kSelMargin) return rule;
+      const float margin = (float)g->tuning("sel_margin_x100") / 100.0f;
+      if (!in_range || logits[best] - logits[dir_class(rule)] < margin) return rule;
       return best == 0 ? 0 : (best == 1 ? 1 : 3);
     }
     default: return rule;

@@ -513,6 +513,8 @@ DirCtl dir_ctl(byo_graph *g, uint64_t maxdeg, hipStream_t s, bool batched) {
     if (!batched && g->sel_loaded) d.sel = g->sel_params.as<float>();
   }
   d.maxdeg = maxdeg;
+  d.margin = (float)g->tuning(batched ? "sel_margin_batch_x100" : "sel_margin_x100") / 100.0f;
+  d.range = std::min(kSelClamp, (float)g->tuning("sel_range_x100") / 100.0f);
   uint8_t *dsched = g->buf("bfs_coop_sched", 64).as<uint8_t>();
   if (g->dir_policy == BYO_DIR_SCHEDULE) BYO_HIP(hipMemcpyAsync(dsched, g->dir_sched, 64, hipMemcpyHostToDevice, s));
   d.sched = dsched;

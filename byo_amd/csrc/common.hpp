@@ -20,9 +20,11 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 // the training set (the trainer applies the same clamp).
 constexpr float kSelClamp = 3.0f;
 // The learned selector overrides the threshold rule only for frontiers whose
-// features all lie within kSelClamp deviations of the training set and when
-// its logit for another direction beats the rule's by at least kSelMargin.
-constexpr float kSelMargin = 1.0f;
+// features all lie within a bound (tuning knob sel_range_x100, at most
+// kSelClamp) of the training set's deviations and when
+// its logit for another direction beats the rule's by at least a margin
+// (tuning knobs sel_margin_x100 and sel_margin_batch_x100, per traversal
+// engine; the trainer stores the value it validated with the weights).
 
 // Selector class (0 push, 1 pull, 2 forward) of a direction 0, 1 or 3.
 __host__ __device__ inline int dir_class(int dir) { return dir == 3 ? 2 : (dir == 1 ? 1 : 0); }
@@ -155,6 +157,9 @@ inline const std::vector<TuneSpec> &tune_specs() {
       {"pr_lanes", 0, 0, 32, "PageRank lanes per vertex (4, 8, 16, 32; 0: per tile from its largest degree)"},
       {"pr_grid", 4096, 256, 65536, "PageRank pull grid (blocks of 256)"},
       {"pr_seg_log2", 40, 10, 40, "log2 of the PageRank source-segment width (40: one segment)"},
+      {"sel_range_x100", 300, 0, 1000, "learned selector in-range bound, training deviations x100"},
+      {"sel_margin_x100", 100, 0, 100000, "learned selector margin, logits x100 (one-source engine)"},
+      {"sel_margin_batch_x100", 100, 0, 100000, "learned selector margin, logits x100 (shared traversal)"},
       {"pr_relabel", 0, 0, 1, "PageRank on a copy relabelled by descending degree (built once per graph)"},
       {"pr_xcd", 0, 0, 1, "PageRank pull split into one source slice per XCD (built once per graph)"},
       {"bfs_engine", 0, 0, 2,

@@ -130,6 +130,8 @@ struct DirCtl {
   const float *sel;      // 1379 params + mean[6] + inv_std[6], or null
   uint64_t maxdeg;
   const uint8_t *sched;  // 64 per-level directions (BYO_DIR_SCHEDULE)
+  float margin;          // logit margin the network must beat the rule by
+  float range;           // in-range bound on the normalised features
 };
 
 struct MlpLds {
@@ -172,7 +174,7 @@ __device__ inline int decide(const DirCtl &a, bool is_dense, uint64_t fs, uint64
     bool in_range = true;
     for (int i = 0; i < 6; i++) {
       const float v = (x[i] - p[kNP + i]) * p[kNP + 6 + i];
-      in_range = in_range && fabsf(v) <= kSelClamp;
+      in_range = in_range && fabsf(v) <= a.range;
       s.x[i] = fminf(fmaxf(v, -kSelClamp), kSelClamp);
     }
     s.in_range = in_range;
@@ -200,8 +202,8 @@ __device__ inline int decide(const DirCtl &a, bool is_dense, uint64_t fs, uint64
   if (!isfinite(z0 + z1 + z2)) return rule;
   const int best = (z1 > z0) ? ((z2 > z1) ? 2 : 1) : ((z2 > z0) ? 2 : 0);
   // the rule stands unless the frontier is in the training range and the
-  // network prefers another direction by at least kSelMargin
-  if (!s.in_range || s.z[best] - s.z[dir_class(rule)] < kSelMargin) return rule;
+  // network prefers another direction by at least the margin
+  if (!s.in_range || s.z[best] - s.z[dir_class(rule)] < a.margin) return rule;
   return best == 0 ? 0 : (best == 1 ? 1 : 3);
 }
 

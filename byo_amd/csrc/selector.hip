@@ -21,7 +21,7 @@ constexpr int kNP = BYO_MLP_NPARAM;
 constexpr int oW1 = 0, ob1 = 192, oW2 = 224, ob2 = 1248, oW3 = 1280, ob3 = 1376;
 static_assert(ob3 + 3 == kNP, "layout");
 
-bool mlp_eval_host(const std::vector<float> &p, const float *xin, float *z) {
+bool mlp_eval_host(const std::vector<float> &p, const float *xin, float *z, float range) {
   // p holds params followed by mean[6] and inv_std[6]
   float x[6];
   // normalised features are clamped to +-kSelClamp standard deviations, so
@@ -30,7 +30,7 @@ bool mlp_eval_host(const std::vector<float> &p, const float *xin, float *z) {
   bool in_range = true;
   for (int i = 0; i < 6; i++) {
     const float v = (xin[i] - p[kNP + i]) * p[kNP + 6 + i];
-    in_range = in_range && std::fabs(v) <= kSelClamp;
+    in_range = in_range && std::fabs(v) <= range;
     x[i] = std::min(std::max(v, -kSelClamp), kSelClamp);
   }
   float h1[32], h2[32];

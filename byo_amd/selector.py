@@ -41,6 +41,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "selector_weights.json")
 WEIGHTS_BATCH = os.path.join(HERE, "selector_weights_batch.json")
 ENGINES = {"single": (SEL_SINGLE, WEIGHTS), "batch": (SEL_BATCH, WEIGHTS_BATCH)}
+# tuning knob holding each engine's decision margin (logits x 100)
+MARGIN_KNOB = {"single": "sel_margin_x100", "batch": "sel_margin_batch_x100"}
 BATCH = 8  # sources per shared traversal when collecting and evaluating
 CLASS_TO_DIR = (0, 1, 3)
 DIR_TO_CLASS = {0: 0, 1: 1, 3: 2}
@@ -189,6 +191,8 @@ def load(G, path=None, engine=None):
             w = json.load(f)
         G.load_selector(np.array(w["params"], np.float32), np.array(w["mean"], np.float32),
                         np.array(w["inv_std"], np.float32), engine=eid)
+        if "margin" in w:
+            G.set_tuning(MARGIN_KNOB[name], int(round(100 * float(w["margin"]))))
         out[name] = w
     return out
 

@@ -338,6 +338,9 @@ int byo_profile_reset(byo_graph_t g);
  *                       that run on XCD x (blockIdx % 8), partial sums
  *                       combined in slice order; slices built on the first
  *                       call after each change of the graph (0)
+ *   sel_margin_x100     0..100000 logit margin x100 by which the learned
+ *                       selector must beat the rule (one-source engine, 100)
+ *   sel_margin_batch_x100  the same for the shared traversal (100)
  *   bfs_engine          0..2 several-source BFS: 0 shared traversal per 64
  *                       sources, 1 one persistent launch taking them in
  *                       turn, 2 host-driven level loop (0)
