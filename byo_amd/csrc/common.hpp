@@ -157,6 +157,7 @@ inline const std::vector<TuneSpec> &tune_specs() {
       {"pr_lanes", 0, 0, 32, "PageRank lanes per vertex (4, 8, 16, 32; 0: per tile from its largest degree)"},
       {"pr_grid", 4096, 256, 65536, "PageRank pull grid (blocks of 256)"},
       {"pr_seg_log2", 40, 10, 40, "log2 of the PageRank source-segment width (40: one segment)"},
+      {"kcore_blocks_per_cu", 3, 1, 8, "blocks per CU of the persistent k-core peel"},
       {"sel_range_x100", 300, 0, 1000, "learned selector in-range bound, training deviations x100"},
       {"sel_margin_x100", 100, 0, 100000, "learned selector margin, logits x100 (one-source engine)"},
       {"sel_margin_batch_x100", 100, 0, 100000, "learned selector margin, logits x100 (shared traversal)"},

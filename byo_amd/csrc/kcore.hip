@@ -70,10 +70,23 @@ __global__ void __launch_bounds__(kCoopBlock) k_kc_persist(KcArgs a) {
   };
   while (removed < a.n) {
     if (cnt == 0) {
-      // next bucket: the smallest live degree ...
+      // next bucket: the smallest live degree ... (scans keep 8 vertices
+      // per lane in flight)
+      constexpr int U = 8;
       uint32_t m = 0xFFFFFFFFu;
-      for (uint64_t v = global_tid(); v < a.n; v += grid_threads())
-        if (!a.dead[v]) m = min(m, a.D[v]);
+      for (uint64_t base = wave * 64 * U; base < a.n; base += nwaves * 64 * U) {
+        uint8_t dv[U];
+        uint32_t Dv[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+          const uint64_t v = min(base + (uint64_t)j * 64 + lane, a.n - 1);
+          dv[j] = a.dead[v];
+          Dv[j] = a.D[v];
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++)
+          if (!dv[j]) m = min(m, Dv[j]);
+      }
       m = wave_min(m);
       if (lane == 0 && m != 0xFFFFFFFFu) atomicMin(&c->mn[p % 3], m);
       next_phase();
@@ -84,38 +97,64 @@ __global__ void __launch_bounds__(kCoopBlock) k_kc_persist(KcArgs a) {
       }
       k = max(k, m);
       // ... and its vertices, dead with coreness k
-      for (uint64_t base = wave * 64; base < a.n; base += nwaves * 64) {
-        const uint64_t v = base + lane;
-        const bool take = v < a.n && !a.dead[v] && a.D[v] <= k;
-        const uint64_t bal = __ballot(take);
-        if (!bal) continue;
-        unsigned int at = 0;
-        if (lane == 0) at = atomicAdd(&c->cnt[p % 3], (unsigned int)__popcll(bal));
-        at = __shfl(at, 0, 64);
-        if (take) {
-          q[at + mbcnt(bal)] = (uint32_t)v;
-          a.dead[v] = 1;
-          a.core[v] = k;
+      for (uint64_t base = wave * 64 * U; base < a.n; base += nwaves * 64 * U) {
+        uint8_t dv[U];
+        uint32_t Dv[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+          const uint64_t v = min(base + (uint64_t)j * 64 + lane, a.n - 1);
+          dv[j] = a.dead[v];
+          Dv[j] = a.D[v];
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+          const uint64_t v = base + (uint64_t)j * 64 + lane;
+          const bool take = v < a.n && !dv[j] && Dv[j] <= k;
+          const uint64_t bal = __ballot(take);
+          if (!bal) continue;
+          unsigned int at = 0;
+          if (lane == 0) at = atomicAdd(&c->cnt[p % 3], (unsigned int)__popcll(bal));
+          at = __shfl(at, 0, 64);
+          if (take) {
+            q[at + mbcnt(bal)] = (uint32_t)v;
+            a.dead[v] = 1;
+            a.core[v] = k;
+          }
         }
       }
       next_phase();
       cnt = ld_u32(&c->cnt[(p - 1) % 3]);
     }
     kmax = max(kmax, k);
-    // one round: a wave per frontier vertex decrements its live neighbours.
-    // (Lanes spread over one list hit distinct vertices; giving a wave 64
-    // low-degree frontier vertices instead makes its atomics collide on the
-    // hubs they share, twice as slow on RMAT.)  Dead neighbours are skipped
-    // to save atomics; a stale flag is harmless, as a dead vertex's degree
-    // no longer matters and, degrees only falling, one that died at level
-    // k0 <= k cannot pass from k+1 to k again.
+    // one round: a wave per frontier vertex decrements its live neighbours
+    // 64 at a time, and the vertices that reach k are appended with one
+    // counter atomic per wave and step (one per push would serialise a
+    // round's millions of pushes on that one address).  Lanes spread over
+    // one list hit distinct vertices; serving 64 low-degree frontier
+    // vertices per instruction instead makes the decrements collide on the
+    // hubs they share (2.4x slower on RMAT, even with the appends
+    // aggregated).  Dead neighbours are skipped to save atomics; a stale
+    // flag is harmless, as a dead vertex's degree no longer matters and,
+    // degrees only falling, one that died at level k0 <= k cannot pass from
+    // k+1 to k again.
     for (uint64_t i = wave; i < cnt; i += nwaves) {
       const uint32_t u = q[i];
-      const uint64_t e1 = a.off[u + 1];
-      for (uint64_t e = a.off[u] + lane; e < e1; e += 64) {
-        const uint32_t v = a.ids[e];
-        if (!a.dead[v] && atomicSub(&a.D[v], 1u) == k + 1) {
-          qn[atomicAdd(&c->cnt[p % 3], 1u)] = v;
+      const uint64_t e0 = a.off[u], e1 = a.off[u + 1];
+      for (uint64_t c0 = e0; c0 < e1; c0 += 64) {
+        const uint64_t e = c0 + lane;
+        bool push = false;
+        uint32_t v = 0;
+        if (e < e1) {
+          v = a.ids[e];
+          push = !a.dead[v] && atomicSub(&a.D[v], 1u) == k + 1;
+        }
+        const uint64_t bal = __ballot(push);
+        if (!bal) continue;
+        unsigned int at = 0;
+        if (lane == 0) at = atomicAdd(&c->cnt[p % 3], (unsigned int)__popcll(bal));
+        at = __shfl(at, 0, 64);
+        if (push) {
+          qn[at + mbcnt(bal)] = v;
           a.dead[v] = 1;
           a.core[v] = k;
         }
@@ -125,6 +164,7 @@ __global__ void __launch_bounds__(kCoopBlock) k_kc_persist(KcArgs a) {
     removed += cnt;
     rounds++;
     cnt = ld_u32(&c->cnt[(p - 1) % 3]);
+
     uint32_t *t = q;
     q = qn;
     qn = t;
@@ -166,7 +206,8 @@ int byo_kcore(byo_graph_t g, uint32_t *core, uint32_t *k_max, uint64_t *rounds, 
     BYO_HIP(hipMemsetAsync(a.ctl, 0, sizeof(KcCtl), s));
     BYO_HIP(hipMemsetAsync(a.ctl->mn, 0xFF, sizeof(a.ctl->mn), s));
     const void *kern = reinterpret_cast<const void *>(k_kc_persist);
-    const int nblocks = coop_grid(kern, kCoopBlock, (int)g->tuning("coop_blocks_per_cu"));
+    // latency-bound per frontier vertex: as many resident waves as fit
+    const int nblocks = coop_grid(kern, kCoopBlock, (int)g->tuning("kcore_blocks_per_cu"));
     void *args[] = {&a};
     PhaseTimer t(s, true);
     BYO_HIP(hipLaunchCooperativeKernel(kern, dim3(nblocks), dim3(kCoopBlock), args, 0, s));

@@ -338,6 +338,9 @@ int byo_profile_reset(byo_graph_t g);
  *                       that run on XCD x (blockIdx % 8), partial sums
  *                       combined in slice order; slices built on the first
  *                       call after each change of the graph (0)
+ *   kcore_blocks_per_cu 1..8  blocks per CU of the persistent k-core peel (3)
+ *   sel_range_x100      0..1000 bound on the selector's normalised features
+ *                       for it to override the rule, deviations x100 (300)
  *   sel_margin_x100     0..100000 logit margin x100 by which the learned
  *                       selector must beat the rule (one-source engine, 100)
  *   sel_margin_batch_x100  the same for the shared traversal (100)
