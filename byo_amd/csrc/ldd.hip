@@ -36,38 +36,69 @@ __global__ void k_ldd_init(uint64_t n, uint32_t *cl, uint32_t *rd) {
   }
 }
 
-__device__ __forceinline__ void ldd_append(uint32_t v, uint64_t deg, uint32_t *q, LddCtl *ctl) {
-  q[atomicAdd(&ctl->cnt, 1u)] = v;
-  atomicAdd(&ctl->deg, (unsigned long long)deg);
-}
-
-// New centres of round r, appended after the frontier already in q.
+// New centres of round r, appended after the frontier already in q (one
+// counter atomic per wave, degrees summed per wave).
 __global__ void k_ldd_centres(const uint64_t *off, uint64_t start, uint64_t end, uint32_t r, uint32_t *cl,
                               uint32_t *rd, uint32_t *q, LddCtl *ctl) {
-  for (uint64_t v = start + global_tid(); v < end; v += grid_threads()) {
-    if (rd[v] == kNone) {
+  const int lane = lane_id();
+  const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
+  unsigned long long dsum = 0;
+  for (uint64_t base = start + wave * 64; base < end; base += nwaves * 64) {
+    const uint64_t v = base + lane;
+    const bool take = v < end && rd[v] == kNone;
+    const uint64_t bal = __ballot(take);
+    if (!bal) continue;
+    unsigned int at = 0;
+    if (lane == 0) at = atomicAdd(&ctl->cnt, (unsigned int)__popcll(bal));
+    at = __shfl(at, 0, 64);
+    if (take) {
       rd[v] = r;
       cl[v] = (uint32_t)v;
-      ldd_append((uint32_t)v, off[v + 1] - off[v], q, ctl);
+      q[at + mbcnt(bal)] = (uint32_t)v;
+      dsum += off[v + 1] - off[v];
     }
   }
+  dsum = wave_sum(dsum);
+  if (lane == 0 && dsum) atomicAdd(&ctl->deg, dsum);
 }
 
+// A wave per frontier vertex, 64 neighbours at a time; claims are appended
+// with one counter atomic per wave and step, degrees summed per wave.
 __global__ void k_ldd_push(const uint64_t *off, const uint32_t *ids, const uint32_t *q, uint32_t cnt, uint32_t r,
                            uint32_t *cl, uint32_t *rd, uint32_t *qn, LddCtl *ctl) {
   const int lane = lane_id();
   const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
+  unsigned long long dsum = 0;
   for (uint64_t i = wave; i < cnt; i += nwaves) {
     const uint32_t u = q[i];
     const uint32_t c = cl[u];
-    for (uint64_t e = off[u] + lane; e < off[u + 1]; e += 64) {
-      const uint32_t v = ids[e];
-      if (rd[v] != kNone && rd[v] != r + 1) continue;
-      const uint32_t old = atomicCAS(&rd[v], kNone, r + 1);
-      if (old == kNone) ldd_append(v, off[v + 1] - off[v], qn, ctl);
-      if (old == kNone || old == r + 1) atomicMin(&cl[v], c);
+    const uint64_t e1 = off[u + 1];
+    for (uint64_t c0 = off[u]; c0 < e1; c0 += 64) {
+      const uint64_t e = c0 + lane;
+      bool won = false;
+      uint32_t v = 0;
+      if (e < e1) {
+        v = ids[e];
+        const uint32_t seen = rd[v];
+        if (seen == kNone || seen == r + 1) {
+          const uint32_t old = atomicCAS(&rd[v], kNone, r + 1);
+          won = old == kNone;
+          if (won || old == r + 1) atomicMin(&cl[v], c);
+        }
+      }
+      const uint64_t bal = __ballot(won);
+      if (!bal) continue;
+      unsigned int at = 0;
+      if (lane == 0) at = atomicAdd(&ctl->cnt, (unsigned int)__popcll(bal));
+      at = __shfl(at, 0, 64);
+      if (won) {
+        qn[at + mbcnt(bal)] = v;
+        dsum += off[v + 1] - off[v];
+      }
     }
   }
+  dsum = wave_sum(dsum);
+  if (lane == 0 && dsum) atomicAdd(&ctl->deg, dsum);
 }
 
 // Pull: every unclaimed vertex takes the smallest cluster among its
@@ -76,6 +107,7 @@ __global__ void k_ldd_pull(const uint64_t *off, const uint32_t *ids, uint64_t n,
                            uint32_t *rd, uint32_t *qn, LddCtl *ctl) {
   const int lane = lane_id();
   const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
+  unsigned long long dsum = 0;
   for (uint64_t base = wave * 64; base < n; base += nwaves * 64) {
     const uint64_t v = base + lane;
     uint32_t best = kNone;
@@ -95,9 +127,11 @@ __global__ void k_ldd_pull(const uint64_t *off, const uint32_t *ids, uint64_t n,
       rd[v] = r + 1;
       cl[v] = best;
       qn[at + mbcnt(bal)] = (uint32_t)v;
-      atomicAdd(&ctl->deg, (unsigned long long)(off[v + 1] - off[v]));
+      dsum += off[v + 1] - off[v];
     }
   }
+  dsum = wave_sum(dsum);
+  if (lane == 0 && dsum) atomicAdd(&ctl->deg, dsum);
 }
 
 __global__ void k_ldd_ctl(LddCtl *ctl, uint32_t cnt) {
