@@ -11,8 +11,8 @@
 // contributions of its source as one contiguous run; W = K rounded up to a
 // power of two lanes serve one vertex, 64/W vertices per wave step, and
 // each lane sums its seed's contributions in adjacency order (eight
-// neighbour ids and gathers in flight per step, the adds in order), so the
-// result is the same as one thread summing the list.  The vertices of a
+// gathers in flight per step with the next eight ids behind them, the adds
+// in order), so the result is the same as one thread summing the list.  The vertices of a
 // sweep are shared out by the traversal's weight-balanced word pieces;
 // iterations are separated by the grid barrier.
 #include "bfs_batch.hpp"
@@ -68,18 +68,26 @@ __device__ __forceinline__ void ppr_body(const PprArgs &q, GridBar *bar) {
         if (v < n && sub < K) {
           const uint64_t b = q.off[v], e = q.off[v + 1];
           Acc<float> a{0.f, 0.f};
-          uint64_t k = b;
-          for (; k + kPprUnroll <= e; k += kPprUnroll) {
+          if (e > b) {
+            // Steps of kPprUnroll edges, the last one partial: ids are read
+            // at clamped positions (so every load is issued), the next
+            // step's ids are requested while this step's gathers are in
+            // flight, and only the edges of the list are added.
+            const uint64_t last = e - 1;
             uint32_t u[kPprUnroll];
-            float f[kPprUnroll];
 #pragma unroll
-            for (int t = 0; t < kPprUnroll; t++) u[t] = q.ids[k + t];
+            for (int t = 0; t < kPprUnroll; t++) u[t] = q.ids[std::min<uint64_t>(b + t, last)];
+            for (uint64_t k = b; k < e; k += kPprUnroll) {
+              float f[kPprUnroll];
 #pragma unroll
-            for (int t = 0; t < kPprUnroll; t++) f[t] = cin[(uint64_t)u[t] * K + sub];
+              for (int t = 0; t < kPprUnroll; t++) f[t] = cin[(uint64_t)u[t] * K + sub];
 #pragma unroll
-            for (int t = 0; t < kPprUnroll; t++) kahan_add(a, f[t]);
+              for (int t = 0; t < kPprUnroll; t++) u[t] = q.ids[std::min<uint64_t>(k + kPprUnroll + t, last)];
+#pragma unroll
+              for (int t = 0; t < kPprUnroll; t++)
+                if (k + t < e) kahan_add(a, f[t]);
+            }
           }
-          for (; k < e; k++) kahan_add(a, cin[(uint64_t)q.ids[k] * K + sub]);
           const float pv = q.damping * (a.s - a.c) + (my_seed == v ? teleport : 0.0f);
           q.p[v * K + sub] = pv;
           cout[v * K + sub] = (e > b) ? pv / (float)(e - b) : 0.0f;
