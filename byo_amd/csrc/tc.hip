@@ -4,13 +4,17 @@
 // 1. rank by (degree, id) -- rankNodes sorts by degree; any tie order gives
 //    the same count, and (degree, id) makes the orientation deterministic;
 // 2. filterGraph(G, rank[u] < rank[v]) (gbbs/interface.h:81-99 ->
-//    graph_mutation.h:15-81): count pass, exclusive scan, order-preserving
-//    fill, giving the oriented graph DG (m/2 ids, each list sorted);
+//    graph_mutation.h:15-81), order-preserving, giving the oriented graph
+//    DG (m/2 edges of a symmetric graph, each list sorted); its lists are
+//    packed in place at the front of each CSR list (one pass, no count or
+//    scan), so u's list is dg_ids[off[u] .. dg_end[u]);
 // 3. count sum_{u->v in DG} |N+(u) & N+(v)| (CountDirectedBalanced,
 //    Triangle.h:99-150, with intersect.h's sorted-list intersection): one
-//    wave per u stages N+(u) in LDS; the pairs (v, w in N+(v)) of all
-//    v in N+(u) are flattened over the lanes and each w is looked up in the
-//    staged list by binary search; the total is a uint64.
+//    wave per u puts N+(u) in an LDS hash table; the pairs (v, w in N+(v))
+//    of all v in N+(u) are flattened over the lanes, four per lane with
+//    their loads in flight together, and each w is looked up in the table;
+//    the total is a uint64.  (Membership of w in N+(u) is what the sorted
+//    intersection counts, so the count is the same.)
 #include <algorithm>
 
 #include "primitives.hpp"
@@ -18,32 +22,27 @@
 namespace byo {
 namespace {
 
-__device__ __forceinline__ bool rank_less(const uint64_t *off, uint32_t u, uint64_t du, uint32_t v) {
-  const uint64_t dv = off[v + 1] - off[v];
-  return du < dv || (du == dv && u < v);
-}
-
-constexpr int kG = 16;  // lanes per vertex in the orientation passes
-
-__global__ void k_tc_outdeg(const uint64_t *off, const uint32_t *ids, uint64_t n, uint64_t *cnt) {
-  const int sub = lane_id() & (kG - 1);
-  const uint64_t group = global_tid() / kG, ngroups = grid_threads() / kG;
-  const uint64_t nrounds = (n + ngroups - 1) / ngroups;
-  for (uint64_t r = 0; r < nrounds; r++) {
-    const uint64_t u = r * ngroups + group;
-    uint64_t c = 0;
-    if (u < n) {
-      const uint64_t b = off[u], e = off[u + 1], du = e - b;
-      for (uint64_t k = b + sub; k < e; k += kG) c += rank_less(off, (uint32_t)u, du, ids[k]);
-    }
-#pragma unroll
-    for (int o = kG / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-    if (u < n && sub == 0) cnt[u] = c;
+// Degree of every vertex as 32 bits (a degree of 2^32 or more is held as
+// 2^32 - 1; (degree, id) is still a strict total order, which is all the
+// orientation needs), so the rank tests gather from a table half the size
+// of the offsets.
+__global__ void k_tc_deg(const uint64_t *off, uint64_t n, uint32_t *deg) {
+  for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
+    const uint64_t d = off[v + 1] - off[v];
+    deg[v] = d < 0xFFFFFFFFull ? (uint32_t)d : 0xFFFFFFFFu;
   }
 }
 
-__global__ void k_tc_fill(const uint64_t *off, const uint32_t *ids, uint64_t n, const uint64_t *dg_off,
-                          uint32_t *dg_ids) {
+constexpr int kG = 16;      // lanes per vertex in the orientation pass
+constexpr int kOrientU = 4;  // steps of kG edges whose loads are in flight together
+
+// filterGraph in one pass, in the CSR's own index space: the edges of u that
+// DG keeps are written, in adjacency order, to dg_ids[off[u] ...], and
+// dg_end[u] = off[u] + their number.  Every lane of the wave runs the same
+// number of steps (ballots below); the ids and degrees of kOrientU steps are
+// loaded at clamped positions before any is used.
+__global__ void k_tc_orient(const uint64_t *off, const uint32_t *ids, const uint32_t *deg, uint64_t n,
+                            uint32_t *dg_ids, uint64_t *dg_end) {
   const int lane = lane_id();
   const int sub = lane & (kG - 1);
   const int gshift = lane & ~(kG - 1);
@@ -52,29 +51,41 @@ __global__ void k_tc_fill(const uint64_t *off, const uint32_t *ids, uint64_t n, 
   for (uint64_t r = 0; r < nrounds; r++) {
     const uint64_t u = r * ngroups + group;
     const bool active = u < n;
-    const uint64_t b = active ? off[u] : 0, e = active ? off[u + 1] : 0, du = e - b;
-    uint64_t w = active ? dg_off[u] : 0;
-    // every lane of the wave runs the same number of steps (ballot below)
-    uint64_t steps = (du + kG - 1) / kG;
-    steps = wave_max(steps);
-    for (uint64_t s = 0; s < steps; s++) {
-      const uint64_t k = b + s * kG + sub;
-      bool keep = false;
-      uint32_t v = 0;
-      if (k < e) {
-        v = ids[k];
-        keep = rank_less(off, (uint32_t)u, du, v);
+    const uint64_t b = active ? off[u] : 0, e = active ? off[u + 1] : 0;
+    const uint32_t du = active ? deg[u] : 0;
+    const uint64_t last = e > b ? e - 1 : 0;  // ids[0] exists: the graph has edges
+    uint64_t w = b;
+    const uint64_t steps = wave_max((e - b + kG - 1) / kG);
+    for (uint64_t s0 = 0; s0 < steps; s0 += kOrientU) {
+      uint32_t v[kOrientU], dv[kOrientU];
+#pragma unroll
+      for (int t = 0; t < kOrientU; t++) v[t] = ids[std::min<uint64_t>(b + (s0 + t) * kG + sub, last)];
+#pragma unroll
+      for (int t = 0; t < kOrientU; t++) dv[t] = deg[v[t]];
+#pragma unroll
+      for (int t = 0; t < kOrientU; t++) {
+        const uint64_t k = b + (s0 + t) * kG + sub;
+        const bool kp = k < e && (du < dv[t] || (du == dv[t] && (uint32_t)u < v[t]));
+        const uint64_t mask = __ballot(kp);
+        const uint64_t gm = (mask >> gshift) & ((1ull << kG) - 1);
+        if (kp) dg_ids[w + __popcll(gm & ((1ull << sub) - 1))] = v[t];
+        w += __popcll(gm);
       }
-      const uint64_t mask = __ballot(keep);
-      const uint64_t gm = (mask >> gshift) & ((1ull << kG) - 1);
-      if (keep) dg_ids[w + __popcll(gm & ((1ull << sub) - 1))] = v;
-      w += __popcll(gm);
     }
+    if (active && sub == 0) dg_end[u] = w;
   }
 }
 
-constexpr int kLdsPerWave = 2048;  // staged N+(u) entries per wave (8 KiB)
+// N+(u) is held in an LDS hash table per wave (open addressing, linear
+// probing) of S slots, S the smallest power of two >= 2|N+(u)| and >= 64,
+// so a lookup costs about one LDS read where a binary search over the list
+// costs log2 |N+(u)| dependent reads.  Lists longer than kHashCap are
+// searched in global memory instead.
+constexpr int kSlots = 1024;  // hash slots per wave (4 KiB)
+constexpr uint32_t kHashCap = kSlots / 2;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kTcWaves = 4;
+constexpr int kPairsPerLane = 4;  // pair lookups per lane with loads in flight together
 
 __device__ __forceinline__ bool find_sorted(const uint32_t *a, uint32_t len, uint32_t x) {
   uint32_t lo = 0, hi = len;
@@ -85,22 +96,27 @@ __device__ __forceinline__ bool find_sorted(const uint32_t *a, uint32_t len, uin
   return lo < len && a[lo] == x;
 }
 
-// The same search over the wave's LDS stage, indexed directly: a pointer
-// that may point to LDS or to global memory would make every probe a flat
-// load.
-template <int N>
-__device__ __forceinline__ bool find_sorted_lds(const uint32_t (&a)[N], uint32_t len, uint32_t x) {
-  uint32_t lo = 0, hi = len;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (a[mid] < x) lo = mid + 1; else hi = mid;
-  }
-  return lo < len && a[lo] == x;
+__device__ __forceinline__ uint32_t slot_of(uint32_t x, uint32_t mask) {
+  return (x * 0x9E3779B1u) >> 11 & mask;  // Fibonacci hashing, bits 11..31
 }
 
-__global__ void __launch_bounds__(256) k_tc_count(const uint64_t *dg_off, const uint32_t *dg_ids, uint64_t n,
+// Probe of the wave's table; the table index is direct so every probe is an
+// LDS read (a pointer that may point to LDS or to global memory would make
+// every probe a flat load).
+__device__ __forceinline__ bool in_table(const uint32_t (&tab)[kSlots], uint32_t mask, uint32_t x) {
+  uint32_t h = slot_of(x, mask);
+  for (;;) {
+    const uint32_t y = tab[h];
+    if (y == x) return true;
+    if (y == kEmpty) return false;
+    h = (h + 1) & mask;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_tc_count(const uint64_t *off, const uint64_t *dg_end, const uint32_t *dg_ids,
+                                                  uint64_t n,
                                                   unsigned long long *next_u, unsigned long long *total) {
-  __shared__ uint32_t stage[kTcWaves][kLdsPerWave];
+  __shared__ uint32_t tab[kTcWaves][kSlots];
   __shared__ uint64_t pre[kTcWaves][64];
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   uint64_t mycount = 0;
@@ -114,53 +130,72 @@ __global__ void __launch_bounds__(256) k_tc_count(const uint64_t *dg_off, const 
     }
     const unsigned long long u = ubase + uoff++;
     if (u >= n) break;
-    const uint64_t bu = dg_off[u], du = dg_off[u + 1] - bu;
+    const uint64_t bu = off[u], du = dg_end[u] - bu;
     if (du < 2) continue;  // a triangle needs two out-neighbours of u
-    const bool in_lds = du <= kLdsPerWave;
-    if (in_lds)
-      for (uint64_t i = lane; i < du; i += 64) stage[wv][i] = dg_ids[bu + i];
+    const bool hashed = du <= kHashCap;
+    uint32_t S = 64;
+    while (S < 2 * du && S < (uint32_t)kSlots) S <<= 1;
+    const uint32_t mask = S - 1;
+    if (hashed) {
+      for (uint32_t i = lane; i < S; i += 64) tab[wv][i] = kEmpty;
+      wave_lds_sync();
+      // ids of one list are distinct, so every insert ends in an empty slot
+      for (uint64_t i = lane; i < du; i += 64) {
+        const uint32_t x = dg_ids[bu + i];
+        uint32_t h = slot_of(x, mask);
+        while (atomicCAS(&tab[wv][h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
+      }
+    }
     // LDS writes by this wave must land before its lanes read them
     wave_lds_sync();
     for (uint64_t c0 = 0; c0 < du; c0 += 64) {
       const uint64_t j = c0 + lane;
-      uint32_t v = 0;
       uint64_t dv = 0, bv = 0;
       if (j < du) {
-        v = dg_ids[bu + j];
-        bv = dg_off[v];
-        dv = dg_off[v + 1] - bv;
+        const uint32_t v = dg_ids[bu + j];
+        bv = off[v];
+        dv = dg_end[v] - bv;
       }
       const uint64_t incl = wave_incl_scan(dv);
       pre[wv][lane] = incl;
       const uint64_t T = __shfl(incl, 63, 64);
+      const uint64_t bv0 = __shfl(bv, 0, 64);
       wave_lds_sync();
-      for (uint64_t x0 = 0; x0 < T; x0 += 64) {
-        const uint64_t x = x0 + lane;
-        // owner slot of pair x: first s with pre[s] > x (all lanes then
-        // fetch the owner's list start with one shuffle)
-        int owner = 0;
-        uint64_t startx = 0;
-        if (x < T) {
-          int lo = 0, hi = 63;
-          while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (pre[wv][mid] > x) hi = mid; else lo = mid + 1;
-          }
-          owner = lo;
-          startx = lo ? pre[wv][lo - 1] : 0;
+      for (uint64_t x0 = 0; x0 < T; x0 += 64 * kPairsPerLane) {
+        // pair x belongs to slot owner = first s with pre[s] > x; the
+        // searches of a lane's pairs advance together, then all their ids
+        // are loaded, then looked up
+        int owner[kPairsPerLane];
+        uint32_t w[kPairsPerLane];
+#pragma unroll
+        for (int t = 0; t < kPairsPerLane; t++) owner[t] = 0;
+#pragma unroll
+        for (int st = 32; st > 0; st >>= 1) {
+          uint64_t pv[kPairsPerLane];
+#pragma unroll
+          for (int t = 0; t < kPairsPerLane; t++) pv[t] = pre[wv][owner[t] + st - 1];
+#pragma unroll
+          for (int t = 0; t < kPairsPerLane; t++)
+            owner[t] += (pv[t] <= x0 + (uint64_t)t * 64 + lane) ? st : 0;
         }
-        const uint64_t obv = __shfl(bv, owner, 64);
-        if (x < T) {
-          const uint32_t w = dg_ids[obv + (x - startx)];
-          if (in_lds)
-            mycount += find_sorted_lds(stage[wv], (uint32_t)du, w);
-          else
-            mycount += find_sorted(dg_ids + bu, (uint32_t)du, w);
+#pragma unroll
+        for (int t = 0; t < kPairsPerLane; t++) {
+          const uint64_t x = x0 + (uint64_t)t * 64 + lane;
+          const int o = owner[t] < 63 ? owner[t] : 63;
+          const uint64_t startx = o ? pre[wv][o - 1] : 0;
+          const uint64_t obv = __shfl(bv, o, 64);
+          // past the end: reread the wave's first pair
+          w[t] = dg_ids[x < T ? obv + (x - startx) : bv0];
+        }
+#pragma unroll
+        for (int t = 0; t < kPairsPerLane; t++) {
+          if (x0 + (uint64_t)t * 64 + lane < T)
+            mycount += hashed ? in_table(tab[wv], mask, w[t]) : find_sorted(dg_ids + bu, (uint32_t)du, w[t]);
         }
       }
       wave_lds_sync();  // pre[] is rewritten by the next chunk
     }
-    wave_lds_sync();  // stage[] is rewritten by the next u
+    wave_lds_sync();  // tab[] is rewritten by the next u
   }
   mycount = wave_sum(mycount);
   if (lane == 0 && mycount) atomicAdd(total, (unsigned long long)mycount);
@@ -185,20 +220,18 @@ int byo_tc(byo_graph_t g, uint64_t *count, double *device_ms) {
     hipStream_t s = g->stream;
     const uint64_t *off = g->off.as<uint64_t>();
     const uint32_t *ids = g->ids.as<uint32_t>();
-    uint64_t *cnt = g->buf("tc_cnt", (n + 1) * 8 + 64).as<uint64_t>();
-    uint64_t *dg_off = g->buf("tc_dgoff", (n + 2) * 8 + 64).as<uint64_t>();
-    uint32_t *dg_ids = g->buf("tc_dgids", (g->m / 2 + 64) * 4).as<uint32_t>();
-    void *stmp = g->buf("tc_scan_tmp", prim::scan_tmp_bytes(n + 1)).p;
+    // DG lives in the CSR's index space: u's kept edges at off[u] ... dg_end[u]
+    uint32_t *dg_ids = g->buf("tc_dgids", (g->m + 64) * 4).as<uint32_t>();
+    uint64_t *dg_end = g->buf("tc_dgend", n * 8 + 64).as<uint64_t>();
+    uint32_t *deg = g->buf("tc_deg", n * 4 + 64).as<uint32_t>();
     unsigned long long *ctr = g->buf("tc_ctr", 64).as<unsigned long long>();
     PhaseTimer t(s, true);
-    const unsigned grid = byo_grid(n * kG, 256, 16384);
-    k_tc_outdeg<<<grid, 256, 0, s>>>(off, ids, n, cnt);
-    prim::exclusive_scan_u64(cnt, dg_off, n, true, stmp, s);
-    k_tc_fill<<<grid, 256, 0, s>>>(off, ids, n, dg_off, dg_ids);
+    k_tc_deg<<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, n, deg);
+    k_tc_orient<<<byo_grid(n * kG, 256, 16384), 256, 0, s>>>(off, ids, deg, n, dg_ids, dg_end);
     BYO_HIP(hipMemsetAsync(ctr, 0, 16, s));
     BYO_LAUNCH_CHECK();
     PhaseTimer tc(s, g->prof.on);
-    k_tc_count<<<2048, 256, 0, s>>>(dg_off, dg_ids, n, ctr, ctr + 1);
+    k_tc_count<<<2048, 256, 0, s>>>(off, dg_end, dg_ids, n, ctr, ctr + 1);
     BYO_LAUNCH_CHECK();
     if (g->prof.on) g->account("tc_count", tc.stop());
     double ms = t.stop();
