@@ -120,82 +120,95 @@ __global__ void __launch_bounds__(256) k_tc_count(const uint64_t *off, const uin
   __shared__ uint64_t pre[kTcWaves][64];
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   uint64_t mycount = 0;
-  constexpr unsigned long long kGrab = 8;  // vertices claimed per atomic
-  unsigned long long ubase = 0, uoff = kGrab;
-  while (true) {
-    if (uoff == kGrab) {
-      if (lane == 0) ubase = atomicAdd(next_u, kGrab);
-      ubase = __shfl(ubase, 0, 64);
-      uoff = 0;
+  constexpr unsigned kGrab = 32;  // vertices claimed per atomic, one per lane
+  for (;;) {
+    unsigned long long ubase = 0;
+    if (lane == 0) ubase = atomicAdd(next_u, (unsigned long long)kGrab);
+    ubase = __shfl(ubase, 0, 64);
+    if (ubase >= n) break;
+    // lane l < kGrab holds vertex ubase + l: its list start and DG degree
+    uint64_t myb = 0, myd = 0;
+    if (lane < (int)kGrab && ubase + lane < n) {
+      myb = off[ubase + lane];
+      myd = dg_end[ubase + lane] - myb;
     }
-    const unsigned long long u = ubase + uoff++;
-    if (u >= n) break;
-    const uint64_t bu = off[u], du = dg_end[u] - bu;
-    if (du < 2) continue;  // a triangle needs two out-neighbours of u
-    const bool hashed = du <= kHashCap;
-    uint32_t S = 64;
-    while (S < 2 * du && S < (uint32_t)kSlots) S <<= 1;
-    const uint32_t mask = S - 1;
-    if (hashed) {
-      for (uint32_t i = lane; i < S; i += 64) tab[wv][i] = kEmpty;
-      wave_lds_sync();
-      // ids of one list are distinct, so every insert ends in an empty slot
-      for (uint64_t i = lane; i < du; i += 64) {
-        const uint32_t x = dg_ids[bu + i];
-        uint32_t h = slot_of(x, mask);
-        while (atomicCAS(&tab[wv][h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
+    uint64_t todo = __ballot(myd >= 2);  // a triangle needs two out-neighbours of u
+    while (todo) {
+      const int l = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const uint64_t bu = __shfl(myb, l, 64), du = __shfl(myd, l, 64);
+      const bool hashed = du <= kHashCap;
+      uint32_t S = 64;
+      while (S < 2 * du && S < (uint32_t)kSlots) S <<= 1;
+      const uint32_t mask = S - 1;
+      if (hashed) {
+        for (uint32_t i = lane; i < S; i += 64) tab[wv][i] = kEmpty;
+        wave_lds_sync();
       }
+      for (uint64_t c0 = 0; c0 < du; c0 += 64) {
+        const uint64_t j = c0 + lane;
+        uint64_t dv = 0, bv = 0;
+        if (j < du) {
+          const uint32_t v = dg_ids[bu + j];
+          bv = off[v];
+          dv = dg_end[v] - bv;
+          // ids of one list are distinct, so every insert ends in an empty slot
+          if (hashed && c0 == 0) {
+            uint32_t h = slot_of(v, mask);
+            while (atomicCAS(&tab[wv][h], kEmpty, v) != kEmpty) h = (h + 1) & mask;
+          }
+        }
+        if (c0 == 0) {
+          if (hashed)
+            for (uint64_t i = 64 + lane; i < du; i += 64) {
+              const uint32_t x = dg_ids[bu + i];
+              uint32_t h = slot_of(x, mask);
+              while (atomicCAS(&tab[wv][h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
+            }
+          // LDS writes by this wave must land before its lanes read them
+          wave_lds_sync();
+        }
+        const uint64_t incl = wave_incl_scan(dv);
+        pre[wv][lane] = incl;
+        const uint64_t T = __shfl(incl, 63, 64);
+        const uint64_t bv0 = __shfl(bv, 0, 64);
+        wave_lds_sync();
+        for (uint64_t x0 = 0; x0 < T; x0 += 64 * kPairsPerLane) {
+          // pair x belongs to slot owner = first s with pre[s] > x; the
+          // searches of a lane's pairs advance together, then all their ids
+          // are loaded, then looked up
+          int owner[kPairsPerLane];
+          uint32_t w[kPairsPerLane];
+  #pragma unroll
+          for (int t = 0; t < kPairsPerLane; t++) owner[t] = 0;
+  #pragma unroll
+          for (int st = 32; st > 0; st >>= 1) {
+            uint64_t pv[kPairsPerLane];
+  #pragma unroll
+            for (int t = 0; t < kPairsPerLane; t++) pv[t] = pre[wv][owner[t] + st - 1];
+  #pragma unroll
+            for (int t = 0; t < kPairsPerLane; t++)
+              owner[t] += (pv[t] <= x0 + (uint64_t)t * 64 + lane) ? st : 0;
+          }
+  #pragma unroll
+          for (int t = 0; t < kPairsPerLane; t++) {
+            const uint64_t x = x0 + (uint64_t)t * 64 + lane;
+            const int o = owner[t] < 63 ? owner[t] : 63;
+            const uint64_t startx = o ? pre[wv][o - 1] : 0;
+            const uint64_t obv = __shfl(bv, o, 64);
+            // past the end: reread the wave's first pair
+            w[t] = dg_ids[x < T ? obv + (x - startx) : bv0];
+          }
+  #pragma unroll
+          for (int t = 0; t < kPairsPerLane; t++) {
+            if (x0 + (uint64_t)t * 64 + lane < T)
+              mycount += hashed ? in_table(tab[wv], mask, w[t]) : find_sorted(dg_ids + bu, (uint32_t)du, w[t]);
+          }
+        }
+        wave_lds_sync();  // pre[] is rewritten by the next chunk
+      }
+      wave_lds_sync();  // tab[] is rewritten by the next u
     }
-    // LDS writes by this wave must land before its lanes read them
-    wave_lds_sync();
-    for (uint64_t c0 = 0; c0 < du; c0 += 64) {
-      const uint64_t j = c0 + lane;
-      uint64_t dv = 0, bv = 0;
-      if (j < du) {
-        const uint32_t v = dg_ids[bu + j];
-        bv = off[v];
-        dv = dg_end[v] - bv;
-      }
-      const uint64_t incl = wave_incl_scan(dv);
-      pre[wv][lane] = incl;
-      const uint64_t T = __shfl(incl, 63, 64);
-      const uint64_t bv0 = __shfl(bv, 0, 64);
-      wave_lds_sync();
-      for (uint64_t x0 = 0; x0 < T; x0 += 64 * kPairsPerLane) {
-        // pair x belongs to slot owner = first s with pre[s] > x; the
-        // searches of a lane's pairs advance together, then all their ids
-        // are loaded, then looked up
-        int owner[kPairsPerLane];
-        uint32_t w[kPairsPerLane];
-#pragma unroll
-        for (int t = 0; t < kPairsPerLane; t++) owner[t] = 0;
-#pragma unroll
-        for (int st = 32; st > 0; st >>= 1) {
-          uint64_t pv[kPairsPerLane];
-#pragma unroll
-          for (int t = 0; t < kPairsPerLane; t++) pv[t] = pre[wv][owner[t] + st - 1];
-#pragma unroll
-          for (int t = 0; t < kPairsPerLane; t++)
-            owner[t] += (pv[t] <= x0 + (uint64_t)t * 64 + lane) ? st : 0;
-        }
-#pragma unroll
-        for (int t = 0; t < kPairsPerLane; t++) {
-          const uint64_t x = x0 + (uint64_t)t * 64 + lane;
-          const int o = owner[t] < 63 ? owner[t] : 63;
-          const uint64_t startx = o ? pre[wv][o - 1] : 0;
-          const uint64_t obv = __shfl(bv, o, 64);
-          // past the end: reread the wave's first pair
-          w[t] = dg_ids[x < T ? obv + (x - startx) : bv0];
-        }
-#pragma unroll
-        for (int t = 0; t < kPairsPerLane; t++) {
-          if (x0 + (uint64_t)t * 64 + lane < T)
-            mycount += hashed ? in_table(tab[wv], mask, w[t]) : find_sorted(dg_ids + bu, (uint32_t)du, w[t]);
-        }
-      }
-      wave_lds_sync();  // pre[] is rewritten by the next chunk
-    }
-    wave_lds_sync();  // tab[] is rewritten by the next u
   }
   mycount = wave_sum(mycount);
   if (lane == 0 && mycount) atomicAdd(total, (unsigned long long)mycount);
