@@ -18,7 +18,9 @@
 //            max_iters rounds.
 // The pull gathers one double per edge from dodF = delta/degree of frontier
 // vertices and 0 elsewhere, so frontier membership costs no second gather;
-// lanes per vertex are picked per tile as in the PageRank pull.
+// lanes per vertex are picked per tile as in the PageRank pull.  With the
+// pr_relabel knob the rounds run on the degree-relabelled graph (the same
+// values per vertex up to summation order) and ranks are mapped back.
 #include <algorithm>
 
 #include "common.hpp"
@@ -171,6 +173,11 @@ __global__ void k_pd_l1(const double *partial, int nb, PdCtl *ctl) {
 
 __global__ void k_pd_clear(PdCtl *ctl) { ctl->cnt = 0; }
 
+// Rank of vertex v from the relabelled vector (perm: old id -> new id).
+__global__ void k_pd_unpermute(const double *in, const uint32_t *perm, uint64_t n, double *out) {
+  for (uint64_t v = global_tid(); v < n; v += grid_threads()) out[v] = in[perm[v]];
+}
+
 }  // namespace
 }  // namespace byo
 
@@ -196,8 +203,14 @@ int byo_pagerank_delta(byo_graph_t g, double eps, double local_eps, int max_iter
     const unsigned fgrid = (unsigned)byo_grid(n, 256, 4096);
     double *partial = g->buf("pd_partial", (fgrid + 8) * 8).as<double>();
     PdCtl *h = static_cast<PdCtl *>(pinned(g, sizeof(PdCtl)));
-    const uint64_t *off = g->off.as<uint64_t>();
-    const uint32_t *ids = g->ids.as<uint32_t>();
+    // the graph the rounds run on: as it is, or its degree-relabelled copy
+    // (tuning knob pr_relabel, shared with PageRank), whose hot deltas sit
+    // together at low ids where they stay in L2
+    const RelabelView view = g->tuning("pr_relabel") && n
+                                 ? relabel_view(g)
+                                 : RelabelView{g->off.as<uint64_t>(), g->ids.as<uint32_t>(), nullptr};
+    const uint64_t *off = view.off;
+    const uint32_t *ids = view.ids;
     PhaseTimer t(s, true);
     int round = 0, ran = 0;
     uint64_t fs = n;  // every vertex starts in the frontier
@@ -224,7 +237,16 @@ int byo_pagerank_delta(byo_graph_t g, double eps, double local_eps, int max_iter
     if (g->prof.on) g->account("pagerank_delta", ms);
     if (device_ms) *device_ms = ms;
     if (rounds_run) *rounds_run = ran;
-    if (p && n) BYO_HIP(hipMemcpyAsync(p, pd, n * 8, hipMemcpyDeviceToHost, s));
+    if (p && n) {
+      const double *res = pd;
+      if (view.perm) {
+        double *un = g->buf("pd_unperm", n * 8 + 64).as<double>();
+        k_pd_unpermute<<<byo_grid(n, 256, 16384), 256, 0, s>>>(pd, view.perm, n, un);
+        BYO_LAUNCH_CHECK();
+        res = un;
+      }
+      BYO_HIP(hipMemcpyAsync(p, res, n * 8, hipMemcpyDeviceToHost, s));
+    }
     BYO_HIP(hipStreamSynchronize(s));
   });
 }

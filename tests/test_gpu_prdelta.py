@@ -3,7 +3,7 @@ restatement (pinned in test_oracle.py): the same rounds and an L1 distance
 of at most 1e-9 on the reference inputs, RMAT (dense rounds then sparse
 pushes), a grid and many-component graphs; one round equals the first
 Jacobi iterate, and with every vertex active r rounds equal r Jacobi
-iterations of the FP64 PageRank."""
+iterations of the FP64 PageRank; the same on the degree-relabelled layout."""
 import os
 
 import numpy as np
@@ -58,3 +58,13 @@ def test_jacobi_pins(B):
     p, r, _ = G.pagerank_delta(0.0, -1e300, 15)
     assert r == 15 and np.abs(p - O.pagerank(g, 0.85, 15)[0]).sum() <= 1e-10
     G.close()
+
+
+def test_relabelled_layout(B):
+    # the rounds on the degree-relabelled graph, ranks mapped back
+    for g in (O.rmat_graph(15), O.er_graph(13, 9000, seed=3)):
+        G = B.CsrGraph.from_csr(g.offsets, g.ids)
+        G.set_tuning("pr_relabel", 1)
+        check(g, G)
+        check(g, G, 1e-7, 0.001, 50)
+        G.close()
