@@ -27,12 +27,13 @@ CANDIDATES = {
     "coop_blocks_per_cu": [2, 3, 4],
     "pieces": [4, 8, 16],
     "ppr_blocks_per_cu": [4, 8],
+    "ppr_pieces": [2, 4, 8],
     "pr_relabel": [0, 1],
     "pr_xcd": [0, 1],
     "pr_lanes": [0, 4, 8, 16, 32],
     "pr_grid": [4096, 8192, 16384, 32768, 65536],
 }
-WORKLOAD = {"coop_blocks_per_cu": "bfs", "pieces": "bfs", "ppr_blocks_per_cu": "ppr", "pr_lanes": "pr",
+WORKLOAD = {"coop_blocks_per_cu": "bfs", "pieces": "bfs", "ppr_blocks_per_cu": "ppr", "ppr_pieces": "ppr", "pr_lanes": "pr",
             "pr_grid": "pr", "pr_relabel": "pr", "pr_xcd": "pr"}
 
 

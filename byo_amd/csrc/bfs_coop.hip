@@ -544,18 +544,21 @@ int coop_grid(const void *kernel, int block, int cap) {
   return nb;
 }
 
-Pieces make_pieces(byo_graph *g, int nblocks) {
+Pieces make_pieces(byo_graph *g, int nblocks, PieceSet set) {
   Pieces pc;
-  pc.per_wave = (uint32_t)g->tuning("pieces");
-  pc.bound = g->buf("bfs_pieces", ((uint64_t)kMaxPieces * 2048 * 4 + 1) * 8 + 64).as<uint64_t>();
-  pc.ready = g->pieces_valid && g->pieces_nblocks == nblocks && g->pieces_per_wave == pc.per_wave;
+  pc.per_wave = (uint32_t)g->tuning(set == kPprPieces ? "ppr_pieces" : "pieces");
+  pc.bound = g->buf(set == kPprPieces ? "ppr_pieces" : "bfs_pieces", ((uint64_t)kMaxPieces * 2048 * 4 + 1) * 8 + 64)
+                 .as<uint64_t>();
+  const byo_graph::PieceCache &c = g->pcache[set];
+  pc.ready = c.valid && c.nblocks == nblocks && c.per_wave == pc.per_wave;
   return pc;
 }
 
-void pieces_launched(byo_graph *g, const Pieces &pc, int nblocks) {
-  g->pieces_valid = true;
-  g->pieces_nblocks = nblocks;
-  g->pieces_per_wave = pc.per_wave;
+void pieces_launched(byo_graph *g, const Pieces &pc, int nblocks, PieceSet set) {
+  byo_graph::PieceCache &c = g->pcache[set];
+  c.valid = true;
+  c.nblocks = nblocks;
+  c.per_wave = pc.per_wave;
 }
 
 // Runs k BFS in one cooperative launch.  Results: k*n parents in scratch

@@ -154,6 +154,7 @@ inline const std::vector<TuneSpec> &tune_specs() {
       {"coop_blocks_per_cu", 4, 1, 4, "blocks per CU of the persistent traversal launches"},
       {"ppr_blocks_per_cu", 4, 1, 8, "blocks per CU of the persistent personalized PageRank launch"},
       {"pieces", 8, 1, 16, "weight-balanced word pieces per wave in vertex-driven passes"},
+      {"ppr_pieces", 4, 1, 16, "weight-balanced word pieces per wave in the personalized PageRank sweeps"},
       {"pr_lanes", 0, 0, 32, "PageRank lanes per vertex (4, 8, 16, 32; 0: per tile from its largest degree)"},
       {"pr_grid", 4096, 256, 65536, "PageRank pull grid (blocks of 256)"},
       {"pr_seg_log2", 40, 10, 40, "log2 of the PageRank source-segment width (40: one segment)"},
@@ -185,9 +186,15 @@ struct byo_graph {
   uint64_t maxdeg = 0;        // maximum degree of the current CSR view
   bool maxdeg_valid = false;  // cleared whenever the CSR view is rebuilt
   bool rl_valid = false;      // degree-relabelled copy (relabel_view) is current
-  bool pieces_valid = false;  // "bfs_pieces" holds the word pieces of this grid:
-  int pieces_nblocks = 0;
-  uint32_t pieces_per_wave = 0;
+  // word pieces cached per set (0: traversals, buffer "bfs_pieces"; 1:
+  // personalized PageRank sweeps, "ppr_pieces"): the buffer holds the pieces
+  // of this grid and pieces per wave
+  struct PieceCache {
+    bool valid = false;
+    int nblocks = 0;
+    uint32_t per_wave = 0;
+  } pcache[2];
+  void pieces_stale() { pcache[0].valid = pcache[1].valid = false; }
   bool xs_valid = false;      // PageRank per-XCD slices are current (for layout xs_layout)
   int xs_layout = 0;
   int64_t last_src = -1;      // last source of byo_bfs / byo_bfs_multi

@@ -215,13 +215,16 @@ DirCtl dir_ctl(byo_graph *g, uint64_t maxdeg, hipStream_t s, bool batched);
 // (cap from the handle's tuning, at most 8; at most 2048 blocks).
 int coop_grid(const void *kernel, int block, int cap);
 
-// Word pieces of a cooperative launch of nblocks blocks (buffer and the
-// "pieces" knob).  The bounds depend only on the graph and the grid, so they
-// are computed by the first launch after either changes and reused by the
-// next ones (ready set); a launcher reports a queued launch with
+// Word pieces of a cooperative launch of nblocks blocks, for the traversal
+// passes (buffer "bfs_pieces", knob "pieces") or the personalized PageRank
+// sweeps (buffer "ppr_pieces", knob "ppr_pieces": their gathers balance
+// best at fewer pieces).  The bounds depend only on the graph and the grid,
+// so they are computed by the first launch after either changes and reused
+// by the next ones (ready set); a launcher reports a queued launch with
 // pieces_launched.
-Pieces make_pieces(byo_graph *g, int nblocks);
+enum PieceSet { kTraversalPieces = 0, kPprPieces = 1 };
+Pieces make_pieces(byo_graph *g, int nblocks, PieceSet set = kTraversalPieces);
 // Records that a launch computing pc's bounds has been queued.
-void pieces_launched(byo_graph *g, const Pieces &pc, int nblocks);
+void pieces_launched(byo_graph *g, const Pieces &pc, int nblocks, PieceSet set = kTraversalPieces);
 
 }  // namespace byo

@@ -758,7 +758,7 @@ static void cpma_build(byo_graph *g, const uint64_t *flat, uint64_t m) {
   BYO_LAUNCH_CHECK();
   g->csr_valid = true;
   g->maxdeg_valid = false;
-  g->pieces_valid = false;
+  g->pieces_stale();
   g->rl_valid = false;
   g->xs_valid = false;
 }
@@ -789,7 +789,7 @@ void cpma_ensure_csr(byo_graph *g) {
   BYO_HIP(hipStreamSynchronize(s));
   g->csr_valid = true;
   g->maxdeg_valid = false;
-  g->pieces_valid = false;
+  g->pieces_stale();
   g->rl_valid = false;
   g->xs_valid = false;
 }
@@ -985,7 +985,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
       if (hctl->changed) {
         g->csr_valid = false;
         g->maxdeg_valid = false;
-        g->pieces_valid = false;
+        g->pieces_stale();
         g->rl_valid = false;
         g->xs_valid = false;
       }
@@ -1019,7 +1019,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
     if (hctl->changed) {
       g->csr_valid = false;
       g->maxdeg_valid = false;
-      g->pieces_valid = false;
+      g->pieces_stale();
       g->rl_valid = false;
       g->xs_valid = false;
     }
@@ -1110,7 +1110,7 @@ static uint32_t insert_graph(byo_graph *g, const uint64_t *keys, const uint64_t 
   if (changed) {
     g->csr_valid = false;
     g->maxdeg_valid = false;
-    g->pieces_valid = false;
+    g->pieces_stale();
     g->rl_valid = false;
     g->xs_valid = false;
   }

@@ -257,7 +257,7 @@ void csr_from_keys(byo_graph *g, uint64_t *keys, uint64_t *alt, uint64_t nk, uin
   BYO_HIP(hipStreamSynchronize(s));
   g->csr_valid = true;
   g->maxdeg_valid = false;
-  g->pieces_valid = false;
+  g->pieces_stale();
   g->rl_valid = false;
   g->xs_valid = false;
 }
@@ -273,7 +273,7 @@ void csr_upload(byo_graph *g, const uint64_t *offsets, const uint32_t *ids, uint
   BYO_HIP(hipStreamSynchronize(g->stream));
   g->csr_valid = true;
   g->maxdeg_valid = false;
-  g->pieces_valid = false;
+  g->pieces_stale();
   g->rl_valid = false;
   g->xs_valid = false;
 }

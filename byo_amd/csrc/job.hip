@@ -12,9 +12,10 @@
 // power of two lanes serve one vertex, 64/W vertices per wave step, and
 // each lane sums its seed's contributions in adjacency order (eight
 // gathers in flight per step with the next eight ids behind them, the adds
-// in order), so the result is the same as one thread summing the list.  The vertices of a
-// sweep are shared out by the traversal's weight-balanced word pieces;
-// iterations are separated by the grid barrier.
+// in order), so the result is the same as one thread summing the list.  The
+// vertices of a sweep are shared out by weight-balanced word pieces of their
+// own (knob ppr_pieces; the sweeps balance best at fewer pieces per wave
+// than the traversal); iterations are separated by the grid barrier.
 #include "bfs_batch.hpp"
 #include "kahan.hpp"
 
@@ -106,13 +107,13 @@ __global__ void __launch_bounds__(kCoopBlock) __attribute__((amdgpu_waves_per_eu
   ppr_body(q, bar);
 }
 
-// The batched job: the shared BFS traversal, a barrier, then the PageRank
-// iterations over the same word pieces (computed by the traversal).
+// The batched job: the shared BFS traversal, the PageRank pieces, a
+// barrier, then the PageRank iterations.
 template <class M>
 __global__ void __launch_bounds__(kCoopBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
     k_job(batch::BatchArgs a, PprArgs q) {
   if (a.k) batch::bfs_batch_body<M>(a);
-  else pieces_init(q.pieces, q.off, q.n, (q.n + 63) >> 6, grid_threads() >> 6);
+  if (q.K) pieces_init(q.pieces, q.off, q.n, (q.n + 63) >> 6, grid_threads() >> 6);
   grid_barrier(a.bar, gridDim.x);
   if (q.K) ppr_body(q, a.bar);
 }
@@ -158,11 +159,12 @@ double job_launch(byo_graph *g, const uint32_t *srcs, uint32_t kb, uint64_t maxd
   const int nblocks = coop_grid(kern, kCoopBlock, cap);
   batch::prepare(g, srcs, std::max<uint32_t>(kb, 1), maxdeg, a, (int)sizeof(M), nblocks);
   a.k = kb;
-  q.pieces = a.pieces;  // one set of pieces for both parts
+  q.pieces = make_pieces(g, nblocks, kPprPieces);
   void *args[] = {&a, &q};
   PhaseTimer t(g->stream, true);
   BYO_HIP(hipLaunchCooperativeKernel(kern, dim3(nblocks), dim3(kCoopBlock), args, 0, g->stream));
-  pieces_launched(g, a.pieces, nblocks);
+  if (kb) pieces_launched(g, a.pieces, nblocks);
+  if (q.K) pieces_launched(g, q.pieces, nblocks, kPprPieces);
   return t.stop();
 }
 
@@ -187,13 +189,13 @@ int byo_ppr_multi(byo_graph_t g, const uint32_t *seeds, uint32_t k, double dampi
     PprArgs q = ppr_args(g, seeds, k, damping, iters);
     const void *kern = reinterpret_cast<const void *>(k_ppr_persist);
     const int nblocks = coop_grid(kern, kCoopBlock, (int)g->tuning("ppr_blocks_per_cu"));
-    q.pieces = make_pieces(g, nblocks);
+    q.pieces = make_pieces(g, nblocks, kPprPieces);
     GridBar *bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
     BYO_HIP(hipMemsetAsync(bar, 0, sizeof(GridBar), g->stream));
     void *args[] = {&q, &bar};
     PhaseTimer t(g->stream, true);
     BYO_HIP(hipLaunchCooperativeKernel(kern, dim3(nblocks), dim3(kCoopBlock), args, 0, g->stream));
-    pieces_launched(g, q.pieces, nblocks);
+    pieces_launched(g, q.pieces, nblocks, kPprPieces);
     const double ms = t.stop();
     if (g->prof.on) g->account("ppr", ms);
     if (device_ms) *device_ms = ms;

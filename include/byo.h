@@ -325,14 +325,16 @@ int byo_profile_reset(byo_graph_t g);
  *   coop_blocks_per_cu  1..4  blocks per CU of the persistent traversals (4)
  *   ppr_blocks_per_cu   1..8  blocks per CU of persistent PPR (4)
  *   pieces              1..16 word pieces per wave in vertex passes (8)
+ *   ppr_pieces          1..16 word pieces per wave in the personalized
+ *                       PageRank sweeps (4)
  *   pr_lanes            0,4,8,16,32 PageRank lanes per vertex (0: per
  *                       tile of 64 vertices from its largest degree)
  *   pr_grid             256..65536 PageRank pull grid (4096)
  *   pr_seg_log2         10..40 PageRank source-segment width (40: one)
- *   pr_relabel          0/1 PageRank on a copy of the graph relabelled by
- *                       descending degree, built on the first call after
- *                       each change of the graph; ranks return in the
- *                       original ids (0)
+ *   pr_relabel          0/1 PageRank and PageRankDelta on a copy of the
+ *                       graph relabelled by descending degree, built on
+ *                       the first call after each change of the graph;
+ *                       ranks return in the original ids (0)
  *   pr_xcd              0/1 PageRank pull split into 8 source slices of
  *                       equal edge count, slice x summed by the workgroups
  *                       that run on XCD x (blockIdx % 8), partial sums

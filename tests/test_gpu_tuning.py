@@ -118,6 +118,11 @@ def test_ppr_blocks_knob(B, s15):
         p, _ = G.ppr(seeds, 0.85, 15)
         assert np.array_equal(p, base)  # one lane sums each list in order: identical
     G.set_tuning("ppr_blocks_per_cu", 4)
+    for v in (1, 16):
+        G.set_tuning("ppr_pieces", v)
+        p, _ = G.ppr(seeds, 0.85, 15)
+        assert np.array_equal(p, base)
+    G.set_tuning("ppr_pieces", 4)
     for j, s in enumerate(seeds):
         assert np.abs(base[j] - O.ppr(g, s, 0.85, 15)).sum() <= 1e-6
 
