@@ -503,7 +503,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
           // scanned wave-wide: the word's unfinished vertices lay their
           // edges end to end (exclusive prefix of remaining degrees in LDS,
           // owner found by a 6-step search) and every window of 512 edges
-          // issues its 4 loads per lane before using any of them.  Each
+          // issues its 8 loads per lane before using any of them.  Each
           // window starts at the first vertex that still has bits to find;
           // bits found are merged into the vertex's LDS mask by atomicOr, and
           // any lane holding a frontier neighbour may write that bit's parent
@@ -524,7 +524,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
               t0 = std::max<uint64_t>(t0, __shfl(scl, __builtin_ctzll(inc), 64));
               // loads are unconditional (idle lanes read edge 0 / vertex
               // ids[0]) so that all of a window's loads are in flight at once
-              // the 4 owner searches advance in lockstep so each step's LDS
+              // the 8 owner searches advance in lockstep so each step's LDS
               // reads are issued together
               uint32_t uu[kPullPerLane];
               int xx[kPullPerLane];
