@@ -77,15 +77,17 @@ __global__ void k_tc_orient(const uint64_t *off, const uint32_t *ids, const uint
 }
 
 // N+(u) is held in an LDS hash table per wave (open addressing, linear
-// probing) of S slots, S the smallest power of two >= 2|N+(u)| and >= 64,
+// probing) of S slots, S the smallest power of two >= 4|N+(u)| and >= 64,
 // so a lookup costs about one LDS read where a binary search over the list
-// costs log2 |N+(u)| dependent reads.  Lists longer than kHashCap are
-// searched in global memory instead.
+// costs log2 |N+(u)| dependent reads; the load of at most 1/4 keeps the
+// longest probe of a wave's 64 lanes short, which is what the wave waits
+// for.  Lists longer than kHashCap are searched in global memory instead.
 constexpr int kSlots = 1024;  // hash slots per wave (4 KiB)
-constexpr uint32_t kHashCap = kSlots / 2;
+constexpr uint32_t kHashCap = kSlots / 4;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kTcWaves = 4;
 constexpr int kPairsPerLane = 4;  // pair lookups per lane with loads in flight together
+constexpr int kWin = 1024;        // pairs per owner table (16 per lane)
 
 __device__ __forceinline__ bool find_sorted(const uint32_t *a, uint32_t len, uint32_t x) {
   uint32_t lo = 0, hi = len;
@@ -100,24 +102,12 @@ __device__ __forceinline__ uint32_t slot_of(uint32_t x, uint32_t mask) {
   return (x * 0x9E3779B1u) >> 11 & mask;  // Fibonacci hashing, bits 11..31
 }
 
-// Probe of the wave's table; the table index is direct so every probe is an
-// LDS read (a pointer that may point to LDS or to global memory would make
-// every probe a flat load).
-__device__ __forceinline__ bool in_table(const uint32_t (&tab)[kSlots], uint32_t mask, uint32_t x) {
-  uint32_t h = slot_of(x, mask);
-  for (;;) {
-    const uint32_t y = tab[h];
-    if (y == x) return true;
-    if (y == kEmpty) return false;
-    h = (h + 1) & mask;
-  }
-}
-
 __global__ void __launch_bounds__(256) k_tc_count(const uint64_t *off, const uint64_t *dg_end, const uint32_t *dg_ids,
                                                   uint64_t n,
                                                   unsigned long long *next_u, unsigned long long *total) {
   __shared__ uint32_t tab[kTcWaves][kSlots];
-  __shared__ uint64_t pre[kTcWaves][64];
+  __shared__ int64_t base[kTcWaves][64];
+  __shared__ __align__(16) uint8_t own[kTcWaves][kWin];
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   uint64_t mycount = 0;
   constexpr unsigned kGrab = 32;  // vertices claimed per atomic, one per lane
@@ -139,7 +129,7 @@ __global__ void __launch_bounds__(256) k_tc_count(const uint64_t *off, const uin
       const uint64_t bu = __shfl(myb, l, 64), du = __shfl(myd, l, 64);
       const bool hashed = du <= kHashCap;
       uint32_t S = 64;
-      while (S < 2 * du && S < (uint32_t)kSlots) S <<= 1;
+      while (S < 4 * du && S < (uint32_t)kSlots) S <<= 1;
       const uint32_t mask = S - 1;
       if (hashed) {
         for (uint32_t i = lane; i < S; i += 64) tab[wv][i] = kEmpty;
@@ -168,44 +158,100 @@ __global__ void __launch_bounds__(256) k_tc_count(const uint64_t *off, const uin
           // LDS writes by this wave must land before its lanes read them
           wave_lds_sync();
         }
+        // the chunk's pairs laid end to end: pair x of slot s (x in
+        // [excl_s, incl_s)) reads dg_ids[base[s] + x]
         const uint64_t incl = wave_incl_scan(dv);
-        pre[wv][lane] = incl;
+        const uint64_t excl = incl - dv;
         const uint64_t T = __shfl(incl, 63, 64);
-        const uint64_t bv0 = __shfl(bv, 0, 64);
-        wave_lds_sync();
-        for (uint64_t x0 = 0; x0 < T; x0 += 64 * kPairsPerLane) {
-          // pair x belongs to slot owner = first s with pre[s] > x; the
-          // searches of a lane's pairs advance together, then all their ids
-          // are loaded, then looked up
-          int owner[kPairsPerLane];
-          uint32_t w[kPairsPerLane];
-  #pragma unroll
-          for (int t = 0; t < kPairsPerLane; t++) owner[t] = 0;
-  #pragma unroll
-          for (int st = 32; st > 0; st >>= 1) {
-            uint64_t pv[kPairsPerLane];
-  #pragma unroll
-            for (int t = 0; t < kPairsPerLane; t++) pv[t] = pre[wv][owner[t] + st - 1];
-  #pragma unroll
-            for (int t = 0; t < kPairsPerLane; t++)
-              owner[t] += (pv[t] <= x0 + (uint64_t)t * 64 + lane) ? st : 0;
+        base[wv][lane] = (int64_t)bv - (int64_t)excl;
+        for (uint64_t X = 0; X < T; X += kWin) {
+          // owner table of pairs [X, X + kWin): each slot marks where its run
+          // starts (slot + 1), a prefix maximum fills the rest; the slot that
+          // owns pair X when its run started earlier is the carry-in
+          const uint64_t Xe = std::min<uint64_t>(X + kWin, T);
+          const uint64_t held = __ballot(dv > 0 && excl <= X && X < incl);
+          const uint32_t carry = held ? (uint32_t)__builtin_ctzll(held) + 1 : 0;
+          reinterpret_cast<uint4 *>(own[wv])[lane] = make_uint4(0, 0, 0, 0);
+          wave_lds_sync();
+          if (dv > 0 && excl > X && excl < Xe) own[wv][excl - X] = (uint8_t)(lane + 1);
+          wave_lds_sync();
+          {
+            uint4 q = reinterpret_cast<uint4 *>(own[wv])[lane];
+            uint32_t wd[4] = {q.x, q.y, q.z, q.w};
+            uint32_t run = 0;  // running maximum over this lane's 16 bytes
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+              for (int k = 0; k < 4; k++) run = std::max(run, (wd[i] >> (8 * k)) & 0xFFu);
+            // maximum over the lanes before this one, then the carry-in
+            uint32_t lm = run;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+              const uint32_t t = __shfl_up(lm, o, 64);
+              if (lane >= o) lm = std::max(lm, t);
+            }
+            uint32_t cur = __shfl_up(lm, 1, 64);
+            cur = std::max(lane ? cur : 0u, carry);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+              uint32_t outw = 0;
+#pragma unroll
+              for (int k = 0; k < 4; k++) {
+                cur = std::max(cur, (wd[i] >> (8 * k)) & 0xFFu);
+                outw |= cur << (8 * k);
+              }
+              wd[i] = outw;
+            }
+            reinterpret_cast<uint4 *>(own[wv])[lane] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
           }
-  #pragma unroll
-          for (int t = 0; t < kPairsPerLane; t++) {
-            const uint64_t x = x0 + (uint64_t)t * 64 + lane;
-            const int o = owner[t] < 63 ? owner[t] : 63;
-            const uint64_t startx = o ? pre[wv][o - 1] : 0;
-            const uint64_t obv = __shfl(bv, o, 64);
-            // past the end: reread the wave's first pair
-            w[t] = dg_ids[x < T ? obv + (x - startx) : bv0];
+          wave_lds_sync();
+          for (uint64_t x0 = X; x0 < Xe; x0 += 64 * kPairsPerLane) {
+            uint32_t w[kPairsPerLane];
+#pragma unroll
+            for (int t = 0; t < kPairsPerLane; t++) {
+              const uint64_t x = x0 + (uint64_t)t * 64 + lane;
+              const uint64_t xc = x < Xe ? x : X;  // past the end: reread pair X
+              const int o = (int)own[wv][xc - X] - 1;
+              w[t] = dg_ids[(uint64_t)(base[wv][o] + (int64_t)xc)];
+            }
+            if (hashed) {
+              // the lane's probes advance together, one LDS read each per
+              // step (the table is indexed directly, so no probe becomes a
+              // flat load)
+              uint32_t h[kPairsPerLane];
+              bool live[kPairsPerLane];
+#pragma unroll
+              for (int t = 0; t < kPairsPerLane; t++) {
+                live[t] = x0 + (uint64_t)t * 64 + lane < Xe;
+                h[t] = slot_of(w[t], mask);
+              }
+              for (;;) {
+                bool any = false;
+#pragma unroll
+                for (int t = 0; t < kPairsPerLane; t++) {
+                  if (!live[t]) continue;
+                  const uint32_t y = tab[wv][h[t]];
+                  if (y == w[t]) {
+                    mycount++;
+                    live[t] = false;
+                  } else if (y == kEmpty) {
+                    live[t] = false;
+                  } else {
+                    h[t] = (h[t] + 1) & mask;
+                  }
+                  any |= live[t];
+                }
+                if (!any) break;
+              }
+            } else {
+#pragma unroll
+              for (int t = 0; t < kPairsPerLane; t++)
+                if (x0 + (uint64_t)t * 64 + lane < Xe) mycount += find_sorted(dg_ids + bu, (uint32_t)du, w[t]);
+            }
           }
-  #pragma unroll
-          for (int t = 0; t < kPairsPerLane; t++) {
-            if (x0 + (uint64_t)t * 64 + lane < T)
-              mycount += hashed ? in_table(tab[wv], mask, w[t]) : find_sorted(dg_ids + bu, (uint32_t)du, w[t]);
-          }
+          wave_lds_sync();  // own[] is rewritten by the next window
         }
-        wave_lds_sync();  // pre[] is rewritten by the next chunk
+        wave_lds_sync();  // base[] is rewritten by the next chunk
       }
       wave_lds_sync();  // tab[] is rewritten by the next u
     }
