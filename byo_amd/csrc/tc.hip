@@ -98,17 +98,16 @@ __device__ __forceinline__ bool find_sorted(const uint32_t *a, uint32_t len, uin
   return lo < len && a[lo] == x;
 }
 
-// First slot of x's probe sequence: always even, so the sequence visits
-// the aligned slot pairs (2b, 2b + 1), (2b + 2, 2b + 3), ... and a lookup
-// reads one pair per step.
+// First slot of x's probe sequence: a multiple of 4, so the sequence
+// visits aligned groups of 4 slots and a lookup reads one group per step.
 __device__ __forceinline__ uint32_t slot_of(uint32_t x, uint32_t mask) {
-  return ((x * 0x9E3779B1u) >> 11) << 1 & mask;  // Fibonacci hashing, bits 11..31
+  return ((x * 0x9E3779B1u) >> 11) << 2 & mask;  // Fibonacci hashing, bits 11..31
 }
 
 __global__ void __launch_bounds__(256) k_tc_count(const uint64_t *off, const uint64_t *dg_end, const uint32_t *dg_ids,
                                                   uint64_t n,
                                                   unsigned long long *next_u, unsigned long long *total) {
-  __shared__ __align__(8) uint32_t tab[kTcWaves][kSlots];
+  __shared__ __align__(16) uint32_t tab[kTcWaves][kSlots];
   __shared__ int64_t base[kTcWaves][64];
   __shared__ __align__(16) uint8_t own[kTcWaves][kWin];
   const int lane = lane_id(), wv = threadIdx.x >> 6;
@@ -233,14 +232,14 @@ __global__ void __launch_bounds__(256) k_tc_count(const uint64_t *off, const uin
 #pragma unroll
                 for (int t = 0; t < kPairsPerLane; t++) {
                   if (!live[t]) continue;
-                  const uint2 y = *reinterpret_cast<const uint2 *>(&tab[wv][h[t]]);
-                  if (y.x == w[t] || y.y == w[t]) {
+                  const uint4 y = *reinterpret_cast<const uint4 *>(&tab[wv][h[t]]);
+                  if (y.x == w[t] || y.y == w[t] || y.z == w[t] || y.w == w[t]) {
                     mycount++;
                     live[t] = false;
-                  } else if (y.x == kEmpty || y.y == kEmpty) {
+                  } else if (y.w == kEmpty) {  // slots fill in order, so an empty slot ends the group's run
                     live[t] = false;
                   } else {
-                    h[t] = (h[t] + 2) & mask;
+                    h[t] = (h[t] + 4) & mask;
                   }
                   any |= live[t];
                 }
