@@ -1,8 +1,9 @@
 // tc.hip -- degree-ordered triangle counting (benchmarks/TriangleCounting/
 // ShunTangwongsan15/Triangle.h:57-204).
 //
-// 1. rank by (degree, id) -- rankNodes sorts by degree; any tie order gives
-//    the same count, and (degree, id) makes the orientation deterministic;
+// 1. rank by (min(degree, 65535), id) -- rankNodes sorts by degree; any
+//    strict total order gives the same count, and this one makes the
+//    orientation deterministic;
 // 2. filterGraph(G, rank[u] < rank[v]) (gbbs/interface.h:81-99 ->
 //    graph_mutation.h:15-81), order-preserving, giving the oriented graph
 //    DG (m/2 edges of a symmetric graph, each list sorted); its lists are
@@ -22,14 +23,15 @@
 namespace byo {
 namespace {
 
-// Degree of every vertex as 32 bits (a degree of 2^32 or more is held as
-// 2^32 - 1; (degree, id) is still a strict total order, which is all the
-// orientation needs), so the rank tests gather from a table half the size
-// of the offsets.
-__global__ void k_tc_deg(const uint64_t *off, uint64_t n, uint32_t *deg) {
+// Degree of every vertex as 16 bits, degrees of 65535 or more held as
+// 65535: (that degree, id) is still a strict total order, which is all the
+// orientation needs for a correct count, and it orders by degree every
+// vertex below the cap (vertices above it, among themselves, by id).  The
+// rank tests then gather from a table a quarter the size of the offsets.
+__global__ void k_tc_deg(const uint64_t *off, uint64_t n, uint16_t *deg) {
   for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
     const uint64_t d = off[v + 1] - off[v];
-    deg[v] = d < 0xFFFFFFFFull ? (uint32_t)d : 0xFFFFFFFFu;
+    deg[v] = d < 0xFFFFull ? (uint16_t)d : (uint16_t)0xFFFFu;
   }
 }
 
@@ -41,7 +43,7 @@ constexpr int kOrientU = 4;  // steps of kG edges whose loads are in flight toge
 // dg_end[u] = off[u] + their number.  Every lane of the wave runs the same
 // number of steps (ballots below); the ids and degrees of kOrientU steps are
 // loaded at clamped positions before any is used.
-__global__ void k_tc_orient(const uint64_t *off, const uint32_t *ids, const uint32_t *deg, uint64_t n,
+__global__ void k_tc_orient(const uint64_t *off, const uint32_t *ids, const uint16_t *deg, uint64_t n,
                             uint32_t *dg_ids, uint64_t *dg_end) {
   const int lane = lane_id();
   const int sub = lane & (kG - 1);
@@ -284,7 +286,7 @@ int byo_tc(byo_graph_t g, uint64_t *count, double *device_ms) {
     // DG lives in the CSR's index space: u's kept edges at off[u] ... dg_end[u]
     uint32_t *dg_ids = g->buf("tc_dgids", (g->m + 64) * 4).as<uint32_t>();
     uint64_t *dg_end = g->buf("tc_dgend", n * 8 + 64).as<uint64_t>();
-    uint32_t *deg = g->buf("tc_deg", n * 4 + 64).as<uint32_t>();
+    uint16_t *deg = g->buf("tc_deg", n * 2 + 64).as<uint16_t>();
     unsigned long long *ctr = g->buf("tc_ctr", 64).as<unsigned long long>();
     PhaseTimer t(s, true);
     k_tc_deg<<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, n, deg);
