@@ -36,7 +36,7 @@ __global__ void k_tc_deg(const uint64_t *off, uint64_t n, uint16_t *deg) {
 }
 
 constexpr int kG = 16;      // lanes per vertex in the orientation pass
-constexpr int kOrientU = 4;  // steps of kG edges whose loads are in flight together
+constexpr int kOrientU = 8;  // steps of kG edges whose loads are in flight together
 
 // filterGraph in one pass, in the CSR's own index space: the edges of u that
 // DG keeps are written, in adjacency order, to dg_ids[off[u] ...], and
