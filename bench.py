@@ -286,6 +286,7 @@ def main():
         ex["kcore_ms"] = round(kc_ms, 3)
         ex["kcore_kmax"] = int(kc_kmax)
         ex["kcore_rounds"] = int(kc_rounds)
+        G.tc()
         tri, tc_ms = G.tc()
         ex["tc_ms"] = round(tc_ms, 3)
         ex["triangles"] = int(tri)
