@@ -97,6 +97,7 @@ SIGNATURES = {
     "byo_tc": (_i32, [_vp, _P(_u64), _P(_dbl)]),
     "byo_bc": (_i32, [_vp, _u32, _vp, _P(_dbl)]),
     "byo_kcore": (_i32, [_vp, _vp, _P(_u32), _P(_u64), _P(_dbl)]),
+    "byo_mis": (_i32, [_vp, _vp, _P(_u32), _P(_dbl)]),
     "byo_ldd": (_i32, [_vp, _dbl, _vp, _P(_u32), _P(_dbl)]),
     "byo_pagerank_delta": (_i32, [_vp, _dbl, _dbl, _i32, _vp, _P(_i32), _P(_dbl)]),
     "byo_map_sum": (_i32, [_vp, _P(_u64), _P(_dbl)]),
@@ -437,6 +438,16 @@ class Graph:
         r, ms = _u32(), _dbl()
         check(lib().byo_ldd(self._h, float(beta), _ptr(cl), C.byref(r), C.byref(ms)))
         return cl[:n], r.value, ms.value
+
+    def mis(self):
+        """Maximal independent set (MaximalIndependentSet_rootset): the
+        lexicographically first MIS of the (splitmix64(v) >> 32, v) order;
+        returns (uint8[n] membership, rounds, device ms)."""
+        n = self.N()
+        out = np.empty(max(n, 1), np.uint8)
+        rounds, ms = _u32(), _dbl()
+        check(lib().byo_mis(self._h, _ptr(out), C.byref(rounds), C.byref(ms)))
+        return out[:n], rounds.value, ms.value
 
     def kcore(self):
         """Coreness of every vertex (KCore); returns (uint32[n], k_max,

@@ -285,6 +285,15 @@ int byo_pagerank_delta(byo_graph_t g, double eps, double local_eps, int max_iter
 int byo_ldd(byo_graph_t g, double beta, uint32_t *cluster, uint32_t *rounds_run,
             double *device_ms);
 
+/* Maximal independent set, MaximalIndependentSet_rootset
+ * (benchmarks/MaximalIndependentSet/RandomGreedy/MaximalIndependentSet.h:
+ * 119-170, run_all's "MIS"): in_mis[n] = 1 for members.  The reference
+ * orders vertices by a random permutation; here the order is that of the
+ * keys (high 32 bits of splitmix64(v), v), and the result is the
+ * lexicographically first MIS of that order (deterministic).  *rounds the
+ * rootset rounds run (may be null).  Symmetric graphs, as in the reference. */
+int byo_mis(byo_graph_t g, uint8_t *in_mis, uint32_t *rounds, double *device_ms);
+
 /* Coreness of every vertex, KCore (benchmarks/KCore/JulienneDBS17/
  * KCore.h:8-56, run_all's "KCore"): core[n]; *k_max the largest coreness
  * (the reference prints it), *rounds the peeling rounds run (either may be

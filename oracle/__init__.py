@@ -68,6 +68,8 @@ def lib():
             "orc_map_sum": (u64, [u64, _u32p]),
             "orc_bc": (None, [u64, _u64p, _u32p, u32, _f64p]),
             "orc_kcore": (None, [u64, _u64p, _u32p, _u32p]),
+            "orc_mis_key": (u64, [u32]),
+            "orc_mis": (None, [u64, _u64p, _u32p, _u8p]),
             "orc_pagerank_delta": (i32, [u64, _u64p, _u32p, dbl, dbl, i32, _f64p]),
             "orc_ldd": (None, [u64, _u64p, _u32p, dbl, _u32p, _u32p]),
             "orc_mlp_forward_backward": (dbl, [_f32p, _f32p, _i32p, i32, _f32p, C.c_void_p]),
@@ -262,6 +264,37 @@ def pagerank_delta(g, eps=1e-4, local_eps=0.01, max_iters=20):
     p = np.empty(max(g.n, 1), np.float64)
     r = lib().orc_pagerank_delta(g.n, g.offsets, g.ids, eps, local_eps, max_iters, p)
     return p[:g.n], r
+
+
+def mis_key(v):
+    """Priority key of vertex v in the MIS order: (splitmix64(v) >> 32, v)."""
+    return int(lib().orc_mis_key(int(v)))
+
+
+def mis(g):
+    """Lexicographically first maximal independent set of the mis_key order
+    (MaximalIndependentSet_rootset, MaximalIndependentSet.h:119-170);
+    uint8[n] membership."""
+    out = np.empty(max(g.n, 1), np.uint8)
+    lib().orc_mis(g.n, g.offsets, g.ids, out)
+    return out[:g.n]
+
+
+def mis_check(g, in_mis):
+    """Number of violations of the lexicographically-first-MIS
+    characterisation in the mis_key order (0 for the set orc_mis defines):
+    a member with a member neighbour of smaller key, or a non-member with no
+    member neighbour of smaller key."""
+    n = g.n
+    key = np.array([mis_key(v) for v in range(n)], np.uint64)
+    src = np.repeat(np.arange(n), np.diff(g.offsets.astype(np.int64)))
+    dst = g.ids.astype(np.int64)
+    earlier_in = (key[dst] < key[src]) & (in_mis[dst] == 1)
+    has = np.zeros(n, bool)
+    np.logical_or.at(has, src[earlier_in], True)
+    bad_in = (in_mis == 1) & has
+    bad_out = (in_mis == 0) & ~has
+    return int(bad_in.sum() + bad_out.sum())
 
 
 def kcore(g):

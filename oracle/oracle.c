@@ -25,6 +25,10 @@
  *   orc_bc            : brute-force dependencies from all-pairs shortest-path
  *       counts (textbook definition) on small graphs; star and path closed
  *       forms; the source's own value = reached - 1.
+ *   orc_mis           : test_mis.cc cases; the characterisation of the
+ *       lexicographically first MIS checked directly (independent; every
+ *       vertex outside has a neighbour inside of smaller key; every vertex
+ *       inside has none of smaller key inside) -- it fixes the set.
  */
 #include "oracle.h"
 
@@ -992,4 +996,38 @@ double orc_mlp_forward_backward(const float *P, const float *x,
   if (G)
     for (int k = 0; k < ORC_MLP_NPARAM; k++) G[k] = (float)g[k];
   return loss / batch;
+}
+
+/* ===================================================================== */
+/* MaximalIndependentSet (rootset)                                       */
+/* ===================================================================== */
+
+/* Priority key of vertex v: the reference draws a random permutation
+ * (parlay::random_permutation, MaximalIndependentSet.h:127); here the order
+ * is that of (high 32 bits of splitmix64(v), v), a fixed pseudo-random
+ * permutation that the device uses too. */
+uint64_t orc_mis_key(uint32_t v) {
+  return ((orc_splitmix64(v) >> 32) << 32) | (uint64_t)v;
+}
+
+/* MaximalIndependentSet_rootset::MaximalIndependentSet
+ * (MaximalIndependentSet.h:119-170) computes the lexicographically first
+ * MIS of the priority order: the rootset rounds add exactly the vertices
+ * that sequential greedy in that order adds.  Restated as the sequential
+ * greedy: vertices in increasing key order join unless a neighbour already
+ * has.  in_mis[n] = 0/1. */
+void orc_mis(uint64_t n, const uint64_t *off, const uint32_t *ids, uint8_t *in_mis) {
+  if (n == 0) return;
+  uint64_t *key = (uint64_t *)malloc(n * sizeof(uint64_t));
+  for (uint64_t v = 0; v < n; v++) key[v] = orc_mis_key((uint32_t)v);
+  /* the low 32 bits of a key are the vertex id */
+  radix_sort_u64(key, n);
+  memset(in_mis, 0, n);
+  for (uint64_t i = 0; i < n; i++) {
+    const uint32_t v = (uint32_t)(key[i] & 0xFFFFFFFFull);
+    int blocked = 0;
+    for (uint64_t e = off[v]; e < off[v + 1] && !blocked; e++) blocked = in_mis[ids[e]];
+    in_mis[v] = (uint8_t)!blocked;
+  }
+  free(key);
 }

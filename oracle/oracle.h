@@ -123,6 +123,10 @@ void orc_orient(uint64_t n, const uint64_t *off, const uint32_t *ids,
 /* Test_Map (Map/Base/Map.h:8-18): sum of dst over all CSR entries. */
 void orc_bc(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t src, double *out);
 void orc_kcore(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t *core);
+/* MaximalIndependentSet_rootset (MaximalIndependentSet.h:119-170) over the
+ * priority order of orc_mis_key: the lexicographically first MIS. */
+uint64_t orc_mis_key(uint32_t v);
+void orc_mis(uint64_t n, const uint64_t *off, const uint32_t *ids, uint8_t *in_mis);
 int orc_pagerank_delta(uint64_t n, const uint64_t *off, const uint32_t *ids, double eps, double local_eps,
                        int max_iters, double *p);
 void orc_ldd(uint64_t n, const uint64_t *off, const uint32_t *ids, double beta, uint32_t *cluster, uint32_t *round);

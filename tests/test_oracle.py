@@ -459,6 +459,52 @@ def test_kcore_matches_definition(which):
     assert np.array_equal(O.kcore(g), O.kcore_brute(g))
 
 
+def test_mis_reference_unit_cases():
+    """test_mis.cc: every vertex of the edgeless graph is in the set; the
+    8-vertex BasicUsage graph gets a valid MIS (CheckMIS: independent, and
+    every vertex outside has a neighbour inside)."""
+    assert list(O.mis(O.sym_csr(3, np.zeros(0, np.uint32), np.zeros(0, np.uint32)))) == [1, 1, 1]
+    e = np.array([[0, 1], [2, 3], [3, 4], [3, 5], [4, 5], [5, 6]], np.uint32)
+    g = O.sym_csr(8, e[:, 0], e[:, 1])
+    m = O.mis(g)
+    adj = [set(int(x) for x in g.ids[g.offsets[v]:g.offsets[v + 1]]) for v in range(8)]
+    assert all(not (m[u] and m[v]) for u in range(8) for v in adj[u])
+    assert all(m[v] or any(m[u] for u in adj[v]) for v in range(8))
+    assert m[7] == 1  # isolated
+
+
+@pytest.mark.parametrize("which", ["rmat", "grid", "er", "inputs"])
+def test_mis_is_lexicographically_first(which):
+    """The set satisfies the characterisation of the lexicographically first
+    MIS of the key order, which determines it uniquely; a vertex moved in or
+    out is caught."""
+    if which == "rmat":
+        g = O.rmat_graph(10, 8)
+    elif which == "grid":
+        g = O.grid3d_graph(8)
+    elif which == "er":
+        g = O.er_graph(9, 1200, seed=6)
+    else:
+        g = O.read_adjacency_graph(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "inputs",
+                                                "rMatGraph_J_5_100"))
+    m = O.mis(g)
+    assert O.mis_check(g, m) == 0
+    deg = np.diff(g.offsets.astype(np.int64))
+    v = int(np.nonzero((m == 0) & (deg > 0))[0][0])
+    bad = m.copy()
+    bad[v] = 1
+    assert O.mis_check(g, bad) > 0
+    bad = m.copy()
+    bad[int(np.nonzero(m)[0][0])] = 0
+    assert O.mis_check(g, bad) > 0
+
+
+def test_mis_key_is_splitmix():
+    # known answer: splitmix64(0) = 0xE220A8397B1DCDAF, so key(0) = 0xE220A839 << 32
+    assert O.mis_key(0) == 0xE220A839 << 32
+    assert O.mis_key(5) & 0xFFFFFFFF == 5
+
+
 def test_pagerank_delta_pins():
     """PageRankDelta (PageRank.h:335-396).  One round is the first Jacobi
     iterate; with every vertex active (local eps -inf) r rounds are r Jacobi
