@@ -14,9 +14,9 @@
 // (high 32 bits of splitmix64(v), v), so the set is deterministic.
 //
 // One cooperative launch: the count pass, then per round a remove phase and
-// a decrement phase, each a grid barrier apart.  A wave takes one queued
-// vertex at a time and its list 64 edges per step; the vertices it adds to
-// a queue are appended with one counter atomic per wave and step.  Counters
+// a decrement phase, each a grid barrier apart.  A wave takes four queued
+// vertices at a time, 16 lanes each walking one list; the vertices it adds
+// to a queue are appended with one counter atomic per wave and step.  Counters
 // rotate over three slots, the one two phases ahead cleared by the leader.
 #include <vector>
 
@@ -26,6 +26,10 @@ namespace byo {
 namespace {
 
 constexpr uint32_t kUndecided = 0, kIn = 1, kOut = 2;
+// Lanes per queued vertex in a round's phases, and queued vertices per wave
+// step (most lists are short: a wave per vertex would leave most lanes idle).
+constexpr int kMisLanes = 16;
+constexpr int kMisVpw = 64 / kMisLanes;
 
 struct MisCtl {
   unsigned int cnt[3];  // vertices appended in a phase (rotating)
@@ -68,6 +72,7 @@ __device__ __forceinline__ void wave_append(bool take, uint32_t v, uint32_t *q, 
 
 __global__ void __launch_bounds__(kCoopBlock) k_mis_persist(MisArgs a) {
   const int lane = lane_id();
+  const int msub = lane & (kMisLanes - 1), mgrp = lane / kMisLanes;
   const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
   const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
   const unsigned nb = gridDim.x;
@@ -99,11 +104,17 @@ __global__ void __launch_bounds__(kCoopBlock) k_mis_persist(MisArgs a) {
   while (roots) {
     decided += roots;
     // remove the roots' undecided neighbours
-    for (uint64_t i = wave; i < roots; i += nwaves) {
-      const uint32_t r = q[i];
-      const uint64_t e0 = a.off[r], e1 = a.off[r + 1];
-      for (uint64_t c0 = e0; c0 < e1; c0 += 64) {
-        const uint64_t e = c0 + lane;
+    for (uint64_t i0 = wave * kMisVpw; i0 < roots; i0 += nwaves * kMisVpw) {
+      const uint64_t i = i0 + mgrp;
+      uint64_t e0 = 0, e1 = 0;
+      if (i < roots) {
+        const uint32_t r = q[i];
+        e0 = a.off[r];
+        e1 = a.off[r + 1];
+      }
+      const uint64_t steps = wave_max((e1 - e0 + kMisLanes - 1) / kMisLanes);
+      for (uint64_t st = 0; st < steps; st++) {
+        const uint64_t e = e0 + st * kMisLanes + msub;
         bool took = false;
         uint32_t d = 0;
         if (e < e1) {
@@ -118,12 +129,18 @@ __global__ void __launch_bounds__(kCoopBlock) k_mis_persist(MisArgs a) {
     decided += removed;
     // every removed vertex releases its undecided neighbours after it; the
     // decrement that reaches zero makes the neighbour a root
-    for (uint64_t i = wave; i < removed; i += nwaves) {
-      const uint32_t s = a.qr[i];
-      const uint64_t ks = mis_key(s);
-      const uint64_t e0 = a.off[s], e1 = a.off[s + 1];
-      for (uint64_t c0 = e0; c0 < e1; c0 += 64) {
-        const uint64_t e = c0 + lane;
+    for (uint64_t i0 = wave * kMisVpw; i0 < removed; i0 += nwaves * kMisVpw) {
+      const uint64_t i = i0 + mgrp;
+      uint64_t e0 = 0, e1 = 0, ks = 0;
+      if (i < removed) {
+        const uint32_t sv = a.qr[i];
+        ks = mis_key(sv);
+        e0 = a.off[sv];
+        e1 = a.off[sv + 1];
+      }
+      const uint64_t steps = wave_max((e1 - e0 + kMisLanes - 1) / kMisLanes);
+      for (uint64_t st = 0; st < steps; st++) {
+        const uint64_t e = e0 + st * kMisLanes + msub;
         bool root = false;
         uint32_t d = 0;
         if (e < e1) {
