@@ -281,6 +281,11 @@ def main():
         ldd_cl, ldd_rounds, ldd_ms = G.ldd(0.2)
         ex["ldd_ms"] = round(ldd_ms, 3)
         ex["ldd_clusters"] = int(len(np.unique(ldd_cl)))
+        G.mis()
+        mis_in, mis_rounds, mis_ms = G.mis()
+        ex["mis_ms"] = round(mis_ms, 3)
+        ex["mis_size"] = int(mis_in.sum())
+        ex["mis_rounds"] = int(mis_rounds)
         G.kcore()
         _, kc_kmax, kc_rounds, kc_ms = G.kcore()
         ex["kcore_ms"] = round(kc_ms, 3)
