@@ -286,6 +286,11 @@ def main():
         ex["mis_ms"] = round(mis_ms, 3)
         ex["mis_size"] = int(mis_in.sum())
         ex["mis_rounds"] = int(mis_rounds)
+        G.coloring()
+        col, col_rounds, col_ms = G.coloring()
+        ex["coloring_ms"] = round(col_ms, 3)
+        ex["coloring_colours"] = int(col.max()) + 1 if len(col) else 0
+        ex["coloring_rounds"] = int(col_rounds)
         G.kcore()
         _, kc_kmax, kc_rounds, kc_ms = G.kcore()
         ex["kcore_ms"] = round(kc_ms, 3)

@@ -98,6 +98,7 @@ SIGNATURES = {
     "byo_bc": (_i32, [_vp, _u32, _vp, _P(_dbl)]),
     "byo_kcore": (_i32, [_vp, _vp, _P(_u32), _P(_u64), _P(_dbl)]),
     "byo_mis": (_i32, [_vp, _vp, _P(_u32), _P(_dbl)]),
+    "byo_coloring": (_i32, [_vp, _vp, _P(_u32), _P(_dbl)]),
     "byo_ldd": (_i32, [_vp, _dbl, _vp, _P(_u32), _P(_dbl)]),
     "byo_pagerank_delta": (_i32, [_vp, _dbl, _dbl, _i32, _vp, _P(_i32), _P(_dbl)]),
     "byo_map_sum": (_i32, [_vp, _P(_u64), _P(_dbl)]),
@@ -447,6 +448,15 @@ class Graph:
         out = np.empty(max(n, 1), np.uint8)
         rounds, ms = _u32(), _dbl()
         check(lib().byo_mis(self._h, _ptr(out), C.byref(rounds), C.byref(ms)))
+        return out[:n], rounds.value, ms.value
+
+    def coloring(self):
+        """Greedy colouring in LLF order (Coloring, lf false); returns
+        (uint32[n] colours, rounds, device ms)."""
+        n = self.N()
+        out = np.empty(max(n, 1), np.uint32)
+        rounds, ms = _u32(), _dbl()
+        check(lib().byo_coloring(self._h, _ptr(out), C.byref(rounds), C.byref(ms)))
         return out[:n], rounds.value, ms.value
 
     def kcore(self):

@@ -294,6 +294,16 @@ int byo_ldd(byo_graph_t g, double beta, uint32_t *cluster, uint32_t *rounds_run,
  * rootset rounds run (may be null).  Symmetric graphs, as in the reference. */
 int byo_mis(byo_graph_t g, uint8_t *in_mis, uint32_t *rounds, double *device_ms);
 
+/* Greedy colouring, Coloring (benchmarks/GraphColoring/Hasenplaugh14/
+ * GraphColoring.h:71-146, run_all's "Coloring", its default LLF order):
+ * color[n].  Vertices are coloured in the order larger log2_up(degree)
+ * first, ties by the keys of byo_mis (the reference breaks them by a random
+ * permutation), each taking coloring::color's value: the smallest colour
+ * in [0, deg) no neighbour holds, deg + 1 when all are taken, 0 when
+ * isolated.  *rounds the rootset rounds run (may be null).  Symmetric
+ * graphs, as in the reference. */
+int byo_coloring(byo_graph_t g, uint32_t *color, uint32_t *rounds, double *device_ms);
+
 /* Coreness of every vertex, KCore (benchmarks/KCore/JulienneDBS17/
  * KCore.h:8-56, run_all's "KCore"): core[n]; *k_max the largest coreness
  * (the reference prints it), *rounds the peeling rounds run (either may be

@@ -70,6 +70,8 @@ def lib():
             "orc_kcore": (None, [u64, _u64p, _u32p, _u32p]),
             "orc_mis_key": (u64, [u32]),
             "orc_mis": (None, [u64, _u64p, _u32p, _u8p]),
+            "orc_coloring": (None, [u64, _u64p, _u32p, _u32p]),
+            "orc_llf_before": (i32, [_u64p, u32, u32]),
             "orc_pagerank_delta": (i32, [u64, _u64p, _u32p, dbl, dbl, i32, _f64p]),
             "orc_ldd": (None, [u64, _u64p, _u32p, dbl, _u32p, _u32p]),
             "orc_mlp_forward_backward": (dbl, [_f32p, _f32p, _i32p, i32, _f32p, C.c_void_p]),
@@ -295,6 +297,35 @@ def mis_check(g, in_mis):
     bad_in = (in_mis == 1) & has
     bad_out = (in_mis == 0) & ~has
     return int(bad_in.sum() + bad_out.sum())
+
+
+def coloring(g):
+    """Greedy colouring in the LLF order (Coloring, GraphColoring.h:71-146)
+    with coloring::color's rule; uint32[n]."""
+    out = np.empty(max(g.n, 1), np.uint32)
+    lib().orc_coloring(g.n, g.offsets, g.ids, out)
+    return out[:g.n]
+
+
+def coloring_check(g, color):
+    """Violations of the colouring's definition: an edge whose ends share a
+    colour, or a vertex whose colour is not the rule (smallest colour in
+    [0, deg) unused by the neighbours before it in LLF order, deg + 1 when
+    none is free, 0 when isolated) applied to those neighbours."""
+    bad = 0
+    for v in range(g.n):
+        nb = g.ids[g.offsets[v]:g.offsets[v + 1]]
+        deg = len(nb)
+        if deg == 0:
+            bad += color[v] != 0
+            continue
+        bad += int(np.sum(color[nb] == color[v]))
+        used = {int(color[u]) for u in nb if lib().orc_llf_before(g.offsets, int(u), v)}
+        c = 0
+        while c < deg and c in used:
+            c += 1
+        bad += color[v] != (c if c < deg else deg + 1)
+    return int(bad)
 
 
 def kcore(g):

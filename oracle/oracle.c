@@ -25,6 +25,9 @@
  *   orc_bc            : brute-force dependencies from all-pairs shortest-path
  *       counts (textbook definition) on small graphs; star and path closed
  *       forms; the source's own value = reached - 1.
+ *   orc_coloring      : a proper colouring, and every colour equals the
+ *       rule applied to the neighbours earlier in the LLF order (checked
+ *       directly, which fixes the colouring); hand-worked triangle and star.
  *   orc_mis           : test_mis.cc cases; the characterisation of the
  *       lexicographically first MIS checked directly (independent; every
  *       vertex outside has a neighbour inside of smaller key; every vertex
@@ -1031,3 +1034,81 @@ void orc_mis(uint64_t n, const uint64_t *off, const uint32_t *ids, uint8_t *in_m
   }
   free(key);
 }
+
+/* ===================================================================== */
+/* Coloring (Hasenplaugh14, LLF)                                         */
+/* ===================================================================== */
+
+static uint32_t log2_up_u64(uint64_t x) { /* parlay::log2_up: ceil(log2 x), 0 for x <= 1 */
+  uint32_t r = 0;
+  while (x > ((uint64_t)1 << r)) r++;
+  return r;
+}
+
+/* u comes before v in the LLF order of Coloring (GraphColoring.h:97-112):
+ * larger log2_up(degree) first, ties by the permutation, which here is the
+ * MIS key order (orc_mis_key). */
+static int llf_before(const uint64_t *off, uint32_t u, uint32_t v) {
+  const uint64_t du = off[u + 1] - off[u], dv = off[v + 1] - off[v];
+  const uint32_t lu = du ? log2_up_u64(du) : 0, lv = dv ? log2_up_u64(dv) : 0;
+  return lu > lv || (lu == lv && orc_mis_key(u) < orc_mis_key(v));
+}
+
+/* Coloring(G, lf = false) (GraphColoring.h:71-146) colours the rootset of
+ * the LLF order round by round; a root's colour is coloring::color
+ * (:30-55): the smallest colour in [0, deg) no neighbour holds, or deg + 1
+ * when all of [0, deg) are taken (the reference's value, kept), 0 for an
+ * isolated vertex.  A root's coloured neighbours are exactly the ones
+ * before it, so this is greedy colouring in LLF order, restated
+ * sequentially. */
+void orc_coloring(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t *color) {
+  if (n == 0) return;
+  /* LLF order: sort by permutation key, then a stable counting sort by
+   * 63 - log2_up(degree) */
+  uint64_t *key = (uint64_t *)malloc(n * sizeof(uint64_t));
+  for (uint64_t v = 0; v < n; v++) key[v] = orc_mis_key((uint32_t)v);
+  radix_sort_u64(key, n); /* by permutation key */
+  uint32_t *order = (uint32_t *)malloc(n * sizeof(uint32_t));
+  uint64_t cntl[66] = {0};
+  for (uint64_t v = 0; v < n; v++) {
+    const uint64_t d = off[v + 1] - off[v];
+    cntl[63 - (d ? log2_up_u64(d) : 0)]++;
+  }
+  uint64_t start[66], acc = 0;
+  for (int l = 0; l < 66; l++) {
+    start[l] = acc;
+    acc += cntl[l];
+  }
+  for (uint64_t i = 0; i < n; i++) { /* stable counting sort by level */
+    const uint32_t v = (uint32_t)(key[i] & 0xFFFFFFFFull);
+    const uint64_t d = off[v + 1] - off[v];
+    order[start[63 - (d ? log2_up_u64(d) : 0)]++] = v;
+  }
+  free(key);
+  for (uint64_t v = 0; v < n; v++) color[v] = 0xFFFFFFFFu;
+  uint8_t *used = (uint8_t *)calloc(n + 2, 1);
+  for (uint64_t i = 0; i < n; i++) {
+    const uint32_t v = order[i];
+    const uint64_t deg = off[v + 1] - off[v];
+    if (deg == 0) {
+      color[v] = 0;
+      continue;
+    }
+    for (uint64_t e = off[v]; e < off[v + 1]; e++) {
+      const uint32_t c = color[ids[e]];
+      if (c < deg) used[c] = 1;
+    }
+    uint64_t c = 0;
+    while (c < deg && used[c]) c++;
+    color[v] = (uint32_t)(c < deg ? c : deg + 1);
+    for (uint64_t e = off[v]; e < off[v + 1]; e++) {
+      const uint32_t cc = color[ids[e]];
+      if (cc < deg) used[cc] = 0;
+    }
+  }
+  free(used);
+  free(order);
+}
+
+/* 1 where u comes before v in the LLF order (for the tests). */
+int orc_llf_before(const uint64_t *off, uint32_t u, uint32_t v) { return llf_before(off, u, v); }

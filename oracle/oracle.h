@@ -127,6 +127,10 @@ void orc_kcore(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t *c
  * priority order of orc_mis_key: the lexicographically first MIS. */
 uint64_t orc_mis_key(uint32_t v);
 void orc_mis(uint64_t n, const uint64_t *off, const uint32_t *ids, uint8_t *in_mis);
+/* Coloring(G, lf = false) (GraphColoring/Hasenplaugh14/GraphColoring.h:
+ * 71-146): greedy colouring in the LLF order with coloring::color's rule. */
+void orc_coloring(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t *color);
+int orc_llf_before(const uint64_t *off, uint32_t u, uint32_t v);
 int orc_pagerank_delta(uint64_t n, const uint64_t *off, const uint32_t *ids, double eps, double local_eps,
                        int max_iters, double *p);
 void orc_ldd(uint64_t n, const uint64_t *off, const uint32_t *ids, double beta, uint32_t *cluster, uint32_t *round);

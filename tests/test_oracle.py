@@ -499,6 +499,39 @@ def test_mis_is_lexicographically_first(which):
     assert O.mis_check(g, bad) > 0
 
 
+def test_coloring_hand_cases():
+    """coloring::color (GraphColoring.h:30-55) on graphs worked by hand: on a
+    triangle (all of one degree level) the vertices in key order take 0, 1
+    and then deg + 1 = 3, as all of [0, 2) is taken; on a star the centre
+    (higher log-degree, first) takes 0 and every leaf, whose one colour slot
+    [0, 1) is taken, takes 2; isolated vertices take 0."""
+    t = O.sym_csr(4, np.array([0, 1, 0], np.uint32), np.array([1, 2, 2], np.uint32))
+    order = sorted(range(3), key=O.mis_key)
+    c = O.coloring(t)
+    assert [int(c[v]) for v in order] == [0, 1, 3] and c[3] == 0
+    star = O.sym_csr(6, np.zeros(5, np.uint32), np.arange(1, 6, dtype=np.uint32))
+    assert list(O.coloring(star)) == [0, 2, 2, 2, 2, 2]
+
+
+@pytest.mark.parametrize("which", ["rmat", "grid", "er", "inputs"])
+def test_coloring_matches_definition(which):
+    if which == "rmat":
+        g = O.rmat_graph(9, 8)
+    elif which == "grid":
+        g = O.grid3d_graph(6)
+    elif which == "er":
+        g = O.er_graph(8, 700, seed=2)
+    else:
+        g = O.read_adjacency_graph(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "inputs",
+                                                "rMatGraph_J_5_100"))
+    c = O.coloring(g)
+    assert O.coloring_check(g, c) == 0
+    bad = c.copy()
+    v = int(np.argmax(np.diff(g.offsets.astype(np.int64))))
+    bad[v] += 1
+    assert O.coloring_check(g, bad) > 0
+
+
 def test_mis_key_is_splitmix():
     # known answer: splitmix64(0) = 0xE220A8397B1DCDAF, so key(0) = 0xE220A839 << 32
     assert O.mis_key(0) == 0xE220A839 << 32
