@@ -286,6 +286,11 @@ def main():
         ex["mis_ms"] = round(mis_ms, 3)
         ex["mis_size"] = int(mis_in.sum())
         ex["mis_rounds"] = int(mis_rounds)
+        G.densest_subgraph()
+        ds_d, ds_rounds, ds_ms = G.densest_subgraph()  # run_all's eps 0.001
+        ex["densest_ms"] = round(ds_ms, 3)
+        ex["densest_density"] = round(ds_d, 6)
+        ex["densest_rounds"] = int(ds_rounds)
         G.coloring()
         col, col_rounds, col_ms = G.coloring()
         ex["coloring_ms"] = round(col_ms, 3)

@@ -99,6 +99,7 @@ SIGNATURES = {
     "byo_kcore": (_i32, [_vp, _vp, _P(_u32), _P(_u64), _P(_dbl)]),
     "byo_mis": (_i32, [_vp, _vp, _P(_u32), _P(_dbl)]),
     "byo_coloring": (_i32, [_vp, _vp, _P(_u32), _P(_dbl)]),
+    "byo_densest_subgraph": (_i32, [_vp, _dbl, _P(_dbl), _P(_u32), _P(_dbl)]),
     "byo_ldd": (_i32, [_vp, _dbl, _vp, _P(_u32), _P(_dbl)]),
     "byo_pagerank_delta": (_i32, [_vp, _dbl, _dbl, _i32, _vp, _P(_i32), _P(_dbl)]),
     "byo_map_sum": (_i32, [_vp, _P(_u64), _P(_dbl)]),
@@ -449,6 +450,13 @@ class Graph:
         rounds, ms = _u32(), _dbl()
         check(lib().byo_mis(self._h, _ptr(out), C.byref(rounds), C.byref(ms)))
         return out[:n], rounds.value, ms.value
+
+    def densest_subgraph(self, eps=0.001):
+        """WorkEfficientDensestSubgraph (ApproxPeelingBKV12); returns
+        (largest density over the peeling rounds, rounds, device ms)."""
+        d, rounds, ms = _dbl(), _u32(), _dbl()
+        check(lib().byo_densest_subgraph(self._h, float(eps), C.byref(d), C.byref(rounds), C.byref(ms)))
+        return d.value, rounds.value, ms.value
 
     def coloring(self):
         """Greedy colouring in LLF order (Coloring, lf false); returns

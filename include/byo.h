@@ -294,6 +294,14 @@ int byo_ldd(byo_graph_t g, double beta, uint32_t *cluster, uint32_t *rounds_run,
  * rootset rounds run (may be null).  Symmetric graphs, as in the reference. */
 int byo_mis(byo_graph_t g, uint8_t *in_mis, uint32_t *rounds, double *device_ms);
 
+/* Approximate densest subgraph by peeling, WorkEfficientDensestSubgraph
+ * (benchmarks/ApproximateDensestSubgraph/ApproxPeelingBKV12/
+ * DensestSubgraph.h:8-145, run_all's default eps 0.001): *density = the
+ * largest sum_R D / |R| over the peeling rounds (both directions of an edge
+ * counted, as the reference prints it), *rounds the rounds run (may be
+ * null).  Symmetric graphs. */
+int byo_densest_subgraph(byo_graph_t g, double eps, double *density, uint32_t *rounds, double *device_ms);
+
 /* Greedy colouring, Coloring (benchmarks/GraphColoring/Hasenplaugh14/
  * GraphColoring.h:71-146, run_all's "Coloring", its default LLF order):
  * color[n].  Vertices are coloured in the order larger log2_up(degree)

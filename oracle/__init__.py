@@ -72,6 +72,7 @@ def lib():
             "orc_mis": (None, [u64, _u64p, _u32p, _u8p]),
             "orc_coloring": (None, [u64, _u64p, _u32p, _u32p]),
             "orc_llf_before": (i32, [_u64p, u32, u32]),
+            "orc_densest": (dbl, [u64, _u64p, _u32p, dbl, C.POINTER(C.c_uint32)]),
             "orc_pagerank_delta": (i32, [u64, _u64p, _u32p, dbl, dbl, i32, _f64p]),
             "orc_ldd": (None, [u64, _u64p, _u32p, dbl, _u32p, _u32p]),
             "orc_mlp_forward_backward": (dbl, [_f32p, _f32p, _i32p, i32, _f32p, C.c_void_p]),
@@ -326,6 +327,27 @@ def coloring_check(g, color):
             c += 1
         bad += color[v] != (c if c < deg else deg + 1)
     return int(bad)
+
+
+def densest(g, eps=0.001):
+    """WorkEfficientDensestSubgraph (DensestSubgraph.h:8-145); returns
+    (largest density over the peeling rounds, rounds)."""
+    r = C.c_uint32()
+    d = lib().orc_densest(g.n, g.offsets, g.ids, eps, C.byref(r))
+    return d, r.value
+
+
+def densest_brute(g):
+    """Largest 2|E(S)|/|S| over all nonempty vertex subsets S (n <= 16)."""
+    n = g.n
+    best = 0.0
+    src = np.repeat(np.arange(n), np.diff(g.offsets.astype(np.int64)))
+    dst = g.ids.astype(np.int64)
+    for mask in range(1, 1 << n):
+        ins = np.array([(mask >> v) & 1 for v in range(n)], bool)
+        e = int(np.sum(ins[src] & ins[dst]))
+        best = max(best, e / int(ins.sum()))
+    return best
 
 
 def kcore(g):

@@ -25,6 +25,9 @@
  *   orc_bc            : brute-force dependencies from all-pairs shortest-path
  *       counts (textbook definition) on small graphs; star and path closed
  *       forms; the source's own value = reached - 1.
+ *   orc_densest       : closed forms (clique k - 1, star 2n/(n+1)); brute
+ *       force over all subsets of small graphs: the result is the density
+ *       of a subset (<= the optimum) and within 2(1+eps) of the optimum.
  *   orc_coloring      : a proper colouring, and every colour equals the
  *       rule applied to the neighbours earlier in the LLF order (checked
  *       directly, which fixes the colouring); hand-worked triangle and star.
@@ -1112,3 +1115,55 @@ void orc_coloring(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t
 
 /* 1 where u comes before v in the LLF order (for the tests). */
 int orc_llf_before(const uint64_t *off, uint32_t u, uint32_t v) { return llf_before(off, u, v); }
+
+/* ===================================================================== */
+/* WorkEfficientDensestSubgraph (ApproxPeelingBKV12)                     */
+/* ===================================================================== */
+
+/* WorkEfficientDensestSubgraph (ApproximateDensestSubgraph/ApproxPeelingBKV12/
+ * DensestSubgraph.h:8-145): rounds over the alive set R with D[v] = alive
+ * neighbours of v; density = sum_R D / |R| (both directions of an edge
+ * counted), the maximum over rounds returned; every alive vertex with
+ * D[v] <= (1 + eps) * density is removed and its alive neighbours lose one
+ * per removed neighbour.  Returns the maximum density; *rounds the rounds
+ * run. */
+double orc_densest(uint64_t n, const uint64_t *off, const uint32_t *ids, double eps, uint32_t *rounds) {
+  uint32_t r = 0;
+  double best = 0.0;
+  if (n == 0) {
+    if (rounds) *rounds = 0;
+    return 0.0;
+  }
+  uint64_t *D = (uint64_t *)malloc(n * sizeof(uint64_t));
+  uint8_t *alive = (uint8_t *)malloc(n);
+  uint32_t *rem = (uint32_t *)malloc(n * sizeof(uint32_t));
+  for (uint64_t v = 0; v < n; v++) {
+    D[v] = off[v + 1] - off[v];
+    alive[v] = 1;
+  }
+  uint64_t left = n;
+  while (left > 0) {
+    uint64_t edges = 0;
+    for (uint64_t v = 0; v < n; v++)
+      if (alive[v]) edges += D[v];
+    const double density = (double)edges / (double)left;
+    const double target = ((1.0 + eps) * (double)edges) / (double)left;
+    if (density > best) best = density;
+    uint64_t k = 0;
+    for (uint64_t v = 0; v < n; v++)
+      if (alive[v] && (double)D[v] <= target) rem[k++] = (uint32_t)v;
+    for (uint64_t i = 0; i < k; i++) alive[rem[i]] = 0;
+    left -= k;
+    for (uint64_t i = 0; i < k; i++) {
+      const uint32_t u = rem[i];
+      for (uint64_t e = off[u]; e < off[u + 1]; e++)
+        if (alive[ids[e]]) D[ids[e]]--;
+    }
+    r++;
+  }
+  free(D);
+  free(alive);
+  free(rem);
+  if (rounds) *rounds = r;
+  return best;
+}

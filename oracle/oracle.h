@@ -131,6 +131,9 @@ void orc_mis(uint64_t n, const uint64_t *off, const uint32_t *ids, uint8_t *in_m
  * 71-146): greedy colouring in the LLF order with coloring::color's rule. */
 void orc_coloring(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t *color);
 int orc_llf_before(const uint64_t *off, uint32_t u, uint32_t v);
+/* WorkEfficientDensestSubgraph (ApproxPeelingBKV12/DensestSubgraph.h:8-145):
+ * the largest density (2|E(R)|/|R|) over the peeling rounds. */
+double orc_densest(uint64_t n, const uint64_t *off, const uint32_t *ids, double eps, uint32_t *rounds);
 int orc_pagerank_delta(uint64_t n, const uint64_t *off, const uint32_t *ids, double eps, double local_eps,
                        int max_iters, double *p);
 void orc_ldd(uint64_t n, const uint64_t *off, const uint32_t *ids, double beta, uint32_t *cluster, uint32_t *round);

@@ -532,6 +532,35 @@ def test_coloring_matches_definition(which):
     assert O.coloring_check(g, bad) > 0
 
 
+def test_densest_closed_forms():
+    """WorkEfficientDensestSubgraph: a k-clique has density k - 1 (every
+    vertex removed in round 1); a star with s leaves has 2s/(s+1) in round 1
+    (leaves go, then the centre alone at 0)."""
+    k = 9
+    src = np.repeat(np.arange(k, dtype=np.uint32), k)
+    dst = np.tile(np.arange(k, dtype=np.uint32), k)
+    d, r = O.densest(O.sym_csr(k, src, dst))
+    assert d == k - 1 and r == 1
+    s = 7
+    d, r = O.densest(O.sym_csr(s + 1, np.zeros(s, np.uint32), np.arange(1, s + 1, dtype=np.uint32)))
+    assert d == 2 * s / (s + 1) and r == 2
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_densest_vs_brute_force(seed):
+    """On small random graphs the result is the density of some subset (at
+    most the optimum) and within the 2(1 + eps) guarantee of it."""
+    rng = np.random.default_rng(seed)
+    n = 12
+    e = rng.integers(0, n, (30, 2)).astype(np.uint32)
+    g = O.sym_csr(n, e[:, 0], e[:, 1])
+    best = O.densest_brute(g)
+    for eps in (0.001, 0.5):
+        d, _ = O.densest(g, eps)
+        assert d <= best + 1e-12
+        assert d >= best / (2 * (1 + eps)) - 1e-12
+
+
 def test_mis_key_is_splitmix():
     # known answer: splitmix64(0) = 0xE220A8397B1DCDAF, so key(0) = 0xE220A839 << 32
     assert O.mis_key(0) == 0xE220A839 << 32
