@@ -286,6 +286,10 @@ def main():
         ex["mis_ms"] = round(mis_ms, 3)
         ex["mis_size"] = int(mis_in.sum())
         ex["mis_rounds"] = int(mis_rounds)
+        G.spanner()
+        sp_e, sp_ms = G.spanner()  # run_all's k = 4
+        ex["spanner_ms"] = round(sp_ms, 3)
+        ex["spanner_edges"] = int(len(sp_e))
         G.densest_subgraph()
         ds_d, ds_rounds, ds_ms = G.densest_subgraph()  # run_all's eps 0.001
         ex["densest_ms"] = round(ds_ms, 3)

@@ -100,6 +100,7 @@ SIGNATURES = {
     "byo_mis": (_i32, [_vp, _vp, _P(_u32), _P(_dbl)]),
     "byo_coloring": (_i32, [_vp, _vp, _P(_u32), _P(_dbl)]),
     "byo_densest_subgraph": (_i32, [_vp, _dbl, _P(_dbl), _P(_u32), _P(_dbl)]),
+    "byo_spanner": (_i32, [_vp, _u32, _vp, _u64, _P(_u64), _P(_dbl)]),
     "byo_ldd": (_i32, [_vp, _dbl, _vp, _P(_u32), _P(_dbl)]),
     "byo_pagerank_delta": (_i32, [_vp, _dbl, _dbl, _i32, _vp, _P(_i32), _P(_dbl)]),
     "byo_map_sum": (_i32, [_vp, _P(_u64), _P(_dbl)]),
@@ -450,6 +451,15 @@ class Graph:
         rounds, ms = _u32(), _dbl()
         check(lib().byo_mis(self._h, _ptr(out), C.byref(rounds), C.byref(ms)))
         return out[:n], rounds.value, ms.value
+
+    def spanner(self, k=4):
+        """MPX spanner (Spanner, beta = ln(n)/(2k)) with fixed choices;
+        returns (sorted uint64 edge keys u << 32 | v, device ms)."""
+        cnt, ms = _u64(), _dbl()
+        check(lib().byo_spanner(self._h, int(k), None, 0, C.byref(cnt), C.byref(ms)))
+        out = np.empty(max(cnt.value, 1), np.uint64)
+        check(lib().byo_spanner(self._h, int(k), _ptr(out), cnt.value, C.byref(cnt), C.byref(ms)))
+        return out[:cnt.value], ms.value
 
     def densest_subgraph(self, eps=0.001):
         """WorkEfficientDensestSubgraph (ApproxPeelingBKV12); returns

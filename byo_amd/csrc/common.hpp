@@ -262,6 +262,9 @@ struct RelabelView {
   const uint32_t *perm;  // null when the graph is used as it is
 };
 RelabelView relabel_view(byo_graph *g);
+// LDD rounds (ldd.hip); leaves clusters and claim rounds in device buffers
+// "ldd_cl" / "ldd_rd".  Caller holds the lock with the CSR view current.
+double ldd_run(byo_graph *g, double beta, uint32_t *rounds_run);
 // CSR construction (graph.hip).
 void csr_from_keys(byo_graph *g, uint64_t *keys, uint64_t *alt, uint64_t nk, uint64_t n);
 void csr_upload(byo_graph *g, const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m);

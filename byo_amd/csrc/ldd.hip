@@ -140,6 +140,69 @@ __global__ void k_ldd_ctl(LddCtl *ctl, uint32_t cnt) {
 }
 
 }  // namespace
+
+// The LDD rounds; clusters and claim rounds stay in device buffers "ldd_cl"
+// and "ldd_rd" (u32[n]).  Returns the device time; *rounds_run the rounds.
+// The caller holds the handle's lock with the CSR view current.
+double ldd_run(byo_graph *g, double beta, uint32_t *rounds_run) {
+  const uint64_t n = g->n, m = g->m;
+  hipStream_t s = g->stream;
+  // generate_shifts: floor(exp(i * beta)) for i < last, exclusive scan
+  std::vector<uint64_t> shifts;
+  if (n) {
+    const uint64_t last = (uint64_t)std::fmin((double)(n + 1), 2.0 + std::ceil(std::log((double)n) / beta));
+    shifts.assign(last + 1, 0);
+    uint64_t acc = 0;
+    for (uint64_t i = 0; i <= last; i++) {
+      const uint64_t x = i < last ? (uint64_t)std::floor(std::exp((double)i * beta)) : 0;
+      shifts[i] = acc;
+      acc += x;
+    }
+  }
+  uint32_t *cl = g->buf("ldd_cl", n * 4 + 64).as<uint32_t>();
+  uint32_t *rd = g->buf("ldd_rd", n * 4 + 64).as<uint32_t>();
+  uint32_t *q = g->buf("ldd_q0", n * 4 + 64).as<uint32_t>();
+  uint32_t *qn = g->buf("ldd_q1", n * 4 + 64).as<uint32_t>();
+  LddCtl *ctl = g->buf("ldd_ctl", sizeof(LddCtl) + 64).as<LddCtl>();
+  LddCtl *h = static_cast<LddCtl *>(pinned(g, sizeof(LddCtl)));
+  const uint64_t *off = g->off.as<uint64_t>();
+  const uint32_t *ids = g->ids.as<uint32_t>();
+  PhaseTimer t(s, true);
+  uint64_t visited = 0, nf = 0, fdeg = 0;
+  uint32_t r = 0;
+  if (n) {
+    k_ldd_init<<<byo_grid(n, 256, 16384), 256, 0, s>>>(n, cl, rd);
+    for (r = 0; visited < n; r++) {
+      BYO_REQUIRE(r + 1 < shifts.size(), BYO_E_STATE, "LDD ran past its last round");
+      const uint64_t start = shifts[r], end = std::min<uint64_t>(shifts[r + 1], n);
+      if (end > start) {
+        k_ldd_ctl<<<1, 1, 0, s>>>(ctl, (uint32_t)nf);
+        k_ldd_centres<<<byo_grid(end - start, 256, 16384), 256, 0, s>>>(off, start, end, r, cl, rd, q, ctl);
+        BYO_LAUNCH_CHECK();
+        BYO_HIP(hipMemcpyAsync(h, ctl, sizeof(LddCtl), hipMemcpyDeviceToHost, s));
+        BYO_HIP(hipStreamSynchronize(s));
+        nf = h->cnt;
+        fdeg += h->deg;
+      }
+      visited += nf;
+      if (visited >= n) break;
+      k_ldd_ctl<<<1, 1, 0, s>>>(ctl, 0);
+      if (nf + fdeg > m / 20)
+        k_ldd_pull<<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, ids, n, r, cl, rd, qn, ctl);
+      else if (nf)
+        k_ldd_push<<<byo_grid(nf * 64, 256, 16384), 256, 0, s>>>(off, ids, q, (uint32_t)nf, r, cl, rd, qn, ctl);
+      BYO_LAUNCH_CHECK();
+      BYO_HIP(hipMemcpyAsync(h, ctl, sizeof(LddCtl), hipMemcpyDeviceToHost, s));
+      BYO_HIP(hipStreamSynchronize(s));
+      nf = h->cnt;
+      fdeg = h->deg;
+      std::swap(q, qn);
+    }
+  }
+  if (rounds_run) *rounds_run = r;
+  return t.stop();
+}
+
 }  // namespace byo
 
 using namespace byo;
@@ -153,66 +216,12 @@ int byo_ldd(byo_graph_t g, double beta, uint32_t *cluster, uint32_t *rounds_run,
     std::lock_guard<std::mutex> lk(g->mu);
     bind(g);
     ensure_csr(g);
-    const uint64_t n = g->n, m = g->m;
-    hipStream_t s = g->stream;
-    // generate_shifts: floor(exp(i * beta)) for i < last, exclusive scan
-    std::vector<uint64_t> shifts;
-    if (n) {
-      const uint64_t last = (uint64_t)std::fmin((double)(n + 1), 2.0 + std::ceil(std::log((double)n) / beta));
-      shifts.assign(last + 1, 0);
-      uint64_t acc = 0;
-      for (uint64_t i = 0; i <= last; i++) {
-        const uint64_t x = i < last ? (uint64_t)std::floor(std::exp((double)i * beta)) : 0;
-        shifts[i] = acc;
-        acc += x;
-      }
-    }
-    uint32_t *cl = g->buf("ldd_cl", n * 4 + 64).as<uint32_t>();
-    uint32_t *rd = g->buf("ldd_rd", n * 4 + 64).as<uint32_t>();
-    uint32_t *q = g->buf("ldd_q0", n * 4 + 64).as<uint32_t>();
-    uint32_t *qn = g->buf("ldd_q1", n * 4 + 64).as<uint32_t>();
-    LddCtl *ctl = g->buf("ldd_ctl", sizeof(LddCtl) + 64).as<LddCtl>();
-    LddCtl *h = static_cast<LddCtl *>(pinned(g, sizeof(LddCtl)));
-    const uint64_t *off = g->off.as<uint64_t>();
-    const uint32_t *ids = g->ids.as<uint32_t>();
-    PhaseTimer t(s, true);
-    uint64_t visited = 0, nf = 0, fdeg = 0;
-    uint32_t r = 0;
-    if (n) {
-      k_ldd_init<<<byo_grid(n, 256, 16384), 256, 0, s>>>(n, cl, rd);
-      for (r = 0; visited < n; r++) {
-        BYO_REQUIRE(r + 1 < shifts.size(), BYO_E_STATE, "LDD ran past its last round");
-        const uint64_t start = shifts[r], end = std::min<uint64_t>(shifts[r + 1], n);
-        if (end > start) {
-          k_ldd_ctl<<<1, 1, 0, s>>>(ctl, (uint32_t)nf);
-          k_ldd_centres<<<byo_grid(end - start, 256, 16384), 256, 0, s>>>(off, start, end, r, cl, rd, q, ctl);
-          BYO_LAUNCH_CHECK();
-          BYO_HIP(hipMemcpyAsync(h, ctl, sizeof(LddCtl), hipMemcpyDeviceToHost, s));
-          BYO_HIP(hipStreamSynchronize(s));
-          nf = h->cnt;
-          fdeg += h->deg;
-        }
-        visited += nf;
-        if (visited >= n) break;
-        k_ldd_ctl<<<1, 1, 0, s>>>(ctl, 0);
-        if (nf + fdeg > m / 20)
-          k_ldd_pull<<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, ids, n, r, cl, rd, qn, ctl);
-        else if (nf)
-          k_ldd_push<<<byo_grid(nf * 64, 256, 16384), 256, 0, s>>>(off, ids, q, (uint32_t)nf, r, cl, rd, qn, ctl);
-        BYO_LAUNCH_CHECK();
-        BYO_HIP(hipMemcpyAsync(h, ctl, sizeof(LddCtl), hipMemcpyDeviceToHost, s));
-        BYO_HIP(hipStreamSynchronize(s));
-        nf = h->cnt;
-        fdeg = h->deg;
-        std::swap(q, qn);
-      }
-    }
-    const double ms = t.stop();
+    const uint64_t n = g->n;
+    const double ms = ldd_run(g, beta, rounds_run);
     if (g->prof.on) g->account("ldd", ms);
     if (device_ms) *device_ms = ms;
-    if (rounds_run) *rounds_run = r;
-    if (n) BYO_HIP(hipMemcpyAsync(cluster, cl, n * 4, hipMemcpyDeviceToHost, s));
-    BYO_HIP(hipStreamSynchronize(s));
+    if (n) BYO_HIP(hipMemcpyAsync(cluster, g->buf("ldd_cl", n * 4 + 64).as<uint32_t>(), n * 4, hipMemcpyDeviceToHost, g->stream));
+    BYO_HIP(hipStreamSynchronize(g->stream));
   });
 }
 

@@ -294,6 +294,18 @@ int byo_ldd(byo_graph_t g, double beta, uint32_t *cluster, uint32_t *rounds_run,
  * rootset rounds run (may be null).  Symmetric graphs, as in the reference. */
 int byo_mis(byo_graph_t g, uint8_t *in_mis, uint32_t *rounds, double *device_ms);
 
+/* Spanner (benchmarks/Spanner/MPXV15/Spanner.h:242-261, run_all's
+ * "Spanner" with k = 4): beta = ln(n) / (2k); the LDD of byo_ldd, each
+ * non-centre's edge to its parent (the smallest neighbour of its cluster
+ * claimed one round before it) and, for every pair of adjacent clusters
+ * c1 < c2, the edge (u, v) with cluster(u) = c1, cluster(v) = c2 smallest
+ * as u << 32 | v (the reference leaves both choices to CAS races).
+ * *count = the number of edges; when edges is not null it receives them
+ * as sorted keys (u << 32 | v) and must hold *count (else BYO_E_INVAL with
+ * *count set).  Symmetric graphs. */
+int byo_spanner(byo_graph_t g, uint32_t k, uint64_t *edges, uint64_t capacity, uint64_t *count,
+                double *device_ms);
+
 /* Approximate densest subgraph by peeling, WorkEfficientDensestSubgraph
  * (benchmarks/ApproximateDensestSubgraph/ApproxPeelingBKV12/
  * DensestSubgraph.h:8-145, run_all's default eps 0.001): *density = the

@@ -10,6 +10,7 @@ Each wrapper names the reference file:line its C function restates; see
 ``oracle/oracle.h`` for the exact semantics and how each is pinned.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -73,6 +74,7 @@ def lib():
             "orc_coloring": (None, [u64, _u64p, _u32p, _u32p]),
             "orc_llf_before": (i32, [_u64p, u32, u32]),
             "orc_densest": (dbl, [u64, _u64p, _u32p, dbl, C.POINTER(C.c_uint32)]),
+            "orc_spanner": (u64, [u64, _u64p, _u32p, u32, _u64p]),
             "orc_pagerank_delta": (i32, [u64, _u64p, _u32p, dbl, dbl, i32, _f64p]),
             "orc_ldd": (None, [u64, _u64p, _u32p, dbl, _u32p, _u32p]),
             "orc_mlp_forward_backward": (dbl, [_f32p, _f32p, _i32p, i32, _f32p, C.c_void_p]),
@@ -327,6 +329,45 @@ def coloring_check(g, color):
             c += 1
         bad += color[v] != (c if c < deg else deg + 1)
     return int(bad)
+
+
+def spanner(g, k=4):
+    """Spanner (MPXV15) with beta = ln(n)/(2k), restated deterministically
+    (orc_spanner); sorted uint64 edge keys (u << 32 | v)."""
+    out = np.empty(max(g.n + g.m, 1), np.uint64)
+    cnt = lib().orc_spanner(g.n, g.offsets, g.ids, k, out)
+    return out[:cnt].copy()
+
+
+def spanner_check(g, keys, k=4):
+    """Violations of the spanner's definition, computed from the LDD alone:
+    every edge in G; one tree edge (v, p) per non-centre v with p the
+    smallest neighbour in v's cluster one round nearer; for every adjacent
+    cluster pair exactly its smallest (u, v) edge; nothing else."""
+    n = g.n
+    if n == 0:
+        return len(keys)
+    cl, rd = ldd(g, math.log(n) / (2 * k))
+    want = set()
+    edges = set()
+    for u in range(n):
+        for v in g.ids[g.offsets[u]:g.offsets[u + 1]]:
+            edges.add((u << 32) | int(v))
+    best = {}
+    for v in range(n):
+        nb = [int(x) for x in g.ids[g.offsets[v]:g.offsets[v + 1]]]
+        if cl[v] != v:
+            p = min(u for u in nb if cl[u] == cl[v] and rd[u] + 1 == rd[v])
+            want.add((v << 32) | p)
+        for u in nb:
+            if cl[v] < cl[u]:
+                key = (int(cl[v]), int(cl[u]))
+                e = (v << 32) | u
+                if key not in best or e < best[key]:
+                    best[key] = e
+    want |= set(best.values())
+    got = set(int(x) for x in keys)
+    return len(got ^ want) + len(got - edges) + (len(got) != len(keys))
 
 
 def densest(g, eps=0.001):

@@ -25,6 +25,10 @@
  *   orc_bc            : brute-force dependencies from all-pairs shortest-path
  *       counts (textbook definition) on small graphs; star and path closed
  *       forms; the source's own value = reached - 1.
+ *   orc_spanner       : checked without its own code on small graphs: a
+ *       subgraph of G with G's components, one tree edge per non-centre to
+ *       a neighbour of its LDD cluster one round nearer the centre, and for
+ *       every pair of adjacent clusters exactly their smallest edge.
  *   orc_densest       : closed forms (clique k - 1, star 2n/(n+1)); brute
  *       force over all subsets of small graphs: the result is the density
  *       of a subset (<= the optimum) and within 2(1+eps) of the optimum.
@@ -1166,4 +1170,71 @@ double orc_densest(uint64_t n, const uint64_t *off, const uint32_t *ids, double 
   free(rem);
   if (rounds) *rounds = r;
   return best;
+}
+
+/* ===================================================================== */
+/* Spanner (MPXV15)                                                      */
+/* ===================================================================== */
+
+typedef struct {
+  uint64_t pair, edge;
+} sp_cand;
+
+static int sp_cand_cmp(const void *a, const void *b) {
+  const sp_cand *x = (const sp_cand *)a, *y = (const sp_cand *)b;
+  if (x->pair != y->pair) return x->pair < y->pair ? -1 : 1;
+  if (x->edge != y->edge) return x->edge < y->edge ? -1 : 1;
+  return 0;
+}
+
+static int u64_cmp(const void *a, const void *b) {
+  const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+  return x < y ? -1 : x > y;
+}
+
+/* Spanner (Spanner/MPXV15/Spanner.h:242-261) with beta = ln(n) / (2k) as
+ * Spanner_runner sets it: LDD_parents, then tree_and_intercluster_edges.
+ * The reference's LDD permutes the centres and lets CAS races pick parents
+ * and one edge per cluster pair; restated deterministically: the LDD of
+ * orc_ldd (identity order, smallest cluster), v's parent the smallest-id
+ * neighbour in v's cluster claimed one round before v, and for every pair
+ * of adjacent clusters c1 < c2 the edge (u, v) with cl[u] = c1, cl[v] = c2
+ * smallest as (u << 32 | v).  out (capacity n + m) receives the edges as
+ * sorted keys (u << 32 | v); returns their number. */
+uint64_t orc_spanner(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t k, uint64_t *out) {
+  if (n == 0) return 0;
+  const double beta = log((double)n) / (2.0 * (double)k);
+  uint32_t *cl = (uint32_t *)malloc(n * sizeof(uint32_t));
+  uint32_t *rd = (uint32_t *)malloc(n * sizeof(uint32_t));
+  orc_ldd(n, off, ids, beta, cl, rd);
+  uint64_t cnt = 0;
+  for (uint64_t v = 0; v < n; v++) {
+    if (cl[v] == v) continue;
+    uint32_t best = 0xFFFFFFFFu;
+    for (uint64_t e = off[v]; e < off[v + 1]; e++) {
+      const uint32_t u = ids[e];
+      if (cl[u] == cl[v] && rd[u] + 1 == rd[v] && u < best) best = u;
+    }
+    out[cnt++] = (v << 32) | best;
+  }
+  const uint64_t m = off[n];
+  sp_cand *c = (sp_cand *)malloc((m + 1) * sizeof(sp_cand));
+  uint64_t nc = 0;
+  for (uint64_t u = 0; u < n; u++)
+    for (uint64_t e = off[u]; e < off[u + 1]; e++) {
+      const uint32_t v = ids[e];
+      if (cl[u] < cl[v]) {
+        c[nc].pair = ((uint64_t)cl[u] << 32) | cl[v];
+        c[nc].edge = (u << 32) | v;
+        nc++;
+      }
+    }
+  qsort(c, nc, sizeof(sp_cand), sp_cand_cmp);
+  for (uint64_t i = 0; i < nc; i++)
+    if (i == 0 || c[i].pair != c[i - 1].pair) out[cnt++] = c[i].edge;
+  free(c);
+  free(cl);
+  free(rd);
+  qsort(out, cnt, sizeof(uint64_t), u64_cmp);
+  return cnt;
 }

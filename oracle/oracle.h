@@ -134,6 +134,9 @@ int orc_llf_before(const uint64_t *off, uint32_t u, uint32_t v);
 /* WorkEfficientDensestSubgraph (ApproxPeelingBKV12/DensestSubgraph.h:8-145):
  * the largest density (2|E(R)|/|R|) over the peeling rounds. */
 double orc_densest(uint64_t n, const uint64_t *off, const uint32_t *ids, double eps, uint32_t *rounds);
+/* Spanner (Spanner/MPXV15/Spanner.h:242-261), beta = ln(n)/(2k): sorted
+ * edge keys (u << 32 | v) into out (capacity n + m); returns the count. */
+uint64_t orc_spanner(uint64_t n, const uint64_t *off, const uint32_t *ids, uint32_t k, uint64_t *out);
 int orc_pagerank_delta(uint64_t n, const uint64_t *off, const uint32_t *ids, double eps, double local_eps,
                        int max_iters, double *p);
 void orc_ldd(uint64_t n, const uint64_t *off, const uint32_t *ids, double beta, uint32_t *cluster, uint32_t *round);

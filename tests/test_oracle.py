@@ -532,6 +532,37 @@ def test_coloring_matches_definition(which):
     assert O.coloring_check(g, bad) > 0
 
 
+@pytest.mark.parametrize("which", ["rmat", "grid", "er", "inputs"])
+def test_spanner_definition_and_connectivity(which):
+    """The spanner equals its definition over the LDD (tree edges to the
+    smallest neighbour one round nearer, the smallest edge of every adjacent
+    cluster pair), is a subgraph of G, and keeps G's components."""
+    import scipy.sparse as sp
+    import scipy.sparse.csgraph as cg
+    if which == "rmat":
+        g = O.rmat_graph(10, 8)
+    elif which == "grid":
+        g = O.grid3d_graph(7)
+    elif which == "er":
+        g = O.er_graph(9, 900, seed=7)
+    else:
+        g = O.read_adjacency_graph(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "inputs",
+                                                "rMatGraph_J_5_100"))
+    keys = O.spanner(g)
+    assert O.spanner_check(g, keys) == 0
+    assert O.spanner_check(g, keys[1:]) > 0
+    n = g.n
+    u = (keys >> np.uint64(32)).astype(np.int64)
+    v = (keys & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    S = sp.csr_matrix((np.ones(len(keys)), (u, v)), shape=(n, n))
+    A = sp.csr_matrix((np.ones(g.m), g.ids.astype(np.int64), g.offsets.astype(np.int64)), shape=(n, n))
+    ns, ls = cg.connected_components(S, directed=False)
+    na, la = cg.connected_components(A, directed=False)
+    assert ns == na
+    assert len(set(zip(ls.tolist(), la.tolist()))) == na
+    assert len(keys) < g.m // 2 + n
+
+
 def test_densest_closed_forms():
     """WorkEfficientDensestSubgraph: a k-clique has density k - 1 (every
     vertex removed in round 1); a star with s leaves has 2s/(s+1) in round 1
