@@ -68,3 +68,24 @@ def test_after_updates(B):
     off, ids = G.export_csr()
     check(O.Graph(off, ids), G, full_check=False)
     G.close()
+
+
+def test_error_paths(B):
+    import ctypes as C
+    g = O.rmat_graph(9)
+    G = B.CsrGraph.from_csr(g.offsets, g.ids)
+    lib = B.lib()
+    cnt, ms = C.c_uint64(), C.c_double()
+    # k = 0 is rejected; a buffer smaller than the spanner is rejected with
+    # the size needed reported
+    assert lib.byo_spanner(G._h, 0, None, 0, C.byref(cnt), C.byref(ms)) == B.E_INVAL
+    assert lib.byo_spanner(G._h, 4, None, 0, C.byref(cnt), C.byref(ms)) == B.OK
+    need = cnt.value
+    small = np.empty(max(need - 1, 1), np.uint64)
+    rc = lib.byo_spanner(G._h, 4, small.ctypes.data_as(C.c_void_p), need - 1, C.byref(cnt), C.byref(ms))
+    assert rc == B.E_INVAL and cnt.value == need
+    d = C.c_double()
+    assert lib.byo_densest_subgraph(G._h, -1.0, C.byref(d), None, None) == B.E_INVAL
+    assert lib.byo_coloring(G._h, None, None, None) == B.E_INVAL
+    assert lib.byo_mis(G._h, None, None, None) == B.E_INVAL
+    G.close()
