@@ -6,8 +6,8 @@
 // coreness.
 //
 // Device peel in one cooperative launch, by rounds.  The frontier is the
-// live vertices of degree <= k; a round has every wave take frontier
-// vertices and decrement their live neighbours with atomics.  The thread
+// live vertices of degree <= k; a round has every wave take four frontier
+// vertices at a time and decrement their live neighbours with atomics.  The thread
 // whose decrement takes a neighbour from k+1 to k pushes it to the next
 // frontier and marks it dead with coreness k right away -- it would be
 // removed next round, and later decrements can only lower its degree, which
@@ -41,12 +41,17 @@ struct KcArgs {
   GridBar *bar;
 };
 
+// Lanes per frontier vertex in a round, and frontier vertices per wave.
+constexpr int kKcLanes = 16;
+constexpr int kKcVpw = 64 / kKcLanes;
+
 __device__ __forceinline__ unsigned int ld_u32(const unsigned int *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void __launch_bounds__(kCoopBlock) k_kc_persist(KcArgs a) {
   const int lane = lane_id();
+  const int ksub = lane & (kKcLanes - 1), kgrp = lane / kKcLanes;
   const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
   const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
   const unsigned nb = gridDim.x;
@@ -126,22 +131,31 @@ __global__ void __launch_bounds__(kCoopBlock) k_kc_persist(KcArgs a) {
       cnt = ld_u32(&c->cnt[(p - 1) % 3]);
     }
     kmax = max(kmax, k);
-    // one round: a wave per frontier vertex decrements its live neighbours
-    // 64 at a time, and the vertices that reach k are appended with one
-    // counter atomic per wave and step (one per push would serialise a
-    // round's millions of pushes on that one address).  Lanes spread over
-    // one list hit distinct vertices; serving 64 low-degree frontier
-    // vertices per instruction instead makes the decrements collide on the
-    // hubs they share (2.4x slower on RMAT, even with the appends
-    // aggregated).  Dead neighbours are skipped to save atomics; a stale
+    // one round: each frontier vertex gets 16 lanes of a wave (four per
+    // wave step) that decrement its live neighbours, and the vertices that
+    // reach k are appended with one counter atomic per wave and step (one
+    // per push would serialise a round's millions of pushes on that one
+    // address).  Four lists per instruction fill the lanes that short lists
+    // left idle under a wave per vertex (Erdos-Renyi 1.4x faster); serving
+    // 64 frontier vertices per instruction instead makes the decrements
+    // collide on the hubs they share (2.4x slower on RMAT, even with the
+    // appends aggregated).  Dead neighbours are skipped to save atomics; a stale
     // flag is harmless, as a dead vertex's degree no longer matters and,
     // degrees only falling, one that died at level k0 <= k cannot pass from
     // k+1 to k again.
-    for (uint64_t i = wave; i < cnt; i += nwaves) {
-      const uint32_t u = q[i];
-      const uint64_t e0 = a.off[u], e1 = a.off[u + 1];
-      for (uint64_t c0 = e0; c0 < e1; c0 += 64) {
-        const uint64_t e = c0 + lane;
+    for (uint64_t i0 = wave * kKcVpw; i0 < cnt; i0 += nwaves * kKcVpw) {
+      // kKcVpw frontier vertices per wave, kKcLanes lanes each; a step
+      // serves kKcLanes edges of every one of them
+      const uint64_t i = i0 + kgrp;
+      uint64_t e0 = 0, e1 = 0;
+      if (i < cnt) {
+        const uint32_t u = q[i];
+        e0 = a.off[u];
+        e1 = a.off[u + 1];
+      }
+      const uint64_t steps = wave_max((e1 - e0 + kKcLanes - 1) / kKcLanes);
+      for (uint64_t st = 0; st < steps; st++) {
+        const uint64_t e = e0 + st * kKcLanes + ksub;
         bool push = false;
         uint32_t v = 0;
         if (e < e1) {
