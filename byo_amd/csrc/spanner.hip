@@ -43,7 +43,8 @@ __global__ void k_sp_tree(const uint64_t *off, const uint32_t *ids, uint64_t n, 
         const uint32_t u = ids[e];
         if (cl[u] == cl[v] && rd[u] + 1 == rd[v] && u < best) best = u;
       }
-      take = true;
+      take = best != kNone;  // an LDD claim always has such a neighbour
+      if (!take) atomicOr(reinterpret_cast<unsigned int *>(cnt + 2), 1u);
       key = (v << 32) | best;
     }
     wave_append64(take, key, out, cnt);
@@ -138,7 +139,7 @@ int byo_spanner(byo_graph_t g, uint32_t k, uint64_t *edges, uint64_t capacity, u
     const uint32_t *rd = g->buf("ldd_rd", n * 4 + 64).as<uint32_t>();
     unsigned long long *ctr = g->buf("sp_ctr", 64).as<unsigned long long>();
     unsigned long long *hc = static_cast<unsigned long long *>(pinned(g, 64));
-    BYO_HIP(hipMemsetAsync(ctr, 0, 16, s));
+    BYO_HIP(hipMemsetAsync(ctr, 0, 24, s));  // edge count, candidate count, no-parent flag
     k_sp_count<<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, ids, n, cl, ctr + 1);
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipMemcpyAsync(hc, ctr + 1, 8, hipMemcpyDeviceToHost, s));
@@ -155,8 +156,9 @@ int byo_spanner(byo_graph_t g, uint32_t k, uint64_t *edges, uint64_t capacity, u
     k_sp_insert<<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, ids, n, cl, tkey, tval, T - 1);
     k_sp_collect<<<byo_grid(T, 256, 16384), 256, 0, s>>>(tkey, tval, T, out, ctr);
     BYO_LAUNCH_CHECK();
-    BYO_HIP(hipMemcpyAsync(hc, ctr, 8, hipMemcpyDeviceToHost, s));
+    BYO_HIP(hipMemcpyAsync(hc, ctr, 24, hipMemcpyDeviceToHost, s));
     BYO_HIP(hipStreamSynchronize(s));
+    BYO_REQUIRE(hc[2] == 0, BYO_E_STATE, "a clustered vertex has no neighbour one round nearer its centre");
     const uint64_t total = hc[0];
     BYO_REQUIRE(total <= n + cand, BYO_E_STATE, "spanner edge count out of range");
     int idbits = 1;
