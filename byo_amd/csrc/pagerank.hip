@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256) k_pr_seg(const uint64_t *lo_pos, const ui
 // contributions, then the Kahan adds of the ones in range.  Tiles of
 // similar degrees -- vertices numbered by degree, as the relabelled copy
 // is -- waste the fewest lanes.
-constexpr int kPrUnroll = 8;
+constexpr int kPrUnroll = 16;
 
 template <class T, int G, bool kComp>
 __device__ __forceinline__ Acc<T> pr_tile(uint64_t lo, uint64_t hi, const uint32_t *ids, const T *contrib_in,
