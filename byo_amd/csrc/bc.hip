@@ -38,7 +38,7 @@ struct alignas(16) BcState {
   uint32_t pad;
 };
 
-constexpr int kBcUnroll = 8;
+constexpr int kBcUnroll = 16;
 
 // Levels are few (tens on small-world graphs) and every reached vertex
 // counts into one, so counting and bucketing go through a per-block LDS
