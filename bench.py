@@ -366,14 +366,19 @@ def main():
             sweep[str(b)] = {"insert_edges_s": round(b / np.mean(ti), 1), "remove_edges_s": round(b / np.mean(tr), 1)}
             b *= 10
         ex["cpma_sweep"] = sweep
+        dkeys.free()
         C.close()
+        if H is not G:
+            H.close()
         if args.job_scale:
             ex["batched_job"] = batched_job(B, args)
         out["extras"] = ex
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(G, srcs, args.cpu_budget)
-    print(json.dumps(out))
+    # every device handle is released here, before the interpreter exits
+    G.close()
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

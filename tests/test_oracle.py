@@ -280,6 +280,41 @@ def test_ppr_mass_and_seed():
     assert p[seed] >= 0.15
 
 
+def test_ppr_dense_solve(fixture_graphs):
+    """The PPR fixed point solves (I - d A D^-1) p = (1 - d) e_s, with
+    D^-1 = 0 for isolated vertices (they spread nothing); after 250 Jacobi
+    iterations the error is below d^250.  A version with damping and teleport
+    swapped, or contributions divided by the target's degree, fails this."""
+    g = fixture_graphs["rMatGraph_J_5_100"]
+    n, d = g.n, 0.85
+    A = np.zeros((n, n))
+    for v in range(n):
+        for u in g.neighbors(v):
+            A[v, u] = 1.0  # p'[v] gathers from its neighbours u
+    deg = g.degrees().astype(np.float64)
+    dinv = np.where(deg > 0, 1.0 / np.maximum(deg, 1), 0.0)
+    for seed in (10, 0, 77):
+        e = np.zeros(n)
+        e[seed] = 1.0
+        exact = np.linalg.solve(np.eye(n) - d * A * dinv[None, :], (1 - d) * e)
+        p = O.ppr(g, seed, d, 250)
+        assert np.abs(p - exact).sum() < 1e-12, seed
+
+
+def test_ppr_linearity_gives_global_pagerank():
+    """PPR is affine in (p0, teleport): the mean over all seeds of PPR after
+    t iterations equals global PageRank (p0 = 1/n, teleport (1-d)/n) after t
+    iterations, isolated vertices included."""
+    g = O.rmat_graph(7)
+    for t in (1, 5, 30):
+        mean = np.zeros(g.n)
+        for s in range(g.n):
+            mean += O.ppr(g, s, 0.85, t)
+        mean /= g.n
+        p, _ = O.pagerank(g, 0.85, t, 0.0)
+        assert np.abs(mean - p).sum() < 1e-13, t
+
+
 # ---- Connectivity (SimpleUnionAsync/Connectivity.h:32-136) -----------------
 
 def test_cc_reference_unit_cases():
