@@ -81,6 +81,7 @@ SIGNATURES = {
     "byo_remove_batch": (_i32, [_vp, _vp, _u64]),
     "byo_insert_batch_dev": (_i32, [_vp, _vp, _u64]),
     "byo_remove_batch_dev": (_i32, [_vp, _vp, _u64]),
+    "byo_insert_batch_dev_noalloc": (_i32, [_vp, _vp, _u64]),
     "byo_update_rmat_keys": (_i32, [_u64, _u64, _u64, _vp]),
     "byo_device_alloc": (_i32, [_u64, _P(_vp)]),
     "byo_device_free": (_i32, [_vp]),
@@ -311,6 +312,11 @@ class Graph:
 
     def remove_batch_dev(self, dev_ptr, count):
         check(lib().byo_remove_batch_dev(self._h, C.c_void_p(dev_ptr), count))
+
+    def insert_batch_dev_noalloc(self, dev_ptr, count):
+        """insert_batch_dev for at most 4096 keys without any allocation by
+        the library (a batch that would need a rebalance raises E_STATE)."""
+        check(lib().byo_insert_batch_dev_noalloc(self._h, C.c_void_p(dev_ptr), count))
 
     def insert_batches_dev(self, dev_ptr, offsets):
         """Batches keys[offsets[i]:offsets[i+1]] (device memory) inserted in

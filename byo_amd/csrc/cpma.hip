@@ -1253,3 +1253,104 @@ int byo_insert_batch(byo_graph_t g, const uint64_t *keys, uint64_t b) { return h
 int byo_remove_batch(byo_graph_t g, const uint64_t *keys, uint64_t b) { return host_batch(g, keys, b, false); }
 
 }  // extern "C"
+
+// ---------------------------------------------- allocation-free small insert
+// byo_insert_batch_dev_noalloc: the small-batch insert path (one workgroup
+// sorts, dedupes and segments; one pass counts; one pass merges in place)
+// with every buffer beyond the container and the caller's keys carved from
+// one file-scope device array.  Each GPU holds its own copy of the array
+// until the process ends; all calls on a GPU share it (the last call's
+// contents stay), so calls that overlap on one GPU race.
+namespace byo {
+namespace {
+constexpr uint64_t kNaMaxBatch = kSmallBatch;
+// layout: ctl | uniq u64[max] | leaf_of u32[max] | starts u32[max] |
+// newc u32[max] | segment count u32
+constexpr size_t kNaCtl = 0;
+constexpr size_t kNaUniq = 256;
+constexpr size_t kNaLeaf = kNaUniq + kNaMaxBatch * 8;
+constexpr size_t kNaStarts = kNaLeaf + kNaMaxBatch * 4;
+constexpr size_t kNaNewc = kNaStarts + kNaMaxBatch * 4;
+constexpr size_t kNaNseg = kNaNewc + kNaMaxBatch * 4;
+constexpr size_t kNaBytes = kNaNseg + 256;  // 86,528 bytes
+static_assert(sizeof(UpdCtl) <= kNaUniq - kNaCtl, "control block fits its slot");
+}  // namespace
+}  // namespace byo
+
+static __device__ __align__(256) unsigned char scratch[byo::kNaBytes];
+
+namespace byo {
+namespace {
+constexpr int kNaMaxDevices = 64;
+std::mutex na_mu;
+unsigned char *na_ptr[kNaMaxDevices] = {};
+
+// Device address of this GPU's copy of `scratch`, looked up once per device
+// with that device current (the symbol's host-side name is not a device
+// address and is never passed to a kernel or copy).
+unsigned char *noalloc_scratch(int device) {
+  BYO_REQUIRE(device >= 0 && device < kNaMaxDevices, BYO_E_INVAL, "device index out of range");
+  std::lock_guard<std::mutex> lk(na_mu);
+  if (!na_ptr[device]) {
+    void *p = nullptr;
+    BYO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    BYO_REQUIRE(p != nullptr, BYO_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    na_ptr[device] = static_cast<unsigned char *>(p);
+  }
+  return na_ptr[device];
+}
+}  // namespace
+}  // namespace byo
+
+extern "C" int byo_insert_batch_dev_noalloc(byo_graph_t g, const uint64_t *keys_dev, uint64_t b) {
+  return guard([&] {
+    BYO_REQUIRE(g && (keys_dev || b == 0), BYO_E_INVAL, "null argument");
+    BYO_REQUIRE(g->kind == 1, BYO_E_STATE, "insert_batch needs a CPMA container");
+    BYO_REQUIRE(b <= kNaMaxBatch, BYO_E_INVAL,
+                "byo_insert_batch_dev_noalloc takes at most 4096 keys per batch");
+    std::lock_guard<std::mutex> lk(g->mu);
+    bind(g);
+    if (b == 0) return;
+    unsigned char *base = noalloc_scratch(g->device);
+    Cpma *c = state(g);
+    hipStream_t s = g->stream;
+    UpdCtl *ctl = reinterpret_cast<UpdCtl *>(base + kNaCtl);
+    uint64_t *uniq = reinterpret_cast<uint64_t *>(base + kNaUniq);
+    uint32_t *leaf = reinterpret_cast<uint32_t *>(base + kNaLeaf);
+    uint32_t *starts = reinterpret_cast<uint32_t *>(base + kNaStarts);
+    uint32_t *newc = reinterpret_cast<uint32_t *>(base + kNaNewc);
+    uint32_t *nseg = reinterpret_cast<uint32_t *>(base + kNaNseg);
+    // the control block keeps the previous call's counts: clear it first
+    k_zero_ctl<<<1, 1, 0, s>>>(ctl, 1);
+    k_small_batch<<<1, kSmallThreads, 0, s>>>(keys_dev, b, g->n, c->sep.as<uint64_t>(), c->nleaves, uniq, leaf,
+                                              starts, nseg, ctl);
+    const UpdDev d{&ctl->pad[0], nseg, nullptr};
+    const unsigned grid = (unsigned)std::min<uint64_t>(b, 16384);
+    k_ins_count<<<grid, 256, 0, s>>>(uniq, d, starts, leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(), newc,
+                                     ctl);
+    BYO_LAUNCH_CHECK();
+    UpdCtl h{};
+    BYO_HIP(hipMemcpyAsync(&h, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
+    BYO_HIP(hipStreamSynchronize(s));
+    BYO_REQUIRE(h.bad == 0, BYO_E_INVAL, "batch holds a vertex id >= n");
+    if (h.pad[0] == 0) return;
+    // a leaf that would overflow needs a rebalance, whose buffers scale with
+    // the container: refuse, with the container unchanged
+    BYO_REQUIRE(h.overflow == 0, BYO_E_STATE,
+                "a leaf would overflow: this batch needs a rebalance, which allocates (use byo_insert_batch_dev)");
+    k_ins_merge<<<grid, 256, 0, s>>>(uniq, d, starts, leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+                                     c->deg.as<uint32_t>(), ctl, nullptr, c->nleaves);
+    BYO_LAUNCH_CHECK();
+    BYO_HIP(hipMemcpyAsync(&h, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
+    BYO_HIP(hipStreamSynchronize(s));
+    c->total += h.changed;
+    g->m = c->total;
+    if (h.changed) {
+      g->csr_valid = false;
+      g->maxdeg_valid = false;
+      g->pieces_stale();
+      g->rl_valid = false;
+      g->xs_valid = false;
+    }
+  });
+}

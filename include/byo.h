@@ -133,6 +133,19 @@ int byo_remove_batch(byo_graph_t g, const uint64_t *keys, uint64_t b);
 /* The same with the batch already in device memory (keys_dev, b entries). */
 int byo_insert_batch_dev(byo_graph_t g, const uint64_t *keys_dev, uint64_t b);
 int byo_remove_batch_dev(byo_graph_t g, const uint64_t *keys_dev, uint64_t b);
+/* Allocation-free variant of byo_insert_batch_dev, for callers that cannot
+ * allocate device or pinned memory (tight serving loops, code under HIP
+ * graph capture, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same result as byo_insert_batch_dev for b <= 4096 keys; the
+ * sorted batch, segments and counters live in a fixed 86,528-byte device
+ * array of the library (one per GPU, held until the process ends), so the
+ * call issues no hipMalloc/hipFree/hipHostMalloc.  Errors: b > 4096 ->
+ * BYO_E_INVAL; an id >= n -> BYO_E_INVAL; a batch that would overflow a
+ * leaf (it needs a rebalance, whose buffers scale with the container) ->
+ * BYO_E_STATE.  On any error the container is unchanged.  All calls on one
+ * GPU share that array: calls that overlap on one GPU (from different
+ * threads or handles) race and must be serialised by the caller. */
+int byo_insert_batch_dev_noalloc(byo_graph_t g, const uint64_t *keys_dev, uint64_t b);
 /* nbatches insert batches held back to back in device memory (batch i is
  * keys_dev[offsets[i] .. offsets[i+1]), offsets on the host), applied in
  * order as ONE captured HIP graph with no host round trip between batches:
