@@ -455,13 +455,23 @@ __global__ void k_max_deg(const uint64_t *off, uint64_t n, unsigned long long *o
   mx = wave_max(mx);
   if (lane_id() == 0 && mx) atomicMax(out, (unsigned long long)mx);
 }
+__global__ void k_max_deg_u32(const uint32_t *deg, uint64_t n, unsigned long long *out) {
+  uint64_t mx = 0;
+  for (uint64_t v = global_tid(); v < n; v += grid_threads()) mx = deg[v] > mx ? deg[v] : mx;
+  mx = wave_max(mx);
+  if (lane_id() == 0 && mx) atomicMax(out, (unsigned long long)mx);
+}
 }  // namespace
+// Maximum degree of the graph (from the container's own degrees).
 uint64_t graph_max_degree(byo_graph *g) {
   if (g->maxdeg_valid) return g->maxdeg;
   auto &b = g->buf("maxdeg", 64);
   BYO_HIP(hipMemsetAsync(b.p, 0, 8, g->stream));
-  if (g->n) k_max_deg<<<byo_grid(g->n, 256, 4096), 256, 0, g->stream>>>(g->off.as<uint64_t>(), g->n,
-                                                                        b.as<unsigned long long>());
+  const AdjView v = adj_view(g);
+  if (g->n && v.deg)
+    k_max_deg_u32<<<byo_grid(g->n, 256, 4096), 256, 0, g->stream>>>(v.deg, g->n, b.as<unsigned long long>());
+  else if (g->n)
+    k_max_deg<<<byo_grid(g->n, 256, 4096), 256, 0, g->stream>>>(v.off, g->n, b.as<unsigned long long>());
   BYO_LAUNCH_CHECK();
   uint64_t *h = static_cast<uint64_t *>(pinned(g, 64));
   BYO_HIP(hipMemcpyAsync(h, b.p, 8, hipMemcpyDeviceToHost, g->stream));
@@ -503,12 +513,14 @@ int byo_bfs_multi(byo_graph_t g, const uint32_t *srcs, uint32_t k, uint32_t *par
     BYO_REQUIRE(g && (srcs || k == 0), BYO_E_INVAL, "null argument");
     std::lock_guard<std::mutex> lk(g->mu);
     bind(g);
-    ensure_csr(g);
     for (uint32_t i = 0; i < k; i++) BYO_REQUIRE(srcs[i] < g->n, BYO_E_INVAL, "source out of range");
     if (k == 0) return;
     const uint64_t mx = (g->dir_policy == BYO_DIR_MLP) ? graph_max_degree(g) : 0;
     const int64_t engine = g->tuning("bfs_engine");
     const bool levels_mode = engine == 2;
+    // the persistent engines read the container itself (adj_view); the
+    // host-driven level loop reads a CSR (exported from a CPMA)
+    if (levels_mode) ensure_csr(g);
     const bool seq_mode = engine == 1;
     if (k > 1 && !levels_mode && !seq_mode) {
       // default for several sources: up to 64 at a time share one traversal

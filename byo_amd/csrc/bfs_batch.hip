@@ -5,9 +5,9 @@
 namespace byo {
 namespace batch {
 
-template <class M>
+template <class M, bool G>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_bfs_batch(BatchArgs a) {
-  bfs_batch_body<M>(a);
+  bfs_batch_body<M, G>(a);
 }
 
 
@@ -16,14 +16,15 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
 constexpr size_t kLogBytes = 65 * sizeof(BLog), kLastOff = kLogBytes, kNlevOff = kLastOff + 64 * 4,
                  kTotOff = kNlevOff + 8, kOutBytes = kTotOff + 64 * 16;
 
-void prepare(byo_graph *g, const uint32_t *srcs, uint32_t k, uint64_t maxdeg, BatchArgs &a, int mask_bytes,
-             int nblocks) {
+void prepare(byo_graph *g, const AdjView &v, const uint32_t *srcs, uint32_t k, uint64_t maxdeg, BatchArgs &a,
+             int mask_bytes, int nblocks) {
   BYO_REQUIRE(k >= 1 && k <= 64, BYO_E_INVAL, "batched BFS takes 1..64 sources per launch");
   const uint64_t n = g->n;
   hipStream_t s = g->stream;
   a = BatchArgs{};
-  a.off = g->off.as<uint64_t>();
-  a.ids = g->ids.as<uint32_t>();
+  a.off = v.off;
+  a.ids = v.ids;
+  a.deg = v.deg;
   a.n = n;
   a.m = g->m;
   a.nm = (n + 63) / 64 * 64;
@@ -32,7 +33,7 @@ void prepare(byo_graph *g, const uint32_t *srcs, uint32_t k, uint64_t maxdeg, Ba
   a.qstride = (n + 63) / 64 * 64;
   a.q = g->buf("bfs_batch_q", 3 * a.qstride * 4 + 64).as<uint32_t>();
   a.scan = g->buf("bfs_scan", (n + 1) * 8 + 64).as<uint64_t>();
-  a.cstart = g->buf("bfs_cstart", (g->m / kChunk + 2) * 4 + 64).as<uint32_t>();
+  a.cstart = g->buf("bfs_cstart", (v.nslots / kChunk + 2) * 4 + 64).as<uint32_t>();
   a.ctr = g->buf("bfs_batch_ctr", 3 * sizeof(BCtr) + 64).as<BCtr>();
   a.k = k;
   char *out = g->buf("bfs_batch_out", kOutBytes + 64).as<char>();
@@ -41,7 +42,7 @@ void prepare(byo_graph *g, const uint32_t *srcs, uint32_t k, uint64_t maxdeg, Ba
   a.nlev = reinterpret_cast<uint32_t *>(out + kNlevOff);
   a.tot = reinterpret_cast<unsigned long long *>(out + kTotOff);
   a.dc = dir_ctl(g, maxdeg, s, true);
-  a.pieces = make_pieces(g, nblocks);
+  a.pieces = make_pieces(g, nblocks, v.deg ? kPmaPieces : kTraversalPieces);
   a.bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
   a.bsum = g->buf("bfs_batch_bsum", (size_t)nblocks * 8 + 64).as<unsigned long long>();
   a.seen = g->buf("bfs_batch_seen", a.nm * mask_bytes + 64).p;
@@ -96,13 +97,16 @@ void finish(byo_graph *g, const BatchArgs &a, byo_bfs_stats *stats, double ms) {
 namespace {
 template <class M>
 double launch(byo_graph *g, const uint32_t *srcs, uint32_t k, uint64_t maxdeg, BatchArgs &a) {
-  const void *kern = reinterpret_cast<const void *>(k_bfs_batch<M>);
+  const AdjView v = adj_view(g);
+  const bool gaps = v.deg != nullptr;
+  const void *kern = gaps ? reinterpret_cast<const void *>(k_bfs_batch<M, true>)
+                          : reinterpret_cast<const void *>(k_bfs_batch<M, false>);
   const int nblocks = coop_grid(kern, kBlock, (int)g->tuning("coop_blocks_per_cu"));
-  prepare(g, srcs, k, maxdeg, a, (int)sizeof(M), nblocks);
+  prepare(g, v, srcs, k, maxdeg, a, (int)sizeof(M), nblocks);
   void *args[] = {&a};
   PhaseTimer t(g->stream, true);
   BYO_HIP(hipLaunchCooperativeKernel(kern, dim3(nblocks), dim3(kBlock), args, 0, g->stream));
-  pieces_launched(g, a.pieces, nblocks);
+  pieces_launched(g, a.pieces, nblocks, gaps ? kPmaPieces : kTraversalPieces);
   return t.stop();
 }
 }  // namespace

@@ -57,8 +57,9 @@ struct BLog {
 };
 
 struct BatchArgs {
-  const uint64_t *off;
+  const uint64_t *off;  // adjacency view (AdjView): CSR, or the PMA's slot starts
   const uint32_t *ids;
+  const uint32_t *deg;  // true degrees when the lists hold empty slots
   uint64_t n, m;
   uint64_t nm;              // mask entries (n rounded up to 64)
   uint32_t *parents;        // parents of source j at parents + j * pstride
@@ -159,7 +160,8 @@ struct WinLds {
 };
 
 // The whole shared traversal, run by every thread of a cooperative grid.
-template <class M>
+// G: the adjacency lists may hold empty slots (kNone), skipped everywhere.
+template <class M, bool G = false>
 __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
   __shared__ unsigned long long lds[8];
   __shared__ MlpLds mlp;
@@ -214,7 +216,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
         for (uint32_t i = 0; i < j; i++) dup = dup || a.srcs[i] == a.srcs[j];
         if (!dup) {
           qb[cnt++] = a.srcs[j];
-          deg += a.off[a.srcs[j] + 1] - a.off[a.srcs[j]];
+          deg += vdeg<G>(a.off, a.deg, a.srcs[j]);
         }
       }
       const unsigned long long all = a.k >= 64 ? ~0ull : ((1ull << a.k) - 1);
@@ -306,8 +308,8 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
             uint32_t tgt = 0;
             if (kk < e) {
               tgt = a.ids[kk];
-              add = relax<M>(a, seen, nmask, u, f, tgt, won);
-              if (add) dsum += a.off[tgt + 1] - a.off[tgt];
+              add = (!G || tgt != kNone) && relax<M>(a, seen, nmask, u, f, tgt, won);
+              if (add) dsum += vdeg<G>(a.off, a.deg, tgt);
             }
             const uint64_t mask = __ballot(add);
             if (mask) {
@@ -357,11 +359,17 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
           }
           pre += tot;
         }
-        if (leader) a.scan[fs] = fdeg;
+        // the scan's total: the frontier's edges, or its slots when the lists
+        // hold empty slots (written by the block holding the last slice)
+        if constexpr (G) {
+          if (threadIdx.x == 0 && s0 < s1 && s1 == fs) a.scan[fs] = pre;
+        } else {
+          if (leader) a.scan[fs] = fdeg;
+        }
         grid_barrier(a.bar, (unsigned)nb);
-        // (3) edge-balanced push, 256 edges per wave step; the LDS window
-        // holds (scan, adjacency base, frontier mask) of 64 queue entries
-        const uint64_t E = fdeg;
+        // (3) edge-balanced push, 256 edges (slots) per wave step; the LDS
+        // window holds (scan, adjacency base, frontier mask) of 64 queue entries
+        const uint64_t E = G ? ld_relaxed(reinterpret_cast<const unsigned long long *>(&a.scan[fs])) : fdeg;
         const uint64_t nch = (E + kChunk - 1) / kChunk;
         for (uint64_t c = wave; c < nch; c += nwaves) {
           const uint64_t e0 = c * kChunk;
@@ -415,8 +423,8 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
                 bs = win.base[wv][lo];
               }
               tgt = a.ids[(uint64_t)(bs + (int64_t)e)];
-              add = relax<M>(a, seen, nmask, owner, f, tgt, won);
-              if (add) dsum += a.off[tgt + 1] - a.off[tgt];
+              add = (!G || tgt != kNone) && relax<M>(a, seen, nmask, owner, f, tgt, won);
+              if (add) dsum += vdeg<G>(a.off, a.deg, tgt);
             }
             tg[t] = tgt;
             ad[t] = add;
@@ -472,6 +480,16 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
             u[st] = a.ids[on ? kk : 0];  // unconditional: all 8 loads in flight
             const M Ux = shfl_m<M>(U, x);  // every lane joins the shuffle
             h[st] = on ? Ux : (M)0;
+          }
+          if constexpr (G) {
+            // an empty slot (or an empty slot 0 read by an idle lane) is no
+            // neighbour; read entry 0's mask instead of an out-of-range one
+#pragma unroll
+            for (int st = 0; st < 8; st++)
+              if (u[st] == kNone) {
+                u[st] = 0;
+                h[st] = 0;
+              }
           }
           M fv[8];
 #pragma unroll
@@ -549,6 +567,10 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
                 const bool ok = idx < R && ax != Ux;
                 hm[t] = ok ? (M)(Ux & (M)~ax) : (M)0;
                 uu[t] = a.ids[ok ? (uint64_t)(bs + (int64_t)idx) : 0];
+                if (G && uu[t] == kNone) {
+                  uu[t] = 0;
+                  hm[t] = 0;
+                }
               }
               M fv[kPullPerLane];
 #pragma unroll
@@ -580,7 +602,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
           if (acc) seen[vl] = s | acc;
         }
         cnt += __popcll(__ballot(acc != 0));
-        if (acc) dsum += el - bl;
+        if (acc) dsum += vdeg<G>(a.off, a.deg, vl);
         won |= acc;
       });
       if (lane == 0 && cnt) atomicAdd(&nx->size, cnt);
@@ -605,8 +627,8 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
             uint32_t t = 0;
             if (kk < e) {
               t = a.ids[kk];
-              add = relax<M>(a, seen, nmask, u, f, t, won);
-              if (add) dsum += a.off[t + 1] - a.off[t];
+              add = (!G || t != kNone) && relax<M>(a, seen, nmask, u, f, t, won);
+              if (add) dsum += vdeg<G>(a.off, a.deg, t);
             }
             const uint64_t mask = __ballot(add);
             if (mask) {
@@ -650,7 +672,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
       const uint64_t vl = w * 64 + lane;
       if (vl < a.n) {
         const uint32_t sm = (uint32_t)seen[vl];
-        const unsigned long long d = a.off[vl + 1] - a.off[vl];
+        const unsigned long long d = vdeg<G>(a.off, a.deg, vl);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
           const unsigned long long bit = (sm >> j) & 1;
@@ -685,7 +707,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
       uint64_t sm = 0, d = 0;
       if (vl < a.n) {
         sm = (uint64_t)seen[vl];
-        d = a.off[vl + 1] - a.off[vl];
+        d = vdeg<G>(a.off, a.deg, vl);
         for (uint32_t j = 0; j < a.k; j++)
           if (!((sm >> j) & 1)) a.parents[(uint64_t)j * a.pstride + vl] = kNone;
       }
@@ -727,8 +749,9 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
 // Host side (bfs_batch.hip): buffers and arguments of one launch for k
 // sources (sources staged, barrier cleared, on g's stream), and the read-back
 // of its per-source statistics.
-void prepare(byo_graph *g, const uint32_t *srcs, uint32_t k, uint64_t maxdeg, BatchArgs &a, int mask_bytes,
-             int nblocks);
+// The traversal reads adjacency view v (G = v.deg != null).
+void prepare(byo_graph *g, const AdjView &v, const uint32_t *srcs, uint32_t k, uint64_t maxdeg, BatchArgs &a,
+             int mask_bytes, int nblocks);
 void finish(byo_graph *g, const BatchArgs &a, byo_bfs_stats *stats, double ms);
 
 }  // namespace batch

@@ -58,8 +58,9 @@ struct LevelLog {
 };
 
 struct CoopArgs {
-  const uint64_t *off;
+  const uint64_t *off;  // adjacency view (AdjView): CSR, or the PMA's slot starts
   const uint32_t *ids;
+  const uint32_t *deg;  // true degrees when the lists hold empty slots
   uint64_t n, m, nw64;
   uint32_t *parents;        // parents of source j at parents + j * pstride
   uint64_t pstride;
@@ -93,6 +94,8 @@ struct PushLds {
   int64_t base[kWaves][64];
 };
 
+// G: the adjacency lists may hold empty slots (kNone), skipped everywhere.
+template <bool G>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) k_bfs_coop(CoopArgs a) {
   __shared__ unsigned long long lds[8];
   __shared__ MlpLds mlp;
@@ -123,7 +126,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
     }
     if (leader) {
       a.q0[0] = src;
-      a.ctr[0] = Ctr{1, a.off[src + 1] - a.off[src], 0, 0};
+      a.ctr[0] = Ctr{1, vdeg<G>(a.off, a.deg, src), 0, 0};
       a.ctr[1] = Ctr{0, 0, 0, 0};
       a.ctr[2] = Ctr{0, 0, 0, 0};
     }
@@ -198,11 +201,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
               uint32_t tgt = 0;
               if (kk < e) {
                 tgt = a.ids[kk];
-                if (!bit_get(a.visited64, tgt) && atomicCAS(&parents[tgt], kNone, u) == kNone) {
+                if ((!G || tgt != kNone) && !bit_get(a.visited64, tgt) &&
+                    atomicCAS(&parents[tgt], kNone, u) == kNone) {
                   win_ = true;
                   atomicOr(reinterpret_cast<unsigned long long *>(&a.visited64[tgt >> 6]), 1ull << (tgt & 63));
                   atomicOr(reinterpret_cast<unsigned long long *>(&nbm[tgt >> 6]), 1ull << (tgt & 63));
-                  dsum += a.off[tgt + 1] - a.off[tgt];
+                  dsum += vdeg<G>(a.off, a.deg, tgt);
                 }
               }
               const uint64_t mask = __ballot(win_);
@@ -264,10 +268,16 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
           }
           pre += tot;
         }
-        if (leader) a.scan[fs] = fdeg;
+        // the scan's total: the frontier's edges, or its slots when the lists
+        // hold empty slots (written by the block holding the last slice)
+        if constexpr (G) {
+          if (threadIdx.x == 0 && s0 < s1 && s1 == fs) a.scan[fs] = pre;
+        } else {
+          if (leader) a.scan[fs] = fdeg;
+        }
         grid_barrier(a.bar, (unsigned)nb);
-        // (3) edge-balanced push, 256 edges per wave step
-        const uint64_t E = fdeg;
+        // (3) edge-balanced push, 256 edges (slots) per wave step
+        const uint64_t E = G ? ld_relaxed(reinterpret_cast<const unsigned long long *>(&a.scan[fs])) : fdeg;
         const uint64_t nch = (E + kChunk - 1) / kChunk;
         unsigned long long dsum = 0;
         for (uint64_t c = wave; c < nch; c += nwaves) {
@@ -315,11 +325,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
                 bs = win.base[wv][lo];
               }
               tgt = a.ids[(uint64_t)(bs + (int64_t)e)];
-              if (!bit_get(a.visited64, tgt) && atomicCAS(&parents[tgt], kNone, (uint32_t)owner_u) == kNone) {
+              if ((!G || tgt != kNone) && !bit_get(a.visited64, tgt) &&
+                  atomicCAS(&parents[tgt], kNone, (uint32_t)owner_u) == kNone) {
                 win_ = true;
                 atomicOr(reinterpret_cast<unsigned long long *>(&a.visited64[tgt >> 6]), 1ull << (tgt & 63));
                 atomicOr(reinterpret_cast<unsigned long long *>(&nbm[tgt >> 6]), 1ull << (tgt & 63));
-                dsum += a.off[tgt + 1] - a.off[tgt];
+                dsum += vdeg<G>(a.off, a.deg, tgt);
               }
             }
             tg[t] = tgt;
@@ -373,6 +384,16 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
               h[st] = ((todo >> x) & 1) && kk < e;
               u[st] = a.ids[h[st] ? kk : 0];  // unconditional: all 8 loads in flight
             }
+            if constexpr (G) {
+              // an empty slot (or an empty slot 0 read by an idle lane) is no
+              // neighbour; probe word 0 instead of an out-of-range one
+#pragma unroll
+              for (int st = 0; st < 8; st++)
+                if (u[st] == kNone) {
+                  u[st] = 0;
+                  h[st] = false;
+                }
+            }
             uint64_t fw[8];
 #pragma unroll
             for (int st = 0; st < 8; st++) fw[st] = fbm[u[st] >> 6];
@@ -412,7 +433,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
                 bool hit = false;
                 if (active && kk < e) {
                   uu = a.ids[kk];
-                  hit = bit_get(fbm, uu);
+                  hit = (!G || uu != kNone) && bit_get(fbm, uu);
                 }
                 const uint32_t gm = (uint32_t)(__ballot(hit) >> (g * 8)) & 0xFFu;
                 const uint32_t pu = __shfl(uu, g * 8 + (gm ? __builtin_ctz(gm) : 0), 64);
@@ -436,7 +457,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
             if (fmask) a.visited64[w] = vis | fmask;
           }
           cnt += __popcll(fmask);
-          if ((fmask >> lane) & 1) dsum += el - bl;
+          if ((fmask >> lane) & 1) dsum += vdeg<G>(a.off, a.deg, vl);
         });
         dsum = wave_sum(dsum);
         if (lane == 0 && cnt) {
@@ -461,7 +482,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
               uint32_t t = 0;
               if (kk < e) {
                 t = a.ids[kk];
-                if (!bit_get(a.visited64, t) && atomicCAS(&parents[t], kNone, u) == kNone) {
+                if ((!G || t != kNone) && !bit_get(a.visited64, t) && atomicCAS(&parents[t], kNone, u) == kNone) {
                   win_ = true;
                   atomicOr(reinterpret_cast<unsigned long long *>(&a.visited64[t >> 6]), 1ull << (t & 63));
                   atomicOr(reinterpret_cast<unsigned long long *>(&nbm[t >> 6]), 1ull << (t & 63));
@@ -469,7 +490,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
               }
               const uint64_t mask = __ballot(win_);
               if (mask) {
-                const unsigned long long d = wave_sum(win_ ? (unsigned long long)(a.off[t + 1] - a.off[t]) : 0ull);
+                const unsigned long long d = wave_sum(win_ ? (unsigned long long)vdeg<G>(a.off, a.deg, t) : 0ull);
                 unsigned long long base = 0;
                 if (lane == 0) {
                   base = atomicAdd(&nx->size, (unsigned long long)__popcll(mask));
@@ -547,7 +568,8 @@ int coop_grid(const void *kernel, int block, int cap) {
 Pieces make_pieces(byo_graph *g, int nblocks, PieceSet set) {
   Pieces pc;
   pc.per_wave = (uint32_t)g->tuning(set == kPprPieces ? "ppr_pieces" : "pieces");
-  pc.bound = g->buf(set == kPprPieces ? "ppr_pieces" : "bfs_pieces", ((uint64_t)kMaxPieces * 2048 * 4 + 1) * 8 + 64)
+  pc.bound = g->buf(set == kPprPieces ? "ppr_pieces" : (set == kPmaPieces ? "pma_pieces" : "bfs_pieces"),
+                    ((uint64_t)kMaxPieces * 2048 * 4 + 1) * 8 + 64)
                  .as<uint64_t>();
   const byo_graph::PieceCache &c = g->pcache[set];
   pc.ready = c.valid && c.nblocks == nblocks && c.per_wave == pc.per_wave;
@@ -569,8 +591,11 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   const uint64_t n = g->n;
   hipStream_t s = g->stream;
   CoopArgs a{};
-  a.off = g->off.as<uint64_t>();
-  a.ids = g->ids.as<uint32_t>();
+  const AdjView v = adj_view(g);
+  const bool gaps = v.deg != nullptr;
+  a.off = v.off;
+  a.ids = v.ids;
+  a.deg = v.deg;
   a.n = n;
   a.m = g->m;
   a.nw64 = (n + 63) / 64 + 1;
@@ -581,7 +606,7 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   a.bm = g->buf("bfs_bms", 3 * a.nw64 * 8).as<uint64_t>();
   a.visited64 = g->buf("bfs_visited", a.nw64 * 8).as<uint64_t>();
   a.scan = g->buf("bfs_scan", (n + 1) * 8 + 64).as<uint64_t>();
-  a.cstart = g->buf("bfs_cstart", (g->m / kChunk + 2) * 4 + 64).as<uint32_t>();
+  a.cstart = g->buf("bfs_cstart", (v.nslots / kChunk + 2) * 4 + 64).as<uint32_t>();
   a.ctr = g->buf("bfs_coop_ctr", 3 * sizeof(Ctr) + 64).as<Ctr>();
   a.k = k;
   uint32_t *dsrc = g->buf("bfs_coop_srcs", (size_t)k * 4 + 64).as<uint32_t>();
@@ -590,17 +615,19 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   a.nlevels = g->buf("bfs_coop_nlev", (size_t)k * 4 + 64).as<uint32_t>();
   a.tot = g->buf("bfs_coop_tot", (size_t)k * 16 + 64).as<unsigned long long>();
   a.dc = dir_ctl(g, maxdeg, s, false);
-  const int nblocks = coop_grid(reinterpret_cast<const void *>(k_bfs_coop), kBlock, (int)g->tuning("coop_blocks_per_cu"));
-  a.pieces = make_pieces(g, nblocks);
+  const void *kern = gaps ? reinterpret_cast<const void *>(k_bfs_coop<true>)
+                         : reinterpret_cast<const void *>(k_bfs_coop<false>);
+  const PieceSet pset = gaps ? kPmaPieces : kTraversalPieces;
+  const int nblocks = coop_grid(kern, kBlock, (int)g->tuning("coop_blocks_per_cu"));
+  a.pieces = make_pieces(g, nblocks, pset);
   a.bsum = g->buf("bfs_coop_bsum", (size_t)nblocks * 8 + 64).as<unsigned long long>();
   a.bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
   BYO_HIP(hipMemsetAsync(a.bar, 0, sizeof(GridBar), s));
   BYO_HIP(hipMemcpyAsync(dsrc, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s));
   void *args[] = {&a};
   PhaseTimer t(s, true);
-  BYO_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_bfs_coop), dim3(nblocks), dim3(kBlock), args, 0,
-                                     s));
-  pieces_launched(g, a.pieces, nblocks);
+  BYO_HIP(hipLaunchCooperativeKernel(kern, dim3(nblocks), dim3(kBlock), args, 0, s));
+  pieces_launched(g, a.pieces, nblocks, pset);
   const double ms = t.stop();
   if (ms_out) *ms_out = ms;
   if (stats) {

@@ -12,6 +12,9 @@
 // symmetric graph every edge leaving the giant component is seen from its
 // other end).  The labels do not depend on the schedule.
 //
+// Both read the container's own adjacency (AdjView): a CSR, or the CPMA's
+// slot index and dst words, whose empty slots (kNone) are skipped.
+//
 // byo_cc_labelprop restates labelprop_cc::CC (LabelPropagation/
 // Connectivity.h:78-146): rounds of dense-forward min propagation over the
 // changed set until it is empty.
@@ -57,7 +60,10 @@ __global__ void k_cc_init(uint32_t *par, uint64_t n) {
 __global__ void k_cc_link_sample(const uint64_t *off, const uint32_t *ids, uint64_t n, uint32_t *par, int r) {
   for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
     const uint64_t b = off[v];
-    if (off[v + 1] - b > (uint64_t)r) uf_unite(par, (uint32_t)v, ids[b + r]);
+    if (off[v + 1] - b > (uint64_t)r) {
+      const uint32_t u = ids[b + r];
+      if (u != kNone) uf_unite(par, (uint32_t)v, u);
+    }
   }
 }
 
@@ -79,7 +85,10 @@ __global__ void k_cc_link_rest(const uint64_t *off, const uint32_t *ids, uint64_
   for (uint64_t v = wave; v < n; v += nwaves) {
     if (uf_find(par, (uint32_t)v) == giant) continue;
     const uint64_t b = off[v] + r0, e = off[v + 1];
-    for (uint64_t k = b + lane; k < e; k += 64) uf_unite(par, (uint32_t)v, ids[k]);
+    for (uint64_t k = b + lane; k < e; k += 64) {
+      const uint32_t u = ids[k];
+      if (u != kNone) uf_unite(par, (uint32_t)v, u);
+    }
   }
 }
 
@@ -110,7 +119,7 @@ __global__ void k_lp_round(const uint64_t *off, const uint32_t *ids, uint64_t n,
       bool any = false;
       for (uint64_t k = off[u] + lane; k < off[u + 1]; k += 64) {
         const uint32_t t = ids[k];
-        if (lu < lab[t]) {
+        if (t != kNone && lu < lab[t]) {
           const uint32_t old = atomicMin(&lab[t], lu);
           if (lu < old) {
             atomicOr(reinterpret_cast<unsigned long long *>(&nbm64[t >> 6]), 1ull << (t & 63));
@@ -139,14 +148,14 @@ int byo_cc(byo_graph_t g, uint32_t *labels, double *device_ms) {
     BYO_REQUIRE(g, BYO_E_INVAL, "null handle");
     std::lock_guard<std::mutex> lk(g->mu);
     bind(g);
-    ensure_csr(g);
     const uint64_t n = g->n;
     if (n == 0) return;
+    const AdjView av = adj_view(g);
     hipStream_t s = g->stream;
     uint32_t *par = g->buf("cc_par", n * 4 + 64).as<uint32_t>();
     uint32_t *samp = g->buf("cc_samp", 1024 * 4).as<uint32_t>();
-    const uint64_t *off = g->off.as<uint64_t>();
-    const uint32_t *ids = g->ids.as<uint32_t>();
+    const uint64_t *off = av.off;
+    const uint32_t *ids = av.ids;
     const unsigned grid = byo_grid(n, 256, 16384);
     PhaseTimer t(s, true);
     k_cc_init<<<grid, 256, 0, s>>>(par, n);
@@ -187,12 +196,12 @@ int byo_cc_labelprop(byo_graph_t g, uint32_t *labels, uint32_t *rounds, double *
     BYO_REQUIRE(g, BYO_E_INVAL, "null handle");
     std::lock_guard<std::mutex> lk(g->mu);
     bind(g);
-    ensure_csr(g);
     const uint64_t n = g->n;
     if (n == 0) {
       if (rounds) *rounds = 0;
       return;
     }
+    const AdjView av = adj_view(g);
     hipStream_t s = g->stream;
     const uint64_t nw = (n + 63) / 64 + 1;
     uint32_t *lab = g->buf("lp_lab", n * 4 + 64).as<uint32_t>();
@@ -200,8 +209,8 @@ int byo_cc_labelprop(byo_graph_t g, uint32_t *labels, uint32_t *rounds, double *
     uint64_t *f1 = g->buf("lp_f1", nw * 8).as<uint64_t>();
     unsigned long long *chg = g->buf("lp_chg", 64).as<unsigned long long>();
     unsigned long long *h = static_cast<unsigned long long *>(pinned(g, 64));
-    const uint64_t *off = g->off.as<uint64_t>();
-    const uint32_t *ids = g->ids.as<uint32_t>();
+    const uint64_t *off = av.off;
+    const uint32_t *ids = av.ids;
     PhaseTimer t(s, true);
     k_lp_init<<<byo_grid(n, 256, 16384), 256, 0, s>>>(lab, n, f0, nw);
     uint32_t r = 0;

@@ -193,8 +193,8 @@ struct byo_graph {
     bool valid = false;
     int nblocks = 0;
     uint32_t per_wave = 0;
-  } pcache[2];
-  void pieces_stale() { pcache[0].valid = pcache[1].valid = false; }
+  } pcache[3];
+  void pieces_stale() { pcache[0].valid = pcache[1].valid = pcache[2].valid = false; }
   bool xs_valid = false;      // PageRank per-XCD slices are current (for layout xs_layout)
   int xs_layout = 0;
   int64_t last_src = -1;      // last source of byo_bfs / byo_bfs_multi
@@ -271,6 +271,24 @@ void csr_upload(byo_graph *g, const uint64_t *offsets, const uint32_t *ids, uint
 void check_csr_input(const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m);
 // Current CSR view of any container (the CPMA materialises one lazily).
 void ensure_csr(byo_graph *g);
+// Adjacency as the traversal kernels read it: v's neighbour list is
+// ids[off[v] .. off[v+1]).  For a CSR the lists are dense and deg is null
+// (degree = off[v+1] - off[v]).  For the CPMA it is the packed-memory array
+// itself: off holds the per-vertex slot starts, ids the PMA's dst words,
+// lists may hold empty slots (kNone) at leaf ends, and deg holds the true
+// degrees.  nslots = off[n].
+struct AdjView {
+  const uint64_t *off;
+  const uint32_t *ids;
+  const uint32_t *deg;
+  uint64_t nslots;
+};
+// The container's own adjacency (CPMA: slot index built on demand after an
+// update); the caller holds the handle's lock.
+AdjView adj_view(byo_graph *g);
+AdjView cpma_view(byo_graph *g);
+// The CPMA's maintained degrees (uint32[n], device).
+const uint32_t *cpma_degrees(byo_graph *g);
 // Internal hooks implemented in other translation units.
 void cpma_destroy(byo_graph *g);
 void cpma_ensure_csr(byo_graph *g);  // refresh g->off/g->ids from the PMA

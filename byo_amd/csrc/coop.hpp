@@ -60,6 +60,14 @@ __device__ __forceinline__ void grid_barrier(GridBar *b, unsigned int nblocks) {
   __syncthreads();
 }
 
+// Degree of v over an adjacency view (G: the lists may hold empty slots, so
+// the true degree comes from deg[]).
+template <bool G>
+__device__ __forceinline__ uint64_t vdeg(const uint64_t *off, const uint32_t *deg, uint64_t v) {
+  if constexpr (G) return deg[v];
+  else return off[v + 1] - off[v];
+}
+
 // Block-wide sum (all threads get the result); lds holds kCoopWaves slots.
 __device__ __forceinline__ unsigned long long block_sum(unsigned long long v, unsigned long long *lds) {
   v = wave_sum(v);
@@ -222,7 +230,7 @@ int coop_grid(const void *kernel, int block, int cap);
 // so they are computed by the first launch after either changes and reused
 // by the next ones (ready set); a launcher reports a queued launch with
 // pieces_launched.
-enum PieceSet { kTraversalPieces = 0, kPprPieces = 1 };
+enum PieceSet { kTraversalPieces = 0, kPprPieces = 1, kPmaPieces = 2 };
 Pieces make_pieces(byo_graph *g, int nblocks, PieceSet set = kTraversalPieces);
 // Records that a launch computing pc's bounds has been queued.
 void pieces_launched(byo_graph *g, const Pieces &pc, int nblocks, PieceSet set = kTraversalPieces);

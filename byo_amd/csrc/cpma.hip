@@ -4,7 +4,11 @@
 // run_structures/run_single_pma.cc:29-140): the edge set is the set of
 // uint64 keys (src<<32)|dst (run_single_pma.cc:83), kept sorted in fixed
 // size leaves of kLeaf slots.  Leaf l holds cnt[l] keys packed at its start
-// (the rest are the empty key ~0), all in [sep[l], sep[l+1]).
+// (the rest are the empty key ~0), all in [sep[l], sep[l+1]).  The slots are
+// stored as two word arrays, src (high word) and dst (low word), so that a
+// neighbour scan over the PMA reads only 4-byte dst words: with the
+// per-vertex slot starts (cpma_index), the dst array is a CSR whose lists
+// may hold empty slots (dst ~0) at leaf ends.
 //
 // insert_batch / remove_batch (Graph::insert_batch, gbbs/graph.h:3071-3124:
 // sort + unique, then insert_sorted_batch):
@@ -37,14 +41,51 @@ constexpr uint64_t kEmpty = ~0ull;
 struct Cpma {
   uint64_t nleaves = 0;
   uint64_t total = 0;
-  DevBuf slots;  // uint64[nleaves * kLeaf]
+  DevBuf hi;     // uint32[nleaves * kLeaf]: src word of every slot (~0 empty)
+  DevBuf lo;     // uint32[nleaves * kLeaf]: dst word of every slot (~0 empty)
   DevBuf cnt;    // uint32[nleaves]
   DevBuf sep;    // uint64[nleaves + 1]
   DevBuf deg;    // uint32[n]
   uint64_t rebalances = 0;
+  // per-vertex slot starts (n + 1; getExtraData's per-vertex index,
+  // run_single_pma.cc:88): v's keys lie in slots [vstart[v], vstart[v+1]),
+  // with empty slots at leaf ends in between; rebuilt after an update
+  DevBuf vstart;
+  bool index_valid = false;
 };
 
 static Cpma *state(byo_graph *g) { return static_cast<Cpma *>(g->cpma); }
+
+// Slot access over the two word arrays.
+struct Leaves {
+  uint32_t *hi, *lo;
+  __device__ __forceinline__ uint64_t get(uint64_t s) const { return ((uint64_t)hi[s] << 32) | lo[s]; }
+  __device__ __forceinline__ void put(uint64_t s, uint64_t k) const {
+    hi[s] = (uint32_t)(k >> 32);
+    lo[s] = (uint32_t)k;
+  }
+  // x among the len sorted keys from slot base
+  __device__ __forceinline__ bool has(uint64_t base, uint64_t len, uint64_t x) const {
+    uint64_t a = 0, b = len;
+    while (a < b) {
+      const uint64_t mid = (a + b) >> 1;
+      if (get(base + mid) < x) a = mid + 1; else b = mid;
+    }
+    return a < len && get(base + a) == x;
+  }
+};
+
+static Leaves leaves(Cpma *c) { return Leaves{c->hi.as<uint32_t>(), c->lo.as<uint32_t>()}; }
+
+// Every view derived from the edge set is stale after a change.
+static void cpma_changed(byo_graph *g) {
+  g->csr_valid = false;
+  g->maxdeg_valid = false;
+  g->pieces_stale();
+  g->rl_valid = false;
+  g->xs_valid = false;
+  state(g)->index_valid = false;
+}
 
 namespace {
 
@@ -63,15 +104,14 @@ __global__ void k_check_ids(const uint64_t *keys, uint64_t b, uint64_t n, UpdCtl
 }
 
 // Leaf l receives keys [l*m/L, (l+1)*m/L) of the sorted unique array.
-__global__ void k_spread(const uint64_t *keys, uint64_t m, uint64_t nleaves, uint64_t *slots, uint32_t *cnt,
+__global__ void k_spread(const uint64_t *keys, uint64_t m, uint64_t nleaves, Leaves lv, uint32_t *cnt,
                          uint64_t *sep) {
   const int lane = lane_id();
   const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
   for (uint64_t l = wave; l < nleaves; l += nwaves) {
     const uint64_t s = (uint64_t)(((unsigned __int128)m * l) / nleaves);
     const uint64_t e = (uint64_t)(((unsigned __int128)m * (l + 1)) / nleaves);
-    uint64_t *leaf = slots + l * kLeaf;
-    for (uint32_t i = lane; i < kLeaf; i += 64) leaf[i] = (s + i < e) ? keys[s + i] : kEmpty;
+    for (uint32_t i = lane; i < kLeaf; i += 64) lv.put(l * kLeaf + i, (s + i < e) ? keys[s + i] : kEmpty);
     if (lane == 0) {
       cnt[l] = (uint32_t)(e - s);
       sep[l] = (l == 0) ? 0ull : (s < m ? keys[s] : kEmpty);
@@ -144,7 +184,7 @@ __device__ __forceinline__ uint32_t lower_rank(const uint64_t *a, uint32_t len, 
 
 // Pass 1 of an insert: count the segment keys absent from their leaf.
 __global__ void __launch_bounds__(256) k_ins_count(const uint64_t *keys, UpdDev d, const uint32_t *starts,
-                                                   const uint32_t *leaf_of, const uint64_t *slots,
+                                                   const uint32_t *leaf_of, Leaves lv,
                                                    const uint32_t *cnt, uint32_t *newc, UpdCtl *ctl) {
   __shared__ uint32_t acc;
   if (d.halt && *d.halt) return;
@@ -152,12 +192,12 @@ __global__ void __launch_bounds__(256) k_ins_count(const uint64_t *keys, UpdDev 
   for (uint64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
     const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
     const uint32_t l = leaf_of[s];
-    const uint64_t *leaf = slots + (uint64_t)l * kLeaf;
+    const uint64_t leaf = (uint64_t)l * kLeaf;
     const uint32_t c = cnt[l];
     if (threadIdx.x == 0) acc = 0;
     __syncthreads();
     uint32_t mine = 0;
-    for (uint64_t i = s + threadIdx.x; i < e; i += blockDim.x) mine += !in_sorted(leaf, c, keys[i]);
+    for (uint64_t i = s + threadIdx.x; i < e; i += blockDim.x) mine += !lv.has(leaf, c, keys[i]);
     mine = wave_sum(mine);
     if ((threadIdx.x & 63) == 0 && mine) atomicAdd(&acc, mine);
     __syncthreads();
@@ -171,7 +211,7 @@ __global__ void __launch_bounds__(256) k_ins_count(const uint64_t *keys, UpdDev 
 
 // Pass 2 of an insert (no leaf overflows): merge in LDS and write back.
 __global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, UpdDev d, const uint32_t *starts,
-                                                   const uint32_t *leaf_of, uint64_t *slots, uint32_t *cnt,
+                                                   const uint32_t *leaf_of, Leaves lv, uint32_t *cnt,
                                                    uint32_t *deg, UpdCtl *ctl, const uint32_t *cand,
                                                    uint64_t nleaves) {
   __shared__ uint64_t old_keys[kLeaf];
@@ -196,9 +236,9 @@ __global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, UpdDev 
         }
       if (inw) continue;
     }
-    uint64_t *leaf = slots + (uint64_t)l * kLeaf;
+    const uint64_t leaf = (uint64_t)l * kLeaf;
     const uint32_t c = cnt[l];
-    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) old_keys[i] = leaf[i];
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) old_keys[i] = lv.get(leaf + i);
     if (threadIdx.x == 0) nadd = 0;
     __syncthreads();
     // stable compaction of the absent keys, 256 at a time
@@ -224,11 +264,11 @@ __global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, UpdDev 
     // merged position of each key = own index + rank in the other list
     for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) {
       const uint64_t k = old_keys[i];
-      leaf[i + lower_rank(add, na, k)] = k;
+      lv.put(leaf + i + lower_rank(add, na, k), k);
     }
     for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) {
       const uint64_t k = add[i];
-      leaf[i + lower_rank(old_keys, c, k)] = k;
+      lv.put(leaf + i + lower_rank(old_keys, c, k), k);
       atomicAdd(&deg[k >> 32], 1u);
     }
     if (threadIdx.x == 0) {
@@ -241,7 +281,7 @@ __global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, UpdDev 
 
 // Removal: each touched leaf keeps the keys absent from its segment.
 __global__ void __launch_bounds__(256) k_remove(const uint64_t *keys, UpdDev d, const uint32_t *starts,
-                                                const uint32_t *leaf_of, uint64_t *slots, uint32_t *cnt,
+                                                const uint32_t *leaf_of, Leaves lv, uint32_t *cnt,
                                                 uint32_t *deg, UpdCtl *ctl) {
   __shared__ uint64_t kept[kLeaf];
   __shared__ uint32_t wcnt[4];
@@ -251,7 +291,7 @@ __global__ void __launch_bounds__(256) k_remove(const uint64_t *keys, UpdDev d, 
   for (uint64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
     const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
     const uint32_t l = leaf_of[s];
-    uint64_t *leaf = slots + (uint64_t)l * kLeaf;
+    const uint64_t leaf = (uint64_t)l * kLeaf;
     const uint32_t c = cnt[l];
     uint32_t run = 0;
     for (uint32_t base = 0; base < c; base += blockDim.x) {
@@ -259,7 +299,7 @@ __global__ void __launch_bounds__(256) k_remove(const uint64_t *keys, UpdDev d, 
       uint64_t k = 0;
       bool keep = false, gone = false;
       if (i < c) {
-        k = leaf[i];
+        k = lv.get(leaf + i);
         gone = in_sorted(keys + s, e - s, k);
         keep = !gone;
       }
@@ -273,7 +313,7 @@ __global__ void __launch_bounds__(256) k_remove(const uint64_t *keys, UpdDev d, 
       run += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
       __syncthreads();
     }
-    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) leaf[i] = i < run ? kept[i] : kEmpty;
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) lv.put(leaf + i, i < run ? kept[i] : kEmpty);
     if (threadIdx.x == 0) {
       cnt[l] = run;
       if (c > run) atomicAdd(&ctl->changed, (unsigned long long)(c - run));
@@ -297,7 +337,7 @@ struct WinArgs {
   const uint64_t *keys;  // sorted unique batch
   UpdDev d;
   const uint32_t *starts, *leaf_of, *newc;
-  uint64_t *slots;
+  Leaves lv;
   uint32_t *cnt;
   uint64_t *sep;
   uint32_t *deg;
@@ -434,7 +474,7 @@ __global__ void k_win_w2(WinArgs w) {
     if (!root_of(w, l, &b)) continue;
     const uint32_t c = w.cnt[l];
     uint64_t *dst = w.stage + b * kLeaf + (w.pc[l] - w.pc[b]);
-    for (uint32_t i = lane; i < c; i += 64) dst[i] = w.slots[l * kLeaf + i];
+    for (uint32_t i = lane; i < c; i += 64) dst[i] = w.lv.get(l * kLeaf + i);
   }
 }
 
@@ -446,7 +486,7 @@ __global__ void k_win_w3(WinArgs w) {
     if (i < bu) {
       const uint32_t l = w.leaf_of[i];
       uint64_t b = 0;
-      if (root_of(w, l, &b)) f = !in_sorted(w.slots + (uint64_t)l * kLeaf, w.cnt[l], w.keys[i]);
+      if (root_of(w, l, &b)) f = !w.lv.has((uint64_t)l * kLeaf, w.cnt[l], w.keys[i]);
     }
     w.nflag[i] = f;
   }
@@ -540,7 +580,7 @@ __global__ void k_win_w6(WinArgs w) {
           si = snext;
           snext = (uint64_t)(((unsigned __int128)(i + 1) * T) / W);
         }
-        w.slots[(b + i) * kLeaf + (q - si)] = key;
+        w.lv.put((b + i) * kLeaf + (q - si), key);
         if (q == si && i > 0) w.sep[b + i] = key;
       }
       p += q1 - q0;
@@ -565,7 +605,7 @@ __global__ void k_win_w7(WinArgs w) {
     const uint64_t i = l - b;
     const uint64_t si = (uint64_t)(((unsigned __int128)i * T) / W);
     const uint64_t ei = (uint64_t)(((unsigned __int128)(i + 1) * T) / W);
-    for (uint64_t x = (ei - si) + lane; x < kLeaf; x += 64) w.slots[l * kLeaf + x] = kEmpty;
+    for (uint64_t x = (ei - si) + lane; x < kLeaf; x += 64) w.lv.put(l * kLeaf + x, kEmpty);
     if (lane == 0) w.cnt[l] = (uint32_t)(ei - si);
   }
 }
@@ -680,15 +720,14 @@ __global__ void k_widen(const uint32_t *a, uint64_t *b, uint64_t n) {
 }
 
 // Compaction of all leaves into a flat sorted array (pos = prefix of cnt).
-__global__ void k_compact(const uint64_t *slots, const uint32_t *cnt, const uint64_t *pos, uint64_t nleaves,
+__global__ void k_compact(Leaves lv, const uint32_t *cnt, const uint64_t *pos, uint64_t nleaves,
                           uint64_t *flat) {
   const int lane = lane_id();
   const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
   for (uint64_t l = wave; l < nleaves; l += nwaves) {
     const uint32_t c = cnt[l];
     const uint64_t p = pos[l];
-    const uint64_t *leaf = slots + l * kLeaf;
-    for (uint32_t i = lane; i < c; i += 64) flat[p + i] = leaf[i];
+    for (uint32_t i = lane; i < c; i += 64) flat[p + i] = lv.get(l * kLeaf + i);
   }
 }
 
@@ -737,13 +776,14 @@ static void cpma_build(byo_graph *g, const uint64_t *flat, uint64_t m) {
   Cpma *c = state(g);
   hipStream_t s = g->stream;
   const uint64_t nl = leaves_for(m);
-  if (c->nleaves != nl || !c->slots.p) {
-    c->slots.alloc(nl * kLeaf * sizeof(uint64_t));
+  if (c->nleaves != nl || !c->lo.p) {
+    c->hi.alloc(nl * kLeaf * sizeof(uint32_t));
+    c->lo.alloc(nl * kLeaf * sizeof(uint32_t));
     c->cnt.alloc(nl * sizeof(uint32_t));
     c->sep.alloc((nl + 1) * sizeof(uint64_t));
     c->nleaves = nl;
   }
-  k_spread<<<byo_grid(nl * 64, 256, 16384), 256, 0, s>>>(flat, m, nl, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+  k_spread<<<byo_grid(nl * 64, 256, 16384), 256, 0, s>>>(flat, m, nl, leaves(c), c->cnt.as<uint32_t>(),
                                                          c->sep.as<uint64_t>());
   BYO_LAUNCH_CHECK();
   c->total = m;
@@ -756,11 +796,8 @@ static void cpma_build(byo_graph *g, const uint64_t *flat, uint64_t m) {
   if (g->n)
     k_degrees_from_off<<<byo_grid(g->n, 256, 16384), 256, 0, s>>>(g->off.as<uint64_t>(), g->n, c->deg.as<uint32_t>());
   BYO_LAUNCH_CHECK();
+  cpma_changed(g);
   g->csr_valid = true;
-  g->maxdeg_valid = false;
-  g->pieces_stale();
-  g->rl_valid = false;
-  g->xs_valid = false;
 }
 
 // Compacts the leaves into flat[total] (sorted, unique).
@@ -773,26 +810,83 @@ static void cpma_flatten(byo_graph *g, uint64_t *flat) {
   // widen counts to uint64 for the scan
   k_widen<<<byo_grid(c->nleaves, 256, 16384), 256, 0, s>>>(c->cnt.as<uint32_t>(), cnt64, c->nleaves);
   prim::exclusive_scan_u64(cnt64, pos, c->nleaves, true, tmp, s);
-  k_compact<<<byo_grid(c->nleaves * 64, 256, 16384), 256, 0, s>>>(c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(), pos,
+  k_compact<<<byo_grid(c->nleaves * 64, 256, 16384), 256, 0, s>>>(leaves(c), c->cnt.as<uint32_t>(), pos,
                                                                   c->nleaves, flat);
   BYO_LAUNCH_CHECK();
 }
 
+// The CSR export of the CPMA (byo_export_csr, byo_neighbors and the
+// consumers that read a CSR); timed as profile phase "cpma_csr_export".
 void cpma_ensure_csr(byo_graph *g) {
   Cpma *c = state(g);
   hipStream_t s = g->stream;
+  PhaseTimer t(s, g->prof.on);
   uint64_t *flat = g->buf("cpma_flat", c->total * 8 + 64).as<uint64_t>();
   cpma_flatten(g, flat);
   g->off.ensure((g->n + 1) * sizeof(uint64_t));
   g->ids.ensure(c->total * sizeof(uint32_t) + 16);
   prim::keys_to_csr(flat, c->total, g->n, g->off.as<uint64_t>(), g->ids.as<uint32_t>(), s);
   BYO_HIP(hipStreamSynchronize(s));
+  if (g->prof.on) g->account("cpma_csr_export", t.stop());
   g->csr_valid = true;
   g->maxdeg_valid = false;
   g->pieces_stale();
   g->rl_valid = false;
   g->xs_valid = false;
 }
+
+namespace {
+// vstart[v] = slot of the key of rank rank[v] (its first key; the next
+// vertex's first key when v has none); ranks at or past the total map to
+// the end of the array.  pc[l] = keys before leaf l (pc[nleaves] = total).
+__global__ void k_vstart(const uint64_t *rank, uint64_t n, const uint64_t *pc, uint64_t nleaves, uint64_t total,
+                         uint64_t *vstart) {
+  for (uint64_t v = global_tid(); v <= n; v += grid_threads()) {
+    const uint64_t r = rank[v];
+    if (r >= total) {
+      vstart[v] = nleaves * kLeaf;
+      continue;
+    }
+    uint64_t lo = 0, hi = nleaves - 1;  // first leaf l with pc[l + 1] > r
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (pc[mid + 1] > r) hi = mid; else lo = mid + 1;
+    }
+    vstart[v] = lo * kLeaf + (r - pc[lo]);
+  }
+}
+}  // namespace
+
+AdjView cpma_view(byo_graph *g) {
+  Cpma *c = state(g);
+  if (!c->index_valid) {
+    hipStream_t s = g->stream;
+    PhaseTimer t(s, g->prof.on);
+    const uint64_t n = g->n, nl = c->nleaves;
+    c->vstart.ensure((n + 1) * 8 + 64);
+    uint64_t *w = g->buf("pidx_w", (std::max(n, nl) + 1) * 8 + 64).as<uint64_t>();
+    uint64_t *rank = g->buf("pidx_rank", (n + 2) * 8 + 64).as<uint64_t>();
+    uint64_t *pc = g->buf("pidx_pc", (nl + 2) * 8 + 64).as<uint64_t>();
+    void *tmp = g->buf("pidx_tmp", prim::scan_tmp_bytes(std::max(n, nl) + 1)).p;
+    // ranks: exclusive prefix of the degrees; leaf starts: of the counts
+    if (n) {
+      k_widen<<<byo_grid(n, 256, 16384), 256, 0, s>>>(c->deg.as<uint32_t>(), w, n);
+      prim::exclusive_scan_u64(w, rank, n, true, tmp, s);
+    } else {
+      prim::fill_u64(rank, 0, 1, s);
+    }
+    k_widen<<<byo_grid(nl, 256, 16384), 256, 0, s>>>(c->cnt.as<uint32_t>(), w, nl);
+    prim::exclusive_scan_u64(w, pc, nl, true, tmp, s);
+    k_vstart<<<byo_grid(n + 1, 256, 16384), 256, 0, s>>>(rank, n, pc, nl, c->total, c->vstart.as<uint64_t>());
+    BYO_LAUNCH_CHECK();
+    BYO_HIP(hipStreamSynchronize(s));
+    if (g->prof.on) g->account("cpma_index", t.stop());
+    c->index_valid = true;
+  }
+  return AdjView{c->vstart.as<uint64_t>(), c->lo.as<uint32_t>(), c->deg.as<uint32_t>(), c->nleaves * kLeaf};
+}
+
+const uint32_t *cpma_degrees(byo_graph *g) { return state(g)->deg.as<uint32_t>(); }
 
 void cpma_destroy(byo_graph *g) {
   delete state(g);
@@ -801,9 +895,9 @@ void cpma_destroy(byo_graph *g) {
 
 uint64_t cpma_bytes(byo_graph *g) {
   Cpma *c = state(g);
-  // leaves + counts + separators + degrees (get_memory_size counts the PMA
-  // and two per-vertex arrays, run_single_pma.cc:126-131)
-  return c->nleaves * kLeaf * 8 + c->nleaves * 4 + (c->nleaves + 1) * 8 + g->n * 4;
+  // leaves + counts + separators + degrees + slot starts (get_memory_size
+  // counts the PMA and two per-vertex arrays, run_single_pma.cc:126-131)
+  return c->nleaves * kLeaf * 8 + c->nleaves * 4 + (c->nleaves + 1) * 8 + g->n * 4 + (g->n + 1) * 8;
 }
 
 namespace {
@@ -886,7 +980,7 @@ static WinArgs win_args(byo_graph *g, const uint64_t *batch, const UpdDev &d, co
   w.starts = u.starts;
   w.leaf_of = u.leaf;
   w.newc = u.newc;
-  w.slots = c->slots.as<uint64_t>();
+  w.lv = leaves(c);
   w.cnt = c->cnt.as<uint32_t>();
   w.sep = c->sep.as<uint64_t>();
   w.deg = c->deg.as<uint32_t>();
@@ -957,7 +1051,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
   const UpdDev d{&ctl->pad[0], u.pos + b, nullptr};
   const unsigned grid = (unsigned)std::min<uint64_t>(b, 16384);
   if (insert) {
-    k_ins_count<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+    k_ins_count<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, leaves(c), c->cnt.as<uint32_t>(),
                                      u.newc, ctl);
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
@@ -973,7 +1067,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
       window_rebuild(g, w);
       cand = w.cand;
     }
-    k_ins_merge<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+    k_ins_merge<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, leaves(c), c->cnt.as<uint32_t>(),
                                      c->deg.as<uint32_t>(), ctl, cand, c->nleaves);
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
@@ -982,13 +1076,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
       if (hctl->overflow) c->rebalances++;
       c->total += hctl->changed;
       g->m = c->total;
-      if (hctl->changed) {
-        g->csr_valid = false;
-        g->maxdeg_valid = false;
-        g->pieces_stale();
-        g->rl_valid = false;
-        g->xs_valid = false;
-      }
+      if (hctl->changed) cpma_changed(g);
     } else {
       // global rebalance: flatten, merge with the batch, dedupe, respread
       c->rebalances++;
@@ -1008,7 +1096,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
       BYO_HIP(hipStreamSynchronize(s));
     }
   } else {
-    k_remove<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+    k_remove<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, leaves(c), c->cnt.as<uint32_t>(),
                                   c->deg.as<uint32_t>(), ctl);
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
@@ -1017,11 +1105,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
     c->total -= hctl->changed;
     g->m = c->total;
     if (hctl->changed) {
-      g->csr_valid = false;
-      g->maxdeg_valid = false;
-      g->pieces_stale();
-      g->rl_valid = false;
-      g->xs_valid = false;
+      cpma_changed(g);
     }
     if (c->nleaves > 1 && (double)c->total < kMinDensity * (double)(c->nleaves * kLeaf)) {
       c->rebalances++;
@@ -1059,13 +1143,13 @@ static uint32_t insert_graph(byo_graph *g, const uint64_t *keys, const uint64_t 
       const uint64_t *batch = sort_and_segment(g, keys + offs[i], b, ctl, u);
       const UpdDev d{&ctl->pad[0], u.pos + b, halt};
       const unsigned grid = (unsigned)std::min<uint64_t>(b, 16384);
-      k_ins_count<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+      k_ins_count<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, leaves(c), c->cnt.as<uint32_t>(),
                                        u.newc, ctl);
       const WinArgs w = win_args(g, batch, d, u, ctl, cap);
       window_pick(g, w, b);
       k_gate<<<1, 1, 0, s>>>(ctl, halt, i + 1);
       window_rebuild(g, w);
-      k_ins_merge<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+      k_ins_merge<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, leaves(c), c->cnt.as<uint32_t>(),
                                        c->deg.as<uint32_t>(), ctl, w.cand, c->nleaves);
     }
     BYO_LAUNCH_CHECK();
@@ -1108,11 +1192,7 @@ static uint32_t insert_graph(byo_graph *g, const uint64_t *keys, const uint64_t 
   c->total += changed;
   g->m = c->total;
   if (changed) {
-    g->csr_valid = false;
-    g->maxdeg_valid = false;
-    g->pieces_stale();
-    g->rl_valid = false;
-    g->xs_valid = false;
+    cpma_changed(g);
   }
   return stop;
 }
@@ -1326,7 +1406,7 @@ extern "C" int byo_insert_batch_dev_noalloc(byo_graph_t g, const uint64_t *keys_
                                               starts, nseg, ctl);
     const UpdDev d{&ctl->pad[0], nseg, nullptr};
     const unsigned grid = (unsigned)std::min<uint64_t>(b, 16384);
-    k_ins_count<<<grid, 256, 0, s>>>(uniq, d, starts, leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(), newc,
+    k_ins_count<<<grid, 256, 0, s>>>(uniq, d, starts, leaf, leaves(c), c->cnt.as<uint32_t>(), newc,
                                      ctl);
     BYO_LAUNCH_CHECK();
     UpdCtl h{};
@@ -1338,7 +1418,7 @@ extern "C" int byo_insert_batch_dev_noalloc(byo_graph_t g, const uint64_t *keys_
     // the container: refuse, with the container unchanged
     BYO_REQUIRE(h.overflow == 0, BYO_E_STATE,
                 "a leaf would overflow: this batch needs a rebalance, which allocates (use byo_insert_batch_dev)");
-    k_ins_merge<<<grid, 256, 0, s>>>(uniq, d, starts, leaf, c->slots.as<uint64_t>(), c->cnt.as<uint32_t>(),
+    k_ins_merge<<<grid, 256, 0, s>>>(uniq, d, starts, leaf, leaves(c), c->cnt.as<uint32_t>(),
                                      c->deg.as<uint32_t>(), ctl, nullptr, c->nleaves);
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipMemcpyAsync(&h, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
@@ -1346,11 +1426,7 @@ extern "C" int byo_insert_batch_dev_noalloc(byo_graph_t g, const uint64_t *keys_
     c->total += h.changed;
     g->m = c->total;
     if (h.changed) {
-      g->csr_valid = false;
-      g->maxdeg_valid = false;
-      g->pieces_stale();
-      g->rl_valid = false;
-      g->xs_valid = false;
+      cpma_changed(g);
     }
   });
 }

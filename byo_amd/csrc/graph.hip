@@ -139,6 +139,12 @@ __global__ void k_update_rmat(uint32_t nn, uint32_t h, uint64_t first, uint64_t 
   }
 }
 
+// H: ids is the CPMA's dst word array, whose empty slots (kNone) add nothing.
+template <bool H>
+__device__ __forceinline__ uint64_t id_val(uint32_t x) {
+  return (H && x == kNone) ? 0 : (uint64_t)x;
+}
+template <bool H>
 __global__ void k_map_sum(const uint32_t *ids, uint64_t m, uint64_t *partial) {
   __shared__ uint64_t lds[16];
   uint64_t acc = 0;
@@ -146,9 +152,9 @@ __global__ void k_map_sum(const uint32_t *ids, uint64_t m, uint64_t *partial) {
   const uint4 *v4 = reinterpret_cast<const uint4 *>(ids);
   for (uint64_t i = global_tid(); i < nvec; i += grid_threads()) {
     uint4 q = v4[i];
-    acc += (uint64_t)q.x + q.y + q.z + q.w;
+    acc += id_val<H>(q.x) + id_val<H>(q.y) + id_val<H>(q.z) + id_val<H>(q.w);
   }
-  for (uint64_t i = nvec * 4 + global_tid(); i < m; i += grid_threads()) acc += ids[i];
+  for (uint64_t i = nvec * 4 + global_tid(); i < m; i += grid_threads()) acc += id_val<H>(ids[i]);
   acc = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
   __syncthreads();
@@ -285,6 +291,11 @@ void check_csr_input(const uint64_t *offsets, const uint32_t *ids, uint64_t n, u
 // Current CSR view of any container (the CPMA materialises one lazily).
 void ensure_csr(byo_graph *g) {
   if (g->kind == 1 && !g->csr_valid) cpma_ensure_csr(g);
+}
+
+AdjView adj_view(byo_graph *g) {
+  if (g->kind == 1) return cpma_view(g);
+  return AdjView{g->off.as<uint64_t>(), g->ids.as<uint32_t>(), nullptr, g->m};
 }
 
 }  // namespace byo
@@ -500,8 +511,13 @@ int byo_degrees(byo_graph_t g, uint32_t *deg) {
     BYO_REQUIRE(g && deg, BYO_E_INVAL, "null argument");
     std::lock_guard<std::mutex> lk(g->mu);
     bind(g);
-    ensure_csr(g);
     if (g->n == 0) return;
+    if (g->kind == 1) {
+      // the CPMA keeps its degrees up to date (no CSR view needed)
+      BYO_HIP(hipMemcpyAsync(deg, cpma_degrees(g), g->n * sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+      BYO_HIP(hipStreamSynchronize(g->stream));
+      return;
+    }
     auto &d = g->buf("degrees_out", g->n * sizeof(uint32_t));
     k_degrees<<<byo_grid(g->n, 256, 16384), 256, 0, g->stream>>>(g->off.as<uint64_t>(), g->n,
                                                                   d.as<uint32_t>());
@@ -517,8 +533,13 @@ int byo_degree(byo_graph_t g, uint32_t v, uint32_t *deg) {
     BYO_REQUIRE(v < g->n, BYO_E_INVAL, "vertex out of range");
     std::lock_guard<std::mutex> lk(g->mu);
     bind(g);
-    ensure_csr(g);
     uint64_t *h = static_cast<uint64_t *>(pinned(g, 64));
+    if (g->kind == 1) {
+      BYO_HIP(hipMemcpyAsync(h, cpma_degrees(g) + v, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+      BYO_HIP(hipStreamSynchronize(g->stream));
+      *deg = *reinterpret_cast<uint32_t *>(h);
+      return;
+    }
     BYO_HIP(hipMemcpyAsync(h, g->off.as<uint64_t>() + v, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                            g->stream));
     BYO_HIP(hipStreamSynchronize(g->stream));
@@ -579,11 +600,15 @@ int byo_map_sum(byo_graph_t g, uint64_t *sum, double *device_ms) {
     BYO_REQUIRE(g && sum, BYO_E_INVAL, "null argument");
     std::lock_guard<std::mutex> lk(g->mu);
     bind(g);
-    ensure_csr(g);
+    // the container's own adjacency: the CSR ids, or every slot of the PMA
+    const AdjView av = adj_view(g);
     const int nb = 2048;
     auto &part = g->buf("map_partial", (nb + 1) * sizeof(uint64_t));
     PhaseTimer t(g->stream, true);
-    k_map_sum<<<nb, 256, 0, g->stream>>>(g->ids.as<uint32_t>(), g->m, part.as<uint64_t>());
+    if (av.deg)
+      k_map_sum<true><<<nb, 256, 0, g->stream>>>(av.ids, av.nslots, part.as<uint64_t>());
+    else
+      k_map_sum<false><<<nb, 256, 0, g->stream>>>(av.ids, g->m, part.as<uint64_t>());
     k_sum_partials<<<1, 1024, 0, g->stream>>>(part.as<uint64_t>(), nb, part.as<uint64_t>() + nb);
     BYO_LAUNCH_CHECK();
     double ms = t.stop();

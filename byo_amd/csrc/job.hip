@@ -157,7 +157,9 @@ double job_launch(byo_graph *g, const uint32_t *srcs, uint32_t kb, uint64_t maxd
   // tuned residencies (the traversal body caps it at its own occupancy)
   const int cap = (int)std::max(g->tuning("coop_blocks_per_cu"), g->tuning("ppr_blocks_per_cu"));
   const int nblocks = coop_grid(kern, kCoopBlock, cap);
-  batch::prepare(g, srcs, std::max<uint32_t>(kb, 1), maxdeg, a, (int)sizeof(M), nblocks);
+  // the job reads the CSR view (exported from a CPMA by the caller)
+  const AdjView csr{g->off.as<uint64_t>(), g->ids.as<uint32_t>(), nullptr, g->m};
+  batch::prepare(g, csr, srcs, std::max<uint32_t>(kb, 1), maxdeg, a, (int)sizeof(M), nblocks);
   a.k = kb;
   q.pieces = make_pieces(g, nblocks, kPprPieces);
   void *args[] = {&a, &q};

@@ -119,7 +119,8 @@ __global__ void __launch_bounds__(256) k_pr_seg(const uint64_t *lo_pos, const ui
 // is -- waste the fewest lanes.
 constexpr int kPrUnroll = 16;
 
-template <class T, int G, bool kComp>
+// H: the lists may hold empty slots (kNone, the PMA's leaf ends): skipped.
+template <class T, int G, bool kComp, bool H>
 __device__ __forceinline__ Acc<T> pr_tile(uint64_t lo, uint64_t hi, const uint32_t *ids, const T *contrib_in,
                                           int lane) {
   constexpr int VPR = 64 / G;  // vertices per round
@@ -133,16 +134,19 @@ __device__ __forceinline__ Acc<T> pr_tile(uint64_t lo, uint64_t hi, const uint32
     for (uint64_t k = b + sub; k < e; k += (uint64_t)kPrUnroll * G) {
       uint32_t u[kPrUnroll];
       T x[kPrUnroll];
+      bool hole[kPrUnroll];
 #pragma unroll
       for (int j = 0; j < kPrUnroll; j++) {
         const uint64_t kk = k + (uint64_t)j * G;
         u[j] = ids[kk < e ? kk : b];
+        hole[j] = H && u[j] == kNone;
+        if (H && hole[j]) u[j] = 0;
       }
 #pragma unroll
       for (int j = 0; j < kPrUnroll; j++) x[j] = contrib_in[u[j]];
 #pragma unroll
       for (int j = 0; j < kPrUnroll; j++) {
-        if (k + (uint64_t)j * G < e) {
+        if (k + (uint64_t)j * G < e && !hole[j]) {
           if (kComp) kahan_add(a, x[j]); else a.s += x[j];
         }
       }
@@ -162,10 +166,13 @@ __device__ __forceinline__ Acc<T> pr_tile(uint64_t lo, uint64_t hi, const uint32
   return mine;
 }
 
-template <class T, bool kComp>
+// H: the adjacency view holds empty slots (the CPMA); degrees come from
+// deg[] (null otherwise).
+template <class T, bool kComp, bool H = false>
 __global__ void __launch_bounds__(256) k_pr_tiles(const uint64_t *off, const uint32_t *ids, uint64_t n,
                                                   const T *contrib_in, const T *p_old, T *p_new, T *contrib_out,
-                                                  double damping, double addc, double *partial) {
+                                                  double damping, double addc, double *partial,
+                                                  const uint32_t *degs = nullptr) {
   __shared__ double lds[4];
   const int lane = lane_id();
   const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
@@ -178,17 +185,17 @@ __global__ void __launch_bounds__(256) k_pr_tiles(const uint64_t *off, const uin
     const uint64_t lo = off[valid ? v : n], hi = off[valid ? v + 1 : n];
     const uint32_t dmax = wave_max((uint32_t)(hi - lo));
     Acc<T> mine;
-    if (dmax <= kPrUnroll) mine = pr_tile<T, 1, kComp>(lo, hi, ids, contrib_in, lane);
-    else if (dmax <= 2 * kPrUnroll) mine = pr_tile<T, 2, kComp>(lo, hi, ids, contrib_in, lane);
-    else if (dmax <= 4 * kPrUnroll) mine = pr_tile<T, 4, kComp>(lo, hi, ids, contrib_in, lane);
-    else if (dmax <= 8 * kPrUnroll) mine = pr_tile<T, 8, kComp>(lo, hi, ids, contrib_in, lane);
-    else if (dmax <= 16 * kPrUnroll) mine = pr_tile<T, 16, kComp>(lo, hi, ids, contrib_in, lane);
-    else if (dmax <= 32 * kPrUnroll) mine = pr_tile<T, 32, kComp>(lo, hi, ids, contrib_in, lane);
-    else mine = pr_tile<T, 64, kComp>(lo, hi, ids, contrib_in, lane);
+    if (dmax <= kPrUnroll) mine = pr_tile<T, 1, kComp, H>(lo, hi, ids, contrib_in, lane);
+    else if (dmax <= 2 * kPrUnroll) mine = pr_tile<T, 2, kComp, H>(lo, hi, ids, contrib_in, lane);
+    else if (dmax <= 4 * kPrUnroll) mine = pr_tile<T, 4, kComp, H>(lo, hi, ids, contrib_in, lane);
+    else if (dmax <= 8 * kPrUnroll) mine = pr_tile<T, 8, kComp, H>(lo, hi, ids, contrib_in, lane);
+    else if (dmax <= 16 * kPrUnroll) mine = pr_tile<T, 16, kComp, H>(lo, hi, ids, contrib_in, lane);
+    else if (dmax <= 32 * kPrUnroll) mine = pr_tile<T, 32, kComp, H>(lo, hi, ids, contrib_in, lane);
+    else mine = pr_tile<T, 64, kComp, H>(lo, hi, ids, contrib_in, lane);
     if (valid) {
       const T sum = kComp ? (mine.s - mine.c) : mine.s;
       const T pv = d * sum + t;
-      const uint64_t deg = hi - lo;
+      const uint64_t deg = H ? (uint64_t)degs[v] : hi - lo;
       p_new[v] = pv;
       contrib_out[v] = deg ? pv / (T)deg : (T)0;
       resid += fabs((double)pv - (double)p_old[v]);
@@ -228,13 +235,13 @@ __global__ void __launch_bounds__(256) k_pr_xcd(const uint32_t *so, const uint32
     const uint64_t lo = sox[valid ? v : n], hi = sox[valid ? v + 1 : n];
     const uint32_t dmax = wave_max((uint32_t)(hi - lo));
     Acc<T> mine;
-    if (dmax <= kPrUnroll) mine = pr_tile<T, 1, kComp>(lo, hi, ids_s, contrib_in, lane);
-    else if (dmax <= 2 * kPrUnroll) mine = pr_tile<T, 2, kComp>(lo, hi, ids_s, contrib_in, lane);
-    else if (dmax <= 4 * kPrUnroll) mine = pr_tile<T, 4, kComp>(lo, hi, ids_s, contrib_in, lane);
-    else if (dmax <= 8 * kPrUnroll) mine = pr_tile<T, 8, kComp>(lo, hi, ids_s, contrib_in, lane);
-    else if (dmax <= 16 * kPrUnroll) mine = pr_tile<T, 16, kComp>(lo, hi, ids_s, contrib_in, lane);
-    else if (dmax <= 32 * kPrUnroll) mine = pr_tile<T, 32, kComp>(lo, hi, ids_s, contrib_in, lane);
-    else mine = pr_tile<T, 64, kComp>(lo, hi, ids_s, contrib_in, lane);
+    if (dmax <= kPrUnroll) mine = pr_tile<T, 1, kComp, false>(lo, hi, ids_s, contrib_in, lane);
+    else if (dmax <= 2 * kPrUnroll) mine = pr_tile<T, 2, kComp, false>(lo, hi, ids_s, contrib_in, lane);
+    else if (dmax <= 4 * kPrUnroll) mine = pr_tile<T, 4, kComp, false>(lo, hi, ids_s, contrib_in, lane);
+    else if (dmax <= 8 * kPrUnroll) mine = pr_tile<T, 8, kComp, false>(lo, hi, ids_s, contrib_in, lane);
+    else if (dmax <= 16 * kPrUnroll) mine = pr_tile<T, 16, kComp, false>(lo, hi, ids_s, contrib_in, lane);
+    else if (dmax <= 32 * kPrUnroll) mine = pr_tile<T, 32, kComp, false>(lo, hi, ids_s, contrib_in, lane);
+    else mine = pr_tile<T, 64, kComp, false>(lo, hi, ids_s, contrib_in, lane);
     if (valid) {
       part_s[(uint64_t)x * n + v] = mine.s;
       if (kComp) part_c[(uint64_t)x * n + v] = mine.c;
@@ -362,10 +369,10 @@ __global__ void k_sum_doubles(const double *partial, int nb, double *out) {
 }
 
 template <class T>
-__global__ void k_pr_init(const uint64_t *off, uint64_t n, T *p, T *contrib) {
+__global__ void k_pr_init(const uint64_t *off, uint64_t n, T *p, T *contrib, const uint32_t *degs) {
   const double p0 = 1.0 / (double)n;
   for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
-    const uint64_t deg = off[v + 1] - off[v];
+    const uint64_t deg = degs ? (uint64_t)degs[v] : off[v + 1] - off[v];
     p[v] = (T)p0;
     contrib[v] = deg ? (T)(p0 / (double)deg) : (T)0;
   }
@@ -463,13 +470,17 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
                    int *iters_run, double *device_ms, const char *phase) {
   const uint64_t n = g->n;
   hipStream_t s = g->stream;
-  const RelabelView view = pr_view(g);
+  // a CPMA is pulled in place (its slot index and true degrees); the
+  // relabelled, per-XCD and segmented layouts are copies of a CSR
+  const AdjView av = adj_view(g);
+  const bool pma = av.deg != nullptr;
+  const RelabelView view = pma ? RelabelView{av.off, av.ids, nullptr} : pr_view(g);
   T *p[2], *c[2];
   p[0] = g->buf("pr_p0", n * sizeof(T) + 64).as<T>();
   p[1] = g->buf("pr_p1", n * sizeof(T) + 64).as<T>();
   c[0] = g->buf("pr_c0", n * sizeof(T) + 64).as<T>();
   c[1] = g->buf("pr_c1", n * sizeof(T) + 64).as<T>();
-  const uint64_t W = segment_width<T>(g, n);
+  const uint64_t W = pma ? n : segment_width<T>(g, n);
   const int S = (int)((n + W - 1) / W);
   T *acc_s = S > 1 ? g->buf("pr_acc_s", n * sizeof(T) + 64).as<T>() : nullptr;
   T *acc_c = (S > 1 && kComp) ? g->buf("pr_acc_c", n * sizeof(T) + 64).as<T>() : nullptr;
@@ -477,9 +488,9 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   // lanes per vertex: picked per tile with one segment and pr_lanes 0;
   // otherwise pr_lanes, or about half the per-segment average degree, in
   // [4, 32]
-  const bool per_tile = S == 1 && !g->tuning("pr_lanes");
+  const bool per_tile = pma || (S == 1 && !g->tuning("pr_lanes"));
   // the XCD split needs the tile pull and 32-bit edge positions
-  const bool per_xcd = per_tile && g->tuning("pr_xcd") && g->m < (1ull << 32) && n > 0;
+  const bool per_xcd = !pma && per_tile && g->tuning("pr_xcd") && g->m < (1ull << 32) && n > 0;
   const double avg = n ? (double)g->m / (double)n / S : 0.0;
   int G = 4;
   while (G < 32 && G < avg / 2) G *= 2;
@@ -499,7 +510,7 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   const uint32_t *ids = view.ids;
   const double addc = (1.0 - damping) * (1.0 / (double)n);
   PhaseTimer timer(s, true);
-  k_pr_init<T><<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, n, p[0], c[0]);
+  k_pr_init<T><<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, n, p[0], c[0], av.deg);
   if (S > 1) k_pr_bounds<<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, ids, n, S, W, pos);
   BYO_LAUNCH_CHECK();
   double *hl1 = static_cast<double *>(pinned(g, 64));
@@ -512,6 +523,9 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
       k_pr_xcd_combine<T, kComp><<<cgrid, 256, 0, s>>>(part_s, part_c, off, n, p[cur], p[cur ^ 1], c[cur ^ 1],
                                                        damping, addc, partial);
       nparts = cgrid;
+    } else if (pma) {
+      k_pr_tiles<T, kComp, true><<<grid, 256, 0, s>>>(off, ids, n, c[cur], p[cur], p[cur ^ 1], c[cur ^ 1], damping,
+                                                      addc, partial, av.deg);
     } else if (per_tile) {
       k_pr_tiles<T, kComp><<<grid, 256, 0, s>>>(off, ids, n, c[cur], p[cur], p[cur ^ 1], c[cur ^ 1], damping, addc,
                                                 partial);
@@ -568,7 +582,6 @@ int byo_pagerank(byo_graph_t g, double damping, int max_iters, double eps, int p
     BYO_REQUIRE(precision == BYO_PR_FP32 || precision == BYO_PR_FP64, BYO_E_INVAL, "unknown precision");
     std::lock_guard<std::mutex> lk(g->mu);
     bind(g);
-    ensure_csr(g);
     if (g->n == 0) {
       if (iters_run) *iters_run = 0;
       return;

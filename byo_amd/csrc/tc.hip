@@ -28,9 +28,9 @@ namespace {
 // orientation needs for a correct count, and it orders by degree every
 // vertex below the cap (vertices above it, among themselves, by id).  The
 // rank tests then gather from a table a quarter the size of the offsets.
-__global__ void k_tc_deg(const uint64_t *off, uint64_t n, uint16_t *deg) {
+__global__ void k_tc_deg(const uint64_t *off, const uint32_t *degs, uint64_t n, uint16_t *deg) {
   for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
-    const uint64_t d = off[v + 1] - off[v];
+    const uint64_t d = degs ? (uint64_t)degs[v] : off[v + 1] - off[v];
     deg[v] = d < 0xFFFFull ? (uint16_t)d : (uint16_t)0xFFFFu;
   }
 }
@@ -38,9 +38,10 @@ __global__ void k_tc_deg(const uint64_t *off, uint64_t n, uint16_t *deg) {
 constexpr int kG = 16;      // lanes per vertex in the orientation pass
 constexpr int kOrientU = 8;  // steps of kG edges whose loads are in flight together
 
-// filterGraph in one pass, in the CSR's own index space: the edges of u that
-// DG keeps are written, in adjacency order, to dg_ids[off[u] ...], and
-// dg_end[u] = off[u] + their number.  Every lane of the wave runs the same
+// filterGraph in one pass, in the container's own index space (CSR
+// positions, or PMA slots whose empty entries, kNone, are dropped): the edges
+// of u that DG keeps are written, in adjacency order, to dg_ids[off[u] ...],
+// and dg_end[u] = off[u] + their number.  Every lane of the wave runs the same
 // number of steps (ballots below); the ids and degrees of kOrientU steps are
 // loaded at clamped positions before any is used.
 __global__ void k_tc_orient(const uint64_t *off, const uint32_t *ids, const uint16_t *deg, uint64_t n,
@@ -60,14 +61,19 @@ __global__ void k_tc_orient(const uint64_t *off, const uint32_t *ids, const uint
     const uint64_t steps = wave_max((e - b + kG - 1) / kG);
     for (uint64_t s0 = 0; s0 < steps; s0 += kOrientU) {
       uint32_t v[kOrientU], dv[kOrientU];
+      bool hole[kOrientU];
 #pragma unroll
-      for (int t = 0; t < kOrientU; t++) v[t] = ids[std::min<uint64_t>(b + (s0 + t) * kG + sub, last)];
+      for (int t = 0; t < kOrientU; t++) {
+        v[t] = ids[std::min<uint64_t>(b + (s0 + t) * kG + sub, last)];
+        hole[t] = v[t] == kNone;  // an empty PMA slot
+        if (hole[t]) v[t] = 0;
+      }
 #pragma unroll
       for (int t = 0; t < kOrientU; t++) dv[t] = deg[v[t]];
 #pragma unroll
       for (int t = 0; t < kOrientU; t++) {
         const uint64_t k = b + (s0 + t) * kG + sub;
-        const bool kp = k < e && (du < dv[t] || (du == dv[t] && (uint32_t)u < v[t]));
+        const bool kp = k < e && !hole[t] && (du < dv[t] || (du == dv[t] && (uint32_t)u < v[t]));
         const uint64_t mask = __ballot(kp);
         const uint64_t gm = (mask >> gshift) & ((1ull << kG) - 1);
         if (kp) dg_ids[w + __popcll(gm & ((1ull << sub) - 1))] = v[t];
@@ -276,20 +282,20 @@ int byo_tc(byo_graph_t g, uint64_t *count, double *device_ms) {
     BYO_REQUIRE(g && count, BYO_E_INVAL, "null argument");
     std::lock_guard<std::mutex> lk(g->mu);
     bind(g);
-    ensure_csr(g);
     const uint64_t n = g->n;
     *count = 0;
     if (n == 0 || g->m == 0) return;
     hipStream_t s = g->stream;
-    const uint64_t *off = g->off.as<uint64_t>();
-    const uint32_t *ids = g->ids.as<uint32_t>();
-    // DG lives in the CSR's index space: u's kept edges at off[u] ... dg_end[u]
-    uint32_t *dg_ids = g->buf("tc_dgids", (g->m + 64) * 4).as<uint32_t>();
+    const AdjView av = adj_view(g);
+    const uint64_t *off = av.off;
+    const uint32_t *ids = av.ids;
+    // DG lives in the container's index space: u's kept edges at off[u] ... dg_end[u]
+    uint32_t *dg_ids = g->buf("tc_dgids", (av.nslots + 64) * 4).as<uint32_t>();
     uint64_t *dg_end = g->buf("tc_dgend", n * 8 + 64).as<uint64_t>();
     uint16_t *deg = g->buf("tc_deg", n * 2 + 64).as<uint16_t>();
     unsigned long long *ctr = g->buf("tc_ctr", 64).as<unsigned long long>();
     PhaseTimer t(s, true);
-    k_tc_deg<<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, n, deg);
+    k_tc_deg<<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, av.deg, n, deg);
     k_tc_orient<<<byo_grid(n * kG, 256, 16384), 256, 0, s>>>(off, ids, deg, n, dg_ids, dg_end);
     BYO_HIP(hipMemsetAsync(ctr, 0, 16, s));
     BYO_LAUNCH_CHECK();

@@ -225,3 +225,76 @@ def test_captured_batches_overflow_and_bad_ids(B):
         _graph_insert(B, G, [ok, bad, later])
     E2 = np.union1d(E, ok)
     assert np.array_equal(keys_of(G)[0], E2)
+
+
+# ---- algorithms over the PMA itself (slot index + dst words, no CSR copy) ----
+
+def _sym(keys):
+    rev = ((keys & np.uint64(0xFFFFFFFF)) << np.uint64(32)) | (keys >> np.uint64(32))
+    return np.concatenate([keys, rev])
+
+
+@pytest.mark.parametrize("scale", [12, 14])
+def test_algorithms_read_the_pma(B, scale):
+    """BFS (one source, 8 sources, forced directions), PageRank FP32/FP64,
+    both connectivity algorithms, triangle counting and the edge scan run on
+    the CPMA's own leaves -- empty slots at leaf ends, empty leaves after
+    removals, rebuilt windows -- and agree with the oracle on the same edge
+    set; none of them materialises the CSR export."""
+    g = O.rmat_graph(scale)
+    G = B.CpmaGraph.from_csr(g.offsets, g.ids)
+    E = O.graph_keys(g)
+    ins = _sym(upd_keys(g.n, 6000))  # RMAT-skewed: overflows hot leaves
+    G.insert_batch(ins)
+    E = np.union1d(E, ins)
+    # remove every edge of a band of vertices: whole leaves empty out
+    band = E[((E >> np.uint64(32)) >= 200) & ((E >> np.uint64(32)) < 900)]
+    band = np.union1d(band, _sym(band))
+    G.remove_batch(band)
+    E = np.setdiff1d(E, band)
+    rem = _sym(upd_keys(g.n, 3000, 100))
+    G.remove_batch(rem)
+    E = np.setdiff1d(E, rem)
+    h = O.keys_to_csr(g.n, E)
+    G.profile(True)
+    G.profile_reset()
+    srcs = O.bfs_sources(h, 8, seed=5)
+    for src in srcs[:3]:
+        lv = O.bfs_levels(h, src)
+        par, st = G.bfs(src)
+        assert O.check_bfs_tree(h, src, par, lv) == 0, src
+        _, dirlog, fs, fd = O.bfs_diropt(h, src)
+        assert st["dir"] == list(dirlog) and st["fsize"] == list(fs) and st["fdeg"] == list(fd)
+        assert np.array_equal(G.bfs_levels(), lv)
+    deg = np.diff(h.offsets)
+    for pol in (B.DIR_THRESHOLD, B.DIR_SPARSE, B.DIR_DENSE):
+        G.set_direction_policy(pol)
+        pars, sts = G.bfs_multi(srcs)
+        for j, src in enumerate(srcs):
+            lv = O.bfs_levels(h, src)
+            assert O.check_bfs_tree(h, src, pars[j], lv) == 0, (pol, src)
+            assert sts[j]["reached"] == int((lv != O.NONE).sum())
+            assert sts[j]["edges_reached"] == int(deg[lv != O.NONE].sum())
+        par, _ = G.bfs(srcs[0])
+        assert O.check_bfs_tree(h, srcs[0], par, O.bfs_levels(h, srcs[0])) == 0, pol
+    G.set_direction_schedule([0, 3, 3, 1, 0])  # push, forward, forward, pull, push
+    par, _ = G.bfs(srcs[1])
+    assert O.check_bfs_tree(h, srcs[1], par, O.bfs_levels(h, srcs[1])) == 0
+    G.set_direction_policy(B.DIR_THRESHOLD)
+    p_ref, _ = O.pagerank(h, 0.85, 20, 0.0)
+    p64, _, _ = G.pagerank(0.85, 20, 0.0, B.PR_FP64)
+    assert np.abs(p64 - p_ref).sum() < 1e-12
+    p32, _, _ = G.pagerank(0.85, 20, 0.0, B.PR_FP32)
+    assert np.abs(p32 - p_ref).sum() <= 1e-6  # the stated FP32 tolerance
+    lab = O.cc(h)
+    assert np.array_equal(G.cc()[0], lab)
+    assert np.array_equal(G.cc_labelprop()[0], lab)
+    assert G.tc()[0] == O.tc(h)
+    assert G.map_sum()[0] == O.map_sum(h)
+    assert np.array_equal(G.degrees(), deg.astype(np.uint32))
+    assert G.profile_get("cpma_csr_export")[0] == 0  # no CSR copy was built
+    assert G.profile_get("cpma_index")[0] >= 1
+    # the export (used only now) holds the same edge set
+    assert np.array_equal(keys_of(G)[0], E)
+    assert G.profile_get("cpma_csr_export")[0] == 1
+    G.close()
