@@ -9,9 +9,13 @@ undirected edges reached / time).  The same JSON line carries the same
 workload under the M/20 threshold rule (learned vs threshold), the
 one-source-at-a-time engine, FP32-Kahan and FP64 PageRank it/s (20
 iterations) with their L1 difference, CC and TC times, CPMA insert/remove
-edges/s at batch 1e7, the roofline of the BFS launch, and the CPU baseline
-(the oracle's OpenMP direction-optimizing BFS on the host, on a bounded
-sample of sources).
+edges/s at batch 1e7 with their roofline (24 b + 2 S_touched bytes), BFS
+and PageRank rerun on the updated CPMA (read from the PMA itself), the
+roofline of the BFS launch, the reference's CPU-runnable case (RMAT-16, BFS
+from 10, PageRank to 1e-6, CC, TC) on the GPU and the host, and the CPU
+baselines: the oracle's OpenMP direction-optimizing BFS on the host on a
+bounded sample of sources (cpu_baseline), and the oracle's PageRank, CC, TC
+and sorted-array batch updates (cpu_baselines_other).
 
     python bench.py [--steps K] [--warmup W] [--scale 22] [--cpma-scale 24]
 """
@@ -61,10 +65,34 @@ def sync():
     return None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def protocol(f, rounds):
+    """run_unweighted.h:135-170: rounds + 1 runs, round 0 untimed, mean of
+    the rest (seconds); returns (mean, last result)."""
+    ts, res = [], None
+    for r in range(rounds + 1):
+        t0 = time.perf_counter()
+        res = f()
+        dt = time.perf_counter() - t0
+        if r:
+            ts.append(dt)
+    return float(np.mean(ts)), res
+
+
 def cpu_baseline(G, sources, budget_s):
     """The oracle's OpenMP direction-optimizing BFS (restating BFS.h over
     edgeMapData) on this host's cores, on as many of the sources as fit in
-    budget_s; returns GTEPS over that sample."""
+    budget_s; returns (GTEPS over that sample, the oracle graph)."""
     import oracle as O
     off, ids = G.export_csr()
     g = O.Graph(off, ids)
@@ -82,8 +110,83 @@ def cpu_baseline(G, sources, budget_s):
         edges += int(fd.sum()) // 2
         done += 1
     return {"value": edges / t_total / 1e9, "unit": "GTEPS", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": "oracle OpenMP diropt BFS (M/20 rule) on the same RMAT-%d graph: %d traversals "
-                      "(passes over the %d sources), %.1f s" % (int(np.log2(g.n)), done, len(sources), t_total)}
+                      "(passes over the %d sources), %.1f s" % (int(np.log2(g.n)), done, len(sources), t_total)}, g
+
+
+def cpu_other(g, H, keys, rounds=1):
+    """The oracle on the host cores for the rest of the path, by the runner
+    protocol with `rounds` timed runs after an untimed one: PageRank FP64
+    (20 iterations), CC and TC on the benchmark graph g (RMAT-22), and the
+    CPMA update as a sorted-array insert / remove (sort + unique + merge,
+    gbbs/graph.h:3071-3124) of the 1e7-key batch into the RMAT-24 edge set."""
+    import oracle as O
+    out = {"cores": os.cpu_count(), "cpu_model": cpu_model(), "kind": "port", "rounds": rounds}
+    t, _ = protocol(lambda: O.pagerank(g, 0.85, 20, 0.0), rounds)
+    out["pr_fp64_it_s"] = round(20 / t, 2)
+    t, _ = protocol(lambda: O.cc(g), rounds)
+    out["cc_ms"] = round(t * 1e3, 1)
+    t, _ = protocol(lambda: O.tc(g), rounds)
+    out["tc_ms"] = round(t * 1e3, 1)
+    if H is not None:
+        off, ids = H.export_csr()
+        E = O.csr_keys(O.Graph(off, ids))
+        del off, ids
+        t, E2 = protocol(lambda: O.batch_insert(E, keys), rounds)
+        out["cpma_insert_edges_s"] = round(len(keys) / t, 1)
+        t, _ = protocol(lambda: O.batch_remove(E2, keys), rounds)
+        out["cpma_remove_edges_s"] = round(len(keys) / t, 1)
+        out["cpma_sample"] = "sorted-array insert/remove of the %d-key batch into the %d-edge RMAT set" % (
+            len(keys), len(E))
+    return out
+
+
+def c1_plain_method(B, rounds=3):
+    """Config C1, the reference's CPU-runnable case, on the GPU and on the
+    host: RMAT scale 16 ef 16 (seed 1), BFS from vertex 10 (the next vertex
+    with edges if 10 has none), PageRank to L1 < 1e-6 (at most 100
+    iterations; FP64 and FP32-Kahan), CC and TC, each by the runner protocol
+    (rounds + 1 runs, round 0 dropped, mean; run_unweighted.h:135-170)."""
+    import oracle as O
+    G = B.CsrGraph.rmat(16, 16)
+    off, ids = G.export_csr()
+    g = O.Graph(off, ids)
+    deg = np.diff(off)
+    src = 10
+    while deg[src] == 0:
+        src += 1
+    gpu, cpu = {}, {}
+    t, (_, st) = protocol(lambda: G.bfs(src, want_parents=False), rounds)
+    gpu["bfs_ms"] = round(t * 1e3, 4)
+    gpu["bfs_gteps"] = round(st["edges_reached"] / 2 / t / 1e9, 3)
+    p_ref, l1_ref = O.pagerank(g, 0.85, 100, 1e-6)
+    for prec, key in ((B.PR_FP64, "fp64"), (B.PR_FP32, "fp32")):
+        t, (p, l1, _) = protocol(lambda: G.pagerank(0.85, 100, 1e-6, prec), rounds)
+        gpu["pr_%s_ms" % key] = round(t * 1e3, 4)
+        gpu["pr_%s_iters" % key] = len(l1)
+        gpu["pr_%s_l1_vs_cpu_fp64" % key] = float(np.abs(p - p_ref).sum())
+    t, (lab, _) = protocol(lambda: G.cc(), rounds)
+    gpu["cc_ms"] = round(t * 1e3, 4)
+    t, (tri, _) = protocol(lambda: G.tc(), rounds)
+    gpu["tc_ms"] = round(t * 1e3, 4)
+    t, (_, _, _, fd) = protocol(lambda: O.bfs_diropt(g, src), rounds)
+    cpu["bfs_ms"] = round(t * 1e3, 3)
+    cpu["bfs_gteps"] = round(int(fd.sum()) / 2 / t / 1e9, 3)
+    t, (_, l1c) = protocol(lambda: O.pagerank(g, 0.85, 100, 1e-6), rounds)
+    cpu["pr_fp64_ms"] = round(t * 1e3, 3)
+    cpu["pr_fp64_iters"] = len(l1c)
+    t, lab_c = protocol(lambda: O.cc(g), rounds)
+    cpu["cc_ms"] = round(t * 1e3, 3)
+    t, tri_c = protocol(lambda: O.tc(g), rounds)
+    cpu["tc_ms"] = round(t * 1e3, 3)
+    res = {"graph": "RMAT-16 ef16 (n=%d, m=%d)" % (g.n, g.m), "src": src,
+           "protocol": "rounds %d + 1 untimed, mean" % rounds, "gpu": gpu,
+           "cpu": dict(cpu, cores=os.cpu_count(), kind="port"),
+           "triangles": int(tri), "components": int(len(np.unique(lab))),
+           "agree": bool(tri == tri_c and np.array_equal(lab, lab_c))}
+    G.close()
+    return res
 
 
 def batched_job(B, args):
@@ -139,7 +242,7 @@ def main():
     ap.add_argument("--job-scale", type=int, default=23, help="scale of the batched job graph (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--cpu-budget", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -333,20 +436,36 @@ def main():
         C.insert_batch_dev(dkeys.ptr, len(keys))  # round 0 (untimed, as Batch_insert_runner)
         C.remove_batch_dev(dkeys.ptr, len(keys))
         C.profile_reset()
-        ins_t, rem_t = [], []
+        ins_t, rem_t, ins_s, rem_s = [], [], [], []
         for r in range(3):
             sync()
+            s0 = C.stats()["touched_bytes"]
             a = time.perf_counter()
             C.insert_batch_dev(dkeys.ptr, len(keys))
             b = time.perf_counter()
+            s1 = C.stats()["touched_bytes"]
             C.remove_batch_dev(dkeys.ptr, len(keys))
             c = time.perf_counter()
+            s2 = C.stats()["touched_bytes"]
             ins_t.append(b - a)
             rem_t.append(c - b)
+            ins_s.append(s1 - s0)
+            rem_s.append(s2 - s1)
         ex["cpma_batch"] = args.cpma_batch
         ex["cpma_insert_edges_s"] = round(args.cpma_batch / np.mean(ins_t), 1)
         ex["cpma_remove_edges_s"] = round(args.cpma_batch / np.mean(rem_t), 1)
         ex["cpma_scale"] = args.cpma_scale
+        # update byte model 24 b + 2 S_touched (batch read, one sort pass in
+        # and out, PMA slots rewritten read + write; S counted by the kernels)
+        bb = args.cpma_batch
+        ib = 24 * bb + 2 * float(np.mean(ins_s))
+        rb = 24 * bb + 2 * float(np.mean(rem_s))
+        ex["cpma_roofline"] = {
+            "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+            "insert": {"achieved": round(ib / np.mean(ins_t) / 1e9, 1), "frac": round(ib / np.mean(ins_t) / 1e9 / HBM_PEAK_GBS, 4),
+                       "s_touched_bytes": int(np.mean(ins_s)), "algorithmic_bytes": int(ib)},
+            "remove": {"achieved": round(rb / np.mean(rem_t) / 1e9, 1), "frac": round(rb / np.mean(rem_t) / 1e9 / HBM_PEAK_GBS, 4),
+                       "s_touched_bytes": int(np.mean(rem_s)), "algorithmic_bytes": int(rb)}}
         # Batch_insert_runner's sweep (run_unweighted.h:781-893): b = 10 ..
         # 1e7 in powers of 10, the same update stream prefix for every size,
         # rounds + 1 with round 0 untimed, insert then remove the batch
@@ -366,16 +485,58 @@ def main():
             sweep[str(b)] = {"insert_edges_s": round(b / np.mean(ti), 1), "remove_edges_s": round(b / np.mean(tr), 1)}
             b *= 10
         ex["cpma_sweep"] = sweep
+        # C3's rerun: BFS and PageRank on the updated graph, read from the PMA
+        # itself (a symmetric batch of 1e7 keys inserted first, so the graph
+        # stays undirected); the slot-index rebuild after the update is timed
+        # separately, and the same algorithms on the CSR of the graph before
+        # the update are given for comparison
+        half = keys[: args.cpma_batch // 2]
+        symk = np.concatenate([half, ((half & np.uint64(0xFFFFFFFF)) << np.uint64(32)) | (half >> np.uint64(32))])
+        dsym = B.DeviceBuffer(symk)
+        C.insert_batch_dev(dsym.ptr, len(symk))
+        C.profile_reset()
+        rr = {"inserted_keys": int(len(symk)), "m": C.M()}
+        csrcs = pick_sources(C.degrees(), args.sources, seed=4)
+        C.apply_tuning()
+        _, cst = C.bfs_multi(csrcs, want_parents=False)  # builds the slot index
+        rr["index_build_ms"] = round(C.profile_get("cpma_index")[1], 3)
+        C.profile_reset()
+        for _ in range(3):
+            _, cst = C.bfs_multi(csrcs, want_parents=False)
+        nl, lms = C.profile_get("bfs_launch")
+        rr["pma_bfs_gteps"] = round(sum(x["edges_reached"] // 2 for x in cst) / (lms / nl * 1e-3) / 1e9, 2)
+        C.pagerank(0.85, 20, 0.0, B.PR_FP32, want_p=False)
+        rr["pma_pr_fp32_it_s"] = round(20 / (C.pagerank(0.85, 20, 0.0, B.PR_FP32, want_p=False)[2] * 1e-3), 1)
+        rr["csr_export_launches"] = C.profile_get("cpma_csr_export")[0]
+        if H is not G:
+            H.apply_tuning()
+            H.profile(True)
+            H.profile_reset()
+            hsrcs = pick_sources(H.degrees(), args.sources, seed=4)
+            for _ in range(3):
+                _, hst = H.bfs_multi(hsrcs, want_parents=False)
+            nl, lms = H.profile_get("bfs_launch")
+            rr["csr_before_update_bfs_gteps"] = round(sum(x["edges_reached"] // 2 for x in hst) / (lms / nl * 1e-3) / 1e9, 2)
+            H.pagerank(0.85, 20, 0.0, B.PR_FP32, want_p=False)
+            rr["csr_before_update_pr_fp32_it_s"] = round(
+                20 / (H.pagerank(0.85, 20, 0.0, B.PR_FP32, want_p=False)[2] * 1e-3), 1)
+        C.remove_batch_dev(dsym.ptr, len(symk))
+        dsym.free()
+        ex["cpma_rerun"] = rr
         dkeys.free()
         C.close()
-        if H is not G:
-            H.close()
+        cpma_keys = keys
         if args.job_scale:
             ex["batched_job"] = batched_job(B, args)
         out["extras"] = ex
+        out["c1_plain_method"] = c1_plain_method(B)
 
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(G, srcs, args.cpu_budget)
+        out["cpu_baseline"], g = cpu_baseline(G, srcs, args.cpu_budget)
+        if not args.no_extras:
+            out["cpu_baselines_other"] = cpu_other(g, H if H is not G else None, cpma_keys)
+    if not args.no_extras and H is not G:
+        H.close()
     # every device handle is released here, before the interpreter exits
     G.close()
     print(json.dumps(out), flush=True)

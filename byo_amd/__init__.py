@@ -82,6 +82,7 @@ SIGNATURES = {
     "byo_insert_batch_dev": (_i32, [_vp, _vp, _u64]),
     "byo_remove_batch_dev": (_i32, [_vp, _vp, _u64]),
     "byo_insert_batch_dev_noalloc": (_i32, [_vp, _vp, _u64]),
+    "byo_cpma_stats": (_i32, [_vp, _P(_u64), _P(_u64), _P(_u64)]),
     "byo_update_rmat_keys": (_i32, [_u64, _u64, _u64, _vp]),
     "byo_device_alloc": (_i32, [_u64, _P(_vp)]),
     "byo_device_free": (_i32, [_vp]),
@@ -582,6 +583,13 @@ class CsrGraph(Graph):
 
 class CpmaGraph(Graph):
     """CPMA-style packed-memory-array graph (run_single_pma.cc:29-140)."""
+
+    def stats(self):
+        """{"touched_bytes", "rebalances", "leaves"}: PMA slot bytes rewritten
+        by all updates so far (S_touched), rebalances so far, leaf count."""
+        t, r, l = _u64(), _u64(), _u64()
+        check(lib().byo_cpma_stats(self._h, C.byref(t), C.byref(r), C.byref(l)))
+        return {"touched_bytes": t.value, "rebalances": r.value, "leaves": l.value}
 
     @classmethod
     def load(cls, path, binary=False):

@@ -47,6 +47,7 @@ struct Cpma {
   DevBuf sep;    // uint64[nleaves + 1]
   DevBuf deg;    // uint32[n]
   uint64_t rebalances = 0;
+  uint64_t touched = 0;  // PMA slot bytes rewritten by all updates so far (S_touched)
   // per-vertex slot starts (n + 1; getExtraData's per-vertex index,
   // run_single_pma.cc:88): v's keys lie in slots [vstart[v], vstart[v+1]),
   // with empty slots at leaf ends in between; rebuilt after an update
@@ -93,6 +94,10 @@ struct UpdCtl {
   unsigned long long changed;   // keys inserted / removed
   unsigned long long overflow;  // a leaf would overflow
   unsigned long long bad;       // an id out of range
+  // pad[0] unique batch keys, pad[1] merged count of a global rebalance,
+  // pad[2] global rebalance needed, pad[3] root windows, pad[4] PMA slot
+  // bytes the update rewrote (S_touched: keys of merged leaves and rebuilt
+  // windows after an insert, keys of compacted leaves before a remove, 8 B each)
   unsigned long long pad[5];
 };
 
@@ -182,30 +187,48 @@ __device__ __forceinline__ uint32_t lower_rank(const uint64_t *a, uint32_t len, 
   return lo;
 }
 
+// The three per-leaf update passes give each touched leaf (one segment of
+// the sorted batch) to one wave: the leaf's keys are staged in the wave's
+// LDS with one coalesced pass and every lookup is an LDS binary search, and
+// the wave synchronises only itself.  (One workgroup per leaf spent most of
+// its time in barriers and dependent global round trips: a batch touches
+// hundreds of thousands of leaves, each holding a few hundred keys.)
+constexpr int kUpdWaves = 4;  // waves per 256-thread workgroup
+
+// x among the len sorted keys of an LDS array.
+__device__ __forceinline__ bool lds_has(const uint64_t *a, uint32_t len, uint64_t x) {
+  uint32_t lo = 0, hi = len;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo < len && a[lo] == x;
+}
+
 // Pass 1 of an insert: count the segment keys absent from their leaf.
 __global__ void __launch_bounds__(256) k_ins_count(const uint64_t *keys, UpdDev d, const uint32_t *starts,
                                                    const uint32_t *leaf_of, Leaves lv,
                                                    const uint32_t *cnt, uint32_t *newc, UpdCtl *ctl) {
-  __shared__ uint32_t acc;
+  __shared__ uint64_t stage[kUpdWaves][kLeaf];
   if (d.halt && *d.halt) return;
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  uint64_t *old = stage[wv];
   const uint64_t b = *d.bu, nseg = *d.nseg;
-  for (uint64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
+  for (uint64_t j = global_tid() >> 6; j < nseg; j += grid_threads() >> 6) {
     const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
     const uint32_t l = leaf_of[s];
     const uint64_t leaf = (uint64_t)l * kLeaf;
     const uint32_t c = cnt[l];
-    if (threadIdx.x == 0) acc = 0;
-    __syncthreads();
+    for (uint32_t i = lane; i < c; i += 64) old[i] = lv.get(leaf + i);
+    wave_lds_sync();
     uint32_t mine = 0;
-    for (uint64_t i = s + threadIdx.x; i < e; i += blockDim.x) mine += !lv.has(leaf, c, keys[i]);
+    for (uint64_t i = s + lane; i < e; i += 64) mine += !lds_has(old, c, keys[i]);
     mine = wave_sum(mine);
-    if ((threadIdx.x & 63) == 0 && mine) atomicAdd(&acc, mine);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      newc[j] = acc;
-      if ((uint64_t)c + acc > kLeaf) atomicOr(&ctl->overflow, 1ull);
+    if (lane == 0) {
+      newc[j] = mine;
+      if ((uint64_t)c + mine > kLeaf) atomicOr(&ctl->overflow, 1ull);
     }
-    __syncthreads();
+    wave_lds_sync();  // the stage is rewritten by the next segment
   }
 }
 
@@ -214,17 +237,17 @@ __global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, UpdDev 
                                                    const uint32_t *leaf_of, Leaves lv, uint32_t *cnt,
                                                    uint32_t *deg, UpdCtl *ctl, const uint32_t *cand,
                                                    uint64_t nleaves) {
-  __shared__ uint64_t old_keys[kLeaf];
-  __shared__ uint64_t add[kLeaf];
-  __shared__ uint32_t wcnt[4];
-  __shared__ uint32_t nadd;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ uint64_t stage_old[kUpdWaves][kLeaf];
+  __shared__ uint64_t stage_add[kUpdWaves][kLeaf];
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  uint64_t *old_keys = stage_old[wv], *add = stage_add[wv];
   // stop on a halted sequence or a global rebalance; the leaves of rebuilt
   // windows (candidate windows, when a leaf overflowed) are skipped
   if ((d.halt && *d.halt) || ctl->pad[2] || ctl->bad) return;
   const bool windows = ctl->overflow != 0;
   const uint64_t b = *d.bu, nseg = *d.nseg;
-  for (uint64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
+  unsigned long long changed = 0, touched = 0;
+  for (uint64_t j = global_tid() >> 6; j < nseg; j += grid_threads() >> 6) {
     const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
     const uint32_t l = leaf_of[s];
     if (windows) {
@@ -238,44 +261,44 @@ __global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, UpdDev 
     }
     const uint64_t leaf = (uint64_t)l * kLeaf;
     const uint32_t c = cnt[l];
-    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) old_keys[i] = lv.get(leaf + i);
-    if (threadIdx.x == 0) nadd = 0;
-    __syncthreads();
-    // stable compaction of the absent keys, 256 at a time
-    for (uint64_t base = s; base < e; base += blockDim.x) {
-      const uint64_t i = base + threadIdx.x;
+    for (uint32_t i = lane; i < c; i += 64) old_keys[i] = lv.get(leaf + i);
+    wave_lds_sync();
+    // stable compaction of the absent keys, 64 at a time (a segment holds at
+    // most kLeaf keys: the unique absent ones fit the leaf, the rest are in it)
+    uint32_t na = 0;
+    for (uint64_t base = s; base < e; base += 64) {
+      const uint64_t i = base + lane;
       uint64_t k = 0;
       bool keep = false;
       if (i < e) {
         k = keys[i];
-        keep = !in_sorted(old_keys, c, k);
+        keep = !lds_has(old_keys, c, k);
       }
       const uint64_t mask = __ballot(keep);
-      if (lane == 0) wcnt[wv] = __popcll(mask);
-      __syncthreads();
-      uint32_t before = nadd;
-      for (int w = 0; w < wv; w++) before += wcnt[w];
-      if (keep) add[before + mbcnt(mask)] = k;
-      __syncthreads();
-      if (threadIdx.x == 0) nadd += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
-      __syncthreads();
+      if (keep) add[na + mbcnt(mask)] = k;
+      na += (uint32_t)__popcll(mask);
     }
-    const uint32_t na = nadd;
+    wave_lds_sync();
     // merged position of each key = own index + rank in the other list
-    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) {
+    for (uint32_t i = lane; i < c; i += 64) {
       const uint64_t k = old_keys[i];
       lv.put(leaf + i + lower_rank(add, na, k), k);
     }
-    for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) {
+    for (uint32_t i = lane; i < na; i += 64) {
       const uint64_t k = add[i];
       lv.put(leaf + i + lower_rank(old_keys, c, k), k);
       atomicAdd(&deg[k >> 32], 1u);
     }
-    if (threadIdx.x == 0) {
-      cnt[l] = c + na;
-      if (na) atomicAdd(&ctl->changed, (unsigned long long)na);
+    if (lane == 0) cnt[l] = c + na;
+    if (na) {
+      changed += na;
+      touched += 8ull * (c + na);
     }
-    __syncthreads();
+    wave_lds_sync();  // the stages are rewritten by the next segment
+  }
+  if (lane == 0 && changed) {
+    atomicAdd(&ctl->changed, changed);
+    atomicAdd(&ctl->pad[4], touched);
   }
 }
 
@@ -283,19 +306,20 @@ __global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, UpdDev 
 __global__ void __launch_bounds__(256) k_remove(const uint64_t *keys, UpdDev d, const uint32_t *starts,
                                                 const uint32_t *leaf_of, Leaves lv, uint32_t *cnt,
                                                 uint32_t *deg, UpdCtl *ctl) {
-  __shared__ uint64_t kept[kLeaf];
-  __shared__ uint32_t wcnt[4];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ uint64_t stage[kUpdWaves][kLeaf];
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  uint64_t *kept = stage[wv];
   if ((d.halt && *d.halt) || ctl->bad) return;
   const uint64_t b = *d.bu, nseg = *d.nseg;
-  for (uint64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
+  unsigned long long changed = 0, touched = 0;
+  for (uint64_t j = global_tid() >> 6; j < nseg; j += grid_threads() >> 6) {
     const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
     const uint32_t l = leaf_of[s];
     const uint64_t leaf = (uint64_t)l * kLeaf;
     const uint32_t c = cnt[l];
     uint32_t run = 0;
-    for (uint32_t base = 0; base < c; base += blockDim.x) {
-      const uint32_t i = base + threadIdx.x;
+    for (uint32_t base = 0; base < c; base += 64) {
+      const uint32_t i = base + lane;
       uint64_t k = 0;
       bool keep = false, gone = false;
       if (i < c) {
@@ -305,20 +329,21 @@ __global__ void __launch_bounds__(256) k_remove(const uint64_t *keys, UpdDev d, 
       }
       if (gone) atomicSub(&deg[k >> 32], 1u);
       const uint64_t mask = __ballot(keep);
-      if (lane == 0) wcnt[wv] = __popcll(mask);
-      __syncthreads();
-      uint32_t before = run;
-      for (int w = 0; w < wv; w++) before += wcnt[w];
-      if (keep) kept[before + mbcnt(mask)] = k;
-      run += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
-      __syncthreads();
+      if (keep) kept[run + mbcnt(mask)] = k;
+      run += (uint32_t)__popcll(mask);
     }
-    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) lv.put(leaf + i, i < run ? kept[i] : kEmpty);
-    if (threadIdx.x == 0) {
-      cnt[l] = run;
-      if (c > run) atomicAdd(&ctl->changed, (unsigned long long)(c - run));
+    wave_lds_sync();
+    if (run < c) {
+      for (uint32_t i = lane; i < c; i += 64) lv.put(leaf + i, i < run ? kept[i] : kEmpty);
+      if (lane == 0) cnt[l] = run;
+      changed += c - run;
+      touched += 8ull * c;
     }
-    __syncthreads();
+    wave_lds_sync();  // the stage is rewritten by the next segment
+  }
+  if (lane == 0 && changed) {
+    atomicAdd(&ctl->changed, changed);
+    atomicAdd(&ctl->pad[4], touched);
   }
 }
 
@@ -531,6 +556,7 @@ __global__ void k_win_w5(WinArgs w) {
     w.rinfo[4 * r + 2] = n0;
     w.rinfo[4 * r + 3] = nn;
     w.rsize[r] = (w.pc[e] - w.pc[b]) + nn;
+    atomicAdd(&w.ctl->pad[4], 8ull * ((w.pc[e] - w.pc[b]) + nn));
     w.cnt64[b] = (w.pc[e] - w.pc[b]) + nn;  // merged size by root base (cnt64 is free after w1's scan)
     if (nn) atomicAdd(&w.ctl->changed, (unsigned long long)nn);
   }
@@ -1074,6 +1100,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
     BYO_HIP(hipStreamSynchronize(s));
     if (!hctl->pad[2]) {
       if (hctl->overflow) c->rebalances++;
+      c->touched += hctl->pad[4];
       c->total += hctl->changed;
       g->m = c->total;
       if (hctl->changed) cpma_changed(g);
@@ -1092,6 +1119,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
       BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
       BYO_HIP(hipStreamSynchronize(s));
       const uint64_t m2 = hctl->pad[1];
+      c->touched += 8 * m2;  // the whole array is rewritten
       cpma_build(g, uniq, m2);
       BYO_HIP(hipStreamSynchronize(s));
     }
@@ -1102,6 +1130,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
     BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
     BYO_HIP(hipStreamSynchronize(s));
     BYO_REQUIRE(hctl->bad == 0, BYO_E_INVAL, "batch holds a vertex id >= n");
+    c->touched += hctl->pad[4];
     c->total -= hctl->changed;
     g->m = c->total;
     if (hctl->changed) {
@@ -1109,6 +1138,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
     }
     if (c->nleaves > 1 && (double)c->total < kMinDensity * (double)(c->nleaves * kLeaf)) {
       c->rebalances++;
+      c->touched += 8 * c->total;
       uint64_t *flat = g->buf("cpma_flat", c->total * 8 + 64).as<uint64_t>();
       cpma_flatten(g, flat);
       cpma_build(g, flat, c->total);
@@ -1188,7 +1218,10 @@ static uint32_t insert_graph(byo_graph *g, const uint64_t *keys, const uint64_t 
   const UpdCtl *hc = reinterpret_cast<const UpdCtl *>(h + 64);
   const uint32_t applied_end = stop ? stop - 1 : nb;
   uint64_t changed = 0;
-  for (uint32_t i = first; i < applied_end; i++) changed += hc[i - first].changed;
+  for (uint32_t i = first; i < applied_end; i++) {
+    changed += hc[i - first].changed;
+    c->touched += hc[i - first].pad[4];
+  }
   c->total += changed;
   g->m = c->total;
   if (changed) {
@@ -1329,6 +1362,18 @@ static int host_batch(byo_graph_t g, const uint64_t *keys, uint64_t b, bool inse
   });
 }
 
+int byo_cpma_stats(byo_graph_t g, uint64_t *touched_bytes, uint64_t *rebalances, uint64_t *leaves) {
+  return guard([&] {
+    BYO_REQUIRE(g, BYO_E_INVAL, "null handle");
+    BYO_REQUIRE(g->kind == 1, BYO_E_STATE, "CPMA statistics need a CPMA container");
+    std::lock_guard<std::mutex> lk(g->mu);
+    const Cpma *c = state(g);
+    if (touched_bytes) *touched_bytes = c->touched;
+    if (rebalances) *rebalances = c->rebalances;
+    if (leaves) *leaves = c->nleaves;
+  });
+}
+
 int byo_insert_batch(byo_graph_t g, const uint64_t *keys, uint64_t b) { return host_batch(g, keys, b, true); }
 int byo_remove_batch(byo_graph_t g, const uint64_t *keys, uint64_t b) { return host_batch(g, keys, b, false); }
 
@@ -1424,6 +1469,7 @@ extern "C" int byo_insert_batch_dev_noalloc(byo_graph_t g, const uint64_t *keys_
     BYO_HIP(hipMemcpyAsync(&h, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
     BYO_HIP(hipStreamSynchronize(s));
     c->total += h.changed;
+    c->touched += h.pad[4];
     g->m = c->total;
     if (h.changed) {
       cpma_changed(g);

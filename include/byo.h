@@ -133,6 +133,14 @@ int byo_remove_batch(byo_graph_t g, const uint64_t *keys, uint64_t b);
 /* The same with the batch already in device memory (keys_dev, b entries). */
 int byo_insert_batch_dev(byo_graph_t g, const uint64_t *keys_dev, uint64_t b);
 int byo_remove_batch_dev(byo_graph_t g, const uint64_t *keys_dev, uint64_t b);
+/* Update statistics of a CPMA: *touched_bytes = PMA slot bytes rewritten by
+ * all updates so far (8 B per key of every merged leaf and rebuilt window
+ * after an insert, of every compacted leaf before a remove, of the whole
+ * array on a global rebalance; the S_touched of the update byte model
+ * 24 b + 2 S_touched), *rebalances = windowed or global rebalances so far,
+ * *leaves = current leaf count (512 slots each).  Any pointer may be NULL.
+ * No counterpart in the reference (CPMA internals are not exposed there). */
+int byo_cpma_stats(byo_graph_t g, uint64_t *touched_bytes, uint64_t *rebalances, uint64_t *leaves);
 /* Allocation-free variant of byo_insert_batch_dev, for callers that cannot
  * allocate device or pinned memory (tight serving loops, code under HIP
  * graph capture, harnesses that forbid hipMalloc).  No counterpart in the

@@ -54,6 +54,9 @@ def lib():
             "orc_sort_unique_u64": (u64, [_u64p, u64]),
             "orc_build_sym_csr": (u64, [u64, u64, _u32p, _u32p, _u64p, _u32p]),
             "orc_keys_to_csr": (None, [u64, _u64p, u64, _u64p, _u32p]),
+            "orc_csr_to_keys": (None, [u64, _u64p, _u32p, _u64p]),
+            "orc_batch_insert": (u64, [_u64p, u64, _u64p, u64, _u64p]),
+            "orc_batch_remove": (u64, [_u64p, u64, _u64p, u64, _u64p]),
             "orc_bfs_levels": (None, [u64, _u64p, _u32p, u32, _u32p]),
             "orc_bfs_diropt": (i32, [u64, u64, _u64p, _u32p, u32, i64, _u32p, _u8p,
                                      _u64p, _u64p, i32]),
@@ -163,6 +166,32 @@ def keys_to_csr(n, keys):
     ids = np.empty(max(len(keys), 1), np.uint32)
     lib().orc_keys_to_csr(n, keys, len(keys), off, ids)
     return Graph(off, ids[:len(keys)].copy())
+
+
+def csr_keys(g):
+    """(src<<32)|dst keys of a CSR, built by the C oracle (OpenMP)."""
+    k = np.empty(max(g.m, 1), np.uint64)
+    lib().orc_csr_to_keys(g.n, g.offsets, g.ids, k)
+    return k[:g.m]
+
+
+def batch_insert(E, batch):
+    """E u unique(batch) over sorted unique keys E (gbbs/graph.h:3071-3097
+    semantics on a sorted array); batch is sorted in place."""
+    E = np.ascontiguousarray(E, np.uint64)
+    b = np.array(batch, np.uint64, copy=True)
+    out = np.empty(len(E) + len(b) + 1, np.uint64)
+    k = lib().orc_batch_insert(E, len(E), b, len(b), out)
+    return out[:k]
+
+
+def batch_remove(E, batch):
+    """E minus batch over sorted unique keys E (gbbs/graph.h:3099-3124)."""
+    E = np.ascontiguousarray(E, np.uint64)
+    b = np.array(batch, np.uint64, copy=True)
+    out = np.empty(len(E) + 1, np.uint64)
+    k = lib().orc_batch_remove(E, len(E), b, len(b), out)
+    return out[:k]
 
 
 def graph_keys(g):

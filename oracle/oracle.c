@@ -227,6 +227,99 @@ void orc_keys_to_csr(uint64_t n, const uint64_t *keys, uint64_t len,
   for (uint64_t i = 0; i < len; i++) ids[i] = (uint32_t)keys[i];
 }
 
+/* (src<<32)|dst keys of a CSR, in CSR order (sorted when the lists are):
+ * the key layout of run_single_pma.cc:83. */
+void orc_csr_to_keys(uint64_t n, const uint64_t *off, const uint32_t *ids, uint64_t *keys) {
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (uint64_t v = 0; v < n; v++)
+    for (uint64_t e = off[v]; e < off[v + 1]; e++) keys[e] = (v << 32) | ids[e];
+}
+
+/* First index of a sorted array a[0..len) holding a value >= x. */
+static uint64_t lower_u64(const uint64_t *a, uint64_t len, uint64_t x) {
+  uint64_t lo = 0, hi = len;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+/* Graph::insert_batch / remove_batch over a sorted-array container
+ * (gbbs/graph.h:3071-3124: parlay::unique(parlay::sort(batch)), then
+ * insert_sorted_batch / remove_sorted_batch): batch[b] is sorted and
+ * deduplicated in place, then merged with the sorted unique edge set E[m]
+ * into out (insert: E u B; remove: E minus B).  The merge is split over the
+ * threads at equal positions of E (each split point found in B by binary
+ * search), so every thread writes a disjoint run of out.  Returns the size
+ * of out (at most m + b entries).  The CPU baseline of the CPMA updates. */
+static uint64_t batch_merge(const uint64_t *E, uint64_t m, uint64_t *batch, uint64_t b, uint64_t *out,
+                            int insert) {
+  const uint64_t bu = orc_sort_unique_u64(batch, b);
+  if (m == 0) {
+    if (insert) memcpy(out, batch, bu * sizeof(uint64_t));
+    return insert ? bu : 0;
+  }
+  const int T = omp_get_max_threads();
+  uint64_t *cnt = (uint64_t *)calloc((size_t)T + 1, sizeof(uint64_t));
+  uint64_t total = 0;
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+    const uint64_t e0 = m * (uint64_t)t / nt, e1 = m * (uint64_t)(t + 1) / nt;
+    /* batch keys below E[e0] belong to the previous thread (or, for t = 0,
+     * to this one); the last thread also takes the keys past E's end */
+    const uint64_t b0 = t == 0 ? 0 : lower_u64(batch, bu, E[e0]);
+    const uint64_t b1 = t == nt - 1 ? bu : lower_u64(batch, bu, E[e1]);
+    for (int pass = 0; pass < 2; pass++) {
+      uint64_t i = e0, j = b0, w = 0;
+      uint64_t *o = NULL;
+      if (pass == 1) {
+#pragma omp barrier
+#pragma omp single
+        {
+          for (int x = 0; x < nt; x++) {
+            uint64_t c = cnt[x];
+            cnt[x] = total;
+            total += c;
+          }
+        }
+        o = out + cnt[t];
+      }
+      while (i < e1 || j < b1) {
+        uint64_t k;
+        int keep;
+        if (j >= b1 || (i < e1 && E[i] < batch[j])) {
+          k = E[i++];
+          keep = 1;
+        } else if (i >= e1 || batch[j] < E[i]) {
+          k = batch[j++];
+          keep = insert;
+        } else {
+          k = E[i++];
+          j++;
+          keep = insert;
+        }
+        if (keep) {
+          if (o) o[w] = k;
+          w++;
+        }
+      }
+      if (pass == 0) cnt[t] = w;
+    }
+  }
+  free(cnt);
+  return total;
+}
+
+uint64_t orc_batch_insert(const uint64_t *E, uint64_t m, uint64_t *batch, uint64_t b, uint64_t *out) {
+  return batch_merge(E, m, batch, b, out, 1);
+}
+
+uint64_t orc_batch_remove(const uint64_t *E, uint64_t m, uint64_t *batch, uint64_t b, uint64_t *out) {
+  return batch_merge(E, m, batch, b, out, 0);
+}
+
 /* graph_io.h:829-847 (sym_graph_from_edges: symmetrize, sort, dedupe, drop
  * self loops), with n fixed by the caller instead of inferred. */
 uint64_t orc_build_sym_csr(uint64_t n, uint64_t ne, const uint32_t *src,

@@ -55,6 +55,13 @@ uint64_t orc_sort_unique_u64(uint64_t *keys, uint64_t n);
 uint64_t orc_build_sym_csr(uint64_t n, uint64_t ne, const uint32_t *src,
                            const uint32_t *dst, uint64_t *offsets,
                            uint32_t *ids);
+/* Keys (src<<32)|dst of a CSR in CSR order (run_single_pma.cc:83). */
+void orc_csr_to_keys(uint64_t n, const uint64_t *off, const uint32_t *ids, uint64_t *keys);
+/* Graph::insert_batch / remove_batch (gbbs/graph.h:3071-3124) over a sorted
+ * array: batch sorted + deduplicated in place, merged with the sorted unique
+ * E[m] into out (E u B, or E minus B); returns |out| (<= m + b).  OpenMP. */
+uint64_t orc_batch_insert(const uint64_t *E, uint64_t m, uint64_t *batch, uint64_t b, uint64_t *out);
+uint64_t orc_batch_remove(const uint64_t *E, uint64_t m, uint64_t *batch, uint64_t b, uint64_t *out);
 /* CSR over sorted unique keys (src<<32|dst) -> offsets[n+1], ids[len]. */
 void orc_keys_to_csr(uint64_t n, const uint64_t *keys, uint64_t len,
                      uint64_t *offsets, uint32_t *ids);

@@ -699,3 +699,22 @@ def test_ldd_properties(which):
                     seen.add(w)
                     st.append(w)
         assert seen == members
+
+
+# ---- sorted-array batch updates (the CPMA's CPU baseline) -----------------
+
+@pytest.mark.parametrize("m,b", [(0, 50), (1, 3), (5000, 0), (20000, 7000), (300000, 120000)])
+def test_batch_insert_remove_match_set_algebra(m, b):
+    rng = np.random.default_rng(m + b)
+    E = np.unique(rng.integers(0, 1 << 40, m, dtype=np.uint64))
+    batch = rng.integers(0, 1 << 40, b, dtype=np.uint64)
+    if len(E) and b:
+        batch[: b // 3] = E[rng.integers(0, len(E), b // 3)]  # present keys
+        batch[b // 3: b // 2] = batch[: b // 2 - b // 3]         # repeats
+    assert np.array_equal(O.batch_insert(E, batch), np.union1d(E, batch))
+    assert np.array_equal(O.batch_remove(E, batch), np.setdiff1d(E, batch))
+
+
+def test_csr_keys_equal_python_keys():
+    g = O.rmat_graph(12)
+    assert np.array_equal(O.csr_keys(g), O.graph_keys(g))
