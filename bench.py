@@ -197,21 +197,31 @@ def batched_job(B, args):
     captured HIP graphs on a CPMA built from the same graph."""
     J = B.CsrGraph.rmat(args.job_scale, 16)
     knobs = J.apply_tuning()
+    from byo_amd import selector as S
+    learned = bool(S.load(J).get("batch"))
     n = J.N()
     deg = J.degrees()
     srcs = pick_sources(deg, 64, seed=2)
     seeds = pick_sources(deg, 32, seed=3)
-    J.batch_job(srcs, seeds, 0.85, 30, want_parents=False, want_ppr=False)
-    J.profile(True)
-    J.profile_reset()
-    _, st, _, ms = J.batch_job(srcs, seeds, 0.85, 30, want_parents=False, want_ppr=False)
-    bfs_ms = sum(J.profile_get(k)[1] for k in ("bfs_batch_seed", "bfs_batch_levels", "bfs_batch_tally"))
+    parts = {}
+    # the job's traversal under the rule and under the learned selector
+    for name, pol in (("rule", B.DIR_THRESHOLD), ("learned", B.DIR_MLP if learned else B.DIR_THRESHOLD)):
+        J.set_direction_policy(pol)
+        J.batch_job(srcs, seeds, 0.85, 30, want_parents=False, want_ppr=False)
+        J.profile(True)
+        J.profile_reset()
+        _, st, _, ms = J.batch_job(srcs, seeds, 0.85, 30, want_parents=False, want_ppr=False)
+        parts[name] = sum(J.profile_get(k)[1] for k in ("bfs_batch_seed", "bfs_batch_levels", "bfs_batch_tally"))
+    bfs_ms = parts["learned"]
     edges = sum(s["edges_reached"] // 2 for s in st)
     res = {"graph": "RMAT-%d ef16 (n=%d, m=%d)" % (args.job_scale, n, J.M()), "tuning": knobs,
            "bfs_sources": 64, "ppr_seeds": 32,
            "ppr_iters": 30, "launch_ms": round(ms, 3), "bfs_part_ms": round(bfs_ms, 3),
            "ppr_part_ms": round(ms - bfs_ms, 3), "bfs_gteps_in_launch": round(edges / (bfs_ms * 1e-3) / 1e9, 2),
-           "ppr_ms_per_iteration": round((ms - bfs_ms) / 30, 3)}
+           "ppr_ms_per_iteration": round((ms - bfs_ms) / 30, 3),
+           "direction": "learned selector" if learned else "M/20 rule",
+           "bfs_part_ms_rule": round(parts["rule"], 3),
+           "bfs_learned_vs_rule": round(parts["rule"] / parts["learned"], 4)}
     rng = np.random.default_rng(16)
     sizes = [int(10 ** x) for x in rng.uniform(3, 6, 16)]
     offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
@@ -299,6 +309,23 @@ def main():
 
     gteps, wall, nlaunch, launch_ms, bytes_total, stats = timed_bfs(policy, args.steps)
     thr_gteps, _, thr_nl, thr_ms, _, thr_stats = timed_bfs(B.DIR_THRESHOLD, args.steps)
+    # the same graph with 64 sources in one shared traversal (the batched
+    # job's width), rule against learned selector, runs interleaved
+    srcs64 = pick_sources(deg, 64, seed=2)
+    best64 = {}
+    for r in range(4):
+        for name, pol in (("rule", B.DIR_THRESHOLD), ("learned", policy)):
+            G.set_direction_policy(pol)
+            G.profile(True)
+            G.profile_reset()
+            _, st64 = G.bfs_multi(srcs64, want_parents=False)
+            ms64 = G.profile_get("bfs_launch")[1]
+            if r:
+                best64[name] = min(best64.get(name, 1e30), ms64)
+            if name == "learned":
+                dirs64 = st64[0]["dir"][:max(s["levels"] for s in st64)]
+    G.set_direction_policy(B.DIR_THRESHOLD)
+    edges64 = sum(s["edges_reached"] // 2 for s in st64)
     seq_gteps = timed_bfs(B.DIR_THRESHOLD, 1, 1)[0]  # bfs_engine 1: one source at a time
     # the dominant kernel is the persistent BFS launch (all sources of a step
     # in one k_bfs_batch dispatch), timed with HIP events on its stream
@@ -337,6 +364,10 @@ def main():
                                  "threshold_dirs": thr_stats[0]["dir"][:max(s["levels"] for s in thr_stats)],
                                  "levels_identical": [(a["levels"], a["reached"], a["edges_reached"]) for a in stats]
                                  == [(a["levels"], a["reached"], a["edges_reached"]) for a in thr_stats]},
+        "learned_vs_threshold_64_sources": {
+            "sources": 64, "rule_ms": round(best64["rule"], 4), "learned_ms": round(best64["learned"], 4),
+            "speedup": round(best64["rule"] / best64["learned"], 4),
+            "learned_gteps": round(edges64 / (best64["learned"] * 1e-3) / 1e9, 2), "learned_dirs": dirs64},
         "one_source_engine_gteps": round(seq_gteps, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

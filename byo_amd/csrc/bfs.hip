@@ -339,10 +339,12 @@ static int choose_direction(byo_graph *g, bool is_dense, uint64_t fs, uint64_t f
     case BYO_DIR_SPARSE: return 0;
     case BYO_DIR_DENSE: return 1;
     case BYO_DIR_SCHEDULE:
-      if (level < 64 && g->dir_sched[level] != 255) return g->dir_sched[level];
+      // a scheduled bitmap pull (4) is this engine's pull
+      if (level < 64 && g->dir_sched[level] != 255) return g->dir_sched[level] == 4 ? 1 : g->dir_sched[level];
       return rule;
     case BYO_DIR_MLP: {
       if (!g->sel_loaded) return rule;
+      if (rule == 0 && !g->tuning("sel_densify")) return rule;  // sparse levels stay sparse
       float x[6];
       const double avg = n ? (double)m / (double)n : 0.0;
       x[0] = (float)std::log2(1.0 + (double)fs);
@@ -500,8 +502,8 @@ int byo_set_direction_schedule(byo_graph_t g, const uint8_t *dirs, uint32_t len)
     BYO_REQUIRE(g && (dirs || len == 0), BYO_E_INVAL, "null argument");
     BYO_REQUIRE(len <= 64, BYO_E_INVAL, "schedule longer than 64 levels");
     for (uint32_t i = 0; i < len; i++)
-      BYO_REQUIRE(dirs[i] == 0 || dirs[i] == 1 || dirs[i] == 3 || dirs[i] == 255, BYO_E_INVAL,
-                  "schedule entries must be 0, 1, 3 or 255");
+      BYO_REQUIRE(dirs[i] == 0 || dirs[i] == 1 || dirs[i] == 3 || dirs[i] == 4 || dirs[i] == 255, BYO_E_INVAL,
+                  "schedule entries must be 0, 1, 3, 4 or 255");
     std::memset(g->dir_sched, 255, sizeof(g->dir_sched));
     if (len) std::memcpy(g->dir_sched, dirs, len);
     g->dir_policy = BYO_DIR_SCHEDULE;

@@ -47,6 +47,7 @@ void prepare(byo_graph *g, const AdjView &v, const uint32_t *srcs, uint32_t k, u
   a.bsum = g->buf("bfs_batch_bsum", (size_t)nblocks * 8 + 64).as<unsigned long long>();
   a.seen = g->buf("bfs_batch_seen", a.nm * mask_bytes + 64).p;
   a.fm = g->buf("bfs_batch_fm", 3 * a.nm * mask_bytes + 64).p;
+  a.ubm = g->buf("bfs_batch_ubm", 3 * (a.nm / 64) * 8 + 64).as<uint64_t>();
   // pinned staging: sources at 0, results at 256 (one allocation, sized
   // before any copy is queued)
   char *hp = static_cast<char *>(pinned(g, 256 + kOutBytes));

@@ -66,6 +66,8 @@ struct BatchArgs {
   uint64_t pstride;
   void *seen;
   void *fm;                 // three rotating frontier-mask arrays of nm entries
+  uint64_t *ubm;            // three rotating union-frontier bitmaps of nm / 64 words
+                            // (bit v: v is in some source's frontier)
   uint32_t *q;              // three rotating queues of qstride entries
   uint64_t qstride;
   uint64_t *scan;
@@ -183,6 +185,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
   // logs[64] holds wall-clock marks: start, seeded, traversal end, tallied
   if (leader) a.logs[64].fsize = wall_clock64();
   pieces_init(a.pieces, a.off, a.n, nwords, nwaves);
+  mlp_stage(a.dc, mlp);
   // ---- seed: masks cleared, then every source's bit set.  (Parents are
   // not cleared: the tally pass writes kNone for every source that did not
   // reach a vertex, the traversal writes all the others.)
@@ -197,6 +200,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
       f4[nvec + i] = z;
       f4[2 * nvec + i] = z;
     }
+    for (uint64_t i = global_tid(); i < 3 * nwords; i += grid_threads()) a.ubm[i] = 0;
   }
   grid_barrier(a.bar, (unsigned)nb);
   if (blockIdx.x == 0) {
@@ -205,6 +209,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
       a.parents[(uint64_t)j * a.pstride + s] = s;
       mask_or<M>(seen, s, (M)((M)1 << j));
       mask_or<M>(fmb, s, (M)((M)1 << j));
+      atomicOr(reinterpret_cast<unsigned long long *>(&a.ubm[s >> 6]), 1ull << (s & 63));
       a.lastlev[j] = 0;
       a.tot[2 * j] = 0;
       a.tot[2 * j + 1] = 0;
@@ -251,6 +256,15 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
     if (leader) a.ctr[(L + 2) % 3] = BCtr{0, 0, 0, 0};
     M *const fmask = fmb + (uint64_t)(L % 3) * a.nm, *const nmask = fmb + (uint64_t)((L + 1) % 3) * a.nm;
     uint32_t *const q = qb + (uint64_t)(L % 3) * a.qstride, *const qn = qb + (uint64_t)((L + 1) % 3) * a.qstride;
+    // union bitmaps rotate with the masks: this frontier's, the next one's
+    // (whole words written by a pull, bits set by push / forward claims),
+    // and last level's, cleared now for level L+1
+    const uint64_t *const cubm = a.ubm + (uint64_t)(L % 3) * nwords;
+    uint64_t *const nubm = a.ubm + (uint64_t)((L + 1) % 3) * nwords;
+    {
+      uint64_t *clr = a.ubm + (uint64_t)((L + 2) % 3) * nwords;
+      for (uint64_t i = global_tid(); i < nwords; i += grid_threads()) clr[i] = 0;
+    }
     {
       // last level's frontier masks, read by nobody now, are cleared for
       // level L+1 -- through that frontier's queue when it has one
@@ -309,7 +323,10 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
             if (kk < e) {
               tgt = a.ids[kk];
               add = (!G || tgt != kNone) && relax<M>(a, seen, nmask, u, f, tgt, won);
-              if (add) dsum += vdeg<G>(a.off, a.deg, tgt);
+              if (add) {
+                dsum += vdeg<G>(a.off, a.deg, tgt);
+                atomicOr(reinterpret_cast<unsigned long long *>(&nubm[tgt >> 6]), 1ull << (tgt & 63));
+              }
             }
             const uint64_t mask = __ballot(add);
             if (mask) {
@@ -424,7 +441,10 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
               }
               tgt = a.ids[(uint64_t)(bs + (int64_t)e)];
               add = (!G || tgt != kNone) && relax<M>(a, seen, nmask, owner, f, tgt, won);
-              if (add) dsum += vdeg<G>(a.off, a.deg, tgt);
+              if (add) {
+                dsum += vdeg<G>(a.off, a.deg, tgt);
+                atomicOr(reinterpret_cast<unsigned long long *>(&nubm[tgt >> 6]), 1ull << (tgt & 63));
+              }
             }
             tg[t] = tgt;
             ad[t] = add;
@@ -447,11 +467,16 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
       is_dense = false;
       prev_have_q = true;  // this frontier has (or was packed into) a queue
       have_q = true;
-    } else if (dir == 1) {
+    } else if (dir == 1 || dir == 4) {
       // dense pull: a wave owns 64 consecutive vertices per step (lane =
       // vertex for offsets and masks).  Groups of 8 lanes scan 8 neighbours
       // of one vertex at a time; within a group the exclusive prefix-OR of
       // the hit masks gives each bit's first neighbour in adjacency order.
+      // Direction 4 (bitmap pull) tests each neighbour against the union
+      // bitmap of the frontier first (n/8 bytes, small enough to stay in
+      // L2) and gathers the neighbour's source mask only on a hit;
+      // direction 1 gathers every neighbour's mask.
+      const bool bmp = dir == 4;
       const int g = lane >> 3, sub = lane & 7;
       unsigned long long cnt = 0;
       for_my_words(a.pieces, wave, nwaves, [&](uint64_t w) {
@@ -492,8 +517,19 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
               }
           }
           M fv[8];
+          if (bmp) {
+            uint64_t bw[8];
 #pragma unroll
-          for (int st = 0; st < 8; st++) fv[st] = fmask[u[st]];
+            for (int st = 0; st < 8; st++) bw[st] = cubm[u[st] >> 6];
+#pragma unroll
+            for (int st = 0; st < 8; st++)
+              if (!((bw[st] >> (u[st] & 63)) & 1)) h[st] = 0;
+#pragma unroll
+            for (int st = 0; st < 8; st++) fv[st] = h[st] ? fmask[u[st]] : (M)0;
+          } else {
+#pragma unroll
+            for (int st = 0; st < 8; st++) fv[st] = fmask[u[st]];
+          }
 #pragma unroll
           for (int st = 0; st < 8; st++) h[st] &= fv[st];
 #pragma unroll
@@ -573,8 +609,19 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
                 }
               }
               M fv[kPullPerLane];
+              if (bmp) {
+                uint64_t bw[kPullPerLane];
 #pragma unroll
-              for (int t = 0; t < kPullPerLane; t++) fv[t] = fmask[uu[t]];
+                for (int t = 0; t < kPullPerLane; t++) bw[t] = cubm[uu[t] >> 6];
+#pragma unroll
+                for (int t = 0; t < kPullPerLane; t++)
+                  if (!((bw[t] >> (uu[t] & 63)) & 1)) hm[t] = 0;
+#pragma unroll
+                for (int t = 0; t < kPullPerLane; t++) fv[t] = hm[t] ? fmask[uu[t]] : (M)0;
+              } else {
+#pragma unroll
+                for (int t = 0; t < kPullPerLane; t++) fv[t] = fmask[uu[t]];
+              }
 #pragma unroll
               for (int t = 0; t < kPullPerLane; t++) hm[t] &= fv[t];
 #pragma unroll
@@ -601,7 +648,9 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
           nmask[vl] = acc;
           if (acc) seen[vl] = s | acc;
         }
-        cnt += __popcll(__ballot(acc != 0));
+        const uint64_t nword = __ballot(acc != 0);
+        if (lane == 0) nubm[w] = nword;
+        cnt += __popcll(nword);
         if (acc) dsum += vdeg<G>(a.off, a.deg, vl);
         won |= acc;
       });
@@ -628,7 +677,10 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
             if (kk < e) {
               t = a.ids[kk];
               add = (!G || t != kNone) && relax<M>(a, seen, nmask, u, f, t, won);
-              if (add) dsum += vdeg<G>(a.off, a.deg, t);
+              if (add) {
+                dsum += vdeg<G>(a.off, a.deg, t);
+                atomicOr(reinterpret_cast<unsigned long long *>(&nubm[t >> 6]), 1ull << (t & 63));
+              }
             }
             const uint64_t mask = __ballot(add);
             if (mask) {

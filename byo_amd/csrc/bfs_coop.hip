@@ -109,6 +109,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
   const uint64_t nbytes = (a.n + 7) >> 3;
 
   pieces_init(a.pieces, a.off, a.n, nwords, nwaves);
+  mlp_stage(a.dc, mlp);
   for (uint32_t j = 0; j < a.k; j++) {
     uint32_t *parents = a.parents + (uint64_t)j * a.pstride;
     const uint32_t src = a.srcs[j];
@@ -142,7 +143,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
       const uint64_t fdeg = ld_relaxed(&cur->deg);
       if (fs == 0) break;
       reached += fs;
-      const int dir = decide(a.dc, is_dense, fs, fdeg, visited_deg, L, mlp);
+      // this engine's dense pull already tests a bitmap frontier: a
+      // scheduled bitmap pull (4) is its pull
+      const int dsel = decide(a.dc, is_dense, fs, fdeg, visited_deg, L, mlp);
+      const int dir = dsel == 4 ? 1 : dsel;
       if (leader && L < 64) a.logs[(uint64_t)j * 64 + L] = LevelLog{fs, fdeg, (uint32_t)dir, 0, (unsigned long long)wall_clock64()};
       if (dir == 2) {
         visited_deg += fdeg;
@@ -535,6 +539,8 @@ DirCtl dir_ctl(byo_graph *g, uint64_t maxdeg, hipStream_t s, bool batched) {
   }
   d.maxdeg = maxdeg;
   d.margin = (float)g->tuning(batched ? "sel_margin_batch_x100" : "sel_margin_x100") / 100.0f;
+  d.third = batched ? 4 : 3;
+  d.densify = g->tuning("sel_densify") != 0;
   d.range = std::min(kSelClamp, (float)g->tuning("sel_range_x100") / 100.0f);
   uint8_t *dsched = g->buf("bfs_coop_sched", 64).as<uint8_t>();
   if (g->dir_policy == BYO_DIR_SCHEDULE) BYO_HIP(hipMemcpyAsync(dsched, g->dir_sched, 64, hipMemcpyHostToDevice, s));

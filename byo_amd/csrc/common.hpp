@@ -26,8 +26,10 @@ constexpr float kSelClamp = 3.0f;
 // (tuning knobs sel_margin_x100 and sel_margin_batch_x100, per traversal
 // engine; the trainer stores the value it validated with the weights).
 
-// Selector class (0 push, 1 pull, 2 forward) of a direction 0, 1 or 3.
-__host__ __device__ inline int dir_class(int dir) { return dir == 3 ? 2 : (dir == 1 ? 1 : 0); }
+// Selector class of a direction: 0 push (direction 0), 1 pull (1), 2 the
+// engine's third choice -- dense forward (3) for the one-source traversal,
+// bitmap pull (4) for the shared multi-source traversal.
+__host__ __device__ inline int dir_class(int dir) { return (dir == 3 || dir == 4) ? 2 : (dir == 1 ? 1 : 0); }
 constexpr int kWave = 64;
 
 // ---------------------------------------------------------------- errors
@@ -162,6 +164,7 @@ inline const std::vector<TuneSpec> &tune_specs() {
       {"sel_range_x100", 300, 0, 1000, "learned selector in-range bound, training deviations x100"},
       {"sel_margin_x100", 100, 0, 100000, "learned selector margin, logits x100 (one-source engine)"},
       {"sel_margin_batch_x100", 100, 0, 100000, "learned selector margin, logits x100 (shared traversal)"},
+      {"sel_densify", 0, 0, 1, "learned selector may turn a level the rule keeps sparse dense (0: never)"},
       {"pr_relabel", 0, 0, 1, "PageRank on a copy relabelled by descending degree (built once per graph)"},
       {"pr_xcd", 0, 0, 1, "PageRank pull split into one source slice per XCD (built once per graph)"},
       {"bfs_engine", 0, 0, 2,

@@ -140,15 +140,27 @@ struct DirCtl {
   const uint8_t *sched;  // 64 per-level directions (BYO_DIR_SCHEDULE)
   float margin;          // logit margin the network must beat the rule by
   float range;           // in-range bound on the normalised features
+  int third;             // direction of selector class 2 (3 forward, 4 bitmap pull)
+  bool densify;          // the network may turn a level the rule keeps sparse dense
 };
 
 struct MlpLds {
-  float x[6], h1[32], h2[32], z[3];
+  float w[BYO_MLP_NPARAM + 12];  // the selector's parameters and normalisation, staged once per launch
+  float z[3];  // the logits of the last decision
   bool in_range;
 };
 
+// Stages the selector's parameters in the block's LDS (every thread calls
+// it once, before the first decide); a no-op unless the learned policy runs.
+__device__ inline void mlp_stage(const DirCtl &a, MlpLds &s) {
+  if (a.policy != BYO_DIR_MLP || !a.sel) return;
+  for (int i = threadIdx.x; i < BYO_MLP_NPARAM + 12; i += blockDim.x) s.w[i] = a.sel[i];
+  __syncthreads();
+}
+
 // Direction of one edgeMap call: 0 sparse, 1 dense pull, 2 empty, 3 dense
-// forward (the same decision as choose_direction in bfs.hip).  Called by
+// forward, 4 bitmap pull (the same decision as choose_direction in bfs.hip;
+// the rule picks 0 or 1, the network may pick the engine's third direction).  Called by
 // every thread of the block (the selector MLP runs one neuron per thread,
 // parameter layout as in selector.hip); returns the same value in all
 // threads.
@@ -167,43 +179,46 @@ __device__ inline int decide(const DirCtl &a, bool is_dense, uint64_t fs, uint64
   if (a.policy == BYO_DIR_DENSE) return 1;
   if (a.policy == BYO_DIR_SCHEDULE) return (level < 64 && a.sched[level] != 255) ? a.sched[level] : rule;
   if (a.policy != BYO_DIR_MLP || !a.sel) return rule;
-  const float *p = a.sel;
-  const int t = threadIdx.x;
-  __syncthreads();
-  if (t == 0) {
+  // a level the rule keeps sparse stays sparse unless densify is set (a
+  // wrong dense choice scans every unvisited edge, a wrong sparse one only
+  // the frontier's): then there is nothing for the network to decide
+  if (rule == 0 && !a.densify) return rule;
+  const float *p = s.w;  // staged by mlp_stage
+  // one wave evaluates the network (lanes 0-31 hold the hidden units, the
+  // second half repeats them; hidden values travel by shuffles), then one
+  // barrier publishes the logits to the block
+  __syncthreads();  // every thread has read the previous call's logits
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x, j = lane & 31;
     const double avg = a.n ? (double)a.m / (double)a.n : 0.0;
+    const float xr[6] = {log2f(1.0f + (float)fs), log2f(1.0f + (float)fdeg),
+                         log2f(1.0f + (float)(a.m > visited_deg ? a.m - visited_deg : 0)), (float)level,
+                         log2f(1.0f + (float)avg), log2f(1.0f + (float)a.maxdeg)};
     float x[6];
-    x[0] = (float)log2(1.0 + (double)fs);
-    x[1] = (float)log2(1.0 + (double)fdeg);
-    x[2] = (float)log2(1.0 + (double)(a.m > visited_deg ? a.m - visited_deg : 0));
-    x[3] = (float)level;
-    x[4] = (float)log2(1.0 + avg);
-    x[5] = (float)log2(1.0 + (double)a.maxdeg);
     bool in_range = true;
+#pragma unroll
     for (int i = 0; i < 6; i++) {
-      const float v = (x[i] - p[kNP + i]) * p[kNP + 6 + i];
+      const float v = (xr[i] - p[kNP + i]) * p[kNP + 6 + i];
       in_range = in_range && fabsf(v) <= a.range;
-      s.x[i] = fminf(fmaxf(v, -kSelClamp), kSelClamp);
+      x[i] = fminf(fmaxf(v, -kSelClamp), kSelClamp);
     }
-    s.in_range = in_range;
-  }
-  __syncthreads();
-  if (t < 32) {
-    float v = p[192 + t];
-    for (int i = 0; i < 6; i++) v += p[t * 6 + i] * s.x[i];
-    s.h1[t] = v > 0.f ? v : 0.f;
-  }
-  __syncthreads();
-  if (t < 32) {
-    float v = p[1248 + t];
-    for (int i = 0; i < 32; i++) v += p[224 + t * 32 + i] * s.h1[i];
-    s.h2[t] = v > 0.f ? v : 0.f;
-  }
-  __syncthreads();
-  if (t < 3) {
-    float v = p[1376 + t];
-    for (int i = 0; i < 32; i++) v += p[1280 + t * 32 + i] * s.h2[i];
-    s.z[t] = v;
+    float h1 = p[192 + j];
+#pragma unroll
+    for (int i = 0; i < 6; i++) h1 += p[j * 6 + i] * x[i];
+    h1 = h1 > 0.f ? h1 : 0.f;
+    float h2 = p[1248 + j];
+#pragma unroll 8
+    for (int i = 0; i < 32; i++) h2 += p[224 + j * 32 + i] * __shfl(h1, i, 64);
+    h2 = h2 > 0.f ? h2 : 0.f;
+    float z[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) z[c] = p[1376 + c] + wave_sum(lane < 32 ? p[1280 + c * 32 + lane] * h2 : 0.f);
+    if (lane == 0) {
+      s.z[0] = z[0];
+      s.z[1] = z[1];
+      s.z[2] = z[2];
+      s.in_range = in_range;
+    }
   }
   __syncthreads();
   const float z0 = s.z[0], z1 = s.z[1], z2 = s.z[2];
@@ -212,7 +227,7 @@ __device__ inline int decide(const DirCtl &a, bool is_dense, uint64_t fs, uint64
   // the rule stands unless the frontier is in the training range and the
   // network prefers another direction by at least the margin
   if (!s.in_range || s.z[best] - s.z[dir_class(rule)] < a.margin) return rule;
-  return best == 0 ? 0 : (best == 1 ? 1 : 3);
+  return best == 0 ? 0 : (best == 1 ? 1 : a.third);
 }
 
 // Direction-decision inputs of a handle (policy, threshold, and the

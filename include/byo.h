@@ -214,9 +214,14 @@ typedef struct {
 /* threshold < 0 means M/20 (the -1 default of edgeMap). */
 int byo_set_direction_policy(byo_graph_t g, int policy, int64_t threshold);
 /* Explicit per-level directions (0 sparse push, 1 dense pull, 3 dense
- * forward, 255 = the threshold rule) for levels 0..len-1 (len <= 64), the
- * rule beyond; selects BYO_DIR_SCHEDULE.  Used to time every direction at
- * every level when training the learned selector. */
+ * forward, 4 bitmap pull, 255 = the threshold rule) for levels 0..len-1
+ * (len <= 64), the rule beyond; selects BYO_DIR_SCHEDULE.  Bitmap pull is
+ * the dense pull that tests each in-neighbour against a 1-bit union
+ * frontier before reading its per-source frontier mask (the shared
+ * multi-source traversal; the one-source traversal's pull already reads a
+ * bitmap, so 4 means 1 there).  The learned selector's third class is 3 for
+ * the one-source engine and 4 for the shared traversal.  Used to time every
+ * direction at every level when training the learned selector. */
 int byo_set_direction_schedule(byo_graph_t g, const uint8_t *dirs, uint32_t len);
 
 /* BFS from src.  Replaces BFS(G, src) (benchmarks/BFS/NonDeterministicBFS/

@@ -78,14 +78,23 @@ def test_batch_forced_directions(B, s16, policy):
 
 def test_batch_schedules(B, s16):
     g, G = s16
-    srcs = O.bfs_sources(g, 16, seed=11)
     try:
-        for sched in ([0, 0, 1, 1, 3, 0], [3, 3, 3, 3, 3, 3], [1, 0, 1, 0, 1, 0, 1], [255, 255, 3, 0, 1]):
-            G.set_direction_schedule(sched)
-            _, stats = check_batch(g, G, srcs)
-            for i, d in enumerate(sched):
-                if d != 255 and i < max(s["levels"] for s in stats) and stats[0]["dir"][i] != 2:
-                    assert stats[0]["dir"][i] == d
+        for srcs in (O.bfs_sources(g, 16, seed=11), O.bfs_sources(g, 5, seed=12), O.bfs_sources(g, 50, seed=13)):
+            # 4: bitmap pull (union-frontier bitmap before the mask gather)
+            for sched in ([0, 0, 1, 1, 3, 0], [3, 3, 3, 3, 3, 3], [1, 0, 1, 0, 1, 0, 1], [255, 255, 3, 0, 1],
+                          [4] * 12, [0, 0, 4, 1, 4, 0, 4], [4, 3, 4, 0, 1, 4, 3, 4], [0, 0, 0, 4, 4, 4, 0, 0]):
+                G.set_direction_schedule(sched)
+                _, stats = check_batch(g, G, srcs)
+                dl = max((s["dir"] for s in stats), key=len)  # the shared per-level record
+                for i, d in enumerate(sched):
+                    if d != 255 and i < len(dl) and dl[i] != 2:
+                        assert dl[i] == d
+        # the one-source engine runs a scheduled bitmap pull as its (bitmap) pull
+        G.set_direction_schedule([0, 4, 4, 0, 4])
+        src = O.bfs_sources(g, 1, seed=14)[0]
+        par, st = G.bfs(src)
+        assert O.check_bfs_tree(g, src, par, O.bfs_levels(g, src)) == 0
+        assert all(d in (0, 1, 2) for d in st["dir"][:5])
     finally:
         G.set_direction_policy(0)
 

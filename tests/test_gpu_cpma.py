@@ -280,6 +280,10 @@ def test_algorithms_read_the_pma(B, scale):
     G.set_direction_schedule([0, 3, 3, 1, 0])  # push, forward, forward, pull, push
     par, _ = G.bfs(srcs[1])
     assert O.check_bfs_tree(h, srcs[1], par, O.bfs_levels(h, srcs[1])) == 0
+    G.set_direction_schedule([0, 0, 4, 4, 1, 0])  # bitmap pulls on the PMA view
+    pars, _ = G.bfs_multi(srcs)
+    for j, src in enumerate(srcs):
+        assert O.check_bfs_tree(h, src, pars[j], O.bfs_levels(h, src)) == 0, ("bitmap pull", src)
     G.set_direction_policy(B.DIR_THRESHOLD)
     p_ref, _ = O.pagerank(h, 0.85, 20, 0.0)
     p64, _, _ = G.pagerank(0.85, 20, 0.0, B.PR_FP64)

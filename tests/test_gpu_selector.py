@@ -50,12 +50,9 @@ def test_training_reduces_loss_and_predict(B):
 
 
 @pytest.mark.parametrize("engine", ["single", "batch"])
-def test_learned_policy_is_bit_exact_and_not_slower(B, engine):
-    """The committed selector of each traversal engine (trained on the GPU on
-    RMAT scale 18-22 timings of that engine) keeps BFS levels bit-exact on
-    graphs it never saw, and its direction choices cost no more device time
-    than the M/20 rule (the gain itself is printed and recorded by
-    bench.py)."""
+def test_learned_policy_is_bit_exact(B, engine):
+    """The committed selector of each traversal engine keeps BFS levels and
+    parent trees valid on graphs it never saw (a 48^3 grid, RMAT-16)."""
     import os
     from byo_amd import selector as S
     if not os.path.exists(S.ENGINES[engine][1]):
@@ -69,10 +66,47 @@ def test_learned_policy_is_bit_exact_and_not_slower(B, engine):
             lv = O.bfs_levels(g, s)
             assert np.array_equal(G.bfs_levels(), lv)
             assert O.check_bfs_tree(g, s, par, lv) == 0
-        par, _ = G.bfs_multi(srcs)
-        for j, s in enumerate(srcs):
-            assert O.check_bfs_tree(g, s, par[j], O.bfs_levels(g, s)) == 0
-        res = S.compare(G, O.bfs_sources(g, 8 if engine == "batch" else 4, seed=22), engine=engine)
-        print("selector gain", engine, res["speedup"], res["learned_dirs"], res["threshold_dirs"])
+        for k in (8, 64):
+            srcs = O.bfs_sources(g, k, seed=21)
+            par, _ = G.bfs_multi(srcs)
+            for j, s in enumerate(srcs):
+                assert O.check_bfs_tree(g, s, par[j], O.bfs_levels(g, s)) == 0
+        G.close()
+
+
+HELD_OUT = {"rmat24_ef32": lambda B: B.CsrGraph.rmat(24, 32), "er23_deg8": lambda B: B.CsrGraph.erdos_renyi(23, 4 << 23),
+            "grid256": lambda B: B.CsrGraph.grid3d(256)}
+
+
+@pytest.mark.parametrize("graph", sorted(HELD_OUT))
+@pytest.mark.parametrize("engine", ["single", "batch"])
+def test_learned_policy_on_held_out_graphs(B, engine, graph):
+    """Config C5: the selector, trained on RMAT scale 18-22 timings, against
+    the M/20 rule on the held-out graphs (RMAT scale 24 ef 32, ER n = 2^23
+    degree 8, the 256^3 grid), with the shared traversal at the benchmark's
+    8 sources and the batched job's 64.  Levels are identical under both
+    policies; where the network changes a direction the traversal is
+    faster (speedup >= 1); where it keeps the rule's directions the work is
+    the same and the time differs only by the network's evaluation (bounded
+    at 3 %); on RMAT-24 ef 32 with 64 sources, where the rule pulls a level
+    too early, the gain reaches the 1.2x target."""
+    import os
+    from byo_amd import selector as S
+    if not os.path.exists(S.ENGINES[engine][1]):
+        pytest.skip("no trained %s-engine selector weights committed" % engine)
+    G = HELD_OUT[graph](B)
+    widths = (8, 64) if engine == "batch" else (4,)
+    for k in widths:
+        srcs = S.sources(G.degrees(), k, seed=5)
+        res = S.compare(G, srcs, engine=engine, repeats=5)
+        print("selector", engine, graph, "sources", k, "speedup %.3f" % res["speedup"],
+              "rule %.3f ms learned %.3f ms" % (res["threshold"], res["learned"]),
+              "dirs", res["learned_dirs"][:12], "rule dirs", res["threshold_dirs"][:12], flush=True)
         assert res["levels_identical"]
-        assert res["speedup"] > 0.8
+        if res["same_directions"]:
+            assert res["speedup"] >= 0.97
+        else:
+            assert res["speedup"] >= 1.0
+        if engine == "batch" and graph == "rmat24_ef32" and k == 64:
+            assert res["speedup"] >= 1.2
+    G.close()
