@@ -239,6 +239,48 @@ def batched_job(B, args):
     return res
 
 
+def only_section(B, G, args, srcs, policy):
+    """One section of the benchmark, args.steps times after args.warmup
+    untimed runs, for profiling a kernel on its own; returns a short record."""
+    n, m = G.N(), G.M()
+    res = {"section": args.only, "graph": "RMAT-%d ef%d" % (args.scale, args.ef), "n": n, "m": m}
+    if args.only == "bfs":
+        G.set_direction_policy(policy)
+        f = lambda: G.bfs_multi(srcs, want_parents=False)
+    elif args.only == "pr":
+        f = lambda: G.pagerank(0.85, 20, 0.0, B.PR_FP32, want_p=False)
+    elif args.only == "map":
+        one = np.zeros(n * 64, np.uint8)
+        one[0] = 1
+
+        def f():
+            G.map_sum()
+            for w in (4, 64):
+                G.map_remote(one, w)
+    elif args.only == "cc":
+        f = G.cc
+    elif args.only == "tc":
+        f = G.tc
+    else:
+        C = B.CpmaGraph.from_graph(G)
+        keys = B.update_rmat_keys(n, args.cpma_batch)
+        d = B.DeviceBuffer(keys)
+
+        def f():
+            C.insert_batch_dev(d.ptr, len(keys))
+            C.remove_batch_dev(d.ptr, len(keys))
+    for _ in range(args.warmup):
+        f()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f()
+    res["ms_per_run"] = round((time.perf_counter() - t0) / args.steps * 1e3, 4)
+    if args.only == "cpma":
+        d.free()
+        C.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +295,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=8.0)
+    ap.add_argument("--only", choices=["bfs", "pr", "map", "cc", "tc", "cpma"], default=None,
+                    help="run one section a few times and print its own short JSON line "
+                         "(for per-kernel profiling runs)")
+    ap.add_argument("--set", action="append", default=[], metavar="KNOB=VALUE",
+                    help="tuning knob override applied after the stored tuning (repeatable)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -273,9 +320,17 @@ def main():
                                                                           time.time() - t), file=sys.stderr)
 
     knobs = G.apply_tuning()  # autotuned block/tile sizes for this graph size, if stored
+    for kv in args.set:
+        k, v = kv.split("=", 1)
+        G.set_tuning(k, int(v))
+        knobs[k] = int(v)
     from byo_amd import selector as S
     learned = bool(S.load(G).get("batch"))
     policy = B.DIR_MLP if learned else B.DIR_THRESHOLD
+    if args.only:
+        print(json.dumps(only_section(B, G, args, srcs, policy)), flush=True)
+        G.close()
+        return
 
     def timed_bfs(pol, steps, mode=None):
         """K steps of the 8-source BFS under a direction policy; returns

@@ -6,7 +6,10 @@ fastest, and stores the winners per graph size in byo_amd/tuning.json.
 Graph.apply_tuning() later applies the entry of the closest size.  What each
 knob is timed on:
 
-    coop_blocks_per_cu, pieces   8-source BFS in one shared traversal (launch ms)
+    coop_blocks_per_cu, pieces   the benchmark's step exactly: its 8 sources
+                                 (SplitMix64 seed 1) in one shared traversal
+                                 under the committed learned selector
+                                 (launch ms, best of 7)
     ppr_blocks_per_cu            32-seed personalized PageRank, 5 iterations
     pr_relabel, pr_xcd,          FP32 PageRank, 5 iterations (the one-time
     pr_lanes, pr_grid            layouts are built before the timing)
@@ -43,10 +46,10 @@ def _sources(deg, k, seed):
 
 
 def measure(G, workload, srcs, seeds, repeats=3):
-    """Best-of-repeats device ms of one workload."""
+    """Best-of-repeats device ms of one workload (BFS: best of 7)."""
     from . import PR_FP32
     best = float("inf")
-    for _ in range(repeats):
+    for _ in range(7 if workload == "bfs" else repeats):
         if workload == "bfs":
             G.profile(True)
             G.profile_reset()
@@ -61,10 +64,16 @@ def measure(G, workload, srcs, seeds, repeats=3):
 
 
 def tune(G, label=""):
-    """Coordinate descent over CANDIDATES; returns (knobs, times)."""
+    """Coordinate descent over CANDIDATES; returns (knobs, times).  The BFS
+    knobs are timed on the launch bench.py times: the same sources and the
+    same direction policy (the learned selector when weights are committed)."""
+    from . import DIR_MLP
+    from . import selector as S
     deg = G.degrees()
-    srcs = _sources(deg, 8, 11)
+    srcs = _sources(deg, 8, 1)
     seeds = _sources(deg, 32, 12)
+    if S.load(G).get("batch"):
+        G.set_direction_policy(DIR_MLP)
     knobs, times = {}, {}
     for key, cands in CANDIDATES.items():
         wl = WORKLOAD[key]

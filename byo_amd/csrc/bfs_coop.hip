@@ -89,6 +89,23 @@ constexpr int kG = 8;  // lanes per vertex in the pull
 
 __device__ __forceinline__ bool bit_get(const uint64_t *bm, uint32_t i) { return (bm[i >> 6] >> (i & 63)) & 1ull; }
 
+// Claims vertex t for the next frontier: t is in neither the visited bitmap
+// nor the current frontier's, and this thread's atomicOr set its bit in the
+// next frontier's bitmap (one winner per vertex, which then writes the
+// parent with a plain store).  The visited bitmap lags by one level: a push
+// or forward level folds the current frontier into it while it runs, a pull
+// level as it writes each word.
+// A relaxed device-scope read of the next bitmap first skips the atomic for a
+// vertex already claimed this level: RMAT hubs take thousands of edges of one
+// frontier, and atomics on one word serialise in L2.
+__device__ __forceinline__ bool claim(const uint64_t *visited, const uint64_t *fbm, uint64_t *nbm, uint32_t t) {
+  const uint64_t w = t >> 6, bit = 1ull << (t & 63);
+  if ((visited[w] | fbm[w]) & bit) return false;
+  unsigned long long *p = reinterpret_cast<unsigned long long *>(&nbm[w]);
+  if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit) return false;
+  return !(atomicOr(p, (unsigned long long)bit) & bit);
+}
+
 struct PushLds {
   uint64_t sc[kWaves][64];
   int64_t base[kWaves][64];
@@ -165,6 +182,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
         uint64_t *clr = a.bm + (uint64_t)((L + 2) % 3) * a.nw64;
         for (uint64_t i = global_tid(); i < a.nw64; i += grid_threads()) clr[i] = 0;
       }
+      if (dir != 1) {
+        // push and forward claim against visited | frontier; the frontier is
+        // folded into the visited bitmap while they run (no other writer)
+        for (uint64_t i = global_tid(); i < a.nw64; i += grid_threads()) {
+          const uint64_t f = fbm[i];
+          if (f) a.visited64[i] |= f;
+        }
+      }
 
       if (dir == 0) {
         if (!have_q) {
@@ -205,11 +230,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
               uint32_t tgt = 0;
               if (kk < e) {
                 tgt = a.ids[kk];
-                if ((!G || tgt != kNone) && !bit_get(a.visited64, tgt) &&
-                    atomicCAS(&parents[tgt], kNone, u) == kNone) {
+                if ((!G || tgt != kNone) && claim(a.visited64, fbm, nbm, tgt)) {
                   win_ = true;
-                  atomicOr(reinterpret_cast<unsigned long long *>(&a.visited64[tgt >> 6]), 1ull << (tgt & 63));
-                  atomicOr(reinterpret_cast<unsigned long long *>(&nbm[tgt >> 6]), 1ull << (tgt & 63));
+                  parents[tgt] = u;
                   dsum += vdeg<G>(a.off, a.deg, tgt);
                 }
               }
@@ -329,11 +352,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
                 bs = win.base[wv][lo];
               }
               tgt = a.ids[(uint64_t)(bs + (int64_t)e)];
-              if ((!G || tgt != kNone) && !bit_get(a.visited64, tgt) &&
-                  atomicCAS(&parents[tgt], kNone, (uint32_t)owner_u) == kNone) {
+              if ((!G || tgt != kNone) && claim(a.visited64, fbm, nbm, tgt)) {
                 win_ = true;
-                atomicOr(reinterpret_cast<unsigned long long *>(&a.visited64[tgt >> 6]), 1ull << (tgt & 63));
-                atomicOr(reinterpret_cast<unsigned long long *>(&nbm[tgt >> 6]), 1ull << (tgt & 63));
+                parents[tgt] = (uint32_t)owner_u;
                 dsum += vdeg<G>(a.off, a.deg, tgt);
               }
             }
@@ -368,7 +389,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
         const int g = lane >> 3, sub = lane & 7;
         unsigned long long cnt = 0, dsum = 0;
         for_my_words(a.pieces, wave, nwaves, [&](uint64_t w) {
-          const uint64_t vis = a.visited64[w];
+          // discovered = visited or in the current frontier (the bitmap
+          // lags by one level after a push)
+          const uint64_t v0 = a.visited64[w];
+          const uint64_t vis = v0 | fbm[w];
           const uint64_t vl = w * 64 + lane;
           uint64_t bl = 0, el = 0;
           if (vl < a.n) {
@@ -458,7 +482,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
           }
           if (lane == 0) {
             nbm[w] = fmask;
-            if (fmask) a.visited64[w] = vis | fmask;
+            if ((vis | fmask) != v0) a.visited64[w] = vis | fmask;
           }
           cnt += __popcll(fmask);
           if ((fmask >> lane) & 1) dsum += vdeg<G>(a.off, a.deg, vl);
@@ -486,10 +510,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
               uint32_t t = 0;
               if (kk < e) {
                 t = a.ids[kk];
-                if ((!G || t != kNone) && !bit_get(a.visited64, t) && atomicCAS(&parents[t], kNone, u) == kNone) {
+                if ((!G || t != kNone) && claim(a.visited64, fbm, nbm, t)) {
                   win_ = true;
-                  atomicOr(reinterpret_cast<unsigned long long *>(&a.visited64[t >> 6]), 1ull << (t & 63));
-                  atomicOr(reinterpret_cast<unsigned long long *>(&nbm[t >> 6]), 1ull << (t & 63));
+                  parents[t] = u;
                 }
               }
               const uint64_t mask = __ballot(win_);

@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(256) k_pr_seg(const uint64_t *lo_pos, const ui
 constexpr int kPrUnroll = 16;
 
 // H: the lists may hold empty slots (kNone, the PMA's leaf ends): skipped.
-template <class T, int G, bool kComp, bool H>
+template <class T, int G, bool kComp, bool H, int U = kPrUnroll>
 __device__ __forceinline__ Acc<T> pr_tile(uint64_t lo, uint64_t hi, const uint32_t *ids, const T *contrib_in,
                                           int lane) {
   constexpr int VPR = 64 / G;  // vertices per round
@@ -131,21 +131,21 @@ __device__ __forceinline__ Acc<T> pr_tile(uint64_t lo, uint64_t hi, const uint32
     const uint64_t b = G == 1 ? lo : __shfl(lo, owner, 64);
     const uint64_t e = G == 1 ? hi : __shfl(hi, owner, 64);
     Acc<T> a{0, 0};
-    for (uint64_t k = b + sub; k < e; k += (uint64_t)kPrUnroll * G) {
-      uint32_t u[kPrUnroll];
-      T x[kPrUnroll];
-      bool hole[kPrUnroll];
+    for (uint64_t k = b + sub; k < e; k += (uint64_t)U * G) {
+      uint32_t u[U];
+      T x[U];
+      bool hole[U];
 #pragma unroll
-      for (int j = 0; j < kPrUnroll; j++) {
+      for (int j = 0; j < U; j++) {
         const uint64_t kk = k + (uint64_t)j * G;
         u[j] = ids[kk < e ? kk : b];
         hole[j] = H && u[j] == kNone;
         if (H && hole[j]) u[j] = 0;
       }
 #pragma unroll
-      for (int j = 0; j < kPrUnroll; j++) x[j] = contrib_in[u[j]];
+      for (int j = 0; j < U; j++) x[j] = contrib_in[u[j]];
 #pragma unroll
-      for (int j = 0; j < kPrUnroll; j++) {
+      for (int j = 0; j < U; j++) {
         if (k + (uint64_t)j * G < e && !hole[j]) {
           if (kComp) kahan_add(a, x[j]); else a.s += x[j];
         }
@@ -164,6 +164,20 @@ __device__ __forceinline__ Acc<T> pr_tile(uint64_t lo, uint64_t hi, const uint32
     if (rel >= 0 && rel < VPR) mine = Acc<T>{gs, gc};
   }
   return mine;
+}
+
+// Lanes per vertex for a tile whose longest list is dmax: the smallest G
+// with G * U >= dmax (a tile of short lists runs a vertex per lane).
+template <class T, bool kComp, bool H, int U>
+__device__ __forceinline__ Acc<T> pr_tile_any(uint32_t dmax, uint64_t lo, uint64_t hi, const uint32_t *ids,
+                                              const T *contrib_in, int lane) {
+  if (dmax <= U) return pr_tile<T, 1, kComp, H, U>(lo, hi, ids, contrib_in, lane);
+  if (dmax <= 2 * U) return pr_tile<T, 2, kComp, H, U>(lo, hi, ids, contrib_in, lane);
+  if (dmax <= 4 * U) return pr_tile<T, 4, kComp, H, U>(lo, hi, ids, contrib_in, lane);
+  if (dmax <= 8 * U) return pr_tile<T, 8, kComp, H, U>(lo, hi, ids, contrib_in, lane);
+  if (dmax <= 16 * U) return pr_tile<T, 16, kComp, H, U>(lo, hi, ids, contrib_in, lane);
+  if (dmax <= 32 * U) return pr_tile<T, 32, kComp, H, U>(lo, hi, ids, contrib_in, lane);
+  return pr_tile<T, 64, kComp, H, U>(lo, hi, ids, contrib_in, lane);
 }
 
 // H: the adjacency view holds empty slots (the CPMA); degrees come from
@@ -219,6 +233,7 @@ __global__ void __launch_bounds__(256) k_pr_tiles(const uint64_t *off, const uin
 // kernel combines the 8 partials in slice order (deterministic) and
 // finalises the ranks.
 constexpr int kXcds = 8;
+constexpr int kXcdUnroll = 2;  // gathers in flight per lane in a slice's short lists (2 measured best of 1-4, 8)
 
 template <class T, bool kComp>
 __global__ void __launch_bounds__(256) k_pr_xcd(const uint32_t *so, const uint32_t *ids_s, uint64_t n,
@@ -234,14 +249,8 @@ __global__ void __launch_bounds__(256) k_pr_xcd(const uint32_t *so, const uint32
     const bool valid = v < n;
     const uint64_t lo = sox[valid ? v : n], hi = sox[valid ? v + 1 : n];
     const uint32_t dmax = wave_max((uint32_t)(hi - lo));
-    Acc<T> mine;
-    if (dmax <= kPrUnroll) mine = pr_tile<T, 1, kComp, false>(lo, hi, ids_s, contrib_in, lane);
-    else if (dmax <= 2 * kPrUnroll) mine = pr_tile<T, 2, kComp, false>(lo, hi, ids_s, contrib_in, lane);
-    else if (dmax <= 4 * kPrUnroll) mine = pr_tile<T, 4, kComp, false>(lo, hi, ids_s, contrib_in, lane);
-    else if (dmax <= 8 * kPrUnroll) mine = pr_tile<T, 8, kComp, false>(lo, hi, ids_s, contrib_in, lane);
-    else if (dmax <= 16 * kPrUnroll) mine = pr_tile<T, 16, kComp, false>(lo, hi, ids_s, contrib_in, lane);
-    else if (dmax <= 32 * kPrUnroll) mine = pr_tile<T, 32, kComp, false>(lo, hi, ids_s, contrib_in, lane);
-    else mine = pr_tile<T, 64, kComp, false>(lo, hi, ids_s, contrib_in, lane);
+    // slices hold a few edges per vertex: short unrolled steps
+    const Acc<T> mine = pr_tile_any<T, kComp, false, kXcdUnroll>(dmax, lo, hi, ids_s, contrib_in, lane);
     if (valid) {
       part_s[(uint64_t)x * n + v] = mine.s;
       if (kComp) part_c[(uint64_t)x * n + v] = mine.c;
