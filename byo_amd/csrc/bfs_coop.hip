@@ -17,8 +17,9 @@
 //  * sparse push (edgeMapSparse, gbbs/edge_map_blocked.h:18-92): the edges
 //    of the frontier are cut into chunks of 256; the scan pass also records
 //    the first frontier index of every chunk, so a wave loads the (scan,
-//    adjacency base) window of its chunk into LDS with one coalesced load and
-//    resolves each edge's owner by a 6-step LDS search;
+//    adjacency base) window of its chunk into LDS with coalesced loads (64
+//    entries, up to 256 for frontiers of low degree) and resolves each
+//    edge's owner by an LDS binary search;
 //  * dense pull (edgeMapDense, edge_map_data.h:47-175): a wave owns one
 //    64-vertex bitmap word per step; 8 lanes per unvisited vertex read 8
 //    consecutive neighbours at once and test them against the bitmap,
@@ -67,6 +68,7 @@ struct CoopArgs {
   uint32_t *q0, *q1;
   uint64_t *bm;              // three rotating bitmaps of nw64 words
   uint64_t *visited64;
+  uint8_t *mark;            // n bytes (+ padding to 16), zero between levels
   uint64_t *scan;
   uint32_t *cstart;         // first frontier index of each 256-edge chunk
   unsigned long long *bsum;
@@ -89,6 +91,17 @@ constexpr int kG = 8;  // lanes per vertex in the pull
 
 __device__ __forceinline__ bool bit_get(const uint64_t *bm, uint32_t i) { return (bm[i >> 6] >> (i & 63)) & 1ull; }
 
+// A per-block filter in LDS of vertices known to be claimed: direct-mapped
+// by a multiplicative hash, an entry holding a vertex id.  An id is stored
+// only once its bit has been seen set in the next-frontier bitmap (or won),
+// so a hit means the vertex is discovered for good and the global tests
+// are skipped.  RMAT hubs take thousands of edges of one frontier; their
+// bitmap words would otherwise be read in L2 by every such edge, and all of
+// those reads queue on the one L2 channel that holds the word.
+constexpr int kFiltLog = 11;
+constexpr int kFilt = 1 << kFiltLog;
+__device__ __forceinline__ uint32_t filt_slot(uint32_t t) { return (t * 2654435761u) >> (32 - kFiltLog); }
+
 // Claims vertex t for the next frontier: t is in neither the visited bitmap
 // nor the current frontier's, and this thread's atomicOr set its bit in the
 // next frontier's bitmap (one winner per vertex, which then writes the
@@ -96,19 +109,153 @@ __device__ __forceinline__ bool bit_get(const uint64_t *bm, uint32_t i) { return
 // or forward level folds the current frontier into it while it runs, a pull
 // level as it writes each word.
 // A relaxed device-scope read of the next bitmap first skips the atomic for a
-// vertex already claimed this level: RMAT hubs take thousands of edges of one
-// frontier, and atomics on one word serialise in L2.
-__device__ __forceinline__ bool claim(const uint64_t *visited, const uint64_t *fbm, uint64_t *nbm, uint32_t t) {
+// vertex already claimed this level.
+__device__ __forceinline__ bool claim(const uint64_t *visited, const uint64_t *fbm, uint64_t *nbm, uint32_t t,
+                                      uint32_t *filt) {
+  const uint32_t slot = filt_slot(t);
+  if (filt[slot] == t) return false;
   const uint64_t w = t >> 6, bit = 1ull << (t & 63);
   if ((visited[w] | fbm[w]) & bit) return false;
   unsigned long long *p = reinterpret_cast<unsigned long long *>(&nbm[w]);
-  if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit) return false;
-  return !(atomicOr(p, (unsigned long long)bit) & bit);
+  const bool won = !(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit) &&
+                   !(atomicOr(p, (unsigned long long)bit) & bit);
+  filt[slot] = t;
+  return won;
 }
 
+// claim() over K candidate targets of one lane, stage by stage: all K filter
+// reads, then all K discovered tests, then all K relaxed reads of the next
+// bitmap, then all K atomicOr, so a lane waits for each stage once.  ok[t]
+// is in: t is a candidate; out: t was claimed by this thread.  Empty slots
+// (kNone) are no candidates when G.
+template <int K, bool G>
+__device__ __forceinline__ void claim_staged(const uint64_t *visited, const uint64_t *fbm, uint64_t *nbm,
+                                             const uint32_t (&tg)[K], bool (&ok)[K], uint32_t *filt) {
+  uint64_t x[K], y[K];
+#pragma unroll
+  for (int t = 0; t < K; t++) {
+    if (G && tg[t] == kNone) ok[t] = false;
+    if (ok[t] && filt[filt_slot(tg[t])] == tg[t]) ok[t] = false;
+  }
+#pragma unroll
+  for (int t = 0; t < K; t++) {
+    x[t] = 0;
+    y[t] = 0;
+    if (ok[t]) {
+      x[t] = visited[tg[t] >> 6];
+      y[t] = fbm[tg[t] >> 6];
+    }
+  }
+  bool tried[K];
+#pragma unroll
+  for (int t = 0; t < K; t++) {
+    const uint64_t bit = 1ull << (tg[t] & 63);
+    ok[t] = ok[t] && !((x[t] | y[t]) & bit);
+    tried[t] = ok[t];
+    if (ok[t])
+      x[t] = __hip_atomic_load(reinterpret_cast<unsigned long long *>(&nbm[tg[t] >> 6]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+  }
+#pragma unroll
+  for (int t = 0; t < K; t++) {
+    const uint64_t bit = 1ull << (tg[t] & 63);
+    ok[t] = ok[t] && !(x[t] & bit);
+    if (ok[t]) x[t] = atomicOr(reinterpret_cast<unsigned long long *>(&nbm[tg[t] >> 6]), (unsigned long long)bit);
+  }
+#pragma unroll
+  for (int t = 0; t < K; t++) {
+    ok[t] = ok[t] && !(x[t] & (1ull << (tg[t] & 63)));
+    if (tried[t]) filt[filt_slot(tg[t])] = tg[t];
+  }
+}
+
+// Push levels with at least this many frontier edges mark instead of
+// claiming.  The bitmap claim costs an L2 atomic per candidate edge, and
+// those atomics queue on the words they share (64 vertices a word, hubs
+// among them): 3.8 M candidates of an RMAT-22 level took 0.3 ms.  A mark is
+// a plain byte store, the same for every writer, and the parent a plain
+// store of whichever frontier neighbour stores last (each is a valid BFS
+// parent: the target is undiscovered and the neighbour in the frontier);
+// one pass over the marks then builds the next bitmap and queue.
+constexpr uint64_t kMarkMinEdges = 1ull << 20;
+
+// The mark counterpart of claim_staged: targets that are in neither the
+// visited bitmap nor the current frontier get mark[t] = 1 and
+// parents[t] = their owner.  The LDS filter skips targets this block has
+// marked already (and any claimed before).
+template <int K, bool G>
+__device__ __forceinline__ void mark_staged(const uint64_t *visited, const uint64_t *fbm, uint8_t *mark,
+                                            uint32_t *parents, const uint32_t (&tg)[K], const uint32_t (&own)[K],
+                                            bool (&ok)[K], uint32_t *filt) {
+  uint64_t x[K];
+#pragma unroll
+  for (int t = 0; t < K; t++) {
+    if (G && tg[t] == kNone) ok[t] = false;
+    if (ok[t] && filt[filt_slot(tg[t])] == tg[t]) ok[t] = false;
+  }
+#pragma unroll
+  for (int t = 0; t < K; t++) {
+    x[t] = ~0ull;
+    if (ok[t]) x[t] = visited[tg[t] >> 6] | fbm[tg[t] >> 6];
+  }
+#pragma unroll
+  for (int t = 0; t < K; t++) {
+    if (ok[t] && !((x[t] >> (tg[t] & 63)) & 1)) {
+      mark[tg[t]] = 1;
+      parents[tg[t]] = own[t];
+      filt[filt_slot(tg[t])] = tg[t];
+    }
+  }
+}
+
+// After a marking push: every wave takes runs of 32 bitmap words, reads the
+// marks of their 2048 vertices (lane = vertex), clears them, writes the next
+// frontier's bitmap words whole and appends the marked vertices to the next
+// queue (in id order within a run, one cursor atomic per run).  Returns the
+// lane's share of the next frontier's degree sum.
+template <bool G>
+__device__ __forceinline__ unsigned long long pack_marks(const CoopArgs &a, uint64_t *nbm, uint32_t *qn,
+                                                         unsigned long long *cursor, uint64_t nwords, uint64_t wave,
+                                                         uint64_t nwaves, int lane) {
+  constexpr int kRun = 32;
+  unsigned long long dsum = 0;
+  for (uint64_t r0 = wave * kRun; r0 < nwords; r0 += nwaves * kRun) {
+    uint64_t mine = 0;  // lane s keeps the word of step s
+#pragma unroll 4
+    for (int st = 0; st < kRun; st++) {
+      const uint64_t w = r0 + st;
+      if (w >= nwords) break;
+      const uint64_t v = w * 64 + lane;
+      const bool on = v < a.n && a.mark[v] != 0;
+      const uint64_t word = __ballot(on);
+      if (on) {
+        a.mark[v] = 0;
+        dsum += vdeg<G>(a.off, a.deg, v);
+      }
+      if (lane == st) mine = word;
+      if (lane == 0) nbm[w] = word;
+    }
+    const uint32_t c = (uint32_t)__popcll(mine);
+    const uint32_t incl = wave_incl_scan(c);
+    const uint32_t tot = __shfl(incl, 63, 64);
+    if (!tot) continue;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(cursor, (unsigned long long)tot);
+    base = __shfl(base, 0, 64) + (incl - c);  // lane s: start of step s's vertices
+    for (int st = 0; st < kRun; st++) {
+      const uint64_t word = __shfl(mine, st, 64);
+      const unsigned long long b = __shfl(base, st, 64);
+      if ((word >> lane) & 1) qn[b + mbcnt(word)] = (uint32_t)((r0 + st) * 64 + lane);
+    }
+  }
+  return dsum;
+}
+
+// (scan, adjacency base) of up to kChunk frontier entries per wave: a chunk
+// of kChunk edges spans at most kChunk entries of positive degree
 struct PushLds {
-  uint64_t sc[kWaves][64];
-  int64_t base[kWaves][64];
+  uint64_t sc[kWaves][kChunk];
+  int64_t base[kWaves][kChunk];
 };
 
 // G: the adjacency lists may hold empty slots (kNone), skipped everywhere.
@@ -117,6 +264,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
   __shared__ unsigned long long lds[8];
   __shared__ MlpLds mlp;
   __shared__ PushLds win;
+  __shared__ uint32_t filt[kFilt];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const uint64_t nb = gridDim.x;
@@ -135,6 +283,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
     // bitmaps start clear elsewhere: the pull writes only the bytes of real
     // vertices, so the padding of the last word must already be zero.
     for (uint64_t i = global_tid(); i < a.n; i += grid_threads()) parents[i] = (i == src) ? src : kNone;
+    for (int i = threadIdx.x; i < kFilt; i += kBlock) filt[i] = kNone;  // synchronised by the barrier below
+    {
+      uint4 *m4 = reinterpret_cast<uint4 *>(a.mark);
+      for (uint64_t i = global_tid(); i < (a.n + 15) / 16; i += grid_threads()) m4[i] = make_uint4(0, 0, 0, 0);
+    }
     const uint64_t sw = src >> 6, sbit = 1ull << (src & 63);
     for (uint64_t i = global_tid(); i < a.nw64; i += grid_threads()) {
       a.visited64[i] = (i == sw) ? sbit : 0;
@@ -230,7 +383,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
               uint32_t tgt = 0;
               if (kk < e) {
                 tgt = a.ids[kk];
-                if ((!G || tgt != kNone) && claim(a.visited64, fbm, nbm, tgt)) {
+                if ((!G || tgt != kNone) && claim(a.visited64, fbm, nbm, tgt, filt)) {
                   win_ = true;
                   parents[tgt] = u;
                   dsum += vdeg<G>(a.off, a.deg, tgt);
@@ -306,61 +459,79 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
         // (3) edge-balanced push, 256 edges (slots) per wave step
         const uint64_t E = G ? ld_relaxed(reinterpret_cast<const unsigned long long *>(&a.scan[fs])) : fdeg;
         const uint64_t nch = (E + kChunk - 1) / kChunk;
+        const bool marks = E >= kMarkMinEdges;
         unsigned long long dsum = 0;
         for (uint64_t c = wave; c < nch; c += nwaves) {
           const uint64_t e0 = c * kChunk;
           const uint64_t i0 = a.cstart[c];
-          {
-            const uint64_t idx = i0 + lane;
+          // the window: 64 entries, and 64 more while the last one loaded
+          // still starts before the chunk's end (low-degree frontiers)
+          int nwin = 0;
+          for (int r = 0; r < (int)(kChunk / 64); r++) {
+            const uint64_t idx = i0 + (uint64_t)r * 64 + lane;
             uint64_t sc = ~0ull;
             int64_t bs = 0;
             if (idx < fs) {
               sc = a.scan[idx];
               bs = (int64_t)a.off[q[idx]] - (int64_t)sc;
             }
-            win.sc[wv][lane] = sc;
-            win.base[wv][lane] = bs;
+            win.sc[wv][r * 64 + lane] = sc;
+            win.base[wv][r * 64 + lane] = bs;
+            nwin += 64;
+            if (__shfl(sc, 63, 64) >= e0 + kChunk) break;
           }
           wave_lds_sync();
-          uint32_t tg[kPerLane];
+          // the lane's kPerLane edges advance through each stage of the
+          // claim together (ids, discovered test, claim, parent and degree),
+          // so every stage has all its loads or atomics in flight at once
+          uint32_t tg[kPerLane], own[kPerLane];
           bool wn[kPerLane];
-          uint32_t total = 0;
 #pragma unroll
           for (int t = 0; t < kPerLane; t++) {
             const uint64_t e = e0 + (uint64_t)t * 64 + lane;
-            bool win_ = false;
-            uint32_t tgt = 0;
-            if (e < E) {
-              int lo = 0, hi = 64;  // sc[lo] <= e (sc[0] <= e0 by construction)
+            wn[t] = e < E;
+            own[t] = 0;
+            tg[t] = 0;
+            if (wn[t]) {
+              int lo = 0, hi = nwin;  // sc[lo] <= e (sc[0] <= e0 by construction)
               while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
                 if (win.sc[wv][mid] <= e) lo = mid; else hi = mid;
               }
-              uint64_t owner_u;
               int64_t bs;
-              if (lo == 63 && i0 + 64 < fs && a.scan[i0 + 64] <= e) {
-                // the chunk spans more than 64 frontier vertices
-                uint64_t x = i0 + 64, y = fs;
+              if (lo == nwin - 1 && i0 + nwin < fs && a.scan[i0 + nwin] <= e) {
+                // the chunk spans more than kChunk frontier entries (entries
+                // of degree 0: a directed graph)
+                uint64_t x = i0 + nwin, y = fs;
                 while (y - x > 1) {
                   const uint64_t mid = (x + y) >> 1;
                   if (a.scan[mid] <= e) x = mid; else y = mid;
                 }
-                owner_u = q[x];
-                bs = (int64_t)a.off[owner_u] - (int64_t)a.scan[x];
+                own[t] = q[x];
+                bs = (int64_t)a.off[own[t]] - (int64_t)a.scan[x];
               } else {
-                owner_u = q[i0 + lo];
+                own[t] = q[i0 + lo];
                 bs = win.base[wv][lo];
               }
-              tgt = a.ids[(uint64_t)(bs + (int64_t)e)];
-              if ((!G || tgt != kNone) && claim(a.visited64, fbm, nbm, tgt)) {
-                win_ = true;
-                parents[tgt] = (uint32_t)owner_u;
-                dsum += vdeg<G>(a.off, a.deg, tgt);
-              }
+              tg[t] = a.ids[(uint64_t)(bs + (int64_t)e)];
             }
-            tg[t] = tgt;
-            wn[t] = win_;
-            total += __popcll(__ballot(win_));
+          }
+          if (marks) {
+            // large frontier: mark, do not claim (the pack pass below builds
+            // the next frontier from the marks)
+            mark_staged<kPerLane, G>(a.visited64, fbm, a.mark, parents, tg, own, wn, filt);
+            wave_lds_sync();  // the window is rewritten by the next chunk
+            continue;
+          }
+          claim_staged<kPerLane, G>(a.visited64, fbm, nbm, tg, wn, filt);
+          uint32_t total = 0;
+#pragma unroll
+          for (int t = 0; t < kPerLane; t++) {
+            if (wn[t]) {
+              parents[tg[t]] = own[t];
+              dsum += vdeg<G>(a.off, a.deg, tg[t]);
+            }
+            total += __popcll(__ballot(wn[t]));
           }
           if (total) {
             unsigned long long base = 0;
@@ -374,6 +545,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
             }
           }
           wave_lds_sync();  // the window is rewritten by the next chunk
+        }
+        if (marks) {
+          grid_barrier(a.bar, (unsigned)nb);
+          dsum += pack_marks<G>(a, nbm, qn, &nx->size, nwords, wave, nwaves, lane);
         }
         dsum = wave_sum(dsum);
         if (lane == 0 && dsum) atomicAdd(&nx->deg, dsum);
@@ -510,7 +685,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
               uint32_t t = 0;
               if (kk < e) {
                 t = a.ids[kk];
-                if ((!G || t != kNone) && claim(a.visited64, fbm, nbm, t)) {
+                if ((!G || t != kNone) && claim(a.visited64, fbm, nbm, t, filt)) {
                   win_ = true;
                   parents[t] = u;
                 }
@@ -634,6 +809,7 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   a.q1 = g->buf("bfs_qn", n * 4 + 64).as<uint32_t>();
   a.bm = g->buf("bfs_bms", 3 * a.nw64 * 8).as<uint64_t>();
   a.visited64 = g->buf("bfs_visited", a.nw64 * 8).as<uint64_t>();
+  a.mark = g->buf("bfs_mark", (n + 15) / 16 * 16 + 64).as<uint8_t>();
   a.scan = g->buf("bfs_scan", (n + 1) * 8 + 64).as<uint64_t>();
   a.cstart = g->buf("bfs_cstart", (v.nslots / kChunk + 2) * 4 + 64).as<uint32_t>();
   a.ctr = g->buf("bfs_coop_ctr", 3 * sizeof(Ctr) + 64).as<Ctr>();
