@@ -58,6 +58,26 @@ def bfs_bytes(n, reached, edges_reached):
     return 8 * reached + 4 * edges_reached + 4 * n + 4 * reached
 
 
+def levels_from_parents(par, src):
+    """BFS levels of one parent tree (parents[src] = src, 0xFFFFFFFF for
+    unreached vertices; -1 there) by pointer jumping: depth[v] accumulates
+    the hops to the root while anc[v] doubles towards it."""
+    n = len(par)
+    idx = np.arange(n, dtype=np.int64)
+    reached = par != 0xFFFFFFFF
+    anc = np.where(reached, par.astype(np.int64), idx)
+    depth = (anc != idx).astype(np.int64)
+    while True:
+        nxt = anc[anc]
+        if np.array_equal(nxt, anc):
+            break
+        depth += depth[anc]
+        anc = nxt
+    depth[~reached] = -1
+    assert anc[src] == src
+    return depth
+
+
 def sync():
     """Every libbyo call returns after its device work has completed (the
     library synchronises its stream), so the timed region needs no extra
@@ -364,6 +384,17 @@ def main():
 
     gteps, wall, nlaunch, launch_ms, bytes_total, stats = timed_bfs(policy, args.steps)
     thr_gteps, _, thr_nl, thr_ms, _, thr_stats = timed_bfs(B.DIR_THRESHOLD, args.steps)
+
+    def level_arrays(pol):
+        """Per-source BFS level arrays of one 8-source launch (parents read
+        back, levels derived on the host; outside the timed steps)."""
+        G.set_direction_policy(pol)
+        par, _ = G.bfs_multi(srcs, want_parents=True)
+        G.set_direction_policy(B.DIR_THRESHOLD)
+        return [levels_from_parents(par[j], srcs[j]) for j in range(len(srcs))]
+
+    levels_identical = all(np.array_equal(x, y) for x, y in zip(level_arrays(policy),
+                                                                level_arrays(B.DIR_THRESHOLD)))
     # the same graph with 64 sources in one shared traversal (the batched
     # job's width), rule against learned selector, runs interleaved
     srcs64 = pick_sources(deg, 64, seed=2)
@@ -417,8 +448,7 @@ def main():
                                  "launch_ms_learned": round(avg_ms, 4),
                                  "launch_ms_threshold": round(thr_ms / max(thr_nl, 1), 4),
                                  "threshold_dirs": thr_stats[0]["dir"][:max(s["levels"] for s in thr_stats)],
-                                 "levels_identical": [(a["levels"], a["reached"], a["edges_reached"]) for a in stats]
-                                 == [(a["levels"], a["reached"], a["edges_reached"]) for a in thr_stats]},
+                                 "levels_identical": levels_identical},
         "learned_vs_threshold_64_sources": {
             "sources": 64, "rule_ms": round(best64["rule"], 4), "learned_ms": round(best64["learned"], 4),
             "speedup": round(best64["rule"] / best64["learned"], 4),
@@ -438,6 +468,9 @@ def main():
         p32, l1_32, ms32 = G.pagerank(0.85, 20, 0.0, B.PR_FP32)  # builds the relabelled copy if knob on
         ex["pr_relabel"] = relabel
         ex["pr_relabel_build_ms"] = round(G.profile_get("relabel")[1], 3)
+        xcd = G.get_tuning("pr_xcd")
+        ex["pr_xcd"] = xcd
+        ex["pr_xcd_slices_build_ms"] = round(G.profile_get("pr_xcd_slices")[1], 3)
         G.profile(False)
         p32, l1_32, ms32 = G.pagerank(0.85, 20, 0.0, B.PR_FP32)
         p64, l1_64, ms64 = G.pagerank(0.85, 20, 0.0, B.PR_FP64)
@@ -455,6 +488,12 @@ def main():
             ex["pr_%s_it_s_relabel_%d" % (key, 1 - relabel)] = round(
                 20 / (G.pagerank(0.85, 20, 0.0, prec, want_p=False)[2] * 1e-3), 1)
         G.set_tuning("pr_relabel", relabel)
+        G.set_tuning("pr_xcd", 1 - xcd)  # with and without the per-XCD source slices
+        for prec, key in ((B.PR_FP32, "fp32"), (B.PR_FP64, "fp64")):
+            G.pagerank(0.85, 20, 0.0, prec, want_p=False)
+            ex["pr_%s_it_s_xcd_%d" % (key, 1 - xcd)] = round(
+                20 / (G.pagerank(0.85, 20, 0.0, prec, want_p=False)[2] * 1e-3), 1)
+        G.set_tuning("pr_xcd", xcd)
         G.cc()
         lab, cc_ms = G.cc()
         ex["cc_ms"] = round(cc_ms, 3)

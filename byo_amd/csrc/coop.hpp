@@ -152,9 +152,20 @@ struct MlpLds {
 
 // Stages the selector's parameters in the block's LDS (every thread calls
 // it once, before the first decide); a no-op unless the learned policy runs.
+// The 32x32 hidden-layer matrix is stored transposed in LDS (entry (j, i) at
+// 224 + i*32 + j), so the 32 lanes computing units j read consecutive words
+// for each input i (no bank conflicts); the other parameters keep their
+// layout.
 __device__ inline void mlp_stage(const DirCtl &a, MlpLds &s) {
   if (a.policy != BYO_DIR_MLP || !a.sel) return;
-  for (int i = threadIdx.x; i < BYO_MLP_NPARAM + 12; i += blockDim.x) s.w[i] = a.sel[i];
+  for (int i = threadIdx.x; i < BYO_MLP_NPARAM + 12; i += blockDim.x) {
+    int dst = i;
+    if (i >= 224 && i < 224 + 1024) {
+      const int r = i - 224;  // (j, i') = (r / 32, r % 32)
+      dst = 224 + (r & 31) * 32 + (r >> 5);
+    }
+    s.w[dst] = a.sel[i];
+  }
   __syncthreads();
 }
 
@@ -208,7 +219,7 @@ __device__ inline int decide(const DirCtl &a, bool is_dense, uint64_t fs, uint64
     h1 = h1 > 0.f ? h1 : 0.f;
     float h2 = p[1248 + j];
 #pragma unroll 8
-    for (int i = 0; i < 32; i++) h2 += p[224 + j * 32 + i] * __shfl(h1, i, 64);
+    for (int i = 0; i < 32; i++) h2 += p[224 + i * 32 + j] * __shfl(h1, i, 64);
     h2 = h2 > 0.f ? h2 : 0.f;
     float z[3];
 #pragma unroll
