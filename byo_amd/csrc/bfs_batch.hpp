@@ -170,6 +170,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
   __shared__ WinLds win;
   __shared__ uint64_t res[kWaves][64];
   __shared__ unsigned long long tl[2][64];
+  __shared__ unsigned long long lvl[3][kWaves];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const uint64_t nb = gridDim.x;
@@ -278,7 +279,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
         for (uint64_t i = global_tid(); i < nvec; i += grid_threads()) c4[i] = make_uint4(0, 0, 0, 0);
       }
     }
-    unsigned long long dsum = 0;
+    unsigned long long dsum = 0, addsize = 0;
     M won = 0;
 
     if (dir == 0) {
@@ -654,7 +655,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
         if (acc) dsum += vdeg<G>(a.off, a.deg, vl);
         won |= acc;
       });
-      if (lane == 0 && cnt) atomicAdd(&nx->size, cnt);
+      addsize = cnt;  // wave-uniform (a sum of ballot counts)
       is_dense = true;
       prev_have_q = have_q;
       have_q = false;
@@ -698,9 +699,25 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
     }
     dsum = wave_sum(dsum);
     won = wave_or<M>(won);
+    // the level's counters, summed per workgroup first: one atomic per block
+    // and counter instead of one per wave (thousands of same-address atomics
+    // at the end of a level queue in one L2 channel)
     if (lane == 0) {
-      if (dsum) atomicAdd(&nx->deg, dsum);
-      if (won) atomicOr(&nx->active, (unsigned long long)won);
+      lvl[0][wv] = addsize;
+      lvl[1][wv] = dsum;
+      lvl[2][wv] = (unsigned long long)won;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long sz = 0, dg = 0, ac = 0;
+      for (int w = 0; w < kWaves; w++) {
+        sz += lvl[0][w];
+        dg += lvl[1][w];
+        ac |= lvl[2][w];
+      }
+      if (sz) atomicAdd(&nx->size, sz);
+      if (dg) atomicAdd(&nx->deg, dg);
+      if (ac) atomicOr(&nx->active, ac);
     }
     grid_barrier(a.bar, (unsigned)nb);
     prev_fs = fs;

@@ -258,6 +258,30 @@ struct PushLds {
   int64_t base[kWaves][kChunk];
 };
 
+// Adds the wave-uniform totals x to *px and y to *py for the whole block,
+// one atomic per counter and block (thousands of same-address atomics at
+// the end of a level, one per wave, queue in one L2 channel).  Every thread
+// of the block calls it; px may be null.
+__device__ __forceinline__ void block_add2(unsigned long long *px, unsigned long long x, unsigned long long *py,
+                                           unsigned long long y, int lane) {
+  __shared__ unsigned long long red[2][kWaves];
+  if (lane == 0) {
+    red[0][threadIdx.x >> 6] = x;
+    red[1][threadIdx.x >> 6] = y;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long sx = 0, sy = 0;
+    for (int w = 0; w < kWaves; w++) {
+      sx += red[0][w];
+      sy += red[1][w];
+    }
+    if (px && sx) atomicAdd(px, sx);
+    if (sy) atomicAdd(py, sy);
+  }
+  __syncthreads();  // red is rewritten by the next call
+}
+
 // G: the adjacency lists may hold empty slots (kNone), skipped everywhere.
 template <bool G>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) k_bfs_coop(CoopArgs a) {
@@ -399,7 +423,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
             }
           }
           dsum = wave_sum(dsum);
-          if (lane == 0 && dsum) atomicAdd(&nx->deg, dsum);
+          block_add2(nullptr, 0, &nx->deg, dsum, lane);
           grid_barrier(a.bar, (unsigned)nb);
           is_dense = false;
           have_q = true;
@@ -551,7 +575,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
           dsum += pack_marks<G>(a, nbm, qn, &nx->size, nwords, wave, nwaves, lane);
         }
         dsum = wave_sum(dsum);
-        if (lane == 0 && dsum) atomicAdd(&nx->deg, dsum);
+        block_add2(nullptr, 0, &nx->deg, dsum, lane);
         grid_barrier(a.bar, (unsigned)nb);
         is_dense = false;
         have_q = true;
@@ -663,10 +687,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
           if ((fmask >> lane) & 1) dsum += vdeg<G>(a.off, a.deg, vl);
         });
         dsum = wave_sum(dsum);
-        if (lane == 0 && cnt) {
-          atomicAdd(&nx->size, cnt);
-          atomicAdd(&nx->deg, dsum);
-        }
+        block_add2(&nx->size, cnt, &nx->deg, dsum, lane);
         grid_barrier(a.bar, (unsigned)nb);
         is_dense = true;
         have_q = false;

@@ -194,6 +194,10 @@ __device__ __forceinline__ uint32_t lower_rank(const uint64_t *a, uint32_t len, 
 // its time in barriers and dependent global round trips: a batch touches
 // hundreds of thousands of leaves, each holding a few hundred keys.)
 constexpr int kUpdWaves = 4;  // waves per 256-thread workgroup
+// Workgroups of a per-leaf pass: enough to fill every CU at the passes'
+// LDS footprint (16-32 KB per workgroup), few enough that each wave takes
+// many leaves and the block tallies stay few.
+constexpr uint64_t kUpdGrid = 4096;
 
 // x among the len sorted keys of an LDS array.
 __device__ __forceinline__ bool lds_has(const uint64_t *a, uint32_t len, uint64_t x) {
@@ -205,6 +209,91 @@ __device__ __forceinline__ bool lds_has(const uint64_t *a, uint32_t len, uint64_
   return lo < len && a[lo] == x;
 }
 
+// A wave's segments are j, j + stride, ...; the bounds of the next one are
+// loaded while the current one is worked on, and its leaf right after the
+// current leaf's slots have been requested, so the dependent chain
+// starts[] -> leaf_of[] -> slots costs one round trip per segment, not three.
+__device__ __forceinline__ void seg_bounds(const uint32_t *starts, uint64_t nseg, uint64_t b, uint64_t j,
+                                           uint64_t &s, uint64_t &e) {
+  if (j < nseg) {
+    s = starts[j];
+    e = (j + 1 < nseg) ? starts[j + 1] : b;
+  }
+}
+
+// All kLeaf slots of the leaf at slot `leaf` into registers (slot t*64 +
+// lane in h[t], l[t]): the loads do not wait for the leaf's count, which
+// comes in parallel.
+__device__ __forceinline__ void load_leaf(const Leaves &lv, uint64_t leaf, uint32_t (&h)[kLeaf / 64],
+                                          uint32_t (&l)[kLeaf / 64], int lane) {
+#pragma unroll
+  for (int t = 0; t < (int)(kLeaf / 64); t++) {
+    h[t] = lv.hi[leaf + t * 64 + lane];
+    l[t] = lv.lo[leaf + t * 64 + lane];
+  }
+}
+
+// The first c keys of a leaf loaded by load_leaf into the wave's LDS array.
+__device__ __forceinline__ void put_leaf(const uint32_t (&h)[kLeaf / 64], const uint32_t (&l)[kLeaf / 64], uint32_t c,
+                                         uint64_t *dst, int lane) {
+#pragma unroll
+  for (int t = 0; t < (int)(kLeaf / 64); t++) {
+    const uint32_t i = t * 64 + lane;
+    if (i < c) dst[i] = ((uint64_t)h[t] << 32) | l[t];
+  }
+}
+
+// Loads up to kLeaf consecutive batch keys [s0, min(s0 + kLeaf, e)) into
+// registers, lane-strided (key s0 + t*64 + lane in k[t]), all loads in flight.
+__device__ __forceinline__ void load_keys(const uint64_t *keys, uint64_t s0, uint64_t e, uint64_t (&k)[kLeaf / 64],
+                                          int lane) {
+#pragma unroll
+  for (int t = 0; t < (int)(kLeaf / 64); t++) {
+    const uint64_t i = s0 + (uint64_t)t * 64 + lane;
+    k[t] = i < e ? keys[i] : 0;
+  }
+}
+
+// deg[src] += sign * (number of keys of src) over the n sorted keys of an
+// LDS array: the first key of each source's run adds the run's length, so
+// a hub with thousands of keys in a batch costs one atomic per leaf it
+// touches, not one per key (per-key atomics on one address queue in L2).
+__device__ __forceinline__ void add_degree_runs(const uint64_t *a, uint32_t n, uint32_t *deg, int sign, int lane) {
+  for (uint32_t i = lane; i < n; i += 64) {
+    const uint32_t src = (uint32_t)(a[i] >> 32);
+    if (i > 0 && (uint32_t)(a[i - 1] >> 32) == src) continue;
+    const uint32_t end = src == 0xFFFFFFFFu ? n : lower_rank(a, n, ((uint64_t)src + 1) << 32);
+    const uint32_t len = end - i;
+    if (sign > 0) atomicAdd(&deg[src], len); else atomicSub(&deg[src], len);
+  }
+}
+
+// Adds the block's keys changed and slot bytes touched to the update's
+// control block: one pair of atomics per workgroup.  (One pair per wave put
+// 65,536 atomics on the same two words for a 1e7-key batch; they queue in
+// one L2 channel and took about a third of the pass.)  Every thread of the
+// block calls it.
+__device__ __forceinline__ void block_tally(UpdCtl *ctl, unsigned long long changed, unsigned long long touched,
+                                            int lane) {
+  __shared__ unsigned long long red[2][kUpdWaves];
+  if (lane == 0) {
+    red[0][threadIdx.x >> 6] = changed;
+    red[1][threadIdx.x >> 6] = touched;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long c = 0, t = 0;
+    for (int w = 0; w < kUpdWaves; w++) {
+      c += red[0][w];
+      t += red[1][w];
+    }
+    if (c) {
+      atomicAdd(&ctl->changed, c);
+      atomicAdd(&ctl->pad[4], t);
+    }
+  }
+}
+
 // Pass 1 of an insert: count the segment keys absent from their leaf.
 __global__ void __launch_bounds__(256) k_ins_count(const uint64_t *keys, UpdDev d, const uint32_t *starts,
                                                    const uint32_t *leaf_of, Leaves lv,
@@ -213,22 +302,38 @@ __global__ void __launch_bounds__(256) k_ins_count(const uint64_t *keys, UpdDev 
   if (d.halt && *d.halt) return;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   uint64_t *old = stage[wv];
-  const uint64_t b = *d.bu, nseg = *d.nseg;
-  for (uint64_t j = global_tid() >> 6; j < nseg; j += grid_threads() >> 6) {
-    const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
-    const uint32_t l = leaf_of[s];
-    const uint64_t leaf = (uint64_t)l * kLeaf;
+  const uint64_t b = *d.bu, nseg = *d.nseg, stride = grid_threads() >> 6;
+  uint64_t j = global_tid() >> 6, s = 0, e = 0;
+  uint32_t l = 0;
+  seg_bounds(starts, nseg, b, j, s, e);
+  if (j < nseg) l = leaf_of[s];
+  for (; j < nseg; j += stride) {
+    uint64_t sn = 0, en = 0;
+    seg_bounds(starts, nseg, b, j + stride, sn, en);
     const uint32_t c = cnt[l];
-    for (uint32_t i = lane; i < c; i += 64) old[i] = lv.get(leaf + i);
+    uint32_t h[kLeaf / 64], lo[kLeaf / 64];
+    load_leaf(lv, (uint64_t)l * kLeaf, h, lo, lane);
+    uint64_t k[kLeaf / 64];
+    load_keys(keys, s, e, k, lane);
+    const uint32_t ln = (j + stride < nseg) ? leaf_of[sn] : 0;
+    put_leaf(h, lo, c, old, lane);
     wave_lds_sync();
     uint32_t mine = 0;
-    for (uint64_t i = s + lane; i < e; i += 64) mine += !lds_has(old, c, keys[i]);
+    for (uint64_t s0 = s; s0 < e; s0 += kLeaf) {
+      if (s0 != s) load_keys(keys, s0, e, k, lane);
+#pragma unroll
+      for (int t = 0; t < (int)(kLeaf / 64); t++)
+        if (s0 + (uint64_t)t * 64 + lane < e) mine += !lds_has(old, c, k[t]);
+    }
     mine = wave_sum(mine);
     if (lane == 0) {
       newc[j] = mine;
       if ((uint64_t)c + mine > kLeaf) atomicOr(&ctl->overflow, 1ull);
     }
     wave_lds_sync();  // the stage is rewritten by the next segment
+    s = sn;
+    e = en;
+    l = ln;
   }
 }
 
@@ -237,69 +342,84 @@ __global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, UpdDev 
                                                    const uint32_t *leaf_of, Leaves lv, uint32_t *cnt,
                                                    uint32_t *deg, UpdCtl *ctl, const uint32_t *cand,
                                                    uint64_t nleaves) {
-  __shared__ uint64_t stage_old[kUpdWaves][kLeaf];
-  __shared__ uint64_t stage_add[kUpdWaves][kLeaf];
+  // one leaf-sized LDS array per wave: the leaf's c keys at its start, the
+  // absent batch keys after them (c + their number <= kLeaf: no leaf
+  // overflows here), so 8 workgroups fit a CU instead of 5
+  __shared__ uint64_t stage[kUpdWaves][kLeaf];
   const int lane = lane_id(), wv = threadIdx.x >> 6;
-  uint64_t *old_keys = stage_old[wv], *add = stage_add[wv];
+  uint64_t *old_keys = stage[wv];
   // stop on a halted sequence or a global rebalance; the leaves of rebuilt
   // windows (candidate windows, when a leaf overflowed) are skipped
   if ((d.halt && *d.halt) || ctl->pad[2] || ctl->bad) return;
   const bool windows = ctl->overflow != 0;
-  const uint64_t b = *d.bu, nseg = *d.nseg;
+  const uint64_t b = *d.bu, nseg = *d.nseg, stride = grid_threads() >> 6;
   unsigned long long changed = 0, touched = 0;
-  for (uint64_t j = global_tid() >> 6; j < nseg; j += grid_threads() >> 6) {
-    const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
-    const uint32_t l = leaf_of[s];
+  uint64_t j = global_tid() >> 6, s = 0, e = 0;
+  uint32_t l = 0;
+  seg_bounds(starts, nseg, b, j, s, e);
+  if (j < nseg) l = leaf_of[s];
+  for (; j < nseg; j += stride) {
+    uint64_t sn = 0, en = 0;
+    seg_bounds(starts, nseg, b, j + stride, sn, en);
+    bool skip = false;
     if (windows) {
+      // levels 1.. of the aligned windows holding l, one per lane
+      const int k = lane + 1;
       bool inw = false;
-      for (int k = 1; k < 40 && (1ull << (k - 1)) < nleaves; k++)
-        if (cand[l & ~((1ull << k) - 1)] >= (uint32_t)k) {
-          inw = true;
-          break;
-        }
-      if (inw) continue;
+      if (k < 40 && (1ull << (k - 1)) < nleaves) inw = cand[l & ~((1ull << k) - 1)] >= (uint32_t)k;
+      skip = __ballot(inw) != 0;
     }
     const uint64_t leaf = (uint64_t)l * kLeaf;
-    const uint32_t c = cnt[l];
-    for (uint32_t i = lane; i < c; i += 64) old_keys[i] = lv.get(leaf + i);
-    wave_lds_sync();
-    // stable compaction of the absent keys, 64 at a time (a segment holds at
-    // most kLeaf keys: the unique absent ones fit the leaf, the rest are in it)
-    uint32_t na = 0;
-    for (uint64_t base = s; base < e; base += 64) {
-      const uint64_t i = base + lane;
-      uint64_t k = 0;
-      bool keep = false;
-      if (i < e) {
-        k = keys[i];
-        keep = !lds_has(old_keys, c, k);
+    uint32_t c = 0, na = 0;
+    uint32_t h[kLeaf / 64], lo[kLeaf / 64];
+    uint64_t k[kLeaf / 64];
+    if (!skip) {
+      c = cnt[l];
+      load_leaf(lv, leaf, h, lo, lane);
+      load_keys(keys, s, e, k, lane);
+    }
+    const uint32_t ln = (j + stride < nseg) ? leaf_of[sn] : 0;
+    if (!skip) {
+      put_leaf(h, lo, c, old_keys, lane);
+      uint64_t *add = old_keys + c;
+      wave_lds_sync();
+      // stable compaction of the absent keys, 64 at a time (a segment holds
+      // at most kLeaf keys: the unique absent ones fit the leaf, the rest
+      // are in it; a longer one is taken kLeaf keys at a time)
+      for (uint64_t s0 = s; s0 < e; s0 += kLeaf) {
+        if (s0 != s) load_keys(keys, s0, e, k, lane);
+#pragma unroll
+        for (int t = 0; t < (int)(kLeaf / 64); t++) {
+          const uint64_t i = s0 + (uint64_t)t * 64 + lane;
+          const bool keep = i < e && !lds_has(old_keys, c, k[t]);
+          const uint64_t mask = __ballot(keep);
+          if (keep) add[na + mbcnt(mask)] = k[t];
+          na += (uint32_t)__popcll(mask);
+        }
       }
-      const uint64_t mask = __ballot(keep);
-      if (keep) add[na + mbcnt(mask)] = k;
-      na += (uint32_t)__popcll(mask);
+      wave_lds_sync();
+      // merged position of each key = own index + rank in the other list
+      for (uint32_t i = lane; i < c; i += 64) {
+        const uint64_t x = old_keys[i];
+        lv.put(leaf + i + lower_rank(add, na, x), x);
+      }
+      for (uint32_t i = lane; i < na; i += 64) {
+        const uint64_t x = add[i];
+        lv.put(leaf + i + lower_rank(old_keys, c, x), x);
+      }
+      add_degree_runs(add, na, deg, 1, lane);
+      if (lane == 0) cnt[l] = c + na;
+      if (na) {
+        changed += na;
+        touched += 8ull * (c + na);
+      }
+      wave_lds_sync();  // the stages are rewritten by the next segment
     }
-    wave_lds_sync();
-    // merged position of each key = own index + rank in the other list
-    for (uint32_t i = lane; i < c; i += 64) {
-      const uint64_t k = old_keys[i];
-      lv.put(leaf + i + lower_rank(add, na, k), k);
-    }
-    for (uint32_t i = lane; i < na; i += 64) {
-      const uint64_t k = add[i];
-      lv.put(leaf + i + lower_rank(old_keys, c, k), k);
-      atomicAdd(&deg[k >> 32], 1u);
-    }
-    if (lane == 0) cnt[l] = c + na;
-    if (na) {
-      changed += na;
-      touched += 8ull * (c + na);
-    }
-    wave_lds_sync();  // the stages are rewritten by the next segment
+    s = sn;
+    e = en;
+    l = ln;
   }
-  if (lane == 0 && changed) {
-    atomicAdd(&ctl->changed, changed);
-    atomicAdd(&ctl->pad[4], touched);
-  }
+  block_tally(ctl, changed, touched, lane);
 }
 
 // Removal: each touched leaf keeps the keys absent from its segment.
@@ -307,32 +427,66 @@ __global__ void __launch_bounds__(256) k_remove(const uint64_t *keys, UpdDev d, 
                                                 const uint32_t *leaf_of, Leaves lv, uint32_t *cnt,
                                                 uint32_t *deg, UpdCtl *ctl) {
   __shared__ uint64_t stage[kUpdWaves][kLeaf];
+  __shared__ uint64_t stage_rm[kUpdWaves][kLeaf];
   const int lane = lane_id(), wv = threadIdx.x >> 6;
-  uint64_t *kept = stage[wv];
+  uint64_t *kept = stage[wv], *rm = stage_rm[wv];
   if ((d.halt && *d.halt) || ctl->bad) return;
-  const uint64_t b = *d.bu, nseg = *d.nseg;
+  const uint64_t b = *d.bu, nseg = *d.nseg, stride = grid_threads() >> 6;
   unsigned long long changed = 0, touched = 0;
-  for (uint64_t j = global_tid() >> 6; j < nseg; j += grid_threads() >> 6) {
-    const uint64_t s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : b;
-    const uint32_t l = leaf_of[s];
+  uint64_t j = global_tid() >> 6, s = 0, e = 0;
+  uint32_t l = 0;
+  seg_bounds(starts, nseg, b, j, s, e);
+  if (j < nseg) l = leaf_of[s];
+  for (; j < nseg; j += stride) {
+    uint64_t sn = 0, en = 0;
+    seg_bounds(starts, nseg, b, j + stride, sn, en);
     const uint64_t leaf = (uint64_t)l * kLeaf;
     const uint32_t c = cnt[l];
+    // the leaf's slots in registers and the segment's keys in LDS (when they
+    // fit; a longer segment is searched in global memory), all loads issued
+    // before any is used
+    constexpr int kSteps = kLeaf / 64;
+    uint32_t h[kSteps], lo[kSteps];
+    load_leaf(lv, leaf, h, lo, lane);
+    const uint64_t ns = e - s;
+    const bool in_lds = ns <= kLeaf;
+    uint64_t k[kLeaf / 64];
+    if (in_lds) load_keys(keys, s, e, k, lane);
+    const uint32_t ln = (j + stride < nseg) ? leaf_of[sn] : 0;
+    if (in_lds) {
+#pragma unroll
+      for (int t = 0; t < kSteps; t++)
+        if ((uint64_t)t * 64 + lane < ns) rm[t * 64 + lane] = k[t];
+    }
+    wave_lds_sync();
     uint32_t run = 0;
-    for (uint32_t base = 0; base < c; base += 64) {
-      const uint32_t i = base + lane;
-      uint64_t k = 0;
-      bool keep = false, gone = false;
+    bool gone[kSteps];
+#pragma unroll
+    for (int t = 0; t < kSteps; t++) {
+      const uint32_t i = t * 64 + lane;
+      const uint64_t x = ((uint64_t)h[t] << 32) | lo[t];
+      bool keep = false;
+      gone[t] = false;
       if (i < c) {
-        k = lv.get(leaf + i);
-        gone = in_sorted(keys + s, e - s, k);
-        keep = !gone;
+        gone[t] = in_lds ? lds_has(rm, (uint32_t)ns, x) : in_sorted(keys + s, ns, x);
+        keep = !gone[t];
       }
-      if (gone) atomicSub(&deg[k >> 32], 1u);
       const uint64_t mask = __ballot(keep);
-      if (keep) kept[run + mbcnt(mask)] = k;
+      if (keep) kept[run + mbcnt(mask)] = x;
       run += (uint32_t)__popcll(mask);
     }
     wave_lds_sync();
+    // the removed keys, compacted in order over the segment's LDS copy (read
+    // by nobody now), lower the degrees one atomic per source run
+    uint32_t ng = 0;
+#pragma unroll
+    for (int t = 0; t < kSteps; t++) {
+      const uint64_t mask = __ballot(gone[t]);
+      if (gone[t]) rm[ng + mbcnt(mask)] = ((uint64_t)h[t] << 32) | lo[t];
+      ng += (uint32_t)__popcll(mask);
+    }
+    wave_lds_sync();
+    add_degree_runs(rm, ng, deg, -1, lane);
     if (run < c) {
       for (uint32_t i = lane; i < c; i += 64) lv.put(leaf + i, i < run ? kept[i] : kEmpty);
       if (lane == 0) cnt[l] = run;
@@ -340,11 +494,11 @@ __global__ void __launch_bounds__(256) k_remove(const uint64_t *keys, UpdDev d, 
       touched += 8ull * c;
     }
     wave_lds_sync();  // the stage is rewritten by the next segment
+    s = sn;
+    e = en;
+    l = ln;
   }
-  if (lane == 0 && changed) {
-    atomicAdd(&ctl->changed, changed);
-    atomicAdd(&ctl->pad[4], touched);
-  }
+  block_tally(ctl, changed, touched, lane);
 }
 
 // ------------------------------------------- windowed rebalance (insert)
@@ -1075,7 +1229,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
   // counts stay on the device; the first read-back also reports a bad id
   // (no kernel writes the container when ctl->bad is set)
   const UpdDev d{&ctl->pad[0], u.pos + b, nullptr};
-  const unsigned grid = (unsigned)std::min<uint64_t>(b, 16384);
+  const unsigned grid = (unsigned)std::min<uint64_t>(b, kUpdGrid);
   if (insert) {
     k_ins_count<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, leaves(c), c->cnt.as<uint32_t>(),
                                      u.newc, ctl);
@@ -1172,7 +1326,7 @@ static uint32_t insert_graph(byo_graph *g, const uint64_t *keys, const uint64_t 
       UpdCtl *ctl = ctls + i;
       const uint64_t *batch = sort_and_segment(g, keys + offs[i], b, ctl, u);
       const UpdDev d{&ctl->pad[0], u.pos + b, halt};
-      const unsigned grid = (unsigned)std::min<uint64_t>(b, 16384);
+      const unsigned grid = (unsigned)std::min<uint64_t>(b, kUpdGrid);
       k_ins_count<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, leaves(c), c->cnt.as<uint32_t>(),
                                        u.newc, ctl);
       const WinArgs w = win_args(g, batch, d, u, ctl, cap);
@@ -1450,7 +1604,7 @@ extern "C" int byo_insert_batch_dev_noalloc(byo_graph_t g, const uint64_t *keys_
     k_small_batch<<<1, kSmallThreads, 0, s>>>(keys_dev, b, g->n, c->sep.as<uint64_t>(), c->nleaves, uniq, leaf,
                                               starts, nseg, ctl);
     const UpdDev d{&ctl->pad[0], nseg, nullptr};
-    const unsigned grid = (unsigned)std::min<uint64_t>(b, 16384);
+    const unsigned grid = (unsigned)std::min<uint64_t>(b, kUpdGrid);
     k_ins_count<<<grid, 256, 0, s>>>(uniq, d, starts, leaf, leaves(c), c->cnt.as<uint32_t>(), newc,
                                      ctl);
     BYO_LAUNCH_CHECK();
