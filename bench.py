@@ -360,22 +360,28 @@ def main():
         if mode:
             G.set_tuning("bfs_engine", mode)
         try:
+            stats = None
             for _ in range(args.warmup):
-                G.bfs_multi(srcs, want_parents=False)
+                _, stats = G.bfs_multi(srcs, want_parents=False)
             G.profile(True)
             G.profile_reset()
             sync()
             t0 = time.perf_counter()
-            edges = 0
-            nbytes = 0
             for _ in range(steps):
-                _, stats = G.bfs_multi(srcs, want_parents=False)
-                for st in stats:
-                    edges += st["edges_reached"] // 2
-                    nbytes += bfs_bytes(n, st["reached"], st["edges_reached"])
+                G.bfs_multi(srcs, want_parents=False, want_stats=False)
             sync()
             wall = time.perf_counter() - t0
             nl, ms = G.profile_get("bfs_launch")
+            # reached vertices and edges are fixed by the sources: the timed
+            # launches read no statistics back; one launch after them counts
+            # them (and matches the warmup's)
+            _, after = G.bfs_multi(srcs, want_parents=False)
+            if stats is not None:
+                assert [(a["reached"], a["edges_reached"]) for a in after] == \
+                    [(a["reached"], a["edges_reached"]) for a in stats], "BFS results changed between launches"
+            stats = after
+            edges = steps * sum(st["edges_reached"] // 2 for st in stats)
+            nbytes = steps * sum(bfs_bytes(n, st["reached"], st["edges_reached"]) for st in stats)
         finally:
             if mode:
                 G.set_tuning("bfs_engine", 0)

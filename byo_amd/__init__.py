@@ -346,13 +346,16 @@ class Graph:
         check(lib().byo_bfs(self._h, src, _ptr(par), C.byref(st)))
         return (par[:n] if want_parents else None), st.as_dict()
 
-    def bfs_multi(self, srcs, want_parents=True):
+    def bfs_multi(self, srcs, want_parents=True, want_stats=True):
+        """k BFS; returns (parents (k, n) or None, per-source stats dicts, or
+        None with want_stats=False: then no statistics are read back)."""
         srcs = np.ascontiguousarray(srcs, np.uint32)
         n, k = self.N(), len(srcs)
         par = np.empty(max(n * k, 1), np.uint32) if want_parents else None
-        st = (BfsStats * max(k, 1))()
-        check(lib().byo_bfs_multi(self._h, _ptr(srcs), k, _ptr(par), C.cast(st, C.c_void_p)))
-        return (par[:n * k].reshape(k, n) if want_parents else None), [st[i].as_dict() for i in range(k)]
+        st = (BfsStats * max(k, 1))() if want_stats else None
+        check(lib().byo_bfs_multi(self._h, _ptr(srcs), k, _ptr(par), C.cast(st, C.c_void_p) if want_stats else None))
+        return ((par[:n * k].reshape(k, n) if want_parents else None),
+                ([st[i].as_dict() for i in range(k)] if want_stats else None))
 
     def bfs_levels(self):
         n = self.N()
