@@ -28,7 +28,7 @@ TUNING = os.path.join(HERE, "tuning.json")
 
 CANDIDATES = {
     "coop_blocks_per_cu": [2, 3, 4],
-    "pieces": [4, 8, 16],
+    "pieces": [4, 8, 16, 32],
     "ppr_blocks_per_cu": [4, 8],
     "ppr_pieces": [2, 4, 8],
     "pr_relabel": [0, 1],

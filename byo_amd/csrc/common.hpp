@@ -155,7 +155,7 @@ inline const std::vector<TuneSpec> &tune_specs() {
   static const std::vector<TuneSpec> specs = {
       {"coop_blocks_per_cu", 4, 1, 4, "blocks per CU of the persistent traversal launches"},
       {"ppr_blocks_per_cu", 4, 1, 8, "blocks per CU of the persistent personalized PageRank launch"},
-      {"pieces", 8, 1, 16, "weight-balanced word pieces per wave in vertex-driven passes"},
+      {"pieces", 8, 1, 64, "weight-balanced word pieces per wave in vertex-driven passes"},
       {"ppr_pieces", 4, 1, 16, "weight-balanced word pieces per wave in the personalized PageRank sweeps"},
       {"pr_lanes", 0, 0, 32, "PageRank lanes per vertex (4, 8, 16, 32; 0: per tile from its largest degree)"},
       {"pr_grid", 4096, 256, 65536, "PageRank pull grid (blocks of 256)"},

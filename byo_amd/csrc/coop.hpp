@@ -91,7 +91,7 @@ __device__ __forceinline__ unsigned long long block_sum(unsigned long long v, un
 // per word.)
 // Pieces per wave is a tuning knob ("pieces", 8 by default, at most
 // kMaxPieces).
-constexpr uint32_t kMaxPieces = 16;
+constexpr uint32_t kMaxPieces = 64;
 struct Pieces {
   uint64_t *bound;  // per_wave * nwaves + 1 word bounds
   uint32_t per_wave;

@@ -29,7 +29,7 @@ def check_bfs(g, G, srcs):
         assert O.check_bfs_tree(g, s, par[j], O.bfs_levels(g, s)) == 0
 
 
-@pytest.mark.parametrize("key,values", [("coop_blocks_per_cu", [1, 2, 3, 4]), ("pieces", [1, 4, 16]),
+@pytest.mark.parametrize("key,values", [("coop_blocks_per_cu", [1, 2, 3, 4]), ("pieces", [1, 4, 16, 32, 64]),
                                         ("bfs_engine", [0, 1, 2])])
 def test_bfs_knobs(B, s15, key, values):
     g, G = s15
