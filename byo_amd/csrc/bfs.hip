@@ -264,13 +264,14 @@ __global__ void k_lv_jump(const uint32_t *anc, const uint32_t *dep, uint32_t *an
 
 }  // namespace
 
-// Per-vertex levels of the tree in "bfs_parents" (the last BFS, rooted at
+// Per-vertex levels of the tree of the last BFS (g->last_tree, rooted at
 // g->last_src) by pointer jumping: every vertex doubles the distance to its
 // known ancestor each pass, so ceil(log2 n) + 1 passes reach the root.
 const uint32_t *bfs_levels_dev(byo_graph *g) {
   const uint32_t src = (uint32_t)g->last_src;
   const uint64_t n = g->n;
-  uint32_t *parents = g->scratch["bfs_parents"].as<uint32_t>();
+  const uint32_t *parents = g->last_tree;
+  BYO_REQUIRE(parents, BYO_E_STATE, "no BFS tree on this handle");
   uint32_t *anc = g->buf("lv_anc", n * 4 + 64).as<uint32_t>();
   uint32_t *dep = g->buf("lv_dep", n * 4 + 64).as<uint32_t>();
   uint32_t *anc2 = g->buf("lv_anc2", n * 4 + 64).as<uint32_t>();
@@ -541,10 +542,7 @@ int byo_bfs_multi(byo_graph_t g, const uint32_t *srcs, uint32_t k, uint32_t *par
         if (parents)
           BYO_HIP(hipMemcpyAsync(parents + (uint64_t)c0 * n, all, (uint64_t)kc * n * 4, hipMemcpyDeviceToHost,
                                  g->stream));
-        if (c0 + kc == k) {
-          uint32_t *last = g->buf("bfs_parents", n * 4 + 64).as<uint32_t>();
-          BYO_HIP(hipMemcpyAsync(last, all + (uint64_t)(kc - 1) * n, n * 4, hipMemcpyDeviceToDevice, g->stream));
-        }
+        if (c0 + kc == k) g->last_tree = all + (uint64_t)(kc - 1) * n;
         BYO_HIP(hipStreamSynchronize(g->stream));
       }
       g->last_src = srcs[k - 1];
@@ -567,10 +565,8 @@ int byo_bfs_multi(byo_graph_t g, const uint32_t *srcs, uint32_t k, uint32_t *par
       if (keep_all) {
         uint32_t *all = g->scratch["bfs_parents_all"].as<uint32_t>();
         BYO_HIP(hipMemcpyAsync(parents, all, (uint64_t)k * n * 4, hipMemcpyDeviceToHost, g->stream));
-        uint32_t *last = g->buf("bfs_parents", n * 4 + 64).as<uint32_t>();
-        BYO_HIP(hipMemcpyAsync(last, all + (uint64_t)(k - 1) * n, n * 4, hipMemcpyDeviceToDevice, g->stream));
       } else if (parents) {
-        BYO_HIP(hipMemcpyAsync(parents, g->scratch["bfs_parents"].p, n * 4, hipMemcpyDeviceToHost, g->stream));
+        BYO_HIP(hipMemcpyAsync(parents, g->last_tree, n * 4, hipMemcpyDeviceToHost, g->stream));
       }
       g->last_src = srcs[k - 1];
       return;
@@ -592,6 +588,7 @@ int byo_bfs_multi(byo_graph_t g, const uint32_t *srcs, uint32_t k, uint32_t *par
     }
     BYO_HIP(hipStreamSynchronize(g->stream));
     g->last_src = srcs[k - 1];
+    g->last_tree = r.parents;
   });
 }
 

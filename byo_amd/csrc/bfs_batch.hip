@@ -43,19 +43,17 @@ void prepare(byo_graph *g, const AdjView &v, const uint32_t *srcs, uint32_t k, u
   a.tot = reinterpret_cast<unsigned long long *>(out + kTotOff);
   a.dc = dir_ctl(g, maxdeg, s, true);
   a.pieces = make_pieces(g, nblocks, v.deg ? kPmaPieces : kTraversalPieces);
-  a.bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
+  a.bar = coop_bar(g);
   a.bsum = g->buf("bfs_batch_bsum", (size_t)nblocks * 8 + 64).as<unsigned long long>();
   a.seen = g->buf("bfs_batch_seen", a.nm * mask_bytes + 64).p;
   a.fm = g->buf("bfs_batch_fm", 3 * a.nm * mask_bytes + 64).p;
   a.ubm = g->buf("bfs_batch_ubm", 3 * (a.nm / 64) * 8 + 64).as<uint64_t>();
-  // pinned staging: sources at 0, results at 256 (one allocation, sized
-  // before any copy is queued)
-  char *hp = static_cast<char *>(pinned(g, 256 + kOutBytes));
-  std::memcpy(hp, srcs, (size_t)k * 4);
-  uint32_t *dsrc = g->buf("bfs_batch_srcs", 64 * 4 + 64).as<uint32_t>();
-  a.srcs = dsrc;
-  BYO_HIP(hipMemsetAsync(a.bar, 0, sizeof(GridBar), s));
-  BYO_HIP(hipMemcpyAsync(dsrc, hp, (size_t)k * 4, hipMemcpyHostToDevice, s));
+  // the sources travel in the kernel arguments (no copy queued before the
+  // launch); the pinned staging of the results (at 256) is sized now, before
+  // any copy is queued
+  std::memcpy(a.srcs, srcs, (size_t)k * 4);
+  (void)pinned(g, 256 + kOutBytes);
+  (void)s;
 }
 
 void finish(byo_graph *g, const BatchArgs &a, byo_bfs_stats *stats, double ms) {

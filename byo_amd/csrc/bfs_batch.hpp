@@ -74,7 +74,7 @@ struct BatchArgs {
   uint32_t *cstart;
   unsigned long long *bsum;  // per-block frontier degree sums (push)
   BCtr *ctr;                // 3 slots
-  const uint32_t *srcs;
+  uint32_t srcs[64];         // the sources, passed in the kernel arguments
   uint32_t k;
   BLog *logs;               // 64
   uint32_t *lastlev;        // k: last level with a nonempty frontier

@@ -746,6 +746,15 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
 
 }  // namespace
 
+GridBar *coop_bar(byo_graph *g) {
+  GridBar *b = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
+  if (!g->coop_bar_ready) {
+    BYO_HIP(hipMemsetAsync(b, 0, sizeof(GridBar), g->stream));
+    g->coop_bar_ready = true;
+  }
+  return b;
+}
+
 DirCtl dir_ctl(byo_graph *g, uint64_t maxdeg, hipStream_t s, bool batched) {
   DirCtl d{};
   d.n = g->n;
@@ -826,6 +835,7 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   a.nw64 = (n + 63) / 64 + 1;
   a.pstride = keep_all ? n : 0;
   a.parents = g->buf(keep_all ? "bfs_parents_all" : "bfs_parents", (keep_all ? k : 1) * n * 4 + 64).as<uint32_t>();
+  g->last_tree = a.parents + (keep_all ? (uint64_t)(k - 1) * n : 0);
   a.q0 = g->buf("bfs_q", n * 4 + 64).as<uint32_t>();
   a.q1 = g->buf("bfs_qn", n * 4 + 64).as<uint32_t>();
   a.bm = g->buf("bfs_bms", 3 * a.nw64 * 8).as<uint64_t>();
@@ -847,8 +857,7 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   const int nblocks = coop_grid(kern, kBlock, (int)g->tuning("coop_blocks_per_cu"));
   a.pieces = make_pieces(g, nblocks, pset);
   a.bsum = g->buf("bfs_coop_bsum", (size_t)nblocks * 8 + 64).as<unsigned long long>();
-  a.bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
-  BYO_HIP(hipMemsetAsync(a.bar, 0, sizeof(GridBar), s));
+  a.bar = coop_bar(g);
   BYO_HIP(hipMemcpyAsync(dsrc, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s));
   void *args[] = {&a};
   PhaseTimer t(s, true);

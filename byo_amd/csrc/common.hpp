@@ -201,6 +201,8 @@ struct byo_graph {
   bool xs_valid = false;      // PageRank per-XCD slices are current (for layout xs_layout)
   int xs_layout = 0;
   int64_t last_src = -1;      // last source of byo_bfs / byo_bfs_multi
+  const uint32_t *last_tree = nullptr;  // device parents of last_src's BFS (in a scratch buffer)
+  bool coop_bar_ready = false;  // "bfs_coop_bar" cleared once: every launch leaves it clear
 
   // CPMA state (kind 1); owned by cpma.hip.
   void *cpma = nullptr;

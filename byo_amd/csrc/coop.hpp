@@ -241,6 +241,12 @@ __device__ inline int decide(const DirCtl &a, bool is_dense, uint64_t fs, uint64
   return best == 0 ? 0 : (best == 1 ? 1 : a.third);
 }
 
+// The grid barrier of the handle's persistent launches ("bfs_coop_bar"),
+// cleared once after it is allocated: a barrier's last arrivals reset its
+// counters and only the generation word moves on, so every launch that
+// returns leaves it ready for the next (no clear queued per launch).
+GridBar *coop_bar(byo_graph *g);
+
 // Direction-decision inputs of a handle (policy, threshold, and the
 // selector weights of the single-source or the batched traversal).
 DirCtl dir_ctl(byo_graph *g, uint64_t maxdeg, hipStream_t s, bool batched);

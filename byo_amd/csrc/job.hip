@@ -192,8 +192,7 @@ int byo_ppr_multi(byo_graph_t g, const uint32_t *seeds, uint32_t k, double dampi
     const void *kern = reinterpret_cast<const void *>(k_ppr_persist);
     const int nblocks = coop_grid(kern, kCoopBlock, (int)g->tuning("ppr_blocks_per_cu"));
     q.pieces = make_pieces(g, nblocks, kPprPieces);
-    GridBar *bar = g->buf("bfs_coop_bar", sizeof(GridBar)).as<GridBar>();
-    BYO_HIP(hipMemsetAsync(bar, 0, sizeof(GridBar), g->stream));
+    GridBar *bar = coop_bar(g);
     void *args[] = {&q, &bar};
     PhaseTimer t(g->stream, true);
     BYO_HIP(hipLaunchCooperativeKernel(kern, dim3(nblocks), dim3(kCoopBlock), args, 0, g->stream));
@@ -239,8 +238,7 @@ int byo_batch_job(byo_graph_t g, const uint32_t *bfs_srcs, uint32_t k_bfs, const
       batch::finish(g, a, stats, ms);
       if (parents)
         BYO_HIP(hipMemcpyAsync(parents, a.parents, (uint64_t)k_bfs * n * 4, hipMemcpyDeviceToHost, g->stream));
-      uint32_t *last = g->buf("bfs_parents", n * 4 + 64).as<uint32_t>();
-      BYO_HIP(hipMemcpyAsync(last, a.parents + (uint64_t)(k_bfs - 1) * n, n * 4, hipMemcpyDeviceToDevice, g->stream));
+      g->last_tree = a.parents + (uint64_t)(k_bfs - 1) * n;
       g->last_src = bfs_srcs[k_bfs - 1];
     }
     if (ppr && k_ppr) ppr_readback(g, q, ppr);
