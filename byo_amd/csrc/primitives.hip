@@ -90,10 +90,48 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_final(const T *in, T *out, 
   if (write_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == blockDim.x - 1) out[n] = ex;
 }
 
+// One workgroup scans the whole array in chunks of 1024 x kScanItems
+// (thread-contiguous items, read before any is written, so in == out is
+// safe).  For arrays up to kSmallScan one launch instead of three: a launch
+// costs ~10 us of dispatch here, more than the scan of such an array.
+constexpr uint64_t kSmallScan = 1ull << 16;
+
+template <class T>
+__global__ void __launch_bounds__(1024) k_scan_small(const T *in, T *out, uint64_t n, int write_total) {
+  __shared__ T lds[1024 / 64 + 1];
+  T carry = 0;
+  for (uint64_t b0 = 0; b0 < n; b0 += 1024 * kScanItems) {
+    const uint64_t base = b0 + threadIdx.x * kScanItems;
+    T v[kScanItems];
+    T acc = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+      const uint64_t i = base + k;
+      v[k] = i < n ? in[i] : 0;
+      acc += v[k];
+    }
+    T tot;
+    T ex = block_excl_scan<T>(acc, lds, &tot) + carry;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+      const uint64_t i = base + k;
+      if (i < n) out[i] = ex;
+      ex += v[k];
+    }
+    carry += tot;
+  }
+  if (write_total && threadIdx.x == 0) out[n] = carry;
+}
+
 template <class T>
 void scan_impl(const T *in, T *out, uint64_t n, bool write_total, void *tmp, hipStream_t s) {
   if (n == 0) {
     if (write_total) BYO_HIP(hipMemsetAsync(out, 0, sizeof(T), s));
+    return;
+  }
+  if (n <= kSmallScan) {
+    k_scan_small<T><<<1, 1024, 0, s>>>(in, out, n, write_total ? 1 : 0);
+    BYO_LAUNCH_CHECK();
     return;
   }
   uint64_t nb = (n + kScanTile - 1) / kScanTile;
