@@ -258,10 +258,13 @@ __global__ void __launch_bounds__(256) k_pr_xcd(const uint32_t *so, const uint32
   }
 }
 
+// degs: true degrees when the view's lists hold empty slots (else null:
+// the offsets give them).
 template <class T, bool kComp>
 __global__ void __launch_bounds__(256) k_pr_xcd_combine(const T *part_s, const T *part_c, const uint64_t *off,
                                                         uint64_t n, const T *p_old, T *p_new, T *contrib_out,
-                                                        double damping, double addc, double *partial) {
+                                                        double damping, double addc, double *partial,
+                                                        const uint32_t *degs) {
   __shared__ double lds[4];
   const T d = (T)damping, t = (T)addc;
   double resid = 0.0;
@@ -274,7 +277,7 @@ __global__ void __launch_bounds__(256) k_pr_xcd_combine(const T *part_s, const T
     }
     const T sum = kComp ? (a.s - a.c) : a.s;
     const T pv = d * sum + t;
-    const uint64_t deg = off[v + 1] - off[v];
+    const uint64_t deg = degs ? (uint64_t)degs[v] : off[v + 1] - off[v];
     p_new[v] = pv;
     contrib_out[v] = deg ? pv / (T)deg : (T)0;
     resid += fabs((double)pv - (double)p_old[v]);
@@ -311,7 +314,10 @@ __device__ __forceinline__ uint64_t list_lower_bound(const uint32_t *ids, uint64
   return lo;
 }
 
-// cnt[x * (n+1) + v] = neighbours of v in slice x (0 at v = n).
+// cnt[x * (n+1) + v] = neighbours of v in slice x (0 at v = n).  H: the
+// lists may hold empty slots (kNone, a PMA slot view), which break their
+// order for a binary search: the slots are walked and the holes skipped.
+template <bool H>
 __global__ void k_xs_count(const uint64_t *off, const uint32_t *ids, uint64_t n, XcdBounds bd, uint32_t *cnt) {
   for (uint64_t v = global_tid(); v <= n; v += grid_threads()) {
     if (v == n) {
@@ -320,14 +326,27 @@ __global__ void k_xs_count(const uint64_t *off, const uint32_t *ids, uint64_t n,
     }
     uint64_t a = off[v];
     const uint64_t e = off[v + 1];
-    for (int x = 0; x < kXcds; x++) {
-      const uint64_t b = x == kXcds - 1 ? e : list_lower_bound(ids, a, e, bd.b[x + 1]);
-      cnt[(uint64_t)x * (n + 1) + v] = (uint32_t)(b - a);
-      a = b;
+    if constexpr (H) {
+      uint32_t c[kXcds] = {};
+      int x = 0;
+      for (uint64_t k = a; k < e; k++) {
+        const uint32_t u = ids[k];
+        if (u == kNone) continue;
+        while (x < kXcds - 1 && u >= bd.b[x + 1]) x++;
+        c[x]++;
+      }
+      for (int y = 0; y < kXcds; y++) cnt[(uint64_t)y * (n + 1) + v] = c[y];
+    } else {
+      for (int x = 0; x < kXcds; x++) {
+        const uint64_t b = x == kXcds - 1 ? e : list_lower_bound(ids, a, e, bd.b[x + 1]);
+        cnt[(uint64_t)x * (n + 1) + v] = (uint32_t)(b - a);
+        a = b;
+      }
     }
   }
 }
 
+template <bool H>
 __global__ void k_xs_scatter(const uint64_t *off, const uint32_t *ids, uint64_t n, XcdBounds bd, const uint32_t *so,
                              uint32_t *ids_s) {
   for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
@@ -335,6 +354,7 @@ __global__ void k_xs_scatter(const uint64_t *off, const uint32_t *ids, uint64_t 
     uint64_t pos = so[v];
     for (uint64_t e = off[v]; e < off[v + 1]; e++) {
       const uint32_t u = ids[e];
+      if (H && u == kNone) continue;
       while (x < kXcds - 1 && u >= bd.b[x + 1]) {
         x++;
         pos = so[(uint64_t)x * (n + 1) + v];
@@ -447,11 +467,11 @@ struct XcdSlices {
   const uint32_t *so, *ids_s;
 };
 
-XcdSlices xcd_slices(byo_graph *g, const RelabelView &view) {
+XcdSlices xcd_slices(byo_graph *g, const RelabelView &view, bool holes) {
   const uint64_t n = g->n, m = g->m;
   uint32_t *so = g->buf("pr_xs_so", (uint64_t)kXcds * (n + 1) * 4 + 64).as<uint32_t>();
   uint32_t *ids_s = g->buf("pr_xs_ids", m * 4 + 64).as<uint32_t>();
-  const int layout = view.perm ? 1 : 0;
+  const int layout = holes ? 2 : (view.perm ? 1 : 0);
   if (!g->xs_valid || g->xs_layout != layout) {
     hipStream_t s = g->stream;
     PhaseTimer t(s, g->prof.on);
@@ -462,9 +482,17 @@ XcdSlices xcd_slices(byo_graph *g, const RelabelView &view) {
     BYO_HIP(hipStreamSynchronize(s));
     const uint64_t cnt_n = (uint64_t)kXcds * (n + 1);
     DevBuf cnt(cnt_n * 4 + 64), tmp(prim::scan_tmp_bytes(cnt_n));
-    k_xs_count<<<byo_grid(n + 1, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd, cnt.as<uint32_t>());
+    if (holes) {
+      k_xs_count<true><<<byo_grid(n + 1, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd, cnt.as<uint32_t>());
+    } else {
+      k_xs_count<false><<<byo_grid(n + 1, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd, cnt.as<uint32_t>());
+    }
     prim::exclusive_scan_u32(cnt.as<uint32_t>(), so, cnt_n, false, tmp.p, s);
-    k_xs_scatter<<<byo_grid(n, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd, so, ids_s);
+    if (holes) {
+      k_xs_scatter<true><<<byo_grid(n, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd, so, ids_s);
+    } else {
+      k_xs_scatter<false><<<byo_grid(n, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd, so, ids_s);
+    }
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipStreamSynchronize(s));
     if (g->prof.on) g->account("pr_xcd_slices", t.stop());
@@ -498,8 +526,9 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   // otherwise pr_lanes, or about half the per-segment average degree, in
   // [4, 32]
   const bool per_tile = pma || (S == 1 && !g->tuning("pr_lanes"));
-  // the XCD split needs the tile pull and 32-bit edge positions
-  const bool per_xcd = !pma && per_tile && g->tuning("pr_xcd") && g->m < (1ull << 32) && n > 0;
+  // the XCD split needs the tile pull and 32-bit edge positions; on a PMA
+  // its slices are built from the slot view (holes dropped)
+  const bool per_xcd = per_tile && g->tuning("pr_xcd") && g->m < (1ull << 32) && n > 0;
   const double avg = n ? (double)g->m / (double)n / S : 0.0;
   int G = 4;
   while (G < 32 && G < avg / 2) G *= 2;
@@ -510,7 +539,7 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   XcdSlices xs{};
   T *part_s = nullptr, *part_c = nullptr;
   if (per_xcd) {
-    xs = xcd_slices(g, view);
+    xs = xcd_slices(g, view, pma);
     part_s = g->buf("pr_xs_part_s", (uint64_t)kXcds * n * sizeof(T) + 64).as<T>();
     if (kComp) part_c = g->buf("pr_xs_part_c", (uint64_t)kXcds * n * sizeof(T) + 64).as<T>();
   }
@@ -530,7 +559,7 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
     if (per_xcd) {
       k_pr_xcd<T, kComp><<<grid / kXcds * kXcds, 256, 0, s>>>(xs.so, xs.ids_s, n, c[cur], part_s, part_c);
       k_pr_xcd_combine<T, kComp><<<cgrid, 256, 0, s>>>(part_s, part_c, off, n, p[cur], p[cur ^ 1], c[cur ^ 1],
-                                                       damping, addc, partial);
+                                                       damping, addc, partial, av.deg);
       nparts = cgrid;
     } else if (pma) {
       k_pr_tiles<T, kComp, true><<<grid, 256, 0, s>>>(off, ids, n, c[cur], p[cur], p[cur ^ 1], c[cur ^ 1], damping,

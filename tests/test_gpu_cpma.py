@@ -290,6 +290,13 @@ def test_algorithms_read_the_pma(B, scale):
     assert np.abs(p64 - p_ref).sum() < 1e-12
     p32, _, _ = G.pagerank(0.85, 20, 0.0, B.PR_FP32)
     assert np.abs(p32 - p_ref).sum() <= 1e-6  # the stated FP32 tolerance
+    # the per-XCD source slices, built from the slot view (holes dropped)
+    G.set_tuning("pr_xcd", 1)
+    p64, _, _ = G.pagerank(0.85, 20, 0.0, B.PR_FP64)
+    assert np.abs(p64 - p_ref).sum() < 1e-12
+    p32, _, _ = G.pagerank(0.85, 20, 0.0, B.PR_FP32)
+    assert np.abs(p32 - p_ref).sum() <= 1e-6
+    G.set_tuning("pr_xcd", 0)
     lab = O.cc(h)
     assert np.array_equal(G.cc()[0], lab)
     assert np.array_equal(G.cc_labelprop()[0], lab)
