@@ -317,6 +317,17 @@ __device__ __forceinline__ uint64_t list_lower_bound(const uint32_t *ids, uint64
 // cnt[x * (n+1) + v] = neighbours of v in slice x (0 at v = n).  H: the
 // lists may hold empty slots (kNone, a PMA slot view), which break their
 // order for a binary search: the slots are walked and the holes skipped.
+// Lists longer than kXsShort are left to one wave each (k_xs_long_*): a
+// thread walking an RMAT hub's list alone held the whole pass.
+constexpr uint64_t kXsShort = 256;
+
+__device__ __forceinline__ int xs_slice(uint32_t u, const XcdBounds &bd) {
+  int x = 0;
+#pragma unroll
+  for (int y = 1; y < kXcds; y++) x += u >= bd.b[y];
+  return x;
+}
+
 template <bool H>
 __global__ void k_xs_count(const uint64_t *off, const uint32_t *ids, uint64_t n, XcdBounds bd, uint32_t *cnt) {
   for (uint64_t v = global_tid(); v <= n; v += grid_threads()) {
@@ -326,6 +337,7 @@ __global__ void k_xs_count(const uint64_t *off, const uint32_t *ids, uint64_t n,
     }
     uint64_t a = off[v];
     const uint64_t e = off[v + 1];
+    if (e - a > kXsShort) continue;  // k_xs_long_count
     if constexpr (H) {
       uint32_t c[kXcds] = {};
       int x = 0;
@@ -350,6 +362,7 @@ template <bool H>
 __global__ void k_xs_scatter(const uint64_t *off, const uint32_t *ids, uint64_t n, XcdBounds bd, const uint32_t *so,
                              uint32_t *ids_s) {
   for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
+    if (off[v + 1] - off[v] > kXsShort) continue;  // k_xs_long_scatter
     int x = 0;
     uint64_t pos = so[v];
     for (uint64_t e = off[v]; e < off[v + 1]; e++) {
@@ -360,6 +373,71 @@ __global__ void k_xs_scatter(const uint64_t *off, const uint32_t *ids, uint64_t 
         pos = so[(uint64_t)x * (n + 1) + v];
       }
       ids_s[pos++] = u;
+    }
+  }
+}
+
+// One wave per long list: every 64 slots, a ballot per slice counts (and
+// ranks) the ids of that slice; slots are visited in order, so each
+// slice's run keeps the list's order.
+template <bool H>
+__global__ void k_xs_long_count(const uint64_t *off, const uint32_t *ids, uint64_t n, XcdBounds bd, uint32_t *cnt) {
+  const int lane = lane_id();
+  const uint64_t wstride = (grid_threads() >> 6) * 64;
+  for (uint64_t v0 = (global_tid() >> 6) * 64; v0 < n; v0 += wstride) {
+    // 64 vertices' bounds at once; the long lists among them one by one
+    const uint64_t vl = v0 + lane;
+    const uint64_t al = vl < n ? off[vl] : 0, el = vl < n ? off[vl + 1] : 0;
+    uint64_t longm = __ballot(vl < n && el - al > kXsShort);
+    while (longm) {
+      const int l = __builtin_ctzll(longm);
+      longm &= longm - 1;
+      const uint64_t v = v0 + l, a = __shfl(al, l, 64), e = __shfl(el, l, 64);
+      uint32_t c = 0;  // lane x < kXcds: the count of slice x
+      for (uint64_t k0 = a; k0 < e; k0 += 64) {
+        const uint64_t k = k0 + lane;
+        const uint32_t u = k < e ? ids[k] : kNone;
+        const bool real = k < e && (!H || u != kNone);
+        const int x = real ? xs_slice(u, bd) : -1;
+#pragma unroll
+        for (int y = 0; y < kXcds; y++) {
+          const uint32_t t = (uint32_t)__popcll(__ballot(x == y));
+          if (lane == y) c += t;
+        }
+      }
+      if (lane < kXcds) cnt[(uint64_t)lane * (n + 1) + v] = c;
+    }
+  }
+}
+
+template <bool H>
+__global__ void k_xs_long_scatter(const uint64_t *off, const uint32_t *ids, uint64_t n, XcdBounds bd,
+                                  const uint32_t *so, uint32_t *ids_s) {
+  const int lane = lane_id();
+  const uint64_t wstride = (grid_threads() >> 6) * 64;
+  for (uint64_t v0 = (global_tid() >> 6) * 64; v0 < n; v0 += wstride) {
+    const uint64_t vl = v0 + lane;
+    const uint64_t al = vl < n ? off[vl] : 0, el = vl < n ? off[vl + 1] : 0;
+    uint64_t longm = __ballot(vl < n && el - al > kXsShort);
+    while (longm) {
+      const int l = __builtin_ctzll(longm);
+      longm &= longm - 1;
+      const uint64_t v = v0 + l, a = __shfl(al, l, 64), e = __shfl(el, l, 64);
+      // lane x < kXcds holds slice x's next output position
+      uint64_t pos = lane < kXcds ? so[(uint64_t)lane * (n + 1) + v] : 0;
+      for (uint64_t k0 = a; k0 < e; k0 += 64) {
+        const uint64_t k = k0 + lane;
+        const uint32_t u = k < e ? ids[k] : kNone;
+        const bool real = k < e && (!H || u != kNone);
+        const int x = real ? xs_slice(u, bd) : -1;
+#pragma unroll
+        for (int y = 0; y < kXcds; y++) {
+          const uint64_t bal = __ballot(x == y);
+          const uint64_t py = __shfl(pos, y, 64);
+          if (x == y) ids_s[py + mbcnt(bal)] = u;
+          if (lane == y) pos += (uint64_t)__popcll(bal);
+        }
+      }
     }
   }
 }
@@ -484,14 +562,20 @@ XcdSlices xcd_slices(byo_graph *g, const RelabelView &view, bool holes) {
     DevBuf cnt(cnt_n * 4 + 64), tmp(prim::scan_tmp_bytes(cnt_n));
     if (holes) {
       k_xs_count<true><<<byo_grid(n + 1, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd, cnt.as<uint32_t>());
+      k_xs_long_count<true><<<byo_grid(n, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd,
+                                                                          cnt.as<uint32_t>());
     } else {
       k_xs_count<false><<<byo_grid(n + 1, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd, cnt.as<uint32_t>());
+      k_xs_long_count<false><<<byo_grid(n, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd,
+                                                                           cnt.as<uint32_t>());
     }
     prim::exclusive_scan_u32(cnt.as<uint32_t>(), so, cnt_n, false, tmp.p, s);
     if (holes) {
       k_xs_scatter<true><<<byo_grid(n, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd, so, ids_s);
+      k_xs_long_scatter<true><<<byo_grid(n, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd, so, ids_s);
     } else {
       k_xs_scatter<false><<<byo_grid(n, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd, so, ids_s);
+      k_xs_long_scatter<false><<<byo_grid(n, 256, 16384), 256, 0, s>>>(view.off, view.ids, n, bd, so, ids_s);
     }
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipStreamSynchronize(s));
