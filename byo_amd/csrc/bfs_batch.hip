@@ -2,11 +2,15 @@
 // bfs_batch.hpp on its own (byo_bfs_multi) and reads its results back.
 #include "bfs_batch.hpp"
 
+#ifndef BYO_BATCH_WPE
+#define BYO_BATCH_WPE 3
+#endif
+
 namespace byo {
 namespace batch {
 
 template <class M, bool G>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_bfs_batch(BatchArgs a) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BYO_BATCH_WPE, 8))) k_bfs_batch(BatchArgs a) {
   bfs_batch_body<M, G>(a);
 }
 
@@ -44,6 +48,8 @@ void prepare(byo_graph *g, const AdjView &v, const uint32_t *srcs, uint32_t k, u
   a.dc = dir_ctl(g, maxdeg, s, true);
   a.pieces = make_pieces(g, nblocks, v.deg ? kPmaPieces : kTraversalPieces);
   a.bar = coop_bar(g);
+  a.dyn_ctr = g->buf("bfs_batch_dyn", 3 * 8192 * 4 + 64).as<unsigned int>();
+  a.dyn = (uint32_t)std::min<int64_t>(g->tuning("pull_dyn"), a.pieces.per_wave);
   a.bsum = g->buf("bfs_batch_bsum", (size_t)nblocks * 8 + 64).as<unsigned long long>();
   a.seen = g->buf("bfs_batch_seen", a.nm * mask_bytes + 64).p;
   a.fm = g->buf("bfs_batch_fm", 3 * a.nm * mask_bytes + 64).p;

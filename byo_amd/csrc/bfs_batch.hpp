@@ -82,6 +82,8 @@ struct BatchArgs {
   unsigned long long *tot;  // k x {vertices reached, sum of their degrees}
   DirCtl dc;
   Pieces pieces;            // weight-balanced word pieces of the grid
+  unsigned int *dyn_ctr;    // 3 rotating sets of 8 piece counters (set at + 8192 s, counter q at + 1024 q)
+  uint32_t dyn;             // pieces per wave dealt by the counters in dense levels (0: all static)
   GridBar *bar;
 };
 
@@ -202,6 +204,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
       f4[2 * nvec + i] = z;
     }
     for (uint64_t i = global_tid(); i < 3 * nwords; i += grid_threads()) a.ubm[i] = 0;
+    if (global_tid() < 24) a.dyn_ctr[(global_tid() >> 3) * 8192 + (global_tid() & 7) * 1024] = 0;
   }
   grid_barrier(a.bar, (unsigned)nb);
   if (blockIdx.x == 0) {
@@ -255,6 +258,15 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
     }
     BCtr *nx = &a.ctr[(L + 1) % 3];
     if (leader) a.ctr[(L + 2) % 3] = BCtr{0, 0, 0, 0};
+    // piece counters of this level's dense pass, and the set of level L+2
+    // cleared (its last use, level L-1, has passed the barrier)
+    unsigned int *const dctr = a.dyn_ctr + (uint64_t)(L % 3) * 8192;
+    // the dynamic tail pays in the first dense level after a sparse one (its
+    // lists are scanned whole, and their lengths are what the static weights
+    // balance least well); later dense levels keep the static share-out,
+    // where claims cost more than they balance
+    const uint32_t ldyn = is_dense ? 0 : a.dyn;
+    if (blockIdx.x == 0 && threadIdx.x < 8) a.dyn_ctr[(uint64_t)((L + 2) % 3) * 8192 + threadIdx.x * 1024] = 0;
     M *const fmask = fmb + (uint64_t)(L % 3) * a.nm, *const nmask = fmb + (uint64_t)((L + 1) % 3) * a.nm;
     uint32_t *const q = qb + (uint64_t)(L % 3) * a.qstride, *const qn = qb + (uint64_t)((L + 1) % 3) * a.qstride;
     // union bitmaps rotate with the masks: this frontier's, the next one's
@@ -284,41 +296,72 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
 
     if (dir == 0) {
       if (!have_q) {
-        // toSparse: every wave packs a run of 2048 mask entries with one
-        // cursor atomic (count pass, then write pass over the same run)
-        constexpr uint64_t kRun = 2048;
-        for (uint64_t r0 = wave * kRun; r0 < a.nm; r0 += nwaves * kRun) {
-          const uint64_t r1 = std::min<uint64_t>(a.nm, r0 + kRun);
+        // toSparse: a block packs runs of kBlock 16-byte vectors of mask
+        // entries (one load per thread), counts them in LDS and takes its
+        // place in the queue with one cursor atomic per run; the queue holds
+        // each run's vertices in id order
+        constexpr int E = 16 / sizeof(M);
+        const uint64_t nvec = a.nm * sizeof(M) / 16;
+        const uint4 *const f4 = reinterpret_cast<const uint4 *>(fmask);
+        for (uint64_t v0 = blockIdx.x * (uint64_t)kBlock; v0 < nvec; v0 += nb * kBlock) {
+          const uint64_t i = v0 + threadIdx.x;
+          uint4 x = make_uint4(0, 0, 0, 0);
+          if (i < nvec) x = f4[i];
+          M e[E];
+          __builtin_memcpy(e, &x, 16);
           uint32_t c = 0;
-          for (uint64_t i = r0 + lane; i < r1; i += 64) c += fmask[i] != 0;
+#pragma unroll
+          for (int t = 0; t < E; t++) c += e[t] != 0;
           const uint32_t incl = wave_incl_scan(c);
-          const uint32_t tot = __shfl(incl, 63, 64);
-          unsigned long long base = 0;
-          if (lane == 0 && tot) base = atomicAdd(&cur->cursor, (unsigned long long)tot);
-          base = __shfl(base, 0, 64);
-          if (!tot) continue;
-          for (uint64_t i0 = r0; i0 < r1; i0 += 64) {
-            const uint64_t i = i0 + lane;
-            const bool nz = i < r1 && fmask[i] != 0;
-            const uint64_t bal = __ballot(nz);
-            if (nz) q[base + mbcnt(bal)] = (uint32_t)i;
-            base += __popcll(bal);
+          if (lane == 63) lds[wv] = incl;
+          __syncthreads();
+          unsigned long long wpre = 0, tot = 0;
+          for (int w = 0; w < kWaves; w++) {
+            if (w < wv) wpre += lds[w];
+            tot += lds[w];
           }
+          if (threadIdx.x == 0 && tot) lds[kWaves] = atomicAdd(&cur->cursor, tot);
+          __syncthreads();
+          if (tot) {
+            uint64_t pos = lds[kWaves] + wpre + incl - c;
+#pragma unroll
+            for (int t = 0; t < E; t++)
+              if (e[t]) q[pos++] = (uint32_t)(i * E + t);
+          }
+          __syncthreads();  // lds is rewritten by the next run
         }
         grid_barrier(a.bar, (unsigned)nb);
       }
       if (fs <= 4 * nb || (fs <= 16 * nwaves && fdeg <= 32 * fs)) {
-        // one block (fs <= #blocks) or one wave per frontier vertex
-        const bool per_block = fs <= nb;
-        const uint64_t unit = per_block ? blockIdx.x : wave, nunits = per_block ? nb : nwaves;
-        const int width = per_block ? kBlock : 64;
-        const int lid = per_block ? (int)threadIdx.x : lane;
-        for (uint64_t i = unit; i < fs; i += nunits) {
-          const uint32_t u = q[i];
-          const M f = fmask[u];
-          const uint64_t b = a.off[u], e = a.off[u + 1];
-          for (uint64_t k0 = b; k0 < e; k0 += width) {
-            const uint64_t kk = k0 + lid;
+        // one block per frontier vertex (fs <= #blocks), or gl lanes per
+        // frontier vertex, gl the power of two >= the frontier's average
+        // degree: a frontier of short lists is then served in one round of
+        // loads instead of a wave per vertex, one after another
+        int gl = kBlock;
+        if (fs > nb) {
+          gl = 1;
+          while (gl < 64 && (uint64_t)gl * fs < fdeg) gl <<= 1;
+        }
+        const int per = gl >= 64 ? 1 : 64 / gl;
+        const int gi = gl >= 64 ? 0 : lane / gl;
+        const int lid = gl == kBlock ? (int)threadIdx.x : (gl == 64 ? lane : lane % gl);
+        const uint64_t first = gl == kBlock ? blockIdx.x : wave * per, step = gl == kBlock ? nb : nwaves * per;
+        const int width = gl == kBlock ? kBlock : gl;
+        for (uint64_t i0 = first; i0 < fs; i0 += step) {
+          const uint64_t i = i0 + gi;
+          uint32_t u = 0;
+          M f = 0;
+          uint64_t b = 0, e = 0;
+          if (i < fs) {
+            u = q[i];
+            f = fmask[u];
+            b = a.off[u];
+            e = a.off[u + 1];
+          }
+          // a wave-uniform trip count (the ballots below need every lane)
+          const uint64_t trips = wave_max<uint64_t>((e - b + width - 1) / width);
+          for (uint64_t t = 0; t < trips; t++) {
+            const uint64_t kk = b + t * width + lid;
             bool add = false;
             uint32_t tgt = 0;
             if (kk < e) {
@@ -478,9 +521,14 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
       // L2) and gathers the neighbour's source mask only on a hit;
       // direction 1 gathers every neighbour's mask.
       const bool bmp = dir == 4;
+      // the one-neighbour round runs in dense levels after the first with up
+      // to 8 sources: there most vertices finish at their first neighbour
+      // (with more sources the many open bits keep them scanning, and the
+      // extra round trip costs more than the gathers it saves)
+      const bool first1 = is_dense && sizeof(M) == 1;
       const int g = lane >> 3, sub = lane & 7;
       unsigned long long cnt = 0;
-      for_my_words(a.pieces, wave, nwaves, [&](uint64_t w) {
+      for_my_words_dyn(a.pieces, wave, nwaves, dctr, ldyn, [&](uint64_t w) {
         const uint64_t vl = w * 64 + lane;
         uint64_t bl = 0, el = 0;
         M s = (M)~(M)0;
@@ -493,64 +541,89 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
         const uint64_t todo = __ballot(U != 0);
         M acc = 0;
         if (todo) {
-          res[wv][lane] = 0;
-          wave_lds_sync();
-          uint32_t u[8];
-          M h[8];
-#pragma unroll
-          for (int st = 0; st < 8; st++) {
-            const int x = st * 8 + g;
-            const uint64_t b = __shfl(bl, x, 64), e = __shfl(el, x, 64);
-            const uint64_t kk = b + sub;
-            const bool on = ((todo >> x) & 1) && kk < e;
-            u[st] = a.ids[on ? kk : 0];  // unconditional: all 8 loads in flight
-            const M Ux = shfl_m<M>(U, x);  // every lane joins the shuffle
-            h[st] = on ? Ux : (M)0;
-          }
-          if constexpr (G) {
-            // an empty slot (or an empty slot 0 read by an idle lane) is no
-            // neighbour; read entry 0's mask instead of an out-of-range one
-#pragma unroll
-            for (int st = 0; st < 8; st++)
-              if (u[st] == kNone) {
-                u[st] = 0;
-                h[st] = 0;
-              }
-          }
-          M fv[8];
-          if (bmp) {
-            uint64_t bw[8];
-#pragma unroll
-            for (int st = 0; st < 8; st++) bw[st] = cubm[u[st] >> 6];
-#pragma unroll
-            for (int st = 0; st < 8; st++)
-              if (!((bw[st] >> (u[st] & 63)) & 1)) h[st] = 0;
-#pragma unroll
-            for (int st = 0; st < 8; st++) fv[st] = h[st] ? fmask[u[st]] : (M)0;
-          } else {
-#pragma unroll
-            for (int st = 0; st < 8; st++) fv[st] = fmask[u[st]];
-          }
-#pragma unroll
-          for (int st = 0; st < 8; st++) h[st] &= fv[st];
-#pragma unroll
-          for (int st = 0; st < 8; st++) {
-            M incl = h[st];
-#pragma unroll
-            for (int d = 1; d < 8; d <<= 1) {
-              const M t = shfl_up8<M>(incl, d);
-              if (sub >= d) incl |= t;
+          // first neighbour of every unfinished vertex (lane = vertex): in
+          // the dense levels after the first, most vertices finish here, and
+          // the 8-wide probe below then gathers only for the others
+          M h1 = 0;
+          if (first1) {
+            const bool p1 = ((todo >> lane) & 1) && el > bl;
+            uint32_t u1 = a.ids[p1 ? bl : 0];
+            h1 = p1 ? U : (M)0;
+            if (G && u1 == kNone) {
+              u1 = 0;
+              h1 = 0;
             }
-            M excl = shfl_up8<M>(incl, 1);
-            if (sub == 0) excl = 0;
-            M nw = h[st] & (M)~excl;
-            const uint64_t v = w * 64 + st * 8 + g;
+            if (bmp && !((cubm[u1 >> 6] >> (u1 & 63)) & 1)) h1 = 0;
+            h1 &= (!bmp || h1) ? fmask[u1] : (M)0;
+            M nw = h1;
             while (nw) {
               const int b = __builtin_ctzll((unsigned long long)nw);
               nw &= (M)(nw - 1);
-              a.parents[(uint64_t)b * a.pstride + v] = u[st];
+              a.parents[(uint64_t)b * a.pstride + vl] = u1;
             }
-            if (sub == 7) res[wv][st * 8 + g] = (uint64_t)incl;
+          }
+          const M U2 = (M)(U & (M)~h1);
+          const uint64_t todo2 = __ballot(((todo >> lane) & 1) && U2 != 0 && el - bl > (first1 ? 1 : 0));
+          res[wv][lane] = (uint64_t)h1;
+          wave_lds_sync();
+          if (todo2) {
+            uint32_t u[8];
+            M h[8];
+#pragma unroll
+            for (int st = 0; st < 8; st++) {
+              const int x = st * 8 + g;
+              const uint64_t b = __shfl(bl, x, 64) + (first1 ? 1 : 0), e = __shfl(el, x, 64);
+              const uint64_t kk = b + sub;
+              const bool on = ((todo2 >> x) & 1) && kk < e;
+              u[st] = a.ids[on ? kk : 0];  // unconditional: all 8 loads in flight
+              const M Ux = shfl_m<M>(U2, x);  // every lane joins the shuffle
+              h[st] = on ? Ux : (M)0;
+            }
+            if constexpr (G) {
+              // an empty slot (or an empty slot 0 read by an idle lane) is no
+              // neighbour; read entry 0's mask instead of an out-of-range one
+#pragma unroll
+              for (int st = 0; st < 8; st++)
+                if (u[st] == kNone) {
+                  u[st] = 0;
+                  h[st] = 0;
+                }
+            }
+            M fv[8];
+            if (bmp) {
+              uint64_t bw[8];
+#pragma unroll
+              for (int st = 0; st < 8; st++) bw[st] = cubm[u[st] >> 6];
+#pragma unroll
+              for (int st = 0; st < 8; st++)
+                if (!((bw[st] >> (u[st] & 63)) & 1)) h[st] = 0;
+#pragma unroll
+              for (int st = 0; st < 8; st++) fv[st] = h[st] ? fmask[u[st]] : (M)0;
+            } else {
+#pragma unroll
+              for (int st = 0; st < 8; st++) fv[st] = fmask[u[st]];
+            }
+#pragma unroll
+            for (int st = 0; st < 8; st++) h[st] &= fv[st];
+#pragma unroll
+            for (int st = 0; st < 8; st++) {
+              M incl = h[st];
+#pragma unroll
+              for (int d = 1; d < 8; d <<= 1) {
+                const M t = shfl_up8<M>(incl, d);
+                if (sub >= d) incl |= t;
+              }
+              M excl = shfl_up8<M>(incl, 1);
+              if (sub == 0) excl = 0;
+              M nw = h[st] & (M)~excl;
+              const uint64_t v = w * 64 + st * 8 + g;
+              while (nw) {
+                const int b = __builtin_ctzll((unsigned long long)nw);
+                nw &= (M)(nw - 1);
+                a.parents[(uint64_t)b * a.pstride + v] = u[st];
+              }
+              if (sub == 7) res[wv][st * 8 + g] |= (uint64_t)incl;
+            }
           }
           wave_lds_sync();
           acc = (M)res[wv][lane];
@@ -562,14 +635,15 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
           // window starts at the first vertex that still has bits to find;
           // bits found are merged into the vertex's LDS mask by atomicOr, and
           // any lane holding a frontier neighbour may write that bit's parent
-          const bool more = ((todo >> lane) & 1) && acc != U && el - bl > 8;
-          const uint64_t r = more ? el - bl - 8 : 0;
+          const uint64_t pre = first1 ? 9 : 8;  // neighbours checked by the two probes
+          const bool more = ((todo >> lane) & 1) && acc != U && el - bl > pre;
+          const uint64_t r = more ? el - bl - pre : 0;
           const uint64_t rin = wave_incl_scan(r);
           const uint64_t R = __shfl(rin, 63, 64);
           if (R) {
             const uint64_t scl = rin - r;
             win.sc[wv][lane] = scl;
-            win.base[wv][lane] = (int64_t)(bl + 8) - (int64_t)scl;
+            win.base[wv][lane] = (int64_t)(bl + pre) - (int64_t)scl;
             win.f[wv][lane] = (uint64_t)U;
             wave_lds_sync();
             uint64_t t0 = 0;
@@ -661,7 +735,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
       have_q = false;
     } else {
       // dense forward: one wave per frontier vertex of each 64-vertex word
-      for_my_words(a.pieces, wave, nwaves, [&](uint64_t w) {
+      for_my_words_dyn(a.pieces, wave, nwaves, dctr, ldyn, [&](uint64_t w) {
         const uint64_t vl = w * 64 + lane;
         const M fv = vl < a.n ? fmask[vl] : (M)0;
         uint64_t word = __ballot(fv != 0);

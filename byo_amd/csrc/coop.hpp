@@ -130,6 +130,65 @@ __device__ __forceinline__ void for_my_words(const Pieces &pc, uint64_t wave, ui
   }
 }
 
+// The same share-out with a dynamic tail: of a wave's pieces, the first
+// per_wave - dyn are its own as above, and the last dyn * nwaves pieces of the
+// grid are handed out by 8 counters (counter q deals pieces q, q + 8, ... of
+// the tail; the counters sit 4 KB apart, ctr[q * 1024], so their atomics
+// queue at different memory channels).  A wave claims from the counter of its
+// block's XCD (workgroups are dealt to the 8 XCDs round-robin) and moves to
+// the first other one that still has pieces when that runs dry (one read of
+// all 8 counters finds it, so a dry counter costs no atomic).  The claim of the next piece is in flight while the
+// wave runs the current one.  The static share balances the edge weights; the
+// tail absorbs what they miss -- the per-word cost of a sweep that finds
+// little to do, or lists that end early.  The counters must be zero on entry.
+template <class F>
+__device__ __forceinline__ void for_my_words_dyn(const Pieces &pc, uint64_t wave, uint64_t nwaves,
+                                                 unsigned int *ctr, uint32_t dyn, F &&body) {
+  const uint32_t own = dyn < pc.per_wave ? pc.per_wave - dyn : 0;
+  const uint64_t base = (uint64_t)own * nwaves, ntail = (uint64_t)pc.per_wave * nwaves - base;
+  const int lane = threadIdx.x & 63;
+  const unsigned int home = blockIdx.x & 7;
+  // one call site of body (a second inlined copy of a large body costs
+  // registers): the static pieces first, then the claimed ones
+  uint64_t i = 0, cnt = 0;
+  unsigned int q = 8, tn = 0;  // q: counter being drained (8: none); tn: lane 0's claim in flight
+  for (;;) {
+    uint64_t p;
+    if (i < own) {
+      p = wave + i * nwaves;
+      i++;
+    } else {
+      if (!ntail) break;
+      unsigned int t = q < 8 ? __shfl(tn, 0, 64) : 0;
+      bool done = false;
+      while (q == 8 || t >= cnt) {
+        // lanes 0-7 read the 8 counters (from the home XCD's on) in one
+        // round trip; the wave drains the first that still has pieces
+        const unsigned int ql = (home + (unsigned int)lane) & 7;
+        const uint64_t cl = ntail > ql ? (ntail - ql + 7) / 8 : 0;
+        const bool avail =
+            lane < 8 && __hip_atomic_load(ctr + ql * 1024, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < cl;
+        const uint64_t am = __ballot(avail);
+        if (!am) {
+          done = true;
+          break;
+        }
+        const int src = __builtin_ctzll(am);
+        q = (home + (unsigned int)src) & 7;
+        cnt = __shfl(cl, src, 64);
+        unsigned int c0 = 0;
+        if (lane == 0) c0 = atomicAdd(ctr + q * 1024, 1u);
+        t = __shfl(c0, 0, 64);
+      }
+      if (done) break;
+      if (lane == 0) tn = atomicAdd(ctr + q * 1024, 1u);  // the next claim, in flight meanwhile
+      p = base + q + 8ull * t;
+    }
+    const uint64_t w1 = pc.bound[p + 1];
+    for (uint64_t w = pc.bound[p]; w < w1; w++) body(w);
+  }
+}
+
 // Inputs of the per-level direction decision.
 struct DirCtl {
   uint64_t n, m;
