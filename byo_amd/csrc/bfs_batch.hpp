@@ -810,21 +810,42 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
   // source j over each word's 64 masks staged in LDS.  Blocks combine lanes
   // in LDS and add their totals with one atomic per source.
   if constexpr (sizeof(M) == 1) {
+    // a thread takes 16 consecutive vertices at a time (one 16-byte load of
+    // their masks and one run of their offsets), so the pass is a couple of
+    // load rounds per thread rather than one per 64-vertex word
     unsigned long long rc[8] = {}, rd[8] = {};
-    for_my_words(a.pieces, wave, nwaves, [&](uint64_t w) {
-      const uint64_t vl = w * 64 + lane;
-      if (vl < a.n) {
-        const uint32_t sm = (uint32_t)seen[vl];
-        const unsigned long long d = vdeg<G>(a.off, a.deg, vl);
+    const uint64_t ngrp = (a.n + 15) / 16;
+    for (uint64_t t = global_tid(); t < ngrp; t += grid_threads()) {
+      const uint64_t v0 = t * 16;
+      const uint4 sv = reinterpret_cast<const uint4 *>(seen)[t];  // nm is a multiple of 64
+      uint8_t sm[16];
+      __builtin_memcpy(sm, &sv, 16);
+      uint64_t d[16];
+      if constexpr (G) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) d[i] = v0 + i < a.n ? a.deg[v0 + i] : 0;
+      } else {
+        uint64_t o[17];
+#pragma unroll
+        for (int i = 0; i <= 16; i++) o[i] = a.off[std::min<uint64_t>(v0 + i, a.n)];
+#pragma unroll
+        for (int i = 0; i < 16; i++) d[i] = o[i + 1] - o[i];
+      }
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        if (v0 + i >= a.n) break;
+        const uint32_t m8 = sm[i];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-          const unsigned long long bit = (sm >> j) & 1;
+          const unsigned long long bit = (m8 >> j) & 1;
           rc[j] += bit;
-          rd[j] += bit * d;
-          if (!bit && (uint32_t)j < a.k) a.parents[(uint64_t)j * a.pstride + vl] = kNone;
+          rd[j] += bit * d[i];
         }
+        const uint32_t miss = ~m8 & ((a.k >= 8) ? 0xffu : ((1u << a.k) - 1));
+        for (uint32_t mm = miss; mm; mm &= mm - 1)
+          a.parents[(uint64_t)__builtin_ctz(mm) * a.pstride + v0 + i] = kNone;
       }
-    });
+    }
     if (threadIdx.x < 64) {
       tl[0][threadIdx.x] = 0;
       tl[1][threadIdx.x] = 0;
