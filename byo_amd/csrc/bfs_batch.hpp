@@ -218,20 +218,27 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
       a.tot[2 * j] = 0;
       a.tot[2 * j + 1] = 0;
     }
-    if (threadIdx.x == 0) {
-      unsigned long long cnt = 0, deg = 0;
-      for (uint32_t j = 0; j < a.k; j++) {
-        bool dup = false;
-        for (uint32_t i = 0; i < j; i++) dup = dup || a.srcs[i] == a.srcs[j];
-        if (!dup) {
-          qb[cnt++] = a.srcs[j];
-          deg += vdeg<G>(a.off, a.deg, a.srcs[j]);
-        }
+    if (threadIdx.x < 64) {
+      // the first frontier: lane j holds source j unless an earlier source
+      // is the same vertex; the queue keeps first occurrences in order
+      bool first = false;
+      unsigned long long d = 0;
+      uint32_t sj = 0;
+      if ((uint32_t)lane < a.k) {
+        sj = a.srcs[lane];
+        first = true;
+        for (int i = 0; i < lane; i++) first = first && a.srcs[i] != sj;
+        if (first) d = vdeg<G>(a.off, a.deg, sj);
       }
-      const unsigned long long all = a.k >= 64 ? ~0ull : ((1ull << a.k) - 1);
-      a.ctr[0] = BCtr{cnt, deg, 0, all};
-      a.ctr[1] = BCtr{0, 0, 0, 0};
-      a.ctr[2] = BCtr{0, 0, 0, 0};
+      const uint64_t fm = __ballot(first);
+      if (first) qb[mbcnt(fm)] = sj;
+      const unsigned long long deg = wave_sum(d);
+      if (lane == 0) {
+        const unsigned long long all = a.k >= 64 ? ~0ull : ((1ull << a.k) - 1);
+        a.ctr[0] = BCtr{(unsigned long long)__popcll(fm), deg, 0, all};
+        a.ctr[1] = BCtr{0, 0, 0, 0};
+        a.ctr[2] = BCtr{0, 0, 0, 0};
+      }
     }
   }
   grid_barrier(a.bar, (unsigned)nb);
@@ -872,19 +879,20 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
       if (vl < a.n) {
         sm = (uint64_t)seen[vl];
         d = vdeg<G>(a.off, a.deg, vl);
-        for (uint32_t j = 0; j < a.k; j++)
-          if (!((sm >> j) & 1)) a.parents[(uint64_t)j * a.pstride + vl] = kNone;
+        const uint64_t kmask = a.k >= 64 ? ~0ull : ((1ull << a.k) - 1);
+        for (uint64_t mm = ~sm & kmask; mm; mm &= mm - 1)
+          a.parents[(uint64_t)__builtin_ctzll(mm) * a.pstride + vl] = kNone;
       }
       if (!__ballot(sm != 0)) return;
       res[wv][lane] = sm;
       win.sc[wv][lane] = d;
       wave_lds_sync();
       if ((uint32_t)lane < a.k) {
+#pragma unroll 16
         for (int x = 0; x < 64; x++) {
-          if ((res[wv][x] >> lane) & 1) {
-            rc++;
-            rd += win.sc[wv][x];
-          }
+          const unsigned long long bit = (res[wv][x] >> lane) & 1;
+          rc += bit;
+          rd += bit * win.sc[wv][x];
         }
       }
       wave_lds_sync();
