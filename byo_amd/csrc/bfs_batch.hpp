@@ -11,10 +11,10 @@
 // union frontier advances every source by one level:
 //
 //   push  (edgeMapSparse, gbbs/edge_map_blocked.h:18-92): for a frontier
-//         vertex u with mask f and an edge u->v, the bits f & ~seen[v] are
-//         claimed by an atomicOr on seen[v]; each bit j that this edge wins
-//         sets parents[j][v] = u, and the thread that makes next[v] nonzero
-//         appends v to the next queue;
+//         vertex u with mask f and an edge u->v, the bits of f that v has
+//         not seen are claimed by one atomicOr on next[v]; each bit j that
+//         this edge wins sets parents[j][v] = u, and the thread that makes
+//         next[v] nonzero appends v to the next queue;
 //   pull  (edgeMapDense, :47-175): a vertex with bits U still unreached
 //         (restricted to sources whose frontier is not empty) scans its
 //         neighbours in adjacency order and takes, for each bit of U, the
@@ -33,7 +33,10 @@
 // Data per vertex: seen and three rotating frontier masks (1, 4 or 8 bytes
 // for <= 8, <= 32, <= 64 sources); queues of the union frontier rotate over
 // three buffers so the frontier of level L-1 can be cleared through its
-// queue while level L runs.
+// queue while level L runs.  A push claims through the next mask alone, so
+// seen lags one level behind: at level L the vertices reached are those of
+// seen | frontier_L, and level L folds its frontier into seen (each frontier
+// vertex by the one thread that takes it; a pull while it rewrites seen).
 #pragma once
 #include <algorithm>
 #include <type_traits>
@@ -138,23 +141,26 @@ __device__ __forceinline__ M wave_or(M v) {
   return (M)x;
 }
 
-// Claims the bits of f that v has not seen; writes the parent of every bit
-// won and returns true for the one claim that makes v's next mask nonzero.
+// Claims the bits of f that v has not reached (seen | this frontier, read
+// plainly: neither changes during a push) with one atomicOr on v's next
+// mask, which settles races between claims of the same level; writes the
+// parent of every bit won and returns true for the one claim that makes v's
+// next mask nonzero.
 template <class M>
-__device__ __forceinline__ bool relax(const BatchArgs &a, M *seen, M *nmask, uint32_t u, M f, uint32_t v, M &won) {
-  const M nb = f & (M)~seen[v];
+__device__ __forceinline__ bool relax(const BatchArgs &a, const M *seen, const M *fcur, M *nmask, uint32_t u, M f,
+                                      uint32_t v, M &won) {
+  const M nb = f & (M)~(seen[v] | fcur[v]);
   if (!nb) return false;
-  const M old = mask_or<M>(seen, v, nb);
+  const M old = mask_or<M>(nmask, v, nb);
   M g = nb & (M)~old;
   if (!g) return false;
-  const M on = mask_or<M>(nmask, v, g);
   won |= g;
   while (g) {
     const int b = __builtin_ctzll((unsigned long long)g);
     g &= (M)(g - 1);
     a.parents[(uint64_t)b * a.pstride + v] = u;
   }
-  return on == 0;
+  return old == 0;
 }
 
 struct WinLds {
@@ -245,6 +251,8 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
   if (leader) a.logs[64].fdeg = wall_clock64();
 
   bool is_dense = false, have_q = true, prev_have_q = true;
+  bool ubm_ok = true;     // this frontier's union bitmap is written (seeding, or a pull before)
+  const M *flast = nullptr;  // a frontier left unfolded into seen when the traversal stops
   uint64_t prev_fs = 0, visited_deg = 0;
   int L = 0;
   for (;; L++) {
@@ -260,6 +268,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
     if (leader && L < 64)
       a.logs[L] = BLog{fs, fdeg, (unsigned long long)act, (uint32_t)dir, 0, (unsigned long long)wall_clock64()};
     if (dir == 2) {
+      flast = fmb + (uint64_t)(L % 3) * a.nm;
       L++;
       break;
     }
@@ -277,9 +286,10 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
     M *const fmask = fmb + (uint64_t)(L % 3) * a.nm, *const nmask = fmb + (uint64_t)((L + 1) % 3) * a.nm;
     uint32_t *const q = qb + (uint64_t)(L % 3) * a.qstride, *const qn = qb + (uint64_t)((L + 1) % 3) * a.qstride;
     // union bitmaps rotate with the masks: this frontier's, the next one's
-    // (whole words written by a pull, bits set by push / forward claims),
-    // and last level's, cleared now for level L+1
-    const uint64_t *const cubm = a.ubm + (uint64_t)(L % 3) * nwords;
+    // (whole words written by a pull; after a push or forward level a bitmap
+    // pull writes its frontier's words first), and last level's, cleared now
+    // for level L+1
+    uint64_t *const cubm = a.ubm + (uint64_t)(L % 3) * nwords;
     uint64_t *const nubm = a.ubm + (uint64_t)((L + 1) % 3) * nwords;
     {
       uint64_t *clr = a.ubm + (uint64_t)((L + 2) % 3) * nwords;
@@ -297,6 +307,14 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
         uint4 *c4 = reinterpret_cast<uint4 *>(clr);
         for (uint64_t i = global_tid(); i < nvec; i += grid_threads()) c4[i] = make_uint4(0, 0, 0, 0);
       }
+    }
+    if (dir == 4 && !ubm_ok) {
+      for (uint64_t w = wave; w < nwords; w += nwaves) {
+        const uint64_t vl = w * 64 + lane;
+        const uint64_t bits = __ballot(vl < a.n && fmask[vl] != 0);
+        if (lane == 0) cubm[w] = bits;
+      }
+      grid_barrier(a.bar, (unsigned)nb);
     }
     unsigned long long dsum = 0, addsize = 0;
     M won = 0;
@@ -364,6 +382,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
             f = fmask[u];
             b = a.off[u];
             e = a.off[u + 1];
+            if (lid == 0) seen[u] = seen[u] | f;  // fold u's frontier bits (u is this group's alone)
           }
           // a wave-uniform trip count (the ballots below need every lane)
           const uint64_t trips = wave_max<uint64_t>((e - b + width - 1) / width);
@@ -373,11 +392,8 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
             uint32_t tgt = 0;
             if (kk < e) {
               tgt = a.ids[kk];
-              add = (!G || tgt != kNone) && relax<M>(a, seen, nmask, u, f, tgt, won);
-              if (add) {
-                dsum += vdeg<G>(a.off, a.deg, tgt);
-                atomicOr(reinterpret_cast<unsigned long long *>(&nubm[tgt >> 6]), 1ull << (tgt & 63));
-              }
+              add = (!G || tgt != kNone) && relax<M>(a, seen, fmask, nmask, u, f, tgt, won);
+              if (add) dsum += vdeg<G>(a.off, a.deg, tgt);
             }
             const uint64_t mask = __ballot(add);
             if (mask) {
@@ -396,6 +412,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
         for (uint64_t i = s0 + threadIdx.x; i < s1; i += kBlock) {
           const uint32_t v = q[i];
           part += a.off[v + 1] - a.off[v];
+          seen[v] = seen[v] | fmask[v];  // fold v's frontier bits (each queue entry once)
         }
         part = block_sum(part, lds);
         if (threadIdx.x == 0) a.bsum[blockIdx.x] = part;
@@ -491,11 +508,8 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
                 bs = win.base[wv][lo];
               }
               tgt = a.ids[(uint64_t)(bs + (int64_t)e)];
-              add = (!G || tgt != kNone) && relax<M>(a, seen, nmask, owner, f, tgt, won);
-              if (add) {
-                dsum += vdeg<G>(a.off, a.deg, tgt);
-                atomicOr(reinterpret_cast<unsigned long long *>(&nubm[tgt >> 6]), 1ull << (tgt & 63));
-              }
+              add = (!G || tgt != kNone) && relax<M>(a, seen, fmask, nmask, owner, f, tgt, won);
+              if (add) dsum += vdeg<G>(a.off, a.deg, tgt);
             }
             tg[t] = tgt;
             ad[t] = add;
@@ -528,6 +542,9 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
       // L2) and gathers the neighbour's source mask only on a hit;
       // direction 1 gathers every neighbour's mask.
       const bool bmp = dir == 4;
+      // after a push or forward level seen lags by this frontier (claims go
+      // through the next masks alone); after a pull it is whole
+      const bool lag = !is_dense || !ubm_ok;
       // the one-neighbour round runs in dense levels after the first with up
       // to 8 sources: there most vertices finish at their first neighbour
       // (with more sources the many open bits keep them scanning, and the
@@ -543,6 +560,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
           bl = a.off[vl];
           el = a.off[vl + 1];
           s = seen[vl];
+          if (lag) s |= fmask[vl];  // this frontier's bits, not folded yet, are reached too
         }
         const M U = (M)(~s & act);
         const uint64_t todo = __ballot(U != 0);
@@ -728,7 +746,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
         }
         if (vl < a.n) {
           nmask[vl] = acc;
-          if (acc) seen[vl] = s | acc;
+          if (acc || lag) seen[vl] = s | acc;
         }
         const uint64_t nword = __ballot(acc != 0);
         if (lane == 0) nubm[w] = nword;
@@ -745,6 +763,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
       for_my_words_dyn(a.pieces, wave, nwaves, dctr, ldyn, [&](uint64_t w) {
         const uint64_t vl = w * 64 + lane;
         const M fv = vl < a.n ? fmask[vl] : (M)0;
+        if (fv) seen[vl] = seen[vl] | fv;  // fold (lane = vertex)
         uint64_t word = __ballot(fv != 0);
         while (word) {
           const int bp = __builtin_ctzll(word);
@@ -758,11 +777,8 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
             uint32_t t = 0;
             if (kk < e) {
               t = a.ids[kk];
-              add = (!G || t != kNone) && relax<M>(a, seen, nmask, u, f, t, won);
-              if (add) {
-                dsum += vdeg<G>(a.off, a.deg, t);
-                atomicOr(reinterpret_cast<unsigned long long *>(&nubm[t >> 6]), 1ull << (t & 63));
-              }
+              add = (!G || t != kNone) && relax<M>(a, seen, fmask, nmask, u, f, t, won);
+              if (add) dsum += vdeg<G>(a.off, a.deg, t);
             }
             const uint64_t mask = __ballot(add);
             if (mask) {
@@ -803,6 +819,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
     grid_barrier(a.bar, (unsigned)nb);
     prev_fs = fs;
     visited_deg += fdeg;
+    ubm_ok = dir == 1 || dir == 4;
   }
   if (leader) {
     const unsigned long long t = wall_clock64();
@@ -824,7 +841,11 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
     const uint64_t ngrp = (a.n + 15) / 16;
     for (uint64_t t = global_tid(); t < ngrp; t += grid_threads()) {
       const uint64_t v0 = t * 16;
-      const uint4 sv = reinterpret_cast<const uint4 *>(seen)[t];  // nm is a multiple of 64
+      uint4 sv = reinterpret_cast<const uint4 *>(seen)[t];  // nm is a multiple of 64
+      if (flast) {
+        const uint4 fl = reinterpret_cast<const uint4 *>(flast)[t];
+        sv = make_uint4(sv.x | fl.x, sv.y | fl.y, sv.z | fl.z, sv.w | fl.w);
+      }
       uint8_t sm[16];
       __builtin_memcpy(sm, &sv, 16);
       uint64_t d[16];
@@ -877,7 +898,7 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
       const uint64_t vl = w * 64 + lane;
       uint64_t sm = 0, d = 0;
       if (vl < a.n) {
-        sm = (uint64_t)seen[vl];
+        sm = (uint64_t)(flast ? (M)(seen[vl] | flast[vl]) : seen[vl]);
         d = vdeg<G>(a.off, a.deg, vl);
         const uint64_t kmask = a.k >= 64 ? ~0ull : ((1ull << a.k) - 1);
         for (uint64_t mm = ~sm & kmask; mm; mm &= mm - 1)
