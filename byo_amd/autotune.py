@@ -7,7 +7,7 @@ Graph.apply_tuning() later applies the entry of the closest size.  What each
 knob is timed on:
 
     coop_blocks_per_cu, pieces,  the benchmark's step exactly: its 8 sources
-    pull_dyn                     (SplitMix64 seed 1) in one shared traversal
+    pull_dyn, coop_grid_sqrt_div (SplitMix64 seed 1) in one shared traversal
                                  under the committed learned selector
                                  (launch ms, best of 7)
     ppr_blocks_per_cu            32-seed personalized PageRank, 5 iterations
@@ -30,6 +30,7 @@ CANDIDATES = {
     "coop_blocks_per_cu": [2, 3, 4],
     "pieces": [4, 8, 16, 32],
     "pull_dyn": [0, 4, 8, 16],
+    "coop_grid_sqrt_div": [0, 8, 14, 20],
     "ppr_blocks_per_cu": [4, 8],
     "ppr_pieces": [2, 4, 8],
     "pr_relabel": [0, 1],
@@ -37,7 +38,7 @@ CANDIDATES = {
     "pr_lanes": [0, 4, 8, 16, 32],
     "pr_grid": [4096, 8192, 16384, 32768, 65536],
 }
-WORKLOAD = {"coop_blocks_per_cu": "bfs", "pieces": "bfs", "pull_dyn": "bfs", "ppr_blocks_per_cu": "ppr", "ppr_pieces": "ppr", "pr_lanes": "pr",
+WORKLOAD = {"coop_blocks_per_cu": "bfs", "pieces": "bfs", "pull_dyn": "bfs", "coop_grid_sqrt_div": "bfs", "ppr_blocks_per_cu": "ppr", "ppr_pieces": "ppr", "pr_lanes": "pr",
             "pr_grid": "pr", "pr_relabel": "pr", "pr_xcd": "pr"}
 
 

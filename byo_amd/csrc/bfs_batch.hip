@@ -106,7 +106,7 @@ double launch(byo_graph *g, const uint32_t *srcs, uint32_t k, uint64_t maxdeg, B
   const bool gaps = v.deg != nullptr;
   const void *kern = gaps ? reinterpret_cast<const void *>(k_bfs_batch<M, true>)
                           : reinterpret_cast<const void *>(k_bfs_batch<M, false>);
-  const int nblocks = coop_grid(kern, kBlock, (int)g->tuning("coop_blocks_per_cu"));
+  const int nblocks = traversal_grid(g, kern, kBlock, v.nslots);
   prepare(g, v, srcs, k, maxdeg, a, (int)sizeof(M), nblocks);
   void *args[] = {&a};
   PhaseTimer t(g->stream, true);

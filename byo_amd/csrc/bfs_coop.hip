@@ -799,6 +799,14 @@ int coop_grid(const void *kernel, int block, int cap) {
   return nb;
 }
 
+int traversal_grid(byo_graph *g, const void *kernel, int block, uint64_t slots) {
+  const int nb = coop_grid(kernel, block, (int)g->tuning("coop_blocks_per_cu"));
+  const uint64_t div = (uint64_t)g->tuning("coop_grid_sqrt_div");
+  if (!div) return nb;
+  const uint64_t want = std::max<uint64_t>(8, ((uint64_t)std::sqrt((double)slots) / div + 7) / 8 * 8);
+  return (int)std::min<uint64_t>((uint64_t)nb, want);
+}
+
 Pieces make_pieces(byo_graph *g, int nblocks, PieceSet set) {
   Pieces pc;
   pc.per_wave = (uint32_t)g->tuning(set == kPprPieces ? "ppr_pieces" : "pieces");
@@ -854,7 +862,7 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   const void *kern = gaps ? reinterpret_cast<const void *>(k_bfs_coop<true>)
                          : reinterpret_cast<const void *>(k_bfs_coop<false>);
   const PieceSet pset = gaps ? kPmaPieces : kTraversalPieces;
-  const int nblocks = coop_grid(kern, kBlock, (int)g->tuning("coop_blocks_per_cu"));
+  const int nblocks = traversal_grid(g, kern, kBlock, v.nslots);
   a.pieces = make_pieces(g, nblocks, pset);
   a.bsum = g->buf("bfs_coop_bsum", (size_t)nblocks * 8 + 64).as<unsigned long long>();
   a.bar = coop_bar(g);

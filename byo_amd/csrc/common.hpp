@@ -156,6 +156,8 @@ inline const std::vector<TuneSpec> &tune_specs() {
       {"coop_blocks_per_cu", 4, 1, 4, "blocks per CU of the persistent traversal launches"},
       {"ppr_blocks_per_cu", 4, 1, 8, "blocks per CU of the persistent personalized PageRank launch"},
       {"pieces", 8, 1, 64, "weight-balanced word pieces per wave in vertex-driven passes"},
+      {"coop_grid_sqrt_div", 14, 0, 1 << 20,
+       "traversal grid at most sqrt(adjacency slots) / this blocks, a multiple of 8 (0: the whole resident grid)"},
       {"pull_dyn", 8, 0, 64,
        "word pieces per wave dealt dynamically (the tail of the share-out) in the shared traversal's dense levels"},
       {"ppr_pieces", 4, 1, 16, "weight-balanced word pieces per wave in the personalized PageRank sweeps"},

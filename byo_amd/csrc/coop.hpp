@@ -313,6 +313,13 @@ DirCtl dir_ctl(byo_graph *g, uint64_t maxdeg, hipStream_t s, bool batched);
 // Grid size of a cooperative kernel: CUs x min(resident blocks per CU, cap)
 // (cap from the handle's tuning, at most 8; at most 2048 blocks).
 int coop_grid(const void *kernel, int block, int cap);
+// The traversal grid for an adjacency of `slots` entries: coop_grid, cut to
+// sqrt(slots) / coop_grid_sqrt_div blocks rounded up to a multiple of 8 (one
+// per XCD) when that knob is set.  A level costs about a * blocks (its grid
+// barrier and per-block setup) + b * slots / blocks (its work), which is
+// least at blocks ~ sqrt(slots); measured optima: ~120 blocks at RMAT-16,
+// 248 at 18, 504 at 20, the whole resident grid (768) from 22 on.
+int traversal_grid(byo_graph *g, const void *kernel, int block, uint64_t slots);
 
 // Word pieces of a cooperative launch of nblocks blocks, for the traversal
 // passes (buffer "bfs_pieces", knob "pieces") or the personalized PageRank

@@ -30,7 +30,8 @@ def check_bfs(g, G, srcs):
 
 
 @pytest.mark.parametrize("key,values", [("coop_blocks_per_cu", [1, 2, 3, 4]), ("pieces", [1, 4, 16, 32, 64]),
-                                        ("pull_dyn", [0, 1, 4, 16, 64]), ("bfs_engine", [0, 1, 2])])
+                                        ("pull_dyn", [0, 1, 4, 16, 64]), ("coop_grid_sqrt_div", [0, 1, 14, 1000]),
+                                        ("bfs_engine", [0, 1, 2])])
 def test_bfs_knobs(B, s15, key, values):
     g, G = s15
     srcs = O.bfs_sources(g, 6, seed=4)
@@ -43,7 +44,7 @@ def test_bfs_knobs(B, s15, key, values):
             par, _ = G.bfs(srcs[0])
             assert O.check_bfs_tree(g, srcs[0], par, O.bfs_levels(g, srcs[0])) == 0
     finally:
-        G.set_tuning(key, {"coop_blocks_per_cu": 4, "pieces": 8, "pull_dyn": 8, "bfs_engine": 0}[key])
+        G.set_tuning(key, {"coop_blocks_per_cu": 4, "pieces": 8, "pull_dyn": 8, "coop_grid_sqrt_div": 14, "bfs_engine": 0}[key])
 
 
 @pytest.mark.parametrize("key,values", [("pr_lanes", [4, 8, 16, 32, 0]), ("pr_grid", [256, 2048, 16384]),
