@@ -68,7 +68,6 @@ struct CoopArgs {
   uint32_t *q0, *q1;
   uint64_t *bm;              // three rotating bitmaps of nw64 words
   uint64_t *visited64;
-  uint8_t *mark;            // n bytes (+ padding to 16), zero between levels
   uint64_t *scan;
   uint32_t *cstart;         // first frontier index of each 256-edge chunk
   unsigned long long *bsum;
@@ -172,21 +171,22 @@ __device__ __forceinline__ void claim_staged(const uint64_t *visited, const uint
 // Push levels with at least this many frontier edges mark instead of
 // claiming.  The bitmap claim costs an L2 atomic per candidate edge, and
 // those atomics queue on the words they share (64 vertices a word, hubs
-// among them): 3.8 M candidates of an RMAT-22 level took 0.3 ms.  A mark is
-// a plain byte store, the same for every writer, and the parent a plain
-// store of whichever frontier neighbour stores last (each is a valid BFS
-// parent: the target is undiscovered and the neighbour in the frontier);
-// one pass over the marks then builds the next bitmap and queue.
+// among them): 3.8 M candidates of an RMAT-22 level took 0.3 ms.  The mark
+// is the parent itself: a plain store of whichever frontier neighbour
+// stores last (each is a valid BFS parent: the target is undiscovered and
+// the neighbour in the frontier), into a parent array that holds kNone for
+// every undiscovered vertex; one pass over the parents of the vertices not
+// yet discovered then builds the next bitmap and queue.
 constexpr uint64_t kMarkMinEdges = 1ull << 20;
 
 // The mark counterpart of claim_staged: targets that are in neither the
-// visited bitmap nor the current frontier get mark[t] = 1 and
-// parents[t] = their owner.  The LDS filter skips targets this block has
-// marked already (and any claimed before).
+// visited bitmap nor the current frontier get parents[t] = their owner.  The
+// LDS filter skips targets this block has marked already (and any claimed
+// before).
 template <int K, bool G>
-__device__ __forceinline__ void mark_staged(const uint64_t *visited, const uint64_t *fbm, uint8_t *mark,
-                                            uint32_t *parents, const uint32_t (&tg)[K], const uint32_t (&own)[K],
-                                            bool (&ok)[K], uint32_t *filt) {
+__device__ __forceinline__ void mark_staged(const uint64_t *visited, const uint64_t *fbm, uint32_t *parents,
+                                            const uint32_t (&tg)[K], const uint32_t (&own)[K], bool (&ok)[K],
+                                            uint32_t *filt) {
   uint64_t x[K];
 #pragma unroll
   for (int t = 0; t < K; t++) {
@@ -201,20 +201,21 @@ __device__ __forceinline__ void mark_staged(const uint64_t *visited, const uint6
 #pragma unroll
   for (int t = 0; t < K; t++) {
     if (ok[t] && !((x[t] >> (tg[t] & 63)) & 1)) {
-      mark[tg[t]] = 1;
       parents[tg[t]] = own[t];
       filt[filt_slot(tg[t])] = tg[t];
     }
   }
 }
 
-// After a marking push: every wave takes runs of 32 bitmap words, reads the
-// marks of their 2048 vertices (lane = vertex), clears them, writes the next
-// frontier's bitmap words whole and appends the marked vertices to the next
-// queue (in id order within a run, one cursor atomic per run).  Returns the
-// lane's share of the next frontier's degree sum.
+// After a marking push: every wave takes runs of 32 bitmap words; a vertex
+// (lane) of a run is new when it has a parent but is neither visited nor in
+// the current frontier; the pass writes the next frontier's bitmap words
+// whole and appends the new vertices to the next queue (in id order within a
+// run, one cursor atomic per run).  Returns the lane's share of the next
+// frontier's degree sum.
 template <bool G>
-__device__ __forceinline__ unsigned long long pack_marks(const CoopArgs &a, uint64_t *nbm, uint32_t *qn,
+__device__ __forceinline__ unsigned long long pack_marks(const CoopArgs &a, const uint32_t *parents,
+                                                         const uint64_t *fbm, uint64_t *nbm, uint32_t *qn,
                                                          unsigned long long *cursor, uint64_t nwords, uint64_t wave,
                                                          uint64_t nwaves, int lane) {
   constexpr int kRun = 32;
@@ -226,12 +227,10 @@ __device__ __forceinline__ unsigned long long pack_marks(const CoopArgs &a, uint
       const uint64_t w = r0 + st;
       if (w >= nwords) break;
       const uint64_t v = w * 64 + lane;
-      const bool on = v < a.n && a.mark[v] != 0;
+      const uint64_t old = a.visited64[w] | fbm[w];
+      const bool on = v < a.n && parents[v] != kNone && !((old >> lane) & 1);
       const uint64_t word = __ballot(on);
-      if (on) {
-        a.mark[v] = 0;
-        dsum += vdeg<G>(a.off, a.deg, v);
-      }
+      if (on) dsum += vdeg<G>(a.off, a.deg, v);
       if (lane == st) mine = word;
       if (lane == 0) nbm[w] = word;
     }
@@ -308,10 +307,6 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
     // vertices, so the padding of the last word must already be zero.
     for (uint64_t i = global_tid(); i < a.n; i += grid_threads()) parents[i] = (i == src) ? src : kNone;
     for (int i = threadIdx.x; i < kFilt; i += kBlock) filt[i] = kNone;  // synchronised by the barrier below
-    {
-      uint4 *m4 = reinterpret_cast<uint4 *>(a.mark);
-      for (uint64_t i = global_tid(); i < (a.n + 15) / 16; i += grid_threads()) m4[i] = make_uint4(0, 0, 0, 0);
-    }
     const uint64_t sw = src >> 6, sbit = 1ull << (src & 63);
     for (uint64_t i = global_tid(); i < a.nw64; i += grid_threads()) {
       a.visited64[i] = (i == sw) ? sbit : 0;
@@ -543,7 +538,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
           if (marks) {
             // large frontier: mark, do not claim (the pack pass below builds
             // the next frontier from the marks)
-            mark_staged<kPerLane, G>(a.visited64, fbm, a.mark, parents, tg, own, wn, filt);
+            mark_staged<kPerLane, G>(a.visited64, fbm, parents, tg, own, wn, filt);
             wave_lds_sync();  // the window is rewritten by the next chunk
             continue;
           }
@@ -572,7 +567,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 
         }
         if (marks) {
           grid_barrier(a.bar, (unsigned)nb);
-          dsum += pack_marks<G>(a, nbm, qn, &nx->size, nwords, wave, nwaves, lane);
+          dsum += pack_marks<G>(a, parents, fbm, nbm, qn, &nx->size, nwords, wave, nwaves, lane);
         }
         dsum = wave_sum(dsum);
         block_add2(nullptr, 0, &nx->deg, dsum, lane);
@@ -848,7 +843,6 @@ void bfs_coop_run(byo_graph *g, const uint32_t *srcs, uint32_t k, bool keep_all,
   a.q1 = g->buf("bfs_qn", n * 4 + 64).as<uint32_t>();
   a.bm = g->buf("bfs_bms", 3 * a.nw64 * 8).as<uint64_t>();
   a.visited64 = g->buf("bfs_visited", a.nw64 * 8).as<uint64_t>();
-  a.mark = g->buf("bfs_mark", (n + 15) / 16 * 16 + 64).as<uint8_t>();
   a.scan = g->buf("bfs_scan", (n + 1) * 8 + 64).as<uint64_t>();
   a.cstart = g->buf("bfs_cstart", (v.nslots / kChunk + 2) * 4 + 64).as<uint32_t>();
   a.ctr = g->buf("bfs_coop_ctr", 3 * sizeof(Ctr) + 64).as<Ctr>();
