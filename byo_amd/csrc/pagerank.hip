@@ -186,8 +186,9 @@ template <class T, bool kComp, bool H = false>
 __global__ void __launch_bounds__(256) k_pr_tiles(const uint64_t *off, const uint32_t *ids, uint64_t n,
                                                   const T *contrib_in, const T *p_old, T *p_new, T *contrib_out,
                                                   double damping, double addc, double *partial,
-                                                  const uint32_t *degs = nullptr) {
+                                                  const uint32_t *degs = nullptr, const uint32_t *stop = nullptr) {
   __shared__ double lds[4];
+  if (stop && *stop) return;  // a batched iteration after convergence
   const int lane = lane_id();
   const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
   const uint64_t ntiles = (n + 63) >> 6;
@@ -237,7 +238,8 @@ constexpr int kXcdUnroll = 2;  // gathers in flight per lane in a slice's short 
 
 template <class T, bool kComp>
 __global__ void __launch_bounds__(256) k_pr_xcd(const uint32_t *so, const uint32_t *ids_s, uint64_t n,
-                                                const T *contrib_in, T *part_s, T *part_c) {
+                                                const T *contrib_in, T *part_s, T *part_c, const uint32_t *stop) {
+  if (stop && *stop) return;  // a batched iteration after convergence
   const int lane = lane_id();
   const uint32_t x = blockIdx.x % kXcds;
   const uint64_t wave = (uint64_t)(blockIdx.x / kXcds) * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -264,8 +266,9 @@ template <class T, bool kComp>
 __global__ void __launch_bounds__(256) k_pr_xcd_combine(const T *part_s, const T *part_c, const uint64_t *off,
                                                         uint64_t n, const T *p_old, T *p_new, T *contrib_out,
                                                         double damping, double addc, double *partial,
-                                                        const uint32_t *degs) {
+                                                        const uint32_t *degs, const uint32_t *stop) {
   __shared__ double lds[4];
+  if (stop && *stop) return;  // a batched iteration after convergence
   const T d = (T)damping, t = (T)addc;
   double resid = 0.0;
   for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
@@ -461,9 +464,13 @@ __global__ void k_pr_bounds(const uint64_t *off, const uint32_t *ids, uint64_t n
   }
 }
 
-__global__ void k_sum_doubles(const double *partial, int nb, double *out) {
+// With stop: an iteration after the one that converged returns at once, and
+// the first L1 below eps records its iteration count (it1) in *stop.
+__global__ void k_sum_doubles(const double *partial, int nb, double *out, uint32_t *stop = nullptr, double eps = 0.0,
+                              uint32_t it1 = 0) {
   // one block, fixed association order
   __shared__ double lds[1024];
+  if (stop && *stop) return;
   double s = 0.0;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[i];
   lds[threadIdx.x] = s;
@@ -472,7 +479,10 @@ __global__ void k_sum_doubles(const double *partial, int nb, double *out) {
     if ((int)threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = lds[0];
+  if (threadIdx.x == 0) {
+    *out = lds[0];
+    if (stop && lds[0] < eps) *stop = it1;
+  }
 }
 
 template <class T>
@@ -636,21 +646,30 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   if (S > 1) k_pr_bounds<<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, ids, n, S, W, pos);
   BYO_LAUNCH_CHECK();
   double *hl1 = static_cast<double *>(pinned(g, 64));
+  // With eps > 0 the iterations are queued in batches of kSpec with a device
+  // stop word: the residual kernel of the first iteration below eps records
+  // the iteration count there, the iterations queued after it return at once,
+  // and the host reads the word once per batch instead of waiting for every
+  // residual (the segmented pull keeps the per-iteration check).
+  constexpr int kSpec = 8;
+  const bool spec = eps > 0 && S == 1;
+  uint32_t *stop = spec ? g->buf("pr_stop", 64).as<uint32_t>() : nullptr;
+  if (spec) prim::fill_u32(stop, 0, 1, s);
   int it = 0, cur = 0;
   while (it < max_iters) {
     PhaseTimer it_t(s, g->prof.on);
     unsigned nparts = grid;
     if (per_xcd) {
-      k_pr_xcd<T, kComp><<<grid / kXcds * kXcds, 256, 0, s>>>(xs.so, xs.ids_s, n, c[cur], part_s, part_c);
+      k_pr_xcd<T, kComp><<<grid / kXcds * kXcds, 256, 0, s>>>(xs.so, xs.ids_s, n, c[cur], part_s, part_c, stop);
       k_pr_xcd_combine<T, kComp><<<cgrid, 256, 0, s>>>(part_s, part_c, off, n, p[cur], p[cur ^ 1], c[cur ^ 1],
-                                                       damping, addc, partial, av.deg);
+                                                       damping, addc, partial, av.deg, stop);
       nparts = cgrid;
     } else if (pma) {
       k_pr_tiles<T, kComp, true><<<grid, 256, 0, s>>>(off, ids, n, c[cur], p[cur], p[cur ^ 1], c[cur ^ 1], damping,
-                                                      addc, partial, av.deg);
+                                                      addc, partial, av.deg, stop);
     } else if (per_tile) {
       k_pr_tiles<T, kComp><<<grid, 256, 0, s>>>(off, ids, n, c[cur], p[cur], p[cur ^ 1], c[cur ^ 1], damping, addc,
-                                                partial);
+                                                partial, nullptr, stop);
     } else {
       for (int sg = 0; sg < S; sg++) {
         const uint64_t *lo = sg == 0 ? off : pos + (uint64_t)(sg - 1) * n;
@@ -659,12 +678,23 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
                             p[cur ^ 1], c[cur ^ 1], damping, addc, partial, grid, s);
       }
     }
-    k_sum_doubles<<<1, 1024, 0, s>>>(partial, nparts, l1dev + it);
+    k_sum_doubles<<<1, 1024, 0, s>>>(partial, nparts, l1dev + it, stop, eps, (uint32_t)(it + 1));
     BYO_LAUNCH_CHECK();
     if (g->prof.on) g->account(phase, it_t.stop());
     cur ^= 1;
     it++;
-    if (eps > 0) {
+    if (spec) {
+      if (it % kSpec == 0 || it == max_iters) {
+        uint32_t *hs = reinterpret_cast<uint32_t *>(hl1);
+        BYO_HIP(hipMemcpyAsync(hs, stop, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        BYO_HIP(hipStreamSynchronize(s));
+        if (*hs) {
+          it = (int)*hs;  // the iterate of iteration it is in p[it % 2]
+          cur = it & 1;
+          break;
+        }
+      }
+    } else if (eps > 0) {
       BYO_HIP(hipMemcpyAsync(hl1, l1dev + it - 1, sizeof(double), hipMemcpyDeviceToHost, s));
       BYO_HIP(hipStreamSynchronize(s));
       if (*hl1 < eps) break;
