@@ -20,6 +20,27 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BYO
 constexpr size_t kLogBytes = 65 * sizeof(BLog), kLastOff = kLogBytes, kNlevOff = kLastOff + 64 * 4,
                  kTotOff = kNlevOff + 8, kOutBytes = kTotOff + 64 * 16;
 
+namespace {
+// fn[v] = the first entry of v's list (a PMA list starts with a key), kNone
+// for an empty list.
+__global__ void k_first_ids(const uint64_t *off, const uint32_t *ids, uint64_t n, uint32_t *fn) {
+  for (uint64_t v = global_tid(); v < n; v += grid_threads()) fn[v] = off[v] < off[v + 1] ? ids[off[v]] : kNone;
+}
+}  // namespace
+
+// The first-neighbour array of view v, built once per adjacency.
+const uint32_t *first_ids(byo_graph *g, const AdjView &v) {
+  uint32_t *fn = g->buf("bfs_first_ids", g->n * 4 + 64).as<uint32_t>();
+  const int view = v.deg ? 1 : 0;
+  if (!g->fn_valid || g->fn_view != view) {
+    k_first_ids<<<byo_grid(g->n, 256, 16384), 256, 0, g->stream>>>(v.off, v.ids, g->n, fn);
+    BYO_LAUNCH_CHECK();
+    g->fn_valid = true;
+    g->fn_view = view;
+  }
+  return fn;
+}
+
 void prepare(byo_graph *g, const AdjView &v, const uint32_t *srcs, uint32_t k, uint64_t maxdeg, BatchArgs &a,
              int mask_bytes, int nblocks) {
   BYO_REQUIRE(k >= 1 && k <= 64, BYO_E_INVAL, "batched BFS takes 1..64 sources per launch");
@@ -29,6 +50,7 @@ void prepare(byo_graph *g, const AdjView &v, const uint32_t *srcs, uint32_t k, u
   a.off = v.off;
   a.ids = v.ids;
   a.deg = v.deg;
+  a.fn = first_ids(g, v);
   a.n = n;
   a.m = g->m;
   a.nm = (n + 63) / 64 * 64;

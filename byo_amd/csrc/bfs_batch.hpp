@@ -63,6 +63,7 @@ struct BatchArgs {
   const uint64_t *off;  // adjacency view (AdjView): CSR, or the PMA's slot starts
   const uint32_t *ids;
   const uint32_t *deg;  // true degrees when the lists hold empty slots
+  const uint32_t *fn;   // first neighbour of every vertex (kNone without one), n entries
   uint64_t n, m;
   uint64_t nm;              // mask entries (n rounded up to 64)
   uint32_t *parents;        // parents of source j at parents + j * pstride
@@ -556,11 +557,13 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
         const uint64_t vl = w * 64 + lane;
         uint64_t bl = 0, el = 0;
         M s = (M)~(M)0;
+        uint32_t f1 = kNone;
         if (vl < a.n) {
           bl = a.off[vl];
           el = a.off[vl + 1];
           s = seen[vl];
           if (lag) s |= fmask[vl];  // this frontier's bits, not folded yet, are reached too
+          if (first1) f1 = a.fn[vl];  // loaded with the word (coalesced), not after its offsets
         }
         const M U = (M)(~s & act);
         const uint64_t todo = __ballot(U != 0);
@@ -571,13 +574,9 @@ __device__ __forceinline__ void bfs_batch_body(const BatchArgs &a) {
           // the 8-wide probe below then gathers only for the others
           M h1 = 0;
           if (first1) {
-            const bool p1 = ((todo >> lane) & 1) && el > bl;
-            uint32_t u1 = a.ids[p1 ? bl : 0];
+            const bool p1 = ((todo >> lane) & 1) && f1 != kNone;
+            const uint32_t u1 = p1 ? f1 : 0;
             h1 = p1 ? U : (M)0;
-            if (G && u1 == kNone) {
-              u1 = 0;
-              h1 = 0;
-            }
             if (bmp && !((cubm[u1 >> 6] >> (u1 & 63)) & 1)) h1 = 0;
             h1 &= (!bmp || h1) ? fmask[u1] : (M)0;
             M nw = h1;

@@ -201,7 +201,14 @@ struct byo_graph {
     int nblocks = 0;
     uint32_t per_wave = 0;
   } pcache[3];
-  void pieces_stale() { pcache[0].valid = pcache[1].valid = pcache[2].valid = false; }
+  // after any change of the adjacency: the word pieces and the
+  // first-neighbour array are rebuilt by their next user
+  void pieces_stale() {
+    pcache[0].valid = pcache[1].valid = pcache[2].valid = false;
+    fn_valid = false;
+  }
+  bool fn_valid = false;      // "bfs_first_ids" holds the first neighbours of view fn_view (0 CSR, 1 PMA)
+  int fn_view = 0;
   bool xs_valid = false;      // PageRank per-XCD slices are current (for layout xs_layout)
   int xs_layout = 0;
   int64_t last_src = -1;      // last source of byo_bfs / byo_bfs_multi
