@@ -224,6 +224,44 @@ def test_pagerank_eps_stop_matches_oracle(B, s16):
     assert np.abs(p - p_ref).sum() < 1e-12
 
 
+@pytest.mark.parametrize("layout", ["plain", "relabel_xcd", "cpma"])
+@pytest.mark.parametrize("eps,iters", [(1e-6, 100), ("at16", 100), (1e-4, 9), (1e-6, 8), (1e-12, 17)])
+def test_pagerank_eps_batches_stop_where_the_oracle_does(B, s16, layout, eps, iters):
+    """Iterations to a residual bound are queued 8 at a time with a device
+    stop word: the iteration count, the residual log and the vector are
+    those of stopping at the first residual below eps (or at iters) -- for
+    stops inside a batch, at a batch end (at16: an eps between the 15th and
+    16th residuals), and at the iteration limit."""
+    g, G = s16
+    if eps == "at16":
+        _, l1_all = O.pagerank(g, 0.85, 30, 0.0)
+        eps = float((l1_all[14] + l1_all[15]) / 2)
+    p_ref, l1_ref = O.pagerank(g, 0.85, iters, eps)
+    if iters == 100 and eps > 1e-6:
+        assert len(l1_ref) == 16
+    if layout == "cpma":
+        H = B.CpmaGraph.from_csr(g.offsets, g.ids)
+    else:
+        H = G
+        H.set_tuning("pr_relabel", 1 if layout == "relabel_xcd" else 0)
+        H.set_tuning("pr_xcd", 1 if layout == "relabel_xcd" else 0)
+    try:
+        for prec, tol in ((B.PR_FP64, 1e-12), (B.PR_FP32, 1e-6)):
+            p, l1, _ = H.pagerank(0.85, iters, eps, prec)
+            assert len(l1) == len(l1_ref), (layout, prec, len(l1), len(l1_ref))
+            assert np.abs(p - p_ref).sum() <= tol
+            if prec == B.PR_FP64:
+                assert np.allclose(l1, l1_ref, rtol=1e-9, atol=1e-15)
+            p2, l2, _ = H.pagerank(0.85, iters, eps, prec)  # the stop word is cleared per call
+            assert len(l2) == len(l1) and np.array_equal(p2, p)
+    finally:
+        if layout == "cpma":
+            H.close()
+        else:
+            H.set_tuning("pr_relabel", 0)
+            H.set_tuning("pr_xcd", 0)
+
+
 def test_ppr_matches_oracle(B, s16):
     g, G = s16
     seeds = O.bfs_sources(g, 5, seed=11)
