@@ -167,15 +167,23 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_hist(const uint64_t *keys,
 
 // Stable scatter: wave w of the block owns keys [tile + w*1024, +1024),
 // visited in 16 rounds of 64 consecutive keys, so ranks follow input order.
+// The tile is first ordered by digit in LDS (its 4096 keys, stable), then
+// written out by consecutive threads, so the keys of one digit leave as runs
+// of consecutive addresses (about 16 keys, one 128-byte line, per digit and
+// tile) instead of one scattered 8-byte store per key.
 __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const uint64_t *keys, uint64_t *out,
                                                                uint64_t n, int shift,
                                                                const uint32_t *base_tbl,
                                                                uint32_t ntiles) {
   __shared__ uint32_t cnt[kWavesPerSortBlock][256];
+  __shared__ uint32_t dstart[256];  // first position of each digit in the tile's sorted order
+  __shared__ uint32_t scan_w[kWavesPerSortBlock + 1];
+  __shared__ uint64_t sorted[kSortTile];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int d = lane; d < 256; d += 64) cnt[wave][d] = 0;
   __syncthreads();
-  const uint64_t wbase = blockIdx.x * kSortTile + (uint64_t)wave * 64 * kSortPerThread;
+  const uint64_t tile0 = blockIdx.x * kSortTile;
+  const uint64_t wbase = tile0 + (uint64_t)wave * 64 * kSortPerThread;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t k_reg[kSortPerThread];
   uint32_t r_reg[kSortPerThread];
@@ -200,22 +208,30 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const uint64_t *ke
     r_reg[r] = valid ? rank : 0xFFFFFFFFu;
   }
   __syncthreads();
-  if (threadIdx.x < 256) {
-    uint32_t run = 0;
-    for (int w = 0; w < kWavesPerSortBlock; w++) {
-      uint32_t t = cnt[w][threadIdx.x];
-      cnt[w][threadIdx.x] = run;
-      run += t;
-    }
+  // per digit (thread = digit): wave offsets within the digit, then the
+  // digit's start in the tile's sorted order
+  uint32_t run = 0;
+  for (int w = 0; w < kWavesPerSortBlock; w++) {
+    uint32_t t = cnt[w][threadIdx.x];
+    cnt[w][threadIdx.x] = run;
+    run += t;
   }
+  uint32_t tot;
+  dstart[threadIdx.x] = block_excl_scan<uint32_t>(run, scan_w, &tot);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kSortPerThread; r++) {
     if (r_reg[r] != 0xFFFFFFFFu) {
       uint32_t d = (uint32_t)(k_reg[r] >> shift) & 0xFF;
-      uint64_t pos = (uint64_t)base_tbl[(uint64_t)d * ntiles + blockIdx.x] + cnt[wave][d] + r_reg[r];
-      out[pos] = k_reg[r];
+      sorted[dstart[d] + cnt[wave][d] + r_reg[r]] = k_reg[r];
     }
+  }
+  __syncthreads();
+  // position i of the sorted tile (digit d) goes to base_tbl[d] + (i - dstart[d])
+  for (uint32_t i = threadIdx.x; i < tot; i += kSortBlock) {
+    const uint64_t key = sorted[i];
+    const uint32_t d = (uint32_t)(key >> shift) & 0xFF;
+    out[(uint64_t)base_tbl[(uint64_t)d * ntiles + blockIdx.x] + (i - dstart[d])] = key;
   }
 }
 
