@@ -295,9 +295,12 @@ __device__ __forceinline__ void block_tally(UpdCtl *ctl, unsigned long long chan
 }
 
 // Pass 1 of an insert: count the segment keys absent from their leaf.
+// done (optional): segments merged already by the one-pass merge (their new
+// keys are in the leaf: 0 counted, nothing loaded).
 __global__ void __launch_bounds__(256) k_ins_count(const uint64_t *keys, UpdDev d, const uint32_t *starts,
                                                    const uint32_t *leaf_of, Leaves lv,
-                                                   const uint32_t *cnt, uint32_t *newc, UpdCtl *ctl) {
+                                                   const uint32_t *cnt, uint32_t *newc, UpdCtl *ctl,
+                                                   const uint8_t *done = nullptr) {
   __shared__ uint64_t stage[kUpdWaves][kLeaf];
   if (d.halt && *d.halt) return;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
@@ -310,6 +313,13 @@ __global__ void __launch_bounds__(256) k_ins_count(const uint64_t *keys, UpdDev 
   for (; j < nseg; j += stride) {
     uint64_t sn = 0, en = 0;
     seg_bounds(starts, nseg, b, j + stride, sn, en);
+    if (done && done[j]) {
+      if (lane == 0) newc[j] = 0;
+      s = sn;
+      e = en;
+      l = (j + stride < nseg) ? leaf_of[sn] : 0;
+      continue;
+    }
     const uint32_t c = cnt[l];
     uint32_t h[kLeaf / 64], lo[kLeaf / 64];
     load_leaf(lv, (uint64_t)l * kLeaf, h, lo, lane);
@@ -338,10 +348,14 @@ __global__ void __launch_bounds__(256) k_ins_count(const uint64_t *keys, UpdDev 
 }
 
 // Pass 2 of an insert (no leaf overflows): merge in LDS and write back.
+// With done: first == true is the one-pass merge, run before any count: it
+// merges every segment whose leaf holds room for all of the segment's keys
+// (c + segment length <= kLeaf, so it cannot overflow, whatever the count)
+// and marks it done; first == false skips the done segments.
 __global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, UpdDev d, const uint32_t *starts,
                                                    const uint32_t *leaf_of, Leaves lv, uint32_t *cnt,
                                                    uint32_t *deg, UpdCtl *ctl, const uint32_t *cand,
-                                                   uint64_t nleaves) {
+                                                   uint64_t nleaves, uint8_t *done = nullptr, bool first = false) {
   // one leaf-sized LDS array per wave: the leaf's c keys at its start, the
   // absent batch keys after them (c + their number <= kLeaf: no leaf
   // overflows here), so 8 workgroups fit a CU instead of 5
@@ -373,6 +387,15 @@ __global__ void __launch_bounds__(256) k_ins_merge(const uint64_t *keys, UpdDev 
     uint32_t c = 0, na = 0;
     uint32_t h[kLeaf / 64], lo[kLeaf / 64];
     uint64_t k[kLeaf / 64];
+    if (done) {
+      if (first) {
+        const bool fits = (uint64_t)cnt[l] + (e - s) <= kLeaf;
+        if (lane == 0) done[j] = fits ? 1 : 0;
+        skip = !fits;
+      } else {
+        skip = skip || done[j];
+      }
+    }
     if (!skip) {
       c = cnt[l];
       load_leaf(lv, leaf, h, lo, lane);
@@ -1097,6 +1120,7 @@ struct UpdBufs {
   uint64_t *k0, *k1;
   void *sort_tmp, *sel_tmp, *scan_tmp;
   uint32_t *leaf, *flag, *pos, *starts, *newc;
+  uint8_t *done;  // per segment: merged by the one-pass merge
 };
 
 static UpdBufs upd_bufs(byo_graph *g, uint64_t cap) {
@@ -1111,6 +1135,7 @@ static UpdBufs upd_bufs(byo_graph *g, uint64_t cap) {
   u.pos = g->buf("upd_pos", (cap + 1) * 4 + 64).as<uint32_t>();
   u.starts = g->buf("upd_starts", cap * 4 + 64).as<uint32_t>();
   u.newc = g->buf("upd_newc", cap * 4 + 64).as<uint32_t>();
+  u.done = g->buf("upd_done", cap + 64).as<uint8_t>();
   return u;
 }
 
@@ -1231,8 +1256,13 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
   const UpdDev d{&ctl->pad[0], u.pos + b, nullptr};
   const unsigned grid = (unsigned)std::min<uint64_t>(b, kUpdGrid);
   if (insert) {
+    // one pass for the segments whose leaf has room for all their keys; the
+    // count (and, on an overflow, the windows) and the merge then cover the
+    // rest (the first pass writes nothing when an id is bad)
+    k_ins_merge<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, leaves(c), c->cnt.as<uint32_t>(),
+                                     c->deg.as<uint32_t>(), ctl, nullptr, c->nleaves, u.done, true);
     k_ins_count<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, leaves(c), c->cnt.as<uint32_t>(),
-                                     u.newc, ctl);
+                                     u.newc, ctl, u.done);
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
     BYO_HIP(hipStreamSynchronize(s));
@@ -1248,7 +1278,7 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
       cand = w.cand;
     }
     k_ins_merge<<<grid, 256, 0, s>>>(batch, d, u.starts, u.leaf, leaves(c), c->cnt.as<uint32_t>(),
-                                     c->deg.as<uint32_t>(), ctl, cand, c->nleaves);
+                                     c->deg.as<uint32_t>(), ctl, cand, c->nleaves, u.done, false);
     BYO_LAUNCH_CHECK();
     BYO_HIP(hipMemcpyAsync(hctl, ctl, sizeof(UpdCtl), hipMemcpyDeviceToHost, s));
     BYO_HIP(hipStreamSynchronize(s));
@@ -1260,7 +1290,10 @@ static void cpma_update(byo_graph *g, const uint64_t *keys_dev, uint64_t b, bool
       if (hctl->changed) cpma_changed(g);
     } else {
       // global rebalance: flatten, merge with the batch, dedupe, respread
+      // (the leaves hold the one-pass merge's keys already: counted first)
       c->rebalances++;
+      c->total += hctl->changed;
+      c->touched += hctl->pad[4];
       const uint64_t tot = c->total + bu;
       uint64_t *flat = g->buf("cpma_flat", c->total * 8 + 64).as<uint64_t>();
       uint64_t *merged = g->buf("cpma_merged", tot * 8 + 64).as<uint64_t>();
