@@ -205,8 +205,8 @@ struct DirCtl {
 
 struct MlpLds {
   float w[BYO_MLP_NPARAM + 12];  // the selector's parameters and normalisation, staged once per launch
-  float z[3];  // the logits of the last decision
-  bool in_range;
+  float z[2][3];  // the logits of the last two decisions (by level parity)
+  bool in_range[2];
 };
 
 // Stages the selector's parameters in the block's LDS (every thread calls
@@ -253,11 +253,18 @@ __device__ inline int decide(const DirCtl &a, bool is_dense, uint64_t fs, uint64
   // wrong dense choice scans every unvisited edge, a wrong sparse one only
   // the frontier's): then there is nothing for the network to decide
   if (rule == 0 && !a.densify) return rule;
+  // the network decides where the rule turns dense (the level after sparse
+  // ones: pull now, later, or from the bitmap); once the traversal is dense
+  // the rule's pull stands, and the network is not evaluated again
+  if (is_dense && rule == 1 && !a.densify) return rule;
   const float *p = s.w;  // staged by mlp_stage
   // one wave evaluates the network (lanes 0-31 hold the hidden units, the
-  // second half repeats them; hidden values travel by shuffles), then one
-  // barrier publishes the logits to the block
-  __syncthreads();  // every thread has read the previous call's logits
+  // second half repeats them), reading the hidden values with v_readlane
+  // (scalar, no LDS round trip); the logits go to the LDS buffer of this
+  // level's parity, so no barrier is needed before writing them (the
+  // previous decide's readers are separated from this one by the grid
+  // barrier ending that level), and one barrier publishes them
+  const int par = level & 1;
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x, j = lane & 31;
     const double avg = a.n ? (double)a.m / (double)a.n : 0.0;
@@ -277,26 +284,33 @@ __device__ inline int decide(const DirCtl &a, bool is_dense, uint64_t fs, uint64
     for (int i = 0; i < 6; i++) h1 += p[j * 6 + i] * x[i];
     h1 = h1 > 0.f ? h1 : 0.f;
     float h2 = p[1248 + j];
-#pragma unroll 8
-    for (int i = 0; i < 32; i++) h2 += p[224 + i * 32 + j] * __shfl(h1, i, 64);
-    h2 = h2 > 0.f ? h2 : 0.f;
-    float z[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) z[c] = p[1376 + c] + wave_sum(lane < 32 ? p[1280 + c * 32 + lane] * h2 : 0.f);
+    for (int i = 0; i < 32; i++)
+      h2 += p[224 + i * 32 + j] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h1), i));
+    h2 = h2 > 0.f ? h2 : 0.f;
+    // every lane sums the 32 hidden units of each output in unit order
+    float z0 = p[1376], z1 = p[1377], z2 = p[1378];
+#pragma unroll
+    for (int i = 0; i < 32; i++) {
+      const float hi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h2), i));
+      z0 += p[1280 + i] * hi;
+      z1 += p[1312 + i] * hi;
+      z2 += p[1344 + i] * hi;
+    }
     if (lane == 0) {
-      s.z[0] = z[0];
-      s.z[1] = z[1];
-      s.z[2] = z[2];
-      s.in_range = in_range;
+      s.z[par][0] = z0;
+      s.z[par][1] = z1;
+      s.z[par][2] = z2;
+      s.in_range[par] = in_range;
     }
   }
   __syncthreads();
-  const float z0 = s.z[0], z1 = s.z[1], z2 = s.z[2];
+  const float z0 = s.z[par][0], z1 = s.z[par][1], z2 = s.z[par][2];
   if (!isfinite(z0 + z1 + z2)) return rule;
   const int best = (z1 > z0) ? ((z2 > z1) ? 2 : 1) : ((z2 > z0) ? 2 : 0);
   // the rule stands unless the frontier is in the training range and the
   // network prefers another direction by at least the margin
-  if (!s.in_range || s.z[best] - s.z[dir_class(rule)] < a.margin) return rule;
+  if (!s.in_range[par] || s.z[par][best] - s.z[par][dir_class(rule)] < a.margin) return rule;
   return best == 0 ? 0 : (best == 1 ? 1 : a.third);
 }
 
