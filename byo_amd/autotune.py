@@ -12,9 +12,11 @@ knob is timed on:
                                  (launch ms, best of 7)
     ppr_blocks_per_cu            32-seed personalized PageRank, 5 iterations
     pr_relabel, pr_xcd,          FP32 PageRank, 5 iterations (the one-time
-    pr_lanes, pr_grid            layouts are built before the timing)
+    pr_xcd_vw, pr_lanes, pr_grid layouts are built before the timing)
 
     python -m byo_amd.autotune --scale 22 --scale 23
+    python -m byo_amd.autotune --scale 22 --only pr_xcd_vw   # one knob, the
+                                 # entry's other knobs as stored
 """
 import argparse
 import json
@@ -35,11 +37,13 @@ CANDIDATES = {
     "ppr_pieces": [2, 4, 8],
     "pr_relabel": [0, 1],
     "pr_xcd": [0, 1],
+    "pr_xcd_vw": [0, 2, 4, 6, 8, 12, 16, 24],
     "pr_lanes": [0, 4, 8, 16, 32],
     "pr_grid": [4096, 8192, 16384, 32768, 65536],
 }
 WORKLOAD = {"coop_blocks_per_cu": "bfs", "pieces": "bfs", "pull_dyn": "bfs", "coop_grid_sqrt_div": "bfs", "ppr_blocks_per_cu": "ppr", "ppr_pieces": "ppr", "pr_lanes": "pr",
-            "pr_grid": "pr", "pr_relabel": "pr", "pr_xcd": "pr"}
+            "pr_grid": "pr", "pr_relabel": "pr", "pr_xcd": "pr",
+            "pr_xcd_vw": "pr"}
 
 
 def _sources(deg, k, seed):
@@ -65,10 +69,11 @@ def measure(G, workload, srcs, seeds, repeats=3):
     return best
 
 
-def tune(G, label=""):
-    """Coordinate descent over CANDIDATES; returns (knobs, times).  The BFS
-    knobs are timed on the launch bench.py times: the same sources and the
-    same direction policy (the learned selector when weights are committed)."""
+def tune(G, label="", only=None, start=None):
+    """Coordinate descent over CANDIDATES (or the keys in only, the others
+    set to start); returns (knobs, times).  The BFS knobs are timed on the
+    launch bench.py times: the same sources and the same direction policy
+    (the learned selector when weights are committed)."""
     from . import DIR_MLP
     from . import selector as S
     deg = G.degrees()
@@ -77,7 +82,11 @@ def tune(G, label=""):
     if S.load(G).get("batch"):
         G.set_direction_policy(DIR_MLP)
     knobs, times = {}, {}
+    for key, v in (start or {}).items():
+        G.set_tuning(key, v)
     for key, cands in CANDIDATES.items():
+        if only and key not in only:
+            continue
         wl = WORKLOAD[key]
         measure(G, wl, srcs, seeds, 1)  # warm
         res = {}
@@ -117,13 +126,21 @@ def main():
     ap.add_argument("--scale", type=int, action="append", default=None)
     ap.add_argument("--ef", type=int, default=16)
     ap.add_argument("--out", default=TUNING)
+    ap.add_argument("--only", action="append", default=None, choices=sorted(CANDIDATES),
+                    help="tune only this knob (repeatable); the stored entry keeps the rest")
     args = ap.parse_args()
     db = load(args.out)
     for scale in args.scale or [22]:
         t0 = time.time()
         G = CsrGraph.rmat(scale, args.ef)
         label = "rmat%d_ef%d" % (scale, args.ef)
-        knobs, times = tune(G, label)
+        old = next((e for e in db["entries"] if e["graph"] == label), None)
+        if args.only and old is None:
+            raise SystemExit("--only needs a stored entry for %s" % label)
+        knobs, times = tune(G, label, args.only, old["knobs"] if args.only else None)
+        if args.only:
+            knobs = dict(old["knobs"], **knobs)
+            times = dict(old.get("times_ms", {}), **times)
         entry = {"graph": label, "n": G.N(), "m": G.M(), "knobs": knobs, "times_ms": times}
         db["entries"] = [e for e in db["entries"] if e["graph"] != label] + [entry]
         G.close()

@@ -171,6 +171,8 @@ inline const std::vector<TuneSpec> &tune_specs() {
       {"sel_densify", 0, 0, 1, "learned selector may turn a level the rule keeps sparse dense (0: never)"},
       {"pr_relabel", 0, 0, 1, "PageRank on a copy relabelled by descending degree (built once per graph)"},
       {"pr_xcd", 0, 0, 1, "PageRank pull split into one source slice per XCD (built once per graph)"},
+      {"pr_xcd_vw", 0, 0, 1024,
+       "weight of a vertex, in edges, when the per-XCD source slices are cut to equal weight (0: equal edges)"},
       {"bfs_engine", 0, 0, 2,
        "BFS of several sources: 0 one shared traversal per 64, 1 one persistent launch taking them in turn, "
        "2 host-driven level loop"},
@@ -211,6 +213,7 @@ struct byo_graph {
   int fn_view = 0;
   bool xs_valid = false;      // PageRank per-XCD slices are current (for layout xs_layout)
   int xs_layout = 0;
+  int64_t xs_vw = 0;          // pr_xcd_vw the slices were cut with
   int64_t last_src = -1;      // last source of byo_bfs / byo_bfs_multi
   const uint32_t *last_tree = nullptr;  // device parents of last_src's BFS (in a scratch buffer)
   bool coop_bar_ready = false;  // "bfs_coop_bar" cleared once: every launch leaves it clear

@@ -226,7 +226,9 @@ __global__ void __launch_bounds__(256) k_pr_tiles(const uint64_t *off, const uin
 // (tuning knob pr_xcd).  Workgroups are dealt to the chip's 8 XCDs
 // round-robin, so workgroup b runs on XCD b % 8; XCD x sums, for every
 // vertex, only the neighbours in source slice x -- a range of ids holding
-// 1/8 of the edges -- so the contributions it gathers form one slice of the
+// 1/8 of the edges, or of the edges plus pr_xcd_vw per vertex (a slice of
+// many low-degree sources has a footprint larger than one L2, and runs
+// slower per edge) -- so the contributions it gathers form one slice of the
 // vector, sized to stay in that XCD's own L2 instead of the whole vector
 // competing for every L2.  The slices' neighbour lists are stored apart
 // (slice x of vertex v at so[x * (n+1) + v] .. so[x * (n+1) + v + 1] of
@@ -236,28 +238,37 @@ __global__ void __launch_bounds__(256) k_pr_tiles(const uint64_t *off, const uin
 constexpr int kXcds = 8;
 constexpr int kXcdUnroll = 2;  // gathers in flight per lane in a slice's short lists (2 measured best of 1-4, 8)
 
+// Sums slice x's neighbours of the 64 vertices of one tile into the
+// slice's partials.
+template <class T, bool kComp>
+__device__ __forceinline__ void pr_xcd_tile(const uint32_t *so, const uint32_t *ids_s, uint64_t n,
+                                            const T *contrib_in, T *part_s, T *part_c, uint32_t x, uint64_t tile,
+                                            int lane) {
+  const uint32_t *sox = so + (uint64_t)x * (n + 1);
+  const uint64_t v = tile * 64 + lane;
+  const bool valid = v < n;
+  const uint64_t lo = sox[valid ? v : n], hi = sox[valid ? v + 1 : n];
+  const uint32_t dmax = wave_max((uint32_t)(hi - lo));
+  // slices hold a few edges per vertex: short unrolled steps
+  const Acc<T> mine = pr_tile_any<T, kComp, false, kXcdUnroll>(dmax, lo, hi, ids_s, contrib_in, lane);
+  if (valid) {
+    part_s[(uint64_t)x * n + v] = mine.s;
+    if (kComp) part_c[(uint64_t)x * n + v] = mine.c;
+  }
+}
+
+// XCD x takes every tile of slice x in a fixed order.
 template <class T, bool kComp>
 __global__ void __launch_bounds__(256) k_pr_xcd(const uint32_t *so, const uint32_t *ids_s, uint64_t n,
                                                 const T *contrib_in, T *part_s, T *part_c, const uint32_t *stop) {
   if (stop && *stop) return;  // a batched iteration after convergence
   const int lane = lane_id();
   const uint32_t x = blockIdx.x % kXcds;
+  const uint64_t ntiles = (n + 63) >> 6;
   const uint64_t wave = (uint64_t)(blockIdx.x / kXcds) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const uint64_t nwaves = (uint64_t)(gridDim.x / kXcds) * (blockDim.x >> 6);
-  const uint64_t ntiles = (n + 63) >> 6;
-  const uint32_t *sox = so + (uint64_t)x * (n + 1);
-  for (uint64_t tile = wave; tile < ntiles; tile += nwaves) {
-    const uint64_t v = tile * 64 + lane;
-    const bool valid = v < n;
-    const uint64_t lo = sox[valid ? v : n], hi = sox[valid ? v + 1 : n];
-    const uint32_t dmax = wave_max((uint32_t)(hi - lo));
-    // slices hold a few edges per vertex: short unrolled steps
-    const Acc<T> mine = pr_tile_any<T, kComp, false, kXcdUnroll>(dmax, lo, hi, ids_s, contrib_in, lane);
-    if (valid) {
-      part_s[(uint64_t)x * n + v] = mine.s;
-      if (kComp) part_c[(uint64_t)x * n + v] = mine.c;
-    }
-  }
+  for (uint64_t tile = wave; tile < ntiles; tile += nwaves)
+    pr_xcd_tile<T, kComp>(so, ids_s, n, contrib_in, part_s, part_c, x, tile, lane);
 }
 
 // degs: true degrees when the view's lists hold empty slots (else null:
@@ -291,15 +302,19 @@ __global__ void __launch_bounds__(256) k_pr_xcd_combine(const T *part_s, const T
   if (threadIdx.x == 0) partial[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
 }
 
-// Slice bounds: bnd[x] = first vertex v with off[v] >= x * m / 8.
-__global__ void k_xs_bounds(const uint64_t *off, uint64_t n, uint32_t *bnd) {
+// Slice bounds: bnd[x] = first vertex v with w(v) >= x * w(n) / 8, where
+// w(v) = off[v] + vw * v weighs each source by its edges plus vw (a slice's
+// gathers cost more as its vertex count, the footprint of its
+// contributions, outgrows one L2).
+__global__ void k_xs_bounds(const uint64_t *off, uint64_t n, uint64_t vw, uint32_t *bnd) {
   const int x = threadIdx.x;
   if (x > kXcds) return;
-  const uint64_t target = off[n] / kXcds * x + (off[n] % kXcds) * x / kXcds;
+  const uint64_t total = off[n] + vw * n;
+  const uint64_t target = total / kXcds * x + (total % kXcds) * x / kXcds;
   uint64_t lo = 0, hi = n;
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
-    if (off[mid] >= target) hi = mid; else lo = mid + 1;
+    if (off[mid] + vw * mid >= target) hi = mid; else lo = mid + 1;
   }
   bnd[x] = x == kXcds ? (uint32_t)n : (uint32_t)lo;
 }
@@ -560,11 +575,12 @@ XcdSlices xcd_slices(byo_graph *g, const RelabelView &view, bool holes) {
   uint32_t *so = g->buf("pr_xs_so", (uint64_t)kXcds * (n + 1) * 4 + 64).as<uint32_t>();
   uint32_t *ids_s = g->buf("pr_xs_ids", m * 4 + 64).as<uint32_t>();
   const int layout = holes ? 2 : (view.perm ? 1 : 0);
-  if (!g->xs_valid || g->xs_layout != layout) {
+  const int64_t vw = g->tuning("pr_xcd_vw");
+  if (!g->xs_valid || g->xs_layout != layout || g->xs_vw != vw) {
     hipStream_t s = g->stream;
     PhaseTimer t(s, g->prof.on);
     uint32_t *dbnd = g->buf("pr_xs_bnd", 64 * 4).as<uint32_t>();
-    k_xs_bounds<<<1, 64, 0, s>>>(view.off, n, dbnd);
+    k_xs_bounds<<<1, 64, 0, s>>>(view.off, n, (uint64_t)vw, dbnd);
     XcdBounds bd{};
     BYO_HIP(hipMemcpyAsync(bd.b, dbnd, sizeof(bd.b), hipMemcpyDeviceToHost, s));
     BYO_HIP(hipStreamSynchronize(s));
@@ -592,6 +608,7 @@ XcdSlices xcd_slices(byo_graph *g, const RelabelView &view, bool holes) {
     if (g->prof.on) g->account("pr_xcd_slices", t.stop());
     g->xs_valid = true;
     g->xs_layout = layout;
+    g->xs_vw = vw;
   }
   return XcdSlices{so, ids_s};
 }

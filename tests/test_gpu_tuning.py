@@ -48,18 +48,27 @@ def test_bfs_knobs(B, s15, key, values):
 
 
 @pytest.mark.parametrize("key,values", [("pr_lanes", [4, 8, 16, 32, 0]), ("pr_grid", [256, 2048, 16384]),
-                                        ("pr_seg_log2", [12, 40])])
+                                        ("pr_seg_log2", [12, 40]), ("pr_xcd_vw", [0, 4, 24, 1024])])
 def test_pagerank_knobs(B, s15, key, values):
+    """pr_xcd_vw moves the per-XCD slice bounds (1024: nearly equal vertex
+    counts, the hubs' slices holding most edges); the slices are rebuilt
+    when it changes."""
     g, G = s15
     ref, _ = O.pagerank(g, 0.85, 20)
+    if key == "pr_xcd_vw":
+        G.set_tuning("pr_relabel", 1)
+        G.set_tuning("pr_xcd", 1)
     try:
         for v in values:
             print(key, v, flush=True)
             G.set_tuning(key, v)
-            p, _, _ = G.pagerank(0.85, 20, 0.0, B.PR_FP32)
-            assert np.abs(p - ref).sum() <= 1e-6
+            for prec in (B.PR_FP32, B.PR_FP64):
+                p, _, _ = G.pagerank(0.85, 20, 0.0, prec)
+                assert np.abs(p - ref).sum() <= 1e-6
     finally:
-        G.set_tuning(key, {"pr_lanes": 0, "pr_grid": 4096, "pr_seg_log2": 40}[key])
+        G.set_tuning(key, {"pr_lanes": 0, "pr_grid": 4096, "pr_seg_log2": 40, "pr_xcd_vw": 0}[key])
+        G.set_tuning("pr_relabel", 0)
+        G.set_tuning("pr_xcd", 0)
 
 
 @pytest.mark.parametrize("precision", ["PR_FP32", "PR_FP64"])
