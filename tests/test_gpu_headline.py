@@ -6,7 +6,9 @@
 * CPMA insert_batch / remove_batch at b = 1e6 and 1e7 (device-resident keys,
   the benchmark's path: full-grid radix sort, windowed rebalances over
   thousands of leaves, leaf-array growth) keep the edge set equal to E u B /
-  E minus B after every batch (Batch_insert_runner, run_unweighted.h:781-893).
+  E minus B after every batch (Batch_insert_runner, run_unweighted.h:781-893),
+  on RMAT-18, RMAT-20 and the benchmark's RMAT-24, where BFS and PageRank
+  then rerun on the PMA after a symmetric batch.
 * RMAT scale 22 ef 16 (the benchmark graph): the device generator equals the
   oracle's graph, the 8-source shared traversal gives valid parent trees
   with the oracle's levels for every source, CC labels and the triangle count
@@ -91,6 +93,87 @@ def test_cpma_large_batches_set_semantics(B, scale, sizes):
                 assert np.array_equal(G.degrees(), np.diff(o).astype(np.uint32))
             finally:
                 d.free()
+    finally:
+        G.close()
+
+
+def test_cpma_rmat24_batches_and_rerun(B):
+    """C3 at the benchmark's own scale, RMAT-24 ef 16 as a CPMA.  First, as
+    the benchmark reruns it, a symmetric 1e7-key batch is inserted and the
+    8-source shared BFS and FP32 PageRank run on the PMA itself (tuning
+    applied, per-XCD slices cut from the slot view) against the oracle on
+    the updated graph, and the batch is removed again.  Then device
+    update-stream batches of 1e6 and 1e7 directed keys are inserted and
+    removed (remove shuffled), with the edge set equal to the oracle's
+    sorted-array E u B / E minus B after each (gbbs/graph.h:3071-3124).  The
+    rerun comes first because removing a directed batch also removes the
+    batch's keys that were edges before, which leaves their reverse edges
+    alone: the graph is no longer symmetric, and a pull reads out-lists as
+    in-lists."""
+    print("cpma RMAT-24: build", flush=True)
+    H = B.CsrGraph.rmat(24, 16)
+    G = B.CpmaGraph.from_graph(H)
+    off, ids = H.export_csr()
+    H.close()
+    E = O.csr_keys(O.Graph(off, ids))
+    del off, ids
+    n = G.N()
+    rng = np.random.default_rng(24)
+
+    def check(step):
+        o, i = G.export_csr()
+        g = O.Graph(o, i)
+        k = O.csr_keys(g)
+        assert len(k) == len(E) and np.array_equal(k, E), step
+        assert G.M() == len(E), step
+        assert np.array_equal(G.degrees(), np.diff(o).astype(np.uint32)), step
+        return g
+
+    def update(keys, insert):
+        d = B.DeviceBuffer(keys)
+        try:
+            if insert:
+                G.insert_batch_dev(d.ptr, len(keys))
+            else:
+                G.remove_batch_dev(d.ptr, len(keys))
+        finally:
+            d.free()
+
+    try:
+        half = B.update_rmat_keys(n, 5_000_000)
+        sym = np.concatenate([half, ((half & np.uint64(0xFFFFFFFF)) << np.uint64(32)) | (half >> np.uint64(32))])
+        print("  symmetric insert %d keys, rerun" % len(sym), flush=True)
+        update(sym, True)
+        E = O.batch_insert(E, sym)
+        g = check("symmetric insert")
+        G.apply_tuning()
+        srcs = O.bfs_sources(g, 8, seed=4)
+        par, st = G.bfs_multi(srcs)
+        for j, s in enumerate(srcs):
+            lv = O.bfs_levels(g, s)
+            assert O.check_bfs_tree(g, s, par[j], lv) == 0, s
+            assert st[j]["reached"] == int((lv != O.NONE).sum())
+        ref, _ = O.pagerank(g, 0.85, 20)
+        p, _, _ = G.pagerank(0.85, 20, 0.0, B.PR_FP32)
+        err = float(np.abs(p - ref).sum())
+        print("  PMA FP32 PageRank L1 vs FP64 oracle", err, flush=True)
+        assert err <= 1e-6
+        del g, par, p, ref
+        update(sym, False)
+        E = O.batch_remove(E, sym)
+        check("symmetric remove")
+        for b in (1_000_000, 10_000_000):
+            keys = B.update_rmat_keys(n, b)
+            print("  insert b=%d" % b, flush=True)
+            update(keys, True)
+            E = O.batch_insert(E, keys)
+            check(("insert", b))
+            sh = keys.copy()
+            rng.shuffle(sh)
+            print("  remove b=%d" % b, flush=True)
+            update(sh, False)
+            E = O.batch_remove(E, keys)
+            check(("remove", b))
     finally:
         G.close()
 
