@@ -477,6 +477,7 @@ def main():
         xcd = G.get_tuning("pr_xcd")
         ex["pr_xcd"] = xcd
         ex["pr_xcd_slices_build_ms"] = round(G.profile_get("pr_xcd_slices")[1], 3)
+        ex["pr_pb_layout_build_ms"] = round(G.profile_get("pr_pb_layout")[1], 3)
         G.profile(False)
         p32, l1_32, ms32 = G.pagerank(0.85, 20, 0.0, B.PR_FP32)
         p64, l1_64, ms64 = G.pagerank(0.85, 20, 0.0, B.PR_FP64)
@@ -500,6 +501,14 @@ def main():
             ex["pr_%s_it_s_xcd_%d" % (key, 1 - xcd)] = round(
                 20 / (G.pagerank(0.85, 20, 0.0, prec, want_p=False)[2] * 1e-3), 1)
         G.set_tuning("pr_xcd", xcd)
+        pb = G.get_tuning("pr_pb")
+        ex["pr_pb"] = pb
+        G.set_tuning("pr_pb", 1 - pb)  # propagation blocking against the pull (other knobs as tuned)
+        for prec, key in ((B.PR_FP32, "fp32"), (B.PR_FP64, "fp64")):
+            G.pagerank(0.85, 20, 0.0, prec, want_p=False)
+            ex["pr_%s_it_s_pb_%d" % (key, 1 - pb)] = round(
+                20 / (G.pagerank(0.85, 20, 0.0, prec, want_p=False)[2] * 1e-3), 1)
+        G.set_tuning("pr_pb", pb)
         G.cc()
         lab, cc_ms = G.cc()
         ex["cc_ms"] = round(cc_ms, 3)

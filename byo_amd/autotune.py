@@ -13,6 +13,7 @@ knob is timed on:
     ppr_blocks_per_cu            32-seed personalized PageRank, 5 iterations
     pr_relabel, pr_xcd,          FP32 PageRank, 5 iterations (the one-time
     pr_xcd_vw, pr_lanes, pr_grid layouts are built before the timing)
+    pr_pb
 
     python -m byo_amd.autotune --scale 22 --scale 23
     python -m byo_amd.autotune --scale 22 --only pr_xcd_vw   # one knob, the
@@ -40,10 +41,11 @@ CANDIDATES = {
     "pr_xcd_vw": [0, 2, 4, 6, 8, 12, 16, 24],
     "pr_lanes": [0, 4, 8, 16, 32],
     "pr_grid": [4096, 8192, 16384, 32768, 65536],
+    "pr_pb": [0, 1],
 }
 WORKLOAD = {"coop_blocks_per_cu": "bfs", "pieces": "bfs", "pull_dyn": "bfs", "coop_grid_sqrt_div": "bfs", "ppr_blocks_per_cu": "ppr", "ppr_pieces": "ppr", "pr_lanes": "pr",
             "pr_grid": "pr", "pr_relabel": "pr", "pr_xcd": "pr",
-            "pr_xcd_vw": "pr"}
+            "pr_xcd_vw": "pr", "pr_pb": "pr"}
 
 
 def _sources(deg, k, seed):

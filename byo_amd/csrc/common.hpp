@@ -171,6 +171,9 @@ inline const std::vector<TuneSpec> &tune_specs() {
       {"sel_densify", 0, 0, 1, "learned selector may turn a level the rule keeps sparse dense (0: never)"},
       {"pr_relabel", 0, 0, 1, "PageRank on a copy relabelled by descending degree (built once per graph)"},
       {"pr_xcd", 0, 0, 1, "PageRank pull split into one source slice per XCD (built once per graph)"},
+      {"pr_pb", 0, 0, 1,
+       "PageRank by propagation blocking: contributions scattered by source into destination bins, each bin "
+       "summed in LDS (built once per graph)"},
       {"pr_xcd_vw", 0, 0, 1024,
        "weight of a vertex, in edges, when the per-XCD source slices are cut to equal weight (0: equal edges)"},
       {"bfs_engine", 0, 0, 2,
@@ -214,6 +217,11 @@ struct byo_graph {
   bool xs_valid = false;      // PageRank per-XCD slices are current (for layout xs_layout)
   int xs_layout = 0;
   int64_t xs_vw = 0;          // pr_xcd_vw the slices were cut with
+  // PageRank propagation-blocking layout (pr_pb): current for layout
+  // pb_layout; pb_items gather items, pb_nsb scatter source blocks
+  bool pb_valid = false;
+  int pb_layout = 0;
+  uint32_t pb_items = 0, pb_nsb = 0;
   int64_t last_src = -1;      // last source of byo_bfs / byo_bfs_multi
   const uint32_t *last_tree = nullptr;  // device parents of last_src's BFS (in a scratch buffer)
   bool coop_bar_ready = false;  // "bfs_coop_bar" cleared once: every launch leaves it clear

@@ -85,6 +85,7 @@ static void cpma_changed(byo_graph *g) {
   g->pieces_stale();
   g->rl_valid = false;
   g->xs_valid = false;
+  g->pb_valid = false;
   state(g)->index_valid = false;
 }
 
@@ -1036,6 +1037,7 @@ void cpma_ensure_csr(byo_graph *g) {
   g->pieces_stale();
   g->rl_valid = false;
   g->xs_valid = false;
+  g->pb_valid = false;
 }
 
 namespace {

@@ -266,6 +266,7 @@ void csr_from_keys(byo_graph *g, uint64_t *keys, uint64_t *alt, uint64_t nk, uin
   g->pieces_stale();
   g->rl_valid = false;
   g->xs_valid = false;
+  g->pb_valid = false;
 }
 
 void csr_upload(byo_graph *g, const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m) {
@@ -282,6 +283,7 @@ void csr_upload(byo_graph *g, const uint64_t *offsets, const uint32_t *ids, uint
   g->pieces_stale();
   g->rl_valid = false;
   g->xs_valid = false;
+  g->pb_valid = false;
 }
 
 void check_csr_input(const uint64_t *offsets, const uint32_t *ids, uint64_t n, uint64_t m) {

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <vector>
 
 #include "kahan.hpp"
 #include "primitives.hpp"
@@ -613,6 +614,365 @@ XcdSlices xcd_slices(byo_graph *g, const RelabelView &view, bool holes) {
   return XcdSlices{so, ids_s};
 }
 
+// ------------------------------------------------ propagation blocking
+// (tuning knob pr_pb).  The pull above pays one scattered L2 request per
+// edge for a 4- or 8-byte contribution.  Here an iteration moves the
+// contributions in two streaming passes over a layout built once per graph:
+//  * destinations are cut into bins of at most kPbV consecutive vertices
+//    and about kPbE in-edges (a bin around a hub holds more and is summed by
+//    several gather items); every edge has a slot in a bin-major array,
+//    ordered by the bin of its destination, then by source, and dl[slot]
+//    (16 bits) is the destination's offset in its bin;
+//  * sources are cut into blocks of at most kPbSV consecutive vertices and
+//    about kPbSE out-edges; a block's slots form one run per bin, and fq /
+//    fs list them in slot order (fq the slot, fs its source's offset in the
+//    block);
+//  * scatter: one workgroup per source block finalises its sources' ranks
+//    from the previous gather, stages their contributions in LDS and writes
+//    its runs slot after slot (consecutive lanes, consecutive slots); the
+//    blocks of one XCD are consecutive, so the runs that meet in a bin meet
+//    in that XCD's L2;
+//  * gather: one workgroup per item streams the item's slots and adds each
+//    value into its destination's accumulator in LDS.
+// The sums are kept in 64-bit fixed point (FP32 values: one word in 2^-62
+// units; FP64: two words, 2^-62 and 2^-94 units).  Every contribution is at
+// most 1 and every in-sum at most the total rank, 1, so the words do not
+// overflow; the LDS atomics add exactly, and the result does not depend on
+// their order (bit-identical between runs).
+#ifndef BYO_PB_E_LOG2
+#define BYO_PB_E_LOG2 17
+#endif
+#ifndef BYO_PB_SE_LOG2
+#define BYO_PB_SE_LOG2 16
+#endif
+#ifndef BYO_PB_GU
+#define BYO_PB_GU 4
+#endif
+#ifndef BYO_PB_SU
+#define BYO_PB_SU 4
+#endif
+constexpr uint32_t kPbV = 4096;                    // destinations per bin
+constexpr uint64_t kPbE = 1ull << BYO_PB_E_LOG2;   // slots per gather item
+constexpr uint32_t kPbSV = 8192;                   // sources per scatter block
+constexpr uint64_t kPbSE = 1ull << BYO_PB_SE_LOG2; // slots per scatter block (a hub's block holds more)
+constexpr int kPbBlock = 512;                      // threads per gather item / scatter block
+constexpr int kPbGU = BYO_PB_GU;                   // slot loads in flight per thread (gather)
+constexpr int kPbSU = BYO_PB_SU;                   // slot stores in flight per thread (scatter)
+
+struct PbItem {
+  uint32_t lo, nv;      // first destination of the bin, destinations in it
+  uint64_t e0, e1;      // slot range
+  uint32_t split, pad;  // 1: the bin spans several items (sums added atomically)
+};
+
+template <class T>
+struct PbWords {
+  static constexpr int k = sizeof(T) == 8 ? 2 : 1;
+};
+
+// x in [0, 1] in fixed point: w[0] = floor(x 2^62); FP64 also w[1] = the
+// rest in 2^-94 units (the products and differences below are exact).
+template <class T>
+__device__ __forceinline__ void pb_fix(T x, unsigned long long *w) {
+  const double q = (double)x * 0x1p62;
+  const unsigned long long hi = (unsigned long long)q;
+  w[0] = hi;
+  if (PbWords<T>::k == 2) w[1] = (unsigned long long)((q - (double)hi) * 0x1p32);
+}
+
+// One workgroup per source block b = [sb[b], sb[b+1]) (workgroup i takes
+// block (i % 8) * per + i / 8, per = ceil(blocks / 8): XCD i % 8 a
+// contiguous range).  kInit: the contributions of the ranks in p_old
+// (iteration 0).  Otherwise p_new = d * (sum gathered in acc) + (1-d)/n,
+// the block's share of the L1 residual, acc cleared for the next gather,
+// and (write_vals) the contributions p_new/deg written to the block's
+// slots.
+template <class T, bool kInit>
+__global__ void __launch_bounds__(kPbBlock) k_pb_scatter(const uint64_t *off, const uint32_t *sb, uint32_t nsb,
+                                                         const uint32_t *fq, const uint16_t *fs, uint64_t n,
+                                                         unsigned long long *acc, const T *p_old, T *p_new, T *val,
+                                                         double damping, double addc, double *partial,
+                                                         int write_vals, const uint32_t *stop) {
+  __shared__ T cs[kPbSV];
+  __shared__ double lds[kPbBlock / 64];
+  if (stop && *stop) return;  // a batched iteration after convergence
+  const uint32_t per = (nsb + kXcds - 1) / kXcds;
+  const uint32_t b = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  double resid = 0.0;
+  if (b < nsb) {
+    const uint32_t v0 = sb[b], v1 = sb[b + 1];
+    for (uint32_t i = threadIdx.x; i < v1 - v0; i += kPbBlock) {
+      const uint64_t v = v0 + i;
+      T pv;
+      if (kInit) {
+        pv = p_old[v];
+      } else {
+        double sum = (double)acc[v] * 0x1p-62;
+        if (PbWords<T>::k == 2) sum += (double)acc[n + v] * 0x1p-94;
+        pv = (T)(damping * sum + addc);
+        p_new[v] = pv;
+        resid += fabs((double)pv - (double)p_old[v]);
+        acc[v] = 0;
+        if (PbWords<T>::k == 2) acc[n + v] = 0;
+      }
+      const uint64_t deg = off[v + 1] - off[v];
+      cs[i] = deg ? pv / (T)deg : (T)0;
+    }
+    if (write_vals) {
+      __syncthreads();
+      // the block's slots: fq/fs entries [off[v0], off[v1]) (a block's
+      // slot count is its sources' edge count)
+      const uint64_t j0 = off[v0], j1 = off[v1];
+      for (uint64_t j = j0 + threadIdx.x; j < j1; j += (uint64_t)kPbBlock * kPbSU) {
+        uint32_t q[kPbSU];
+        uint16_t l[kPbSU];
+#pragma unroll
+        for (int u = 0; u < kPbSU; u++) {
+          const uint64_t jj = j + (uint64_t)u * kPbBlock;
+          const uint64_t at = jj < j1 ? jj : j;
+          q[u] = fq[at];
+          l[u] = fs[at];
+        }
+#pragma unroll
+        for (int u = 0; u < kPbSU; u++)
+          if (j + (uint64_t)u * kPbBlock < j1) val[q[u]] = cs[l[u]];
+      }
+    }
+  }
+  if (kInit) return;
+  resid = wave_sum(resid);
+  if (lane_id() == 0) lds[threadIdx.x >> 6] = resid;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kPbBlock / 64; w++) t += lds[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// One workgroup per item: the item's slots added into LDS accumulators of
+// its bin's destinations, then written to acc (word w of destination v at
+// acc[w * n + v]); a split bin's items add theirs atomically (acc is clear).
+template <class T>
+__global__ void __launch_bounds__(kPbBlock) k_pb_gather(const PbItem *items, const T *val, const uint16_t *dl,
+                                                        unsigned long long *acc, uint64_t n, const uint32_t *stop) {
+  constexpr int W = PbWords<T>::k;
+  __shared__ unsigned long long s[W * kPbV];
+  if (stop && *stop) return;  // a batched iteration after convergence
+  const PbItem it = items[blockIdx.x];
+#pragma unroll
+  for (int w = 0; w < W; w++)
+    for (uint32_t i = threadIdx.x; i < it.nv; i += kPbBlock) s[w * kPbV + i] = 0;
+  __syncthreads();
+  for (uint64_t e = it.e0 + threadIdx.x; e < it.e1; e += (uint64_t)kPbBlock * kPbGU) {
+    T xv[kPbGU];
+    uint32_t d[kPbGU];
+#pragma unroll
+    for (int j = 0; j < kPbGU; j++) {
+      const uint64_t kk = e + (uint64_t)j * kPbBlock;
+      const uint64_t at = kk < it.e1 ? kk : e;
+      xv[j] = val[at];
+      d[j] = dl[at];
+    }
+#pragma unroll
+    for (int j = 0; j < kPbGU; j++) {
+      if (e + (uint64_t)j * kPbBlock < it.e1) {
+        unsigned long long w[2];
+        pb_fix<T>(xv[j], w);
+        atomicAdd(&s[d[j]], w[0]);
+        if (W == 2) atomicAdd(&s[kPbV + d[j]], w[1]);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < W; w++) {
+    for (uint32_t i = threadIdx.x; i < it.nv; i += kPbBlock) {
+      const unsigned long long a = s[w * kPbV + i];
+      unsigned long long *dst = acc + (uint64_t)w * n + it.lo + i;
+      if (!it.split) *dst = a;
+      else if (a) atomicAdd(dst, a);
+    }
+  }
+}
+
+// Layout build.  cut[j] = first vertex v with off[v] >= j * step.
+__global__ void k_pb_cuts(const uint64_t *off, uint64_t n, uint64_t step, uint64_t ncut, uint32_t *cut) {
+  for (uint64_t j = global_tid(); j < ncut; j += grid_threads()) {
+    const uint64_t target = j * step;
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (off[mid] >= target) hi = mid; else lo = mid + 1;
+    }
+    cut[j] = (uint32_t)lo;
+  }
+}
+// eb[k] = off[lo[k]]; which[v] = the range k with lo[k] <= v < lo[k+1].
+__global__ void k_pb_range_edges(const uint64_t *off, const uint32_t *lo, uint64_t nr1, uint64_t *eb) {
+  for (uint64_t k = global_tid(); k < nr1; k += grid_threads()) eb[k] = off[lo[k]];
+}
+__global__ void k_pb_which(const uint32_t *lo, uint64_t nr, uint64_t n, uint32_t *which) {
+  for (uint64_t v = global_tid(); v < n; v += grid_threads()) {
+    uint64_t a = 0, b = nr;  // last k with lo[k] <= v
+    while (b - a > 1) {
+      const uint64_t mid = (a + b) >> 1;
+      if (lo[mid] <= v) a = mid; else b = mid;
+    }
+    which[v] = (uint32_t)a;
+  }
+}
+// keys[e] = (bin of ids[e]) << 32 | e: in edge order, so a stable sort on
+// the bin digits leaves each bin's slots in source order.
+__global__ void k_pb_keys(const uint32_t *ids, uint64_t m, const uint32_t *vbin, uint64_t *keys) {
+  for (uint64_t e = global_tid(); e < m; e += grid_threads()) keys[e] = ((uint64_t)vbin[ids[e]] << 32) | e;
+}
+// Slot q holds edge e = low word of sorted[q] (bin k = high word): its
+// destination offset dl[q], and the key (source block << 32 | q) of the
+// second sort, which lists every source block's slots in slot order.
+__global__ void k_pb_slots(const uint64_t *sorted, uint64_t m, const uint32_t *ids, const uint64_t *ekeys,
+                           const uint32_t *bin_lo, const uint32_t *vsb, uint16_t *dl, uint32_t *ssrc,
+                           uint64_t *keys2) {
+  for (uint64_t q = global_tid(); q < m; q += grid_threads()) {
+    const uint64_t key = sorted[q];
+    const uint32_t e = (uint32_t)key, k = (uint32_t)(key >> 32);
+    const uint32_t src = (uint32_t)(ekeys[e] >> 32);
+    dl[q] = (uint16_t)(ids[e] - bin_lo[k]);
+    ssrc[q] = src;
+    keys2[q] = ((uint64_t)vsb[src] << 32) | q;
+  }
+}
+__global__ void k_pb_flist(const uint64_t *sorted2, uint64_t m, const uint32_t *ssrc, const uint32_t *sb_lo,
+                           uint32_t *fq, uint16_t *fs) {
+  for (uint64_t j = global_tid(); j < m; j += grid_threads()) {
+    const uint64_t key = sorted2[j];
+    const uint32_t q = (uint32_t)key, b = (uint32_t)(key >> 32);
+    fq[j] = q;
+    fs[j] = (uint16_t)(ssrc[q] - sb_lo[b]);
+  }
+}
+
+struct PbLayout {
+  const PbItem *items;
+  const uint32_t *sb, *fq;
+  const uint16_t *dl, *fs;
+  uint32_t nitems, nsb;
+};
+
+// Cuts [0, n) into ranges of about step edges and at most width vertices
+// (the ranges' first vertices, ending with n).
+std::vector<uint32_t> pb_ranges(byo_graph *g, const uint64_t *off, uint64_t step, uint64_t width) {
+  const uint64_t n = g->n, m = g->m;
+  hipStream_t s = g->stream;
+  const uint64_t ncut = (m + step - 1) / step + 1;
+  DevBuf dcut(ncut * 4 + 64);
+  k_pb_cuts<<<byo_grid(ncut, 256, 4096), 256, 0, s>>>(off, n, step, ncut, dcut.as<uint32_t>());
+  BYO_LAUNCH_CHECK();
+  std::vector<uint32_t> cut(ncut);
+  BYO_HIP(hipMemcpyAsync(cut.data(), dcut.p, ncut * 4, hipMemcpyDeviceToHost, s));
+  BYO_HIP(hipStreamSynchronize(s));
+  cut.push_back((uint32_t)n);
+  std::vector<uint32_t> lo{0};
+  uint64_t prev = 0;
+  for (uint32_t c : cut) {
+    if (c <= prev) continue;
+    const uint64_t w = c - prev, parts = (w + width - 1) / width;
+    for (uint64_t i = 1; i < parts; i++) lo.push_back((uint32_t)(prev + w * i / parts));
+    lo.push_back(c);
+    prev = c;
+  }
+  return lo;
+}
+
+// Stable sort of keys[m] on the digits of the high word below 2^bits;
+// returns the sorted buffer (keys or alt).
+const uint64_t *pb_sort_high(uint64_t *keys, uint64_t *alt, uint64_t m, uint64_t maxhigh, void *tmp,
+                             hipStream_t s) {
+  int shifts[4], np = 0;
+  for (int b = 0; b < 32 && (maxhigh >> b); b += 8) shifts[np++] = 32 + b;
+  return np ? prim::radix_sort_u64(keys, alt, m, shifts, np, tmp, s) : keys;
+}
+
+// The propagation-blocking layout of the pull's graph view, built once per
+// graph and layout (relabelled or not).
+PbLayout pb_layout(byo_graph *g, const RelabelView &view) {
+  const uint64_t n = g->n, m = g->m;
+  uint16_t *dl = g->buf("pr_pb_dl", m * 2 + 64).as<uint16_t>();
+  uint32_t *fq = g->buf("pr_pb_fq", m * 4 + 64).as<uint32_t>();
+  uint16_t *fs = g->buf("pr_pb_fs", m * 2 + 64).as<uint16_t>();
+  const int layout = view.perm ? 1 : 0;
+  if (!g->pb_valid || g->pb_layout != layout) {
+    hipStream_t s = g->stream;
+    PhaseTimer t(s, g->prof.on);
+    const std::vector<uint32_t> lo = pb_ranges(g, view.off, kPbE, kPbV);     // destination bins
+    const std::vector<uint32_t> sbl = pb_ranges(g, view.off, kPbSE, kPbSV);  // source blocks
+    const uint64_t nbins = lo.size() - 1, nsb = sbl.size() - 1;
+    DevBuf dlo(lo.size() * 4 + 64), deb(lo.size() * 8 + 64);
+    uint32_t *dsb = g->buf("pr_pb_sb", sbl.size() * 4 + 64).as<uint32_t>();
+    BYO_HIP(hipMemcpyAsync(dlo.p, lo.data(), lo.size() * 4, hipMemcpyHostToDevice, s));
+    BYO_HIP(hipMemcpyAsync(dsb, sbl.data(), sbl.size() * 4, hipMemcpyHostToDevice, s));
+    k_pb_range_edges<<<byo_grid(lo.size(), 256, 4096), 256, 0, s>>>(view.off, dlo.as<uint32_t>(), lo.size(),
+                                                                  deb.as<uint64_t>());
+    BYO_LAUNCH_CHECK();
+    std::vector<uint64_t> eb(lo.size());
+    BYO_HIP(hipMemcpyAsync(eb.data(), deb.p, lo.size() * 8, hipMemcpyDeviceToHost, s));
+    BYO_HIP(hipStreamSynchronize(s));
+    std::vector<PbItem> items;
+    for (uint64_t k = 0; k < nbins; k++) {
+      const uint64_t e0 = eb[k], e1 = eb[k + 1];
+      if (e1 == e0) continue;  // no in-edges: its sums stay 0
+      const uint64_t parts = (e1 - e0 + kPbE - 1) / kPbE;
+      for (uint64_t i = 0; i < parts; i++) {
+        PbItem it{};
+        it.lo = lo[k];
+        it.nv = lo[k + 1] - lo[k];
+        it.e0 = e0 + (e1 - e0) * i / parts;
+        it.e1 = e0 + (e1 - e0) * (i + 1) / parts;
+        it.split = parts > 1;
+        items.push_back(it);
+      }
+    }
+    PbItem *ditems = g->buf("pr_pb_items", items.size() * sizeof(PbItem) + 64).as<PbItem>();
+    BYO_HIP(hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(PbItem), hipMemcpyHostToDevice, s));
+    {
+      DevBuf vbin(n * 4 + 64), vsb(n * 4 + 64), ekeys(m * 8 + 64), keys(m * 8 + 64), alt(m * 8 + 64),
+          ssrc(m * 4 + 64), tmp(prim::radix_tmp_bytes(m));
+      k_pb_which<<<byo_grid(n, 256, 16384), 256, 0, s>>>(dlo.as<uint32_t>(), nbins, n, vbin.as<uint32_t>());
+      k_pb_which<<<byo_grid(n, 256, 16384), 256, 0, s>>>(dsb, nsb, n, vsb.as<uint32_t>());
+      k_pb_keys<<<byo_grid(m, 256, 16384), 256, 0, s>>>(view.ids, m, vbin.as<uint32_t>(), keys.as<uint64_t>());
+      prim::csr_to_keys(view.off, view.ids, n, m, ekeys.as<uint64_t>(), s);  // edge sources
+      BYO_LAUNCH_CHECK();
+      // slots: the edges sorted (stably) on their destination's bin
+      const uint64_t *sorted = pb_sort_high(keys.as<uint64_t>(), alt.as<uint64_t>(), m, nbins - 1, tmp.p, s);
+      uint64_t *keys2 = sorted == keys.as<uint64_t>() ? alt.as<uint64_t>() : keys.as<uint64_t>();
+      k_pb_slots<<<byo_grid(m, 256, 16384), 256, 0, s>>>(sorted, m, view.ids, ekeys.as<uint64_t>(),
+                                                        dlo.as<uint32_t>(), vsb.as<uint32_t>(), dl,
+                                                        ssrc.as<uint32_t>(), keys2);
+      BYO_LAUNCH_CHECK();
+      // every source block's slots in slot order: a stable sort on the block
+      uint64_t *other = keys2 == keys.as<uint64_t>() ? alt.as<uint64_t>() : keys.as<uint64_t>();
+      const uint64_t *sorted2 = pb_sort_high(keys2, other, m, nsb - 1, tmp.p, s);
+      k_pb_flist<<<byo_grid(m, 256, 16384), 256, 0, s>>>(sorted2, m, ssrc.as<uint32_t>(), dsb, fq, fs);
+      BYO_LAUNCH_CHECK();
+      BYO_HIP(hipStreamSynchronize(s));
+    }
+    g->pb_items = (uint32_t)items.size();
+    g->pb_nsb = (uint32_t)nsb;
+    if (g->prof.on) g->account("pr_pb_layout", t.stop());
+    g->pb_valid = true;
+    g->pb_layout = layout;
+  }
+  PbLayout L{};
+  L.items = g->buf("pr_pb_items", 64).as<PbItem>();
+  L.sb = g->buf("pr_pb_sb", 64).as<uint32_t>();
+  L.fq = fq;
+  L.fs = fs;
+  L.dl = dl;
+  L.nitems = g->pb_items;
+  L.nsb = g->pb_nsb;
+  return L;
+}
+
 template <class T, bool kComp>
 void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, double *p_out, double *l1_log,
                    int *iters_run, double *device_ms, const char *phase) {
@@ -639,7 +999,10 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   const bool per_tile = pma || (S == 1 && !g->tuning("pr_lanes"));
   // the XCD split needs the tile pull and 32-bit edge positions; on a PMA
   // its slices are built from the slot view (holes dropped)
-  const bool per_xcd = per_tile && g->tuning("pr_xcd") && g->m < (1ull << 32) && n > 0;
+  // propagation blocking (CSR views with 32-bit slot numbers) replaces the
+  // pull, and the XCD split with it
+  const bool use_pb = per_tile && !pma && g->tuning("pr_pb") && g->m > 0 && g->m < 0xFFFFFFFFull;
+  const bool per_xcd = !use_pb && per_tile && g->tuning("pr_xcd") && g->m < (1ull << 32) && n > 0;
   const double avg = n ? (double)g->m / (double)n / S : 0.0;
   int G = 4;
   while (G < 32 && G < avg / 2) G *= 2;
@@ -654,6 +1017,18 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
     part_s = g->buf("pr_xs_part_s", (uint64_t)kXcds * n * sizeof(T) + 64).as<T>();
     if (kComp) part_c = g->buf("pr_xs_part_c", (uint64_t)kXcds * n * sizeof(T) + 64).as<T>();
   }
+  PbLayout pb{};
+  T *pb_val = nullptr;
+  unsigned long long *pb_acc = nullptr;
+  if (use_pb) {
+    pb = pb_layout(g, view);
+    // the scatter's residual partials: one per source-block workgroup
+    partial = g->buf("pr_partial", ((size_t)std::max({grid, cgrid, (pb.nsb + kXcds - 1) / kXcds * kXcds}) + 8) *
+                                       sizeof(double)).as<double>();
+    pb_val = g->buf("pr_pb_val", g->m * sizeof(T) + 64).as<T>();
+    pb_acc = g->buf("pr_pb_acc", (uint64_t)PbWords<T>::k * n * 8 + 64).as<unsigned long long>();
+    prim::fill_u64(reinterpret_cast<uint64_t *>(pb_acc), 0, (uint64_t)PbWords<T>::k * n, s);
+  }
   double *l1dev = g->buf("pr_l1", (size_t)(max_iters + 1) * sizeof(double)).as<double>();
   const uint64_t *off = view.off;
   const uint32_t *ids = view.ids;
@@ -661,6 +1036,10 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   PhaseTimer timer(s, true);
   k_pr_init<T><<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, n, p[0], c[0], av.deg);
   if (S > 1) k_pr_bounds<<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, ids, n, S, W, pos);
+  const unsigned sgrid = use_pb ? (pb.nsb + kXcds - 1) / kXcds * kXcds : 0;
+  if (use_pb)
+    k_pb_scatter<T, true><<<sgrid, kPbBlock, 0, s>>>(off, pb.sb, pb.nsb, pb.fq, pb.fs, n, pb_acc, p[0], nullptr,
+                                                     pb_val, damping, addc, partial, 1, nullptr);
   BYO_LAUNCH_CHECK();
   double *hl1 = static_cast<double *>(pinned(g, 64));
   // With eps > 0 the iterations are queued in batches of kSpec with a device
@@ -676,7 +1055,13 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   while (it < max_iters) {
     PhaseTimer it_t(s, g->prof.on);
     unsigned nparts = grid;
-    if (per_xcd) {
+    if (use_pb) {
+      k_pb_gather<T><<<pb.nitems, kPbBlock, 0, s>>>(pb.items, pb_val, pb.dl, pb_acc, n, stop);
+      k_pb_scatter<T, false><<<sgrid, kPbBlock, 0, s>>>(off, pb.sb, pb.nsb, pb.fq, pb.fs, n, pb_acc, p[cur],
+                                                        p[cur ^ 1], pb_val, damping, addc, partial,
+                                                        it + 1 < max_iters, stop);
+      nparts = sgrid;
+    } else if (per_xcd) {
       k_pr_xcd<T, kComp><<<grid / kXcds * kXcds, 256, 0, s>>>(xs.so, xs.ids_s, n, c[cur], part_s, part_c, stop);
       k_pr_xcd_combine<T, kComp><<<cgrid, 256, 0, s>>>(part_s, part_c, off, n, p[cur], p[cur ^ 1], c[cur ^ 1],
                                                        damping, addc, partial, av.deg, stop);

@@ -22,7 +22,7 @@ def test_candidates_are_valid_knob_values():
     # the values the tuner tries lie in the ranges byo_set_tuning accepts
     ranges = {"coop_blocks_per_cu": (1, 4), "ppr_blocks_per_cu": (1, 8), "pieces": (1, 64), "ppr_pieces": (1, 16),
               "pr_lanes": (0, 32), "pr_grid": (256, 65536), "pr_relabel": (0, 1), "pr_xcd": (0, 1),
-              "pull_dyn": (0, 64), "coop_grid_sqrt_div": (0, 1 << 20), "pr_xcd_vw": (0, 1024)}
+              "pull_dyn": (0, 64), "coop_grid_sqrt_div": (0, 1 << 20), "pr_xcd_vw": (0, 1024), "pr_pb": (0, 1)}
     for k, vals in autotune.CANDIDATES.items():
         lo, hi = ranges[k]
         assert all(lo <= v <= hi for v in vals), k

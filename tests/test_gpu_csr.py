@@ -224,14 +224,16 @@ def test_pagerank_eps_stop_matches_oracle(B, s16):
     assert np.abs(p - p_ref).sum() < 1e-12
 
 
-@pytest.mark.parametrize("layout", ["plain", "relabel_xcd", "cpma"])
+@pytest.mark.parametrize("layout", ["plain", "relabel_xcd", "relabel_pb", "plain_pb", "cpma"])
 @pytest.mark.parametrize("eps,iters", [(1e-6, 100), ("at16", 100), (1e-4, 9), (1e-6, 8), (1e-12, 17)])
 def test_pagerank_eps_batches_stop_where_the_oracle_does(B, s16, layout, eps, iters):
     """Iterations to a residual bound are queued 8 at a time with a device
     stop word: the iteration count, the residual log and the vector are
     those of stopping at the first residual below eps (or at iters) -- for
     stops inside a batch, at a batch end (at16: an eps between the 15th and
-    16th residuals), and at the iteration limit."""
+    16th residuals), and at the iteration limit -- in every layout: the
+    plain pull, the relabelled copy with per-XCD slices, propagation
+    blocking on either, and the CPMA."""
     g, G = s16
     if eps == "at16":
         _, l1_all = O.pagerank(g, 0.85, 30, 0.0)
@@ -243,8 +245,9 @@ def test_pagerank_eps_batches_stop_where_the_oracle_does(B, s16, layout, eps, it
         H = B.CpmaGraph.from_csr(g.offsets, g.ids)
     else:
         H = G
-        H.set_tuning("pr_relabel", 1 if layout == "relabel_xcd" else 0)
+        H.set_tuning("pr_relabel", 1 if layout.startswith("relabel") else 0)
         H.set_tuning("pr_xcd", 1 if layout == "relabel_xcd" else 0)
+        H.set_tuning("pr_pb", 1 if layout.endswith("_pb") else 0)
     try:
         for prec, tol in ((B.PR_FP64, 1e-12), (B.PR_FP32, 1e-6)):
             p, l1, _ = H.pagerank(0.85, iters, eps, prec)
@@ -260,6 +263,7 @@ def test_pagerank_eps_batches_stop_where_the_oracle_does(B, s16, layout, eps, it
         else:
             H.set_tuning("pr_relabel", 0)
             H.set_tuning("pr_xcd", 0)
+            H.set_tuning("pr_pb", 0)
 
 
 def test_ppr_matches_oracle(B, s16):

@@ -228,11 +228,12 @@ def test_s22_cc_tc_pagerank(B, s22):
     assert cnt == O.tc(g)
     print("RMAT-22 triangles", cnt, "components", len(np.unique(lab)), flush=True)
     pr, _ = O.pagerank(g, 0.85, 20, 0.0)
-    for relabel in (0, 1):
+    for relabel, pb in ((0, 0), (1, 0), (1, 1)):
         G.set_tuning("pr_relabel", relabel)
+        G.set_tuning("pr_pb", pb)
         p32, _, _ = G.pagerank(0.85, 20, 0.0, B.PR_FP32)
         err = float(np.abs(p32 - pr).sum())
-        print("relabel", relabel, "FP32-Kahan L1 vs FP64 oracle", err, flush=True)
+        print("relabel", relabel, "propagation blocking", pb, "FP32 L1 vs FP64 oracle", err, flush=True)
         assert err <= 1e-6  # the stated FP32 tolerance
         p64, _, _ = G.pagerank(0.85, 20, 0.0, B.PR_FP64)
         assert float(np.abs(p64 - pr).sum()) <= 1e-12
