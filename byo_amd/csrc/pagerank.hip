@@ -3,15 +3,16 @@
 // PageRank.h:125-193) with the Jacobi semantics of PageRank_edgeMap
 // (PageRank.h:80-123), and batched personalized PageRank.
 //
-// FP32 path: ranks and contributions p/deg are FP32; each vertex sums its
-// in-neighbour contributions with Kahan compensation per lane and a
-// TwoSum-compensated combine across the lanes of its group; the global L1
-// residual is reduced in FP64 in a fixed order (deterministic).  FP64 path:
-// the same kernel with double values and no compensation.
-//
-// Per iteration one kernel reads offsets (8 B/vertex), ids (4 B/edge), one
-// contribution per edge (w B), the old rank (w B/vertex), and writes the new
-// rank and contribution (2w B/vertex).
+// Two ways to an iteration.  The pull: FP32 ranks and contributions p/deg,
+// each vertex summing its in-neighbour contributions with Kahan
+// compensation per lane and a TwoSum-compensated combine across the lanes
+// of its group (FP64: double values, no compensation); one kernel reads
+// offsets (8 B/vertex), ids (4 B/edge), one gathered contribution per edge
+// (w B), the old rank (w B/vertex), and writes the new rank and contribution
+// (2w B/vertex).  Propagation blocking (below, knob pr_pb): contributions in
+// FP32 or FP64 moved through destination bins in two streaming passes and
+// summed exactly in 64-bit fixed point.  Either way the global L1 residual is
+// reduced in FP64 in a fixed order (deterministic).
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
