@@ -652,11 +652,31 @@ XcdSlices xcd_slices(byo_graph *g, const RelabelView &view, bool holes) {
 #ifndef BYO_PB_SU
 #define BYO_PB_SU 4
 #endif
+#ifndef BYO_PB_SV
+#define BYO_PB_SV 8192
+#endif
+#ifndef BYO_PB_SB32
+#define BYO_PB_SB32 1024
+#endif
+#ifndef BYO_PB_SB64
+#define BYO_PB_SB64 512
+#endif
+#ifndef BYO_PB_GB32
+#define BYO_PB_GB32 1024
+#endif
+#ifndef BYO_PB_GB64
+#define BYO_PB_GB64 1024
+#endif
 constexpr uint32_t kPbV = 4096;                    // destinations per bin
 constexpr uint64_t kPbE = 1ull << BYO_PB_E_LOG2;   // slots per gather item
-constexpr uint32_t kPbSV = 8192;                   // sources per scatter block
+constexpr uint32_t kPbSV = BYO_PB_SV;              // sources per scatter block
 constexpr uint64_t kPbSE = 1ull << BYO_PB_SE_LOG2; // slots per scatter block (a hub's block holds more)
-constexpr int kPbBlock = 512;                      // threads per gather item / scatter block
+// threads per scatter block and per gather item, by value width
+template <class T>
+struct PbThreads {
+  static constexpr int scatter = sizeof(T) == 4 ? BYO_PB_SB32 : BYO_PB_SB64;
+  static constexpr int gather = sizeof(T) == 4 ? BYO_PB_GB32 : BYO_PB_GB64;
+};
 constexpr int kPbGU = BYO_PB_GU;                   // slot loads in flight per thread (gather)
 constexpr int kPbSU = BYO_PB_SU;                   // slot stores in flight per thread (scatter)
 
@@ -689,20 +709,20 @@ __device__ __forceinline__ void pb_fix(T x, unsigned long long *w) {
 // and (write_vals) the contributions p_new/deg written to the block's
 // slots.
 template <class T, bool kInit>
-__global__ void __launch_bounds__(kPbBlock) k_pb_scatter(const uint64_t *off, const uint32_t *sb, uint32_t nsb,
+__global__ void __launch_bounds__(PbThreads<T>::scatter) k_pb_scatter(const uint64_t *off, const uint32_t *sb, uint32_t nsb,
                                                          const uint32_t *fq, const uint16_t *fs, uint64_t n,
                                                          unsigned long long *acc, const T *p_old, T *p_new, T *val,
                                                          double damping, double addc, double *partial,
                                                          int write_vals, const uint32_t *stop) {
   __shared__ T cs[kPbSV];
-  __shared__ double lds[kPbBlock / 64];
+  __shared__ double lds[PbThreads<T>::scatter / 64];
   if (stop && *stop) return;  // a batched iteration after convergence
   const uint32_t per = (nsb + kXcds - 1) / kXcds;
   const uint32_t b = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
   double resid = 0.0;
   if (b < nsb) {
     const uint32_t v0 = sb[b], v1 = sb[b + 1];
-    for (uint32_t i = threadIdx.x; i < v1 - v0; i += kPbBlock) {
+    for (uint32_t i = threadIdx.x; i < v1 - v0; i += PbThreads<T>::scatter) {
       const uint64_t v = v0 + i;
       T pv;
       if (kInit) {
@@ -724,19 +744,19 @@ __global__ void __launch_bounds__(kPbBlock) k_pb_scatter(const uint64_t *off, co
       // the block's slots: fq/fs entries [off[v0], off[v1]) (a block's
       // slot count is its sources' edge count)
       const uint64_t j0 = off[v0], j1 = off[v1];
-      for (uint64_t j = j0 + threadIdx.x; j < j1; j += (uint64_t)kPbBlock * kPbSU) {
+      for (uint64_t j = j0 + threadIdx.x; j < j1; j += (uint64_t)PbThreads<T>::scatter * kPbSU) {
         uint32_t q[kPbSU];
         uint16_t l[kPbSU];
 #pragma unroll
         for (int u = 0; u < kPbSU; u++) {
-          const uint64_t jj = j + (uint64_t)u * kPbBlock;
+          const uint64_t jj = j + (uint64_t)u * PbThreads<T>::scatter;
           const uint64_t at = jj < j1 ? jj : j;
           q[u] = fq[at];
           l[u] = fs[at];
         }
 #pragma unroll
         for (int u = 0; u < kPbSU; u++)
-          if (j + (uint64_t)u * kPbBlock < j1) val[q[u]] = cs[l[u]];
+          if (j + (uint64_t)u * PbThreads<T>::scatter < j1) val[q[u]] = cs[l[u]];
       }
     }
   }
@@ -746,7 +766,7 @@ __global__ void __launch_bounds__(kPbBlock) k_pb_scatter(const uint64_t *off, co
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0.0;
-    for (int w = 0; w < kPbBlock / 64; w++) t += lds[w];
+    for (int w = 0; w < PbThreads<T>::scatter / 64; w++) t += lds[w];
     partial[blockIdx.x] = t;
   }
 }
@@ -755,7 +775,7 @@ __global__ void __launch_bounds__(kPbBlock) k_pb_scatter(const uint64_t *off, co
 // its bin's destinations, then written to acc (word w of destination v at
 // acc[w * n + v]); a split bin's items add theirs atomically (acc is clear).
 template <class T>
-__global__ void __launch_bounds__(kPbBlock) k_pb_gather(const PbItem *items, const T *val, const uint16_t *dl,
+__global__ void __launch_bounds__(PbThreads<T>::gather) k_pb_gather(const PbItem *items, const T *val, const uint16_t *dl,
                                                         unsigned long long *acc, uint64_t n, const uint32_t *stop) {
   constexpr int W = PbWords<T>::k;
   __shared__ unsigned long long s[W * kPbV];
@@ -763,21 +783,21 @@ __global__ void __launch_bounds__(kPbBlock) k_pb_gather(const PbItem *items, con
   const PbItem it = items[blockIdx.x];
 #pragma unroll
   for (int w = 0; w < W; w++)
-    for (uint32_t i = threadIdx.x; i < it.nv; i += kPbBlock) s[w * kPbV + i] = 0;
+    for (uint32_t i = threadIdx.x; i < it.nv; i += PbThreads<T>::gather) s[w * kPbV + i] = 0;
   __syncthreads();
-  for (uint64_t e = it.e0 + threadIdx.x; e < it.e1; e += (uint64_t)kPbBlock * kPbGU) {
+  for (uint64_t e = it.e0 + threadIdx.x; e < it.e1; e += (uint64_t)PbThreads<T>::gather * kPbGU) {
     T xv[kPbGU];
     uint32_t d[kPbGU];
 #pragma unroll
     for (int j = 0; j < kPbGU; j++) {
-      const uint64_t kk = e + (uint64_t)j * kPbBlock;
+      const uint64_t kk = e + (uint64_t)j * PbThreads<T>::gather;
       const uint64_t at = kk < it.e1 ? kk : e;
       xv[j] = val[at];
       d[j] = dl[at];
     }
 #pragma unroll
     for (int j = 0; j < kPbGU; j++) {
-      if (e + (uint64_t)j * kPbBlock < it.e1) {
+      if (e + (uint64_t)j * PbThreads<T>::gather < it.e1) {
         unsigned long long w[2];
         pb_fix<T>(xv[j], w);
         atomicAdd(&s[d[j]], w[0]);
@@ -788,7 +808,7 @@ __global__ void __launch_bounds__(kPbBlock) k_pb_gather(const PbItem *items, con
   __syncthreads();
 #pragma unroll
   for (int w = 0; w < W; w++) {
-    for (uint32_t i = threadIdx.x; i < it.nv; i += kPbBlock) {
+    for (uint32_t i = threadIdx.x; i < it.nv; i += PbThreads<T>::gather) {
       const unsigned long long a = s[w * kPbV + i];
       unsigned long long *dst = acc + (uint64_t)w * n + it.lo + i;
       if (!it.split) *dst = a;
@@ -1039,7 +1059,7 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   if (S > 1) k_pr_bounds<<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, ids, n, S, W, pos);
   const unsigned sgrid = use_pb ? (pb.nsb + kXcds - 1) / kXcds * kXcds : 0;
   if (use_pb)
-    k_pb_scatter<T, true><<<sgrid, kPbBlock, 0, s>>>(off, pb.sb, pb.nsb, pb.fq, pb.fs, n, pb_acc, p[0], nullptr,
+    k_pb_scatter<T, true><<<sgrid, PbThreads<T>::scatter, 0, s>>>(off, pb.sb, pb.nsb, pb.fq, pb.fs, n, pb_acc, p[0], nullptr,
                                                      pb_val, damping, addc, partial, 1, nullptr);
   BYO_LAUNCH_CHECK();
   double *hl1 = static_cast<double *>(pinned(g, 64));
@@ -1057,8 +1077,8 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
     PhaseTimer it_t(s, g->prof.on);
     unsigned nparts = grid;
     if (use_pb) {
-      k_pb_gather<T><<<pb.nitems, kPbBlock, 0, s>>>(pb.items, pb_val, pb.dl, pb_acc, n, stop);
-      k_pb_scatter<T, false><<<sgrid, kPbBlock, 0, s>>>(off, pb.sb, pb.nsb, pb.fq, pb.fs, n, pb_acc, p[cur],
+      k_pb_gather<T><<<pb.nitems, PbThreads<T>::gather, 0, s>>>(pb.items, pb_val, pb.dl, pb_acc, n, stop);
+      k_pb_scatter<T, false><<<sgrid, PbThreads<T>::scatter, 0, s>>>(off, pb.sb, pb.nsb, pb.fq, pb.fs, n, pb_acc, p[cur],
                                                         p[cur ^ 1], pb_val, damping, addc, partial,
                                                         it + 1 < max_iters, stop);
       nparts = sgrid;
