@@ -653,9 +653,14 @@ def main():
             _, cst = C.bfs_multi(csrcs, want_parents=False)
         nl, lms = C.profile_get("bfs_launch")
         rr["pma_bfs_gteps"] = round(sum(x["edges_reached"] // 2 for x in cst) / (lms / nl * 1e-3) / 1e9, 2)
-        C.pagerank(0.85, 20, 0.0, B.PR_FP32, want_p=False)  # builds the per-XCD slices when pr_xcd is on
+        rr["csr_export_launches_bfs"] = C.profile_get("cpma_csr_export")[0]
+        C.pagerank(0.85, 20, 0.0, B.PR_FP32, want_p=False)  # builds the PageRank layout (slices / blocking)
+        rr["pma_pr_pb"] = C.get_tuning("pr_pb")
         rr["pma_pr_xcd"] = C.get_tuning("pr_xcd")
         rr["pma_pr_xcd_slices_build_ms"] = round(C.profile_get("pr_xcd_slices")[1], 3)
+        # propagation blocking cuts its layout from the CPMA's CSR export (once
+        # per update, timed with it); the iterations read the layout only
+        rr["pma_pr_pb_layout_build_ms"] = round(C.profile_get("pr_pb_layout")[1] + C.profile_get("cpma_csr_export")[1], 3)
         rr["pma_pr_fp32_it_s"] = round(20 / (C.pagerank(0.85, 20, 0.0, B.PR_FP32, want_p=False)[2] * 1e-3), 1)
         rr["csr_export_launches"] = C.profile_get("cpma_csr_export")[0]
         if H is not G:

@@ -914,14 +914,13 @@ const uint64_t *pb_sort_high(uint64_t *keys, uint64_t *alt, uint64_t m, uint64_t
   return np ? prim::radix_sort_u64(keys, alt, m, shifts, np, tmp, s) : keys;
 }
 
-// The propagation-blocking layout of the pull's graph view, built once per
-// graph and layout (relabelled or not).
-PbLayout pb_layout(byo_graph *g, const RelabelView &view) {
+// The propagation-blocking layout of a CSR view, built once per graph and
+// layout (0: the CSR, 1: its relabelled copy, 2: the CSR export of a CPMA).
+PbLayout pb_layout(byo_graph *g, const RelabelView &view, int layout) {
   const uint64_t n = g->n, m = g->m;
   uint16_t *dl = g->buf("pr_pb_dl", m * 2 + 64).as<uint16_t>();
   uint32_t *fq = g->buf("pr_pb_fq", m * 4 + 64).as<uint32_t>();
   uint16_t *fs = g->buf("pr_pb_fs", m * 2 + 64).as<uint16_t>();
-  const int layout = view.perm ? 1 : 0;
   if (!g->pb_valid || g->pb_layout != layout) {
     hipStream_t s = g->stream;
     PhaseTimer t(s, g->prof.on);
@@ -1022,7 +1021,7 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   // its slices are built from the slot view (holes dropped)
   // propagation blocking (CSR views with 32-bit slot numbers) replaces the
   // pull, and the XCD split with it
-  const bool use_pb = per_tile && !pma && g->tuning("pr_pb") && g->m > 0 && g->m < 0xFFFFFFFFull;
+  const bool use_pb = per_tile && g->tuning("pr_pb") && g->m > 0 && g->m < 0xFFFFFFFFull;
   const bool per_xcd = !use_pb && per_tile && g->tuning("pr_xcd") && g->m < (1ull << 32) && n > 0;
   const double avg = n ? (double)g->m / (double)n / S : 0.0;
   int G = 4;
@@ -1039,10 +1038,19 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
     if (kComp) part_c = g->buf("pr_xs_part_c", (uint64_t)kXcds * n * sizeof(T) + 64).as<T>();
   }
   PbLayout pb{};
+  const uint64_t *pb_off = nullptr;
   T *pb_val = nullptr;
   unsigned long long *pb_acc = nullptr;
   if (use_pb) {
-    pb = pb_layout(g, view);
+    // a CPMA's layout is cut from its CSR export (built once per update, like
+    // the layout); the iterations read the layout and its compact offsets
+    RelabelView pbv = view;
+    if (pma) {
+      ensure_csr(g);
+      pbv = RelabelView{g->off.as<uint64_t>(), g->ids.as<uint32_t>(), nullptr};
+    }
+    pb = pb_layout(g, pbv, pma ? 2 : (view.perm ? 1 : 0));
+    pb_off = pbv.off;
     // the scatter's residual partials: one per source-block workgroup
     partial = g->buf("pr_partial", ((size_t)std::max({grid, cgrid, (pb.nsb + kXcds - 1) / kXcds * kXcds}) + 8) *
                                        sizeof(double)).as<double>();
@@ -1059,7 +1067,7 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
   if (S > 1) k_pr_bounds<<<byo_grid(n, 256, 16384), 256, 0, s>>>(off, ids, n, S, W, pos);
   const unsigned sgrid = use_pb ? (pb.nsb + kXcds - 1) / kXcds * kXcds : 0;
   if (use_pb)
-    k_pb_scatter<T, true><<<sgrid, PbThreads<T>::scatter, 0, s>>>(off, pb.sb, pb.nsb, pb.fq, pb.fs, n, pb_acc, p[0], nullptr,
+    k_pb_scatter<T, true><<<sgrid, PbThreads<T>::scatter, 0, s>>>(pb_off, pb.sb, pb.nsb, pb.fq, pb.fs, n, pb_acc, p[0], nullptr,
                                                      pb_val, damping, addc, partial, 1, nullptr);
   BYO_LAUNCH_CHECK();
   double *hl1 = static_cast<double *>(pinned(g, 64));
@@ -1078,7 +1086,7 @@ void pagerank_impl(byo_graph *g, double damping, int max_iters, double eps, doub
     unsigned nparts = grid;
     if (use_pb) {
       k_pb_gather<T><<<pb.nitems, PbThreads<T>::gather, 0, s>>>(pb.items, pb_val, pb.dl, pb_acc, n, stop);
-      k_pb_scatter<T, false><<<sgrid, PbThreads<T>::scatter, 0, s>>>(off, pb.sb, pb.nsb, pb.fq, pb.fs, n, pb_acc, p[cur],
+      k_pb_scatter<T, false><<<sgrid, PbThreads<T>::scatter, 0, s>>>(pb_off, pb.sb, pb.nsb, pb.fq, pb.fs, n, pb_acc, p[cur],
                                                         p[cur ^ 1], pb_val, damping, addc, partial,
                                                         it + 1 < max_iters, stop);
       nparts = sgrid;

@@ -224,7 +224,7 @@ def test_pagerank_eps_stop_matches_oracle(B, s16):
     assert np.abs(p - p_ref).sum() < 1e-12
 
 
-@pytest.mark.parametrize("layout", ["plain", "relabel_xcd", "relabel_pb", "plain_pb", "cpma"])
+@pytest.mark.parametrize("layout", ["plain", "relabel_xcd", "relabel_pb", "plain_pb", "cpma", "cpma_pb"])
 @pytest.mark.parametrize("eps,iters", [(1e-6, 100), ("at16", 100), (1e-4, 9), (1e-6, 8), (1e-12, 17)])
 def test_pagerank_eps_batches_stop_where_the_oracle_does(B, s16, layout, eps, iters):
     """Iterations to a residual bound are queued 8 at a time with a device
@@ -241,8 +241,9 @@ def test_pagerank_eps_batches_stop_where_the_oracle_does(B, s16, layout, eps, it
     p_ref, l1_ref = O.pagerank(g, 0.85, iters, eps)
     if iters == 100 and eps > 1e-6:
         assert len(l1_ref) == 16
-    if layout == "cpma":
+    if layout.startswith("cpma"):
         H = B.CpmaGraph.from_csr(g.offsets, g.ids)
+        H.set_tuning("pr_pb", 1 if layout.endswith("_pb") else 0)
     else:
         H = G
         H.set_tuning("pr_relabel", 1 if layout.startswith("relabel") else 0)
@@ -258,7 +259,7 @@ def test_pagerank_eps_batches_stop_where_the_oracle_does(B, s16, layout, eps, it
             p2, l2, _ = H.pagerank(0.85, iters, eps, prec)  # the stop word is cleared per call
             assert len(l2) == len(l1) and np.array_equal(p2, p)
     finally:
-        if layout == "cpma":
+        if layout.startswith("cpma"):
             H.close()
         else:
             H.set_tuning("pr_relabel", 0)

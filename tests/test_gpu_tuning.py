@@ -176,13 +176,16 @@ def test_pagerank_pb_shapes(B, shape):
         G.close()
 
 
-def test_pagerank_layouts_isolated_and_updates(B):
-    """Isolated vertices and ties in degree; the relabelled copy and the XCD
-    slices follow the graph through insert and remove batches."""
+@pytest.mark.parametrize("pb", [0, 1])
+def test_pagerank_layouts_isolated_and_updates(B, pb):
+    """Isolated vertices and ties in degree; the relabelled copy, the XCD
+    slices and the propagation-blocking layout follow the graph through
+    insert and remove batches."""
     g = O.rmat_graph(12)
     G = B.CpmaGraph.from_csr(g.offsets, g.ids)
     G.set_tuning("pr_relabel", 1)
     G.set_tuning("pr_xcd", 1)
+    G.set_tuning("pr_pb", pb)
     s, d = O.update_rmat(g.n, 5000, 3)
     one = (s.astype(np.uint64) << np.uint64(32)) | d.astype(np.uint64)
     rev = (d.astype(np.uint64) << np.uint64(32)) | s.astype(np.uint64)
