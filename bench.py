@@ -268,7 +268,9 @@ def only_section(B, G, args, srcs, policy):
         G.set_direction_policy(policy)
         f = lambda: G.bfs_multi(srcs, want_parents=False)
     elif args.only == "pr":
-        f = lambda: G.pagerank(0.85, 20, 0.0, B.PR_FP32, want_p=False)
+        def f():
+            for prec in (B.PR_FP32, B.PR_FP64):
+                G.pagerank(0.85, 20, 0.0, prec, want_p=False)
     elif args.only == "map":
         one = np.zeros(n * 64, np.uint8)
         one[0] = 1
