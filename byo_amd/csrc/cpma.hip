@@ -133,17 +133,32 @@ __global__ void k_degrees_from_off(const uint64_t *off, uint64_t n, uint32_t *de
 // leaf index of each batch key: last l with sep[l] <= key.  The unique key
 // count *bu is read on the device (keys at and past it are skipped), so the
 // same launch serves a batch whose size is known only after deduplication.
+// The keys are sorted, so the leaves of a wave's 64 keys lie between the
+// leaves of its first and last key: lanes 0 and 1 search those two over all
+// leaves, and every lane then searches only that range.
+__device__ __forceinline__ uint64_t leaf_search(const uint64_t *sep, uint64_t lo, uint64_t hi, uint64_t k) {
+  // sep[lo] <= k < sep[hi] (sep[nleaves] = ~0 > any key)
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (sep[mid] <= k) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 __global__ void k_leaf_of(const uint64_t *keys, const unsigned long long *bu, const uint64_t *sep, uint64_t nleaves,
                           uint32_t *leaf) {
   const uint64_t b = *bu;
-  for (uint64_t i = global_tid(); i < b; i += grid_threads()) {
-    const uint64_t k = keys[i];
-    uint64_t lo = 0, hi = nleaves;  // sep[lo] <= k < sep[hi] (sep[nleaves] = ~0 > any key)
-    while (hi - lo > 1) {
-      uint64_t mid = (lo + hi) >> 1;
-      if (sep[mid] <= k) lo = mid; else hi = mid;
-    }
-    leaf[i] = (uint32_t)lo;
+  const int lane = lane_id();
+  const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
+  for (uint64_t base = wave * 64; base < b; base += nwaves * 64) {
+    const uint64_t i = base + lane;
+    const uint64_t last = std::min<uint64_t>(b - 1, base + 63);
+    const uint64_t k = keys[i < b ? i : last];
+    // lane 0: the leaf of the wave's first key, lane 1: of its last
+    const uint64_t kend = lane == 1 ? keys[last] : k;
+    const uint64_t mine = lane < 2 ? leaf_search(sep, 0, nleaves, kend) : 0;
+    const uint64_t l0 = __shfl(mine, 0, 64), l1 = __shfl(mine, 1, 64);
+    if (i < b) leaf[i] = (uint32_t)leaf_search(sep, l0, l1 + 1, k);
   }
 }
 
