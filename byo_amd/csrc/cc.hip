@@ -76,18 +76,26 @@ __global__ void k_cc_gather(const uint32_t *par, uint64_t n, uint32_t *out, int 
   if (i < k) out[i] = par[((uint64_t)i * 2654435761ull + 12345ull) % n];
 }
 
-// Remaining edges of vertices outside the giant component: one wave per
-// vertex (neighbours strided over lanes), grid-stride.
+// Remaining edges of vertices outside the giant component.  A wave takes 64
+// vertices, one per lane, finds which are outside the giant component (the
+// finds run side by side), then gives each of those its neighbours strided
+// over the lanes.  (A vertex the wave's earlier unites have just joined to
+// the giant component is still processed: unites are idempotent.)
 __global__ void k_cc_link_rest(const uint64_t *off, const uint32_t *ids, uint64_t n, uint32_t *par, uint32_t giant,
                                int r0) {
   const int lane = lane_id();
   const uint64_t wave = global_tid() >> 6, nwaves = grid_threads() >> 6;
-  for (uint64_t v = wave; v < n; v += nwaves) {
-    if (uf_find(par, (uint32_t)v) == giant) continue;
-    const uint64_t b = off[v] + r0, e = off[v + 1];
-    for (uint64_t k = b + lane; k < e; k += 64) {
-      const uint32_t u = ids[k];
-      if (u != kNone) uf_unite(par, (uint32_t)v, u);
+  for (uint64_t base = wave * 64; base < n; base += nwaves * 64) {
+    const uint64_t mine = base + lane;
+    unsigned long long out = __ballot(mine < n && uf_find(par, (uint32_t)mine) != giant);
+    while (out) {
+      const uint64_t v = base + __builtin_ctzll(out);
+      out &= out - 1;
+      const uint64_t b = off[v] + r0, e = off[v + 1];
+      for (uint64_t k = b + lane; k < e; k += 64) {
+        const uint32_t u = ids[k];
+        if (u != kNone) uf_unite(par, (uint32_t)v, u);
+      }
     }
   }
 }
