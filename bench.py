@@ -285,6 +285,9 @@ def only_section(B, G, args, srcs, policy):
         f = G.tc
     else:
         C = B.CpmaGraph.from_graph(G)
+        for kv in args.set:
+            k, v = kv.split("=", 1)
+            C.set_tuning(k, int(v))
         keys = B.update_rmat_keys(n, args.cpma_batch)
         d = B.DeviceBuffer(keys)
 
@@ -575,6 +578,9 @@ def main():
         else:
             H = B.CsrGraph.rmat(args.cpma_scale, args.ef)
         C = B.CpmaGraph.from_graph(H)
+        for kv in args.set:  # --set knobs reach the container's update path too
+            k, v = kv.split("=", 1)
+            C.set_tuning(k, int(v))
         keys = B.update_rmat_keys(C.N(), args.cpma_batch)
         dkeys = B.DeviceBuffer(keys)
         C.profile(True)
